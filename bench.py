@@ -1,0 +1,231 @@
+"""Benchmark: StaticMap int64->int64 Insert + Find at load factor 0.8 on
+2^28 requested slots (268,435,460 slots x 16 B = 4.3 GB) on one MI355X.
+
+One step = Clear + Insert of round(0.8 * capacity) distinct uniform 64-bit
+keys (value = key*7+1) + Find of 2^28 queries at a 50 % hit rate, all
+device-resident before the timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ops-collections_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--slots-log2", type=int, default=28)
+    ap.add_argument("--lf", type=float, default=0.8)
+    ap.add_argument("--group", type=int, default=0, help="lanes per key (0 = library default)")
+    ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (PMC byte-counter calibration)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def copy_calibration():
+    import torch
+    a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copied 3 x 1 GiB")
+
+
+def make_workload(torch, cap, lf, seed=1234):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    n_ins = int(round(lf * cap))
+    n_find = 1 << 28 if cap >= (1 << 28) else cap
+    # inserted keys in [0, 2^62); absent queries in [2^62, 2^63): the sentinel -1 never occurs
+    keys = torch.randint(0, 1 << 62, (n_ins,), device="cuda", dtype=torch.int64, generator=g)
+    keys = torch.unique(keys)  # distinct (a 2^28-of-2^62 draw almost never collides)
+    n_ins = keys.numel()
+    keys = keys[torch.randperm(n_ins, device="cuda", generator=g)]
+    vals = keys * 7 + 1
+    hits = keys[torch.randint(0, n_ins, (n_find // 2,), device="cuda", generator=g)]
+    miss = torch.randint(1 << 62, (1 << 63) - 1, (n_find - n_find // 2,), device="cuda", dtype=torch.int64,
+                         generator=g)
+    q = torch.cat([hits, miss])
+    q = q[torch.randperm(n_find, device="cuda", generator=g)]
+    return keys, vals, q
+
+
+def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
+    cap_req = 1 << args.slots_log2
+    m = mxco.StaticMap(cap_req, -1, -1, torch.int64, torch.int64, group_size=group)
+    cap = m.capacity()
+    keys, vals, q = make_workload(torch, cap, lf)
+    pairs = mxco.make_pairs(keys, vals)
+    out = torch.empty(q.numel(), dtype=torch.int64, device="cuda")
+    s = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+
+    def step(e=None):
+        if e:
+            e[0].record(s)
+        m.clear_async()
+        if e:
+            e[1].record(s)
+        m.insert_async(pairs)
+        if e:
+            e[2].record(s)
+        m.find_async(q, out=out)
+        if e:
+            e[3].record(s)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_clear = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    t_ins = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    t_find = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
+
+    res = {"capacity": cap, "n_insert": keys.numel(), "n_find": q.numel(), "wall_ms_per_step": wall * 1e3 / steps,
+           "clear_ms": t_clear, "insert_ms": t_ins, "find_ms": t_find, "group": group, "lf": lf}
+    if verify:
+        # Inserted keys hit with f(key); absent keys miss; no insert failed.
+        exp = torch.where(q < (1 << 62), q * 7 + 1, torch.full_like(q, -1))
+        res["verified"] = bool(torch.equal(out, exp))
+        m.clear()
+        res["insert_failed"] = m.insert(pairs)
+        ln_f, sl_f = m.probe_stats(q)
+        ln_i, sl_i = m.probe_stats(keys)
+        res.update({"find_lines": ln_f, "find_slots": sl_f, "insert_lines": ln_i, "insert_slots": sl_i})
+    del m
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(args):
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cap_req = 1 << 24
+    rng = np.random.default_rng(5)
+    t = O.OpenAddressing(np.int64, np.int64, cap_req, -1, -1)
+    n_ins = int(round(0.8 * t.capacity))
+    keys = rng.choice(1 << 62, n_ins, replace=False).astype(np.int64)
+    vals = keys * 7 + 1
+    q = np.concatenate([rng.choice(keys, cap_req // 2), rng.integers(1 << 62, (1 << 63) - 1, cap_req // 2)])
+    rng.shuffle(q)
+    t0 = time.perf_counter()
+    failed = t.mt_insert(keys, vals, threads)
+    t1 = time.perf_counter()
+    out = t.mt_find(q, threads)
+    t2 = time.perf_counter()
+    ok = failed == 0 and np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    keys_total = n_ins + len(q)
+    return {"value": keys_total / (t2 - t0) / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
+            "sample": "oracle restatement (std-thread-style pthreads, __atomic CAS) of StaticMap int64 Insert of "
+                      "%d keys + Find of %d queries (50%% hit) on a 2^24-slot table at LF 0.8" % (n_ins, len(q)),
+            "insert_s": t1 - t0, "find_s": t2 - t1, "verified": bool(ok), "cpu_model": model}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (see DESIGN.md on sharding)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    import mxco
+    log("bench: StaticMap int64 LF %.2f, 2^%d slots requested, group %d" % (args.lf, args.slots_log2, args.group))
+    r = run_gpu(args, torch, mxco, args.lf, args.group, args.steps, args.warmup)
+    log("bench: gpu done %s" % json.dumps(r))
+    n_keys = r["n_insert"] + r["n_find"]
+    ms = r["wall_ms_per_step"]
+    value = n_keys / (ms * 1e-3) / 1e9
+    # Algorithmic bytes (128 B per distinct line the reference's walk reads,
+    # plus the streamed key/value in and out).
+    find_bytes = 128.0 * r["find_lines"] + r["n_find"] * (8 + 8)
+    ins_bytes = 128.0 * r["insert_lines"] + r["n_insert"] * (128 + 16)
+    find_gbs = find_bytes / (r["find_ms"] * 1e-3) / 1e9
+    ins_gbs = ins_bytes / (r["insert_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "oa_traffic_i64_lf08.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf))
+    dominant_insert = r["insert_ms"] >= r["find_ms"]
+    roof_find = {"kernel": "oa_kernel<8,8,G,OP_FIND>", "bound": "hbm", "achieved": round(find_gbs, 1),
+                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(find_gbs / PEAK_HBM_GBS, 4),
+                 "traffic": traffic.get("find") if traffic else None,
+                 "lines_per_query": r["find_lines"] / r["n_find"], "launch_ms": r["find_ms"],
+                 "algorithmic_bytes": find_bytes}
+    roof_ins = {"kernel": "oa_kernel<8,8,G,OP_INSERT>", "bound": "hbm", "achieved": round(ins_gbs, 1),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ins_gbs / PEAK_HBM_GBS, 4),
+                "traffic": traffic.get("insert") if traffic else None,
+                "lines_per_key": r["insert_lines"] / r["n_insert"], "launch_ms": r["insert_ms"],
+                "algorithmic_bytes": ins_bytes}
+    line = {
+        "metric": "Gkeys/s StaticMap Insert/Find int64 @LF0.8; batched vs plain speedup; p99 ms",
+        "value": round(value, 4), "unit": "Gkeys/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (uniform 64-bit keys, seeded torch generator)",
+        "config": {"workload": "StaticMap int64->int64, %d slots (2^%d requested, bucket 5, linear probing, "
+                               "murmurhash3_32), LF %.2f: Clear + Insert %d + Find %d (50%% hit) per step"
+                               % (r["capacity"], args.slots_log2, args.lf, r["n_insert"], r["n_find"]),
+                   "parallelism": "single GPU"},
+        "insert_gkeys_s": round(r["n_insert"] / (r["insert_ms"] * 1e-3) / 1e9, 4),
+        "find_gkeys_s": round(r["n_find"] / (r["find_ms"] * 1e-3) / 1e9, 4),
+        "clear_ms": round(r["clear_ms"], 4),
+        "verified": r["verified"] and r["insert_failed"] == 0,
+        "roofline": roof_ins if dominant_insert else roof_find,
+        "roofline_other": roof_find if dominant_insert else roof_ins,
+        "group": r["group"],
+    }
+    if args.sweep:
+        sw = []
+        for lf in (0.5, 0.7, 0.8, 0.9):
+            for g in (1, 2, 4, 8):
+                x = run_gpu(args, torch, mxco, lf, g, 3, 1, verify=False)
+                sw.append({"lf": lf, "group": g, "insert_ms": round(x["insert_ms"], 3),
+                           "find_ms": round(x["find_ms"], 3),
+                           "insert_gkeys_s": round(x["n_insert"] / x["insert_ms"] / 1e6, 3),
+                           "find_gkeys_s": round(x["n_find"] / x["find_ms"] / 1e6, 3)})
+                log("sweep %s" % json.dumps(sw[-1]))
+        line["sweep"] = sw
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(args)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

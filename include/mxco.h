@@ -1,0 +1,196 @@
+/*
+ * mxco.h -- C ABI of libmxco.so, the MI355X (gfx950) implementation of the
+ * ops-collections concurrent-container bulk operations.
+ *
+ * Every entry point takes plain pointers, sizes and a hipStream_t.  All key,
+ * value, stencil and output buffers are caller-owned device pointers; the
+ * library never copies user data between host and device.  Each entry point
+ * cites the reference interface it replaces (file:line in ops-collections).
+ *
+ * Error model (reference: include/utility/kernel_launch_utils.h:19-29 and
+ * the throwing sites listed per function): every call returns an
+ * mxco_status.  The C++ headers in include/mxco/ turn the codes back into the
+ * exception types the reference throws (std::invalid_argument,
+ * std::length_error, std::bad_alloc, std::runtime_error, std::logic_error).
+ * mxco_last_error() returns the calling thread's last message.
+ *
+ * Threading: calls on one object must be serialised by the caller, exactly
+ * as in the reference (per-object scratch counters,
+ * include/detail/open_addressing/open_addressing_impl.h:548-558).
+ */
+#ifndef MXCO_H
+#define MXCO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MXCO_API __attribute__((visibility("default")))
+
+typedef enum mxco_status {
+    MXCO_OK = 0,
+    MXCO_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    MXCO_LENGTH_ERROR = 2,     /* std::length_error (batch beyond the index range) */
+    MXCO_OUT_OF_MEMORY = 3,    /* std::bad_alloc (device allocation failed) */
+    MXCO_RUNTIME_ERROR = 4,    /* std::runtime_error (HIP runtime failure) */
+    MXCO_LOGIC_ERROR = 5,      /* std::logic_error (moved-from / uninitialised object) */
+    MXCO_UNSUPPORTED = 6       /* configuration this build does not provide */
+} mxco_status;
+
+/* Last error message of the calling thread ("" after success). */
+MXCO_API const char* mxco_last_error(void);
+MXCO_API const char* mxco_status_string(mxco_status status);
+/* Library version, e.g. "0.1.0". */
+MXCO_API const char* mxco_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Shared enums                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* ProbingScheme (include/probing_scheme.h:37-122). */
+typedef enum mxco_probing { MXCO_PROBING_LINEAR = 0, MXCO_PROBING_DOUBLE = 1 } mxco_probing;
+
+/* Hasher families (include/hash_functions.h:17-30).  Seeds are fixed as in
+ * the reference, which rejects stateful functors
+ * (open_addressing_impl.h:527-532): seed 0, and seed 1 for the second hash
+ * of DoubleHashing (probing_scheme.h:99). */
+typedef enum mxco_hash {
+    MXCO_HASH_MURMUR3_32 = 0, /* murmurhash3_32<Key>, StaticMap default */
+    MXCO_HASH_XXHASH_32 = 1,  /* xxhash_32<Key>, StaticSet default */
+    MXCO_HASH_XXHASH_64 = 2,  /* xxhash_64<Key> */
+    MXCO_HASH_FMIX_32 = 3,    /* murmurhash3_fmix32<Key>, 4-byte keys */
+    MXCO_HASH_FMIX_64 = 4     /* murmurhash3_fmix64<Key>, 8-byte keys */
+} mxco_hash;
+
+/* KeyEqual: integral keys compare bitwise (== on the integer), floating
+ * keys with operator== on float/double (utility/math_utils.h:67).  The
+ * sentinel test is always bitwise (detail/bitwise_compare.h:133-145). */
+typedef enum mxco_key_kind { MXCO_KEY_INTEGRAL = 0, MXCO_KEY_FLOAT = 1 } mxco_key_kind;
+
+/* Built-in stencil predicates.  The reference takes a Predicate *type*
+ * default-constructed on the device (kernels.h:66); through a C ABI the
+ * predicate is one of these, applied to each stencil element. */
+typedef enum mxco_pred {
+    MXCO_PRED_ALWAYS = 0,  /* AlwaysTrue (open_addressing_impl.h:31) */
+    MXCO_PRED_NONZERO = 1, /* stencil != 0 */
+    MXCO_PRED_ODD = 2,     /* stencil % 2 != 0 */
+    MXCO_PRED_EVEN = 3     /* stencil % 2 == 0 */
+} mxco_pred;
+
+typedef struct mxco_stencil {
+    const void* data;   /* device pointer, one element per input */
+    uint32_t elem_size; /* 1, 4 or 8 bytes (signed integers) */
+    uint32_t pred;      /* mxco_pred */
+} mxco_stencil;
+
+/* ---------------------------------------------------------------------- */
+/* Open addressing: StaticMap (value_size > 0) and StaticSet (== 0)       */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_oa_desc {
+    uint32_t key_size;    /* 4 or 8 */
+    uint32_t value_size;  /* 0 (set), 4 or 8; slot = Pair<K,V> layout (pair.h:41) */
+    uint32_t key_kind;    /* mxco_key_kind */
+    uint32_t bucket_size; /* Storage<B>, 1..64 (storage.h:16-20) */
+    uint32_t probing;     /* mxco_probing */
+    uint32_t hash;        /* mxco_hash */
+    uint32_t group_size;  /* lanes per key: 0 = tuned default, or 1, 2, 4, 8 */
+    uint32_t reserved;
+} mxco_oa_desc;
+
+typedef struct mxco_oa mxco_oa;
+
+/* Capacity the container will use for a requested capacity
+ * (MakeValidExtent, detail/extent/extent.inl:21-63). */
+MXCO_API mxco_status mxco_valid_extent(uint64_t requested, uint32_t bucket_size, uint32_t probing,
+                                       uint64_t* capacity);
+
+/* StaticMap(capacity, emptyKey, emptyValue, ..., stream) static_map.h:78-84,
+ * StaticSet(capacity, emptyKey, ..., stream) static_set.h:73-78.
+ * empty_value is ignored for sets.  Allocates the slot array and clears it
+ * (synchronous, as the reference's constructor, open_addressing_impl.h:77-94). */
+MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity, const void* empty_key,
+                                    const void* empty_value, hipStream_t stream, mxco_oa** out);
+/* ~StaticMap (frees the slot array). */
+MXCO_API mxco_status mxco_oa_destroy(mxco_oa* table);
+/* Capacity() static_map.h:585, Data() static_map.h:604. */
+MXCO_API mxco_status mxco_oa_capacity(const mxco_oa* table, uint64_t* capacity);
+MXCO_API mxco_status mxco_oa_data(const mxco_oa* table, void** slots);
+MXCO_API mxco_status mxco_oa_slot_size(const mxco_oa* table, uint32_t* slot_size);
+
+/* SetErasedKey(erasedKey, stream) static_map.h:97. */
+MXCO_API mxco_status mxco_oa_set_erased_key(mxco_oa* table, const void* erased_key, hipStream_t stream);
+
+/* Clear / ClearAsync static_map.h:288-292 (open_addressing_impl.h:432-452). */
+MXCO_API mxco_status mxco_oa_clear(mxco_oa* table, hipStream_t stream, int sync);
+
+/* Insert / InsertAsync / InsertIf / InsertIfAsync
+ * (static_map.h:116,131,216,232; open_addressing_impl.h:102-221).
+ * pairs: n slot-layout elements.  stencil == NULL means AlwaysTrue.
+ * sync != 0: waits and stores the number of inputs not inserted
+ * (duplicates, the reserved empty key, table full) in *num_failed.
+ * A NULL pairs/stencil pointer counts every input as failed (sync) or does
+ * nothing (async), as the reference does. */
+MXCO_API mxco_status mxco_oa_insert(mxco_oa* table, const void* pairs, const mxco_stencil* stencil, uint64_t n,
+                                    hipStream_t stream, int sync, uint64_t* num_failed);
+
+/* InsertOrAssign(values, n, stream[, assignedNum]) / InsertOrAssignAsync
+ * (static_map.h:151-175; open_addressing_impl.h:113-157).  Maps only. */
+MXCO_API mxco_status mxco_oa_insert_or_assign(mxco_oa* table, const void* pairs, uint64_t n, hipStream_t stream,
+                                              int sync, uint64_t* num_failed, uint64_t* num_assigned);
+
+/* AssignIfAsync<StencilT, Predicate> (static_map.h:190; kernels.h:185-210):
+ * overwrites the payload of keys already present, never inserts.  Maps only. */
+MXCO_API mxco_status mxco_oa_assign_if(mxco_oa* table, const void* pairs, const mxco_stencil* stencil, uint64_t n,
+                                       hipStream_t stream, int sync);
+
+/* InsertAndFind / InsertAndFindAsync (static_map.h:256-271;
+ * open_addressing_ref_impl.h:302-330): out_found[i] = payload now stored
+ * for the key (map) or the stored key (set), out_inserted[i] = 1 when this
+ * input inserted it. */
+MXCO_API mxco_status mxco_oa_insert_and_find(mxco_oa* table, const void* pairs, void* out_found,
+                                             uint8_t* out_inserted, uint64_t n, hipStream_t stream, int sync);
+
+/* Find / FindAsync / FindIf / FindIfAsync (static_map.h:350-400;
+ * kernels.h:314-347): out[i] = payload (map) or stored key (set) of
+ * keys[i], or the empty value/key on a miss or a false predicate. */
+MXCO_API mxco_status mxco_oa_find(mxco_oa* table, const void* keys, const mxco_stencil* stencil, void* out,
+                                  uint64_t n, hipStream_t stream, int sync);
+
+/* Contains / ContainsAsync / ContainsIf / ContainsIfAsync
+ * (static_map.h:439-490; kernels.h:350-377): out[i] = 1 byte bool. */
+MXCO_API mxco_status mxco_oa_contains(mxco_oa* table, const void* keys, const mxco_stencil* stencil, uint8_t* out,
+                                      uint64_t n, hipStream_t stream, int sync);
+
+/* Count(keys, n, stream) static_map.h:575 (kernels.h:428-467): number of
+ * query keys present, synchronous. */
+MXCO_API mxco_status mxco_oa_count(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
+                                   uint64_t* count);
+
+/* Erase / EraseAsync (static_map.h:278-286; open_addressing_ref_impl.h:341-379):
+ * writes the tombstone set by mxco_oa_set_erased_key, or an empty slot when
+ * none is set.  sync != 0 stores the number of keys not erased. */
+MXCO_API mxco_status mxco_oa_erase(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream, int sync,
+                                   uint64_t* num_failed);
+
+/* Probe statistics of Find's walk over `keys` (no reference counterpart;
+ * a diagnostic for roofline accounting): *lines = distinct 128-byte lines
+ * and *slots = slots the reference's sequential walk reads
+ * (open_addressing_ref_impl.h:568-605), summed over the batch.  Synchronous. */
+MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
+                                         uint64_t* lines, uint64_t* slots);
+
+/* Number of compute units of the current device (the persistent grids and
+ * the tuner size themselves from it). */
+MXCO_API mxco_status mxco_device_cu_count(int* cus);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MXCO_H */

@@ -1,0 +1,467 @@
+// C ABI of StaticMap / StaticSet (include/mxco.h, mxco_oa_*).
+//
+// Host semantics follow OpenAddressingImpl
+// (include/detail/open_addressing/open_addressing_impl.h:58-559): zero-sized
+// batches are no-ops, null inputs count as failures on the synchronous
+// counting calls and are ignored by the asynchronous ones, oversized batches
+// raise a length error, and a synchronous call is the asynchronous call plus
+// a stream synchronisation and a read-back of the per-object counters.
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "mxco_common.h"
+#include "oa_launch.h"
+
+using namespace mxco;
+
+struct mxco_oa {
+    mxco_oa_desc d;
+    uint64_t cap;
+    uint32_t ss;
+    uint32_t voff;
+    uint8_t* slots;
+    unsigned long long* counters;       // device [failed, assigned]
+    unsigned long long* host_counters;  // pinned mirror
+    uint64_t empty_key;
+    uint64_t empty_val;
+    uint64_t erased_key;
+    bool has_tomb;
+    int device;
+    int group;
+    unsigned blocks_per_cu;
+};
+
+namespace {
+
+uint64_t load_le(const void* p, uint32_t n)
+{
+    uint64_t v = 0;
+    memcpy(&v, p, n);
+    return v;
+}
+
+void pair_layout(uint32_t ks, uint32_t vs, uint32_t* ss, uint32_t* voff)
+{
+    if (vs == 0) {
+        *ss = ks;
+        *voff = 0;
+        return;
+    }
+    *ss = (ks == 4 && vs == 4) ? 8 : 16;
+    *voff = (ks == 4 && vs == 4) ? 4 : 8;
+}
+
+int env_int(const char* name, int dflt)
+{
+    const char* s = getenv(name);
+    return s && *s ? atoi(s) : dflt;
+}
+
+// Lanes per key when the descriptor leaves it to the library.  Chosen by
+// the tuner's sweep (bench.py --tune) on MI355X: see the group-width table
+// in the documentation.
+int default_group(uint32_t ss)
+{
+    int g = env_int("MXCO_OA_GROUP", 0);
+    if (g == 1 || g == 2 || g == 4 || g == 8) return g;
+    return ss == 16 ? 2 : 4;
+}
+
+OaArgs base_args(const mxco_oa* t)
+{
+    OaArgs a;
+    memset(&a, 0, sizeof(a));
+    a.table = t->slots;
+    a.cap = (uint32_t)t->cap;
+    a.bucket = t->d.bucket_size;
+    a.nbuckets = (uint32_t)(t->cap / t->d.bucket_size);
+    a.probing = t->d.probing;
+    a.hash = t->d.hash;
+    a.key_float = t->d.key_kind == MXCO_KEY_FLOAT;
+    a.empty_key = t->empty_key;
+    a.empty_val = t->empty_val;
+    a.erased_key = t->has_tomb ? t->erased_key : t->empty_key;
+    a.has_tombstone = t->has_tomb;
+    return a;
+}
+
+mxco_status check_table(const mxco_oa* t, const char* fn)
+{
+    if (!t || !t->slots) return fail(MXCO_LOGIC_ERROR, std::string(fn) + ": moved-from or null container");
+    return MXCO_OK;
+}
+
+mxco_status check_batch(uint64_t n, const char* fn)
+{
+    if (n > max_batch())
+        return fail(MXCO_LENGTH_ERROR, std::string(fn) + ": batch size exceeds the kernel index range");
+    return MXCO_OK;
+}
+
+mxco_status set_stencil(OaArgs& a, const mxco_stencil* st, const char* fn)
+{
+    if (!st || st->pred == MXCO_PRED_ALWAYS) return MXCO_OK;
+    if (st->elem_size != 1 && st->elem_size != 4 && st->elem_size != 8)
+        return fail(MXCO_INVALID_ARGUMENT, std::string(fn) + ": stencil element size must be 1, 4 or 8");
+    if (st->pred > MXCO_PRED_EVEN) return fail(MXCO_INVALID_ARGUMENT, std::string(fn) + ": unknown predicate");
+    a.stencil = (const uint8_t*)st->data;
+    a.stencil_size = st->elem_size;
+    a.stencil_pred = st->pred;
+    return MXCO_OK;
+}
+
+mxco_status launch(const mxco_oa* t, int op, OaArgs& a, hipStream_t stream, const char* fn)
+{
+    const unsigned grid = grid_for(a.n * (uint64_t)t->group, t->blocks_per_cu);
+    hipError_t e = launch_oa(t->d.key_size, t->d.value_size, t->group, op, a, dim3(grid), stream);
+    if (e != hipSuccess) return hip_fail(e, fn);
+    return MXCO_OK;
+}
+
+mxco_status zero_counters(mxco_oa* t, hipStream_t stream)
+{
+    zero_counters_kernel<<<1, 64, 0, stream>>>(t->counters, 2);
+    MXCO_HIP(hipGetLastError(), "zero counters");
+    return MXCO_OK;
+}
+
+mxco_status read_counters(mxco_oa* t, hipStream_t stream, uint64_t* c0, uint64_t* c1)
+{
+    MXCO_HIP(hipMemcpyAsync(t->host_counters, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            stream),
+             "read counters");
+    MXCO_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (c0) *c0 = t->host_counters[0];
+    if (c1) *c1 = t->host_counters[1];
+    return MXCO_OK;
+}
+
+mxco_status fill(mxco_oa* t, hipStream_t stream)
+{
+    uint8_t img[16];
+    memset(img, 0, sizeof(img));
+    for (uint32_t off = 0; off < 16; off += t->ss) {
+        memcpy(img + off, &t->empty_key, t->d.key_size);
+        if (t->d.value_size) memcpy(img + off + t->voff, &t->empty_val, t->d.value_size);
+    }
+    u64x2 pat;
+    memcpy(&pat, img, 16);
+    const uint64_t bytes = t->cap * t->ss;
+    const unsigned grid = grid_for(bytes / 16, 8);
+    oa_fill_kernel<<<grid, 256, 0, stream>>>(t->slots, bytes, pat);
+    MXCO_HIP(hipGetLastError(), "Clear");
+    return MXCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity, const void* empty_key,
+                                    const void* empty_value, hipStream_t stream, mxco_oa** out)
+{
+    if (!desc || !out || !empty_key) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: null argument");
+    *out = nullptr;
+    const mxco_oa_desc& d = *desc;
+    if (d.key_size != 4 && d.key_size != 8)
+        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: key size must be 4 or 8 bytes");
+    if (d.value_size != 0 && d.value_size != 4 && d.value_size != 8)
+        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: payload size must be 0, 4 or 8 bytes");
+    if (d.value_size && !empty_value) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: null empty value");
+    if (d.bucket_size < 1 || d.bucket_size > 1024)
+        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: bucket size must be in [1, 1024]");
+    if (d.probing > MXCO_PROBING_DOUBLE) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: unknown probing scheme");
+    if (d.hash > MXCO_HASH_FMIX_64) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: unknown hash");
+    if ((d.hash == MXCO_HASH_FMIX_32 && d.key_size != 4) || (d.hash == MXCO_HASH_FMIX_64 && d.key_size != 8))
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: fmix hash width differs from the key width");
+    if (d.key_kind > MXCO_KEY_FLOAT) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: unknown key kind");
+    if (d.group_size != 0 && d.group_size != 1 && d.group_size != 2 && d.group_size != 4 && d.group_size != 8)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: group size must be 0, 1, 2, 4 or 8");
+
+    const uint64_t cap = valid_extent(capacity, d.bucket_size, d.probing);
+    if (cap == 0) return fail(MXCO_INVALID_ARGUMENT, "Invalid input Extent");
+    if (cap > 0xffffffffull)
+        return fail(MXCO_INVALID_ARGUMENT,
+                    "mxco_oa_create: capacity exceeds the 32-bit slot index of the device probe sequence");
+
+    mxco_oa* t = new (std::nothrow) mxco_oa();
+    if (!t) return fail(MXCO_OUT_OF_MEMORY, "mxco_oa_create: host allocation");
+    t->d = d;
+    t->cap = cap;
+    pair_layout(d.key_size, d.value_size, &t->ss, &t->voff);
+    t->empty_key = load_le(empty_key, d.key_size);
+    t->empty_val = d.value_size ? load_le(empty_value, d.value_size) : 0;
+    t->erased_key = t->empty_key;
+    t->has_tomb = false;
+    t->group = d.group_size ? (int)d.group_size : default_group(t->ss);
+    t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 16);
+    (void)hipGetDevice(&t->device);
+
+    hipError_t e = hipMalloc((void**)&t->slots, cap * t->ss);
+    if (e != hipSuccess) {
+        delete t;
+        return hip_fail(e, "mxco_oa_create: slot array");
+    }
+    e = hipMalloc((void**)&t->counters, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&t->host_counters, 2 * sizeof(unsigned long long), 0);
+    if (e != hipSuccess) {
+        (void)hipFree(t->slots);
+        if (t->counters) (void)hipFree(t->counters);
+        delete t;
+        return hip_fail(e, "mxco_oa_create: counters");
+    }
+    mxco_status st = fill(t, stream);
+    if (st == MXCO_OK) {
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) st = hip_fail(e, "mxco_oa_create: clear");
+    }
+    if (st != MXCO_OK) {
+        (void)hipFree(t->slots);
+        (void)hipFree(t->counters);
+        (void)hipHostFree(t->host_counters);
+        delete t;
+        return st;
+    }
+    *out = t;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_destroy(mxco_oa* t)
+{
+    if (!t) return ok();
+    if (t->slots) (void)hipFree(t->slots);
+    if (t->counters) (void)hipFree(t->counters);
+    if (t->host_counters) (void)hipHostFree(t->host_counters);
+    delete t;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_capacity(const mxco_oa* t, uint64_t* capacity)
+{
+    if (mxco_status s = check_table(t, "Capacity")) return s;
+    if (!capacity) return fail(MXCO_INVALID_ARGUMENT, "Capacity: null");
+    *capacity = t->cap;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_data(const mxco_oa* t, void** slots)
+{
+    if (mxco_status s = check_table(t, "Data")) return s;
+    if (!slots) return fail(MXCO_INVALID_ARGUMENT, "Data: null");
+    *slots = t->slots;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_slot_size(const mxco_oa* t, uint32_t* ss)
+{
+    if (mxco_status s = check_table(t, "slot_size")) return s;
+    if (!ss) return fail(MXCO_INVALID_ARGUMENT, "slot_size: null");
+    *ss = t->ss;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_set_erased_key(mxco_oa* t, const void* erased_key, hipStream_t stream)
+{
+    if (mxco_status s = check_table(t, "SetErasedKey")) return s;
+    if (!erased_key) return fail(MXCO_INVALID_ARGUMENT, "SetErasedKey: null key");
+    t->erased_key = load_le(erased_key, t->d.key_size);
+    t->has_tomb = t->erased_key != t->empty_key;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_clear(mxco_oa* t, hipStream_t stream, int sync)
+{
+    if (mxco_status s = check_table(t, "Clear")) return s;
+    if (mxco_status s = fill(t, stream)) return s;
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "Clear: hipStreamSynchronize");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_insert(mxco_oa* t, const void* pairs, const mxco_stencil* stencil, uint64_t n,
+                                    hipStream_t stream, int sync, uint64_t* num_failed)
+{
+    if (mxco_status s = check_table(t, "Insert")) return s;
+    if (num_failed) *num_failed = 0;
+    if (n == 0) return ok();
+    const bool stencil_null = stencil && stencil->pred != MXCO_PRED_ALWAYS && !stencil->data;
+    if (!pairs || stencil_null) {
+        if (sync && num_failed) *num_failed = n;
+        return ok();
+    }
+    if (mxco_status s = check_batch(n, "Insert")) return s;
+    OaArgs a = base_args(t);
+    if (mxco_status s = set_stencil(a, stencil, "InsertIf")) return s;
+    a.in = (const uint8_t*)pairs;
+    a.n = n;
+    if (sync) {
+        if (mxco_status s = zero_counters(t, stream)) return s;
+        a.counters = t->counters;
+    }
+    if (mxco_status s = launch(t, OP_INSERT, a, stream, "Insert")) return s;
+    if (sync) {
+        uint64_t f = 0;
+        if (mxco_status s = read_counters(t, stream, &f, nullptr)) return s;
+        if (num_failed) *num_failed = f;
+    }
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_insert_or_assign(mxco_oa* t, const void* pairs, uint64_t n, hipStream_t stream, int sync,
+                                              uint64_t* num_failed, uint64_t* num_assigned)
+{
+    if (mxco_status s = check_table(t, "InsertOrAssign")) return s;
+    if (t->d.value_size == 0) return fail(MXCO_UNSUPPORTED, "InsertOrAssign: containers with a payload only");
+    if (num_failed) *num_failed = 0;
+    if (num_assigned) *num_assigned = 0;
+    if (n == 0) return ok();
+    if (!pairs) {
+        if (sync && num_failed) *num_failed = n;
+        return ok();
+    }
+    if (mxco_status s = check_batch(n, "InsertOrAssign")) return s;
+    OaArgs a = base_args(t);
+    a.in = (const uint8_t*)pairs;
+    a.n = n;
+    if (sync) {
+        if (mxco_status s = zero_counters(t, stream)) return s;
+        a.counters = t->counters;
+    }
+    if (mxco_status s = launch(t, OP_INSERT_OR_ASSIGN, a, stream, "InsertOrAssign")) return s;
+    if (sync) {
+        uint64_t f = 0, g = 0;
+        if (mxco_status s = read_counters(t, stream, &f, &g)) return s;
+        if (num_failed) *num_failed = f;
+        if (num_assigned) *num_assigned = g;
+    }
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_assign_if(mxco_oa* t, const void* pairs, const mxco_stencil* stencil, uint64_t n,
+                                       hipStream_t stream, int sync)
+{
+    if (mxco_status s = check_table(t, "AssignIf")) return s;
+    if (t->d.value_size == 0) return fail(MXCO_UNSUPPORTED, "AssignIf: containers with a payload only");
+    if (n == 0 || !pairs) return ok();
+    if (stencil && stencil->pred != MXCO_PRED_ALWAYS && !stencil->data) return ok();
+    if (mxco_status s = check_batch(n, "AssignIf")) return s;
+    OaArgs a = base_args(t);
+    if (mxco_status s = set_stencil(a, stencil, "AssignIf")) return s;
+    a.in = (const uint8_t*)pairs;
+    a.n = n;
+    if (mxco_status s = launch(t, OP_ASSIGN, a, stream, "AssignIf")) return s;
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "AssignIf: hipStreamSynchronize");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_insert_and_find(mxco_oa* t, const void* pairs, void* out_found, uint8_t* out_inserted,
+                                             uint64_t n, hipStream_t stream, int sync)
+{
+    if (mxco_status s = check_table(t, "InsertAndFind")) return s;
+    if (n == 0 || !pairs) return ok();
+    if (!out_found || !out_inserted) return fail(MXCO_INVALID_ARGUMENT, "InsertAndFind: null output");
+    if (mxco_status s = check_batch(n, "InsertAndFind")) return s;
+    OaArgs a = base_args(t);
+    a.in = (const uint8_t*)pairs;
+    a.out = (uint8_t*)out_found;
+    a.out2 = out_inserted;
+    a.n = n;
+    if (mxco_status s = launch(t, OP_INSERT_AND_FIND, a, stream, "InsertAndFind")) return s;
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "InsertAndFind: hipStreamSynchronize");
+    return ok();
+}
+
+static mxco_status find_like(mxco_oa* t, int op, const void* keys, const mxco_stencil* stencil, void* out, uint64_t n,
+                             hipStream_t stream, int sync, const char* fn)
+{
+    if (mxco_status s = check_table(t, fn)) return s;
+    if (n == 0 || !keys || !out) return ok();
+    if (stencil && stencil->pred != MXCO_PRED_ALWAYS && !stencil->data) return ok();
+    if (mxco_status s = check_batch(n, fn)) return s;
+    OaArgs a = base_args(t);
+    if (mxco_status s = set_stencil(a, stencil, fn)) return s;
+    a.in = (const uint8_t*)keys;
+    a.out = (uint8_t*)out;
+    a.n = n;
+    if (mxco_status s = launch(t, op, a, stream, fn)) return s;
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_find(mxco_oa* t, const void* keys, const mxco_stencil* stencil, void* out, uint64_t n,
+                                  hipStream_t stream, int sync)
+{
+    return find_like(t, OP_FIND, keys, stencil, out, n, stream, sync, "Find");
+}
+
+MXCO_API mxco_status mxco_oa_contains(mxco_oa* t, const void* keys, const mxco_stencil* stencil, uint8_t* out,
+                                      uint64_t n, hipStream_t stream, int sync)
+{
+    return find_like(t, OP_CONTAINS, keys, stencil, out, n, stream, sync, "Contains");
+}
+
+MXCO_API mxco_status mxco_oa_count(mxco_oa* t, const void* keys, uint64_t n, hipStream_t stream, uint64_t* count)
+{
+    if (mxco_status s = check_table(t, "Count")) return s;
+    if (!count) return fail(MXCO_INVALID_ARGUMENT, "Count: null output");
+    *count = 0;
+    if (n == 0 || !keys) return ok();
+    if (mxco_status s = check_batch(n, "Count")) return s;
+    OaArgs a = base_args(t);
+    a.in = (const uint8_t*)keys;
+    a.n = n;
+    if (mxco_status s = zero_counters(t, stream)) return s;
+    a.counters = t->counters;
+    if (mxco_status s = launch(t, OP_COUNT, a, stream, "Count")) return s;
+    if (mxco_status s = read_counters(t, stream, count, nullptr)) return s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_erase(mxco_oa* t, const void* keys, uint64_t n, hipStream_t stream, int sync,
+                                   uint64_t* num_failed)
+{
+    if (mxco_status s = check_table(t, "Erase")) return s;
+    if (num_failed) *num_failed = 0;
+    if (n == 0) return ok();
+    if (!keys) {
+        if (sync && num_failed) *num_failed = n;
+        return ok();
+    }
+    if (mxco_status s = check_batch(n, "Erase")) return s;
+    OaArgs a = base_args(t);
+    a.in = (const uint8_t*)keys;
+    a.n = n;
+    if (sync) {
+        if (mxco_status s = zero_counters(t, stream)) return s;
+        a.counters = t->counters;
+    }
+    if (mxco_status s = launch(t, OP_ERASE, a, stream, "Erase")) return s;
+    if (sync) {
+        uint64_t f = 0;
+        if (mxco_status s = read_counters(t, stream, &f, nullptr)) return s;
+        if (num_failed) *num_failed = f;
+    }
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* t, const void* keys, uint64_t n, hipStream_t stream,
+                                         uint64_t* lines, uint64_t* slots)
+{
+    if (mxco_status s = check_table(t, "probe_stats")) return s;
+    if (!lines || !slots) return fail(MXCO_INVALID_ARGUMENT, "probe_stats: null output");
+    *lines = *slots = 0;
+    if (n == 0 || !keys) return ok();
+    if (mxco_status s = check_batch(n, "probe_stats")) return s;
+    OaArgs a = base_args(t);
+    a.in = (const uint8_t*)keys;
+    a.n = n;
+    if (mxco_status s = zero_counters(t, stream)) return s;
+    a.counters = t->counters;
+    if (mxco_status s = launch(t, OP_PROBE_STATS, a, stream, "probe_stats")) return s;
+    if (mxco_status s = read_counters(t, stream, lines, slots)) return s;
+    return ok();
+}
+
+}  // extern "C"

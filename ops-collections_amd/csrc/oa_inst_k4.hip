@@ -1,0 +1,9 @@
+// Open-addressing kernels for the 4-byte key / 0-byte payload slot layout.
+#include "oa_launch.h"
+
+namespace mxco {
+hipError_t launch_oa_k4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s)
+{
+    return launch_oa_layout<4, 0>(g, op, a, grid, s);
+}
+}  // namespace mxco

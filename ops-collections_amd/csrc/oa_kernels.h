@@ -1,0 +1,550 @@
+// Open-addressing bulk kernels (StaticMap / StaticSet) for gfx950.
+//
+// One kernel template serves every bulk operation.  A key is owned by a
+// cooperative group of G lanes (G = 1, 2, 4, 8) of a 64-lane wavefront.
+// The group walks the reference's probe sequence one *window* at a time:
+//   - LinearProbing: the walk is a contiguous slot scan from the start
+//     bucket, wrapping at the capacity (probing_scheme_impl.inl:76-84,
+//     118-128), so a window is the rest of the current 128-byte line;
+//   - DoubleHashing: a window is (a piece of) the current bucket
+//     (probing_scheme_impl.inl:177-189).
+// The G lanes load the window's slots with independent vector loads (one
+// 128-byte line per round trip at most), classify every slot, and take the
+// *first* slot in walk order that ends the scan with a wave ballot -- the
+// same slot the reference's sequential per-slot loop stops at
+// (open_addressing_ref_impl.h:490-605).  A failed CAS resumes the walk at
+// the next slot, as the reference's loop does.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mxco_hash.h"
+
+namespace mxco {
+
+enum OaOp : int {
+    OP_FIND = 0,
+    OP_CONTAINS = 1,
+    OP_COUNT = 2,
+    OP_INSERT = 3,
+    OP_INSERT_OR_ASSIGN = 4,
+    OP_ASSIGN = 5,
+    OP_INSERT_AND_FIND = 6,
+    OP_ERASE = 7,
+    OP_PROBE_STATS = 8,  // Find's walk, counting 128-byte lines and slots read
+};
+
+struct OaArgs {
+    uint8_t* table;
+    const uint8_t* in;       // slot-layout pairs (insert family) or keys
+    const uint8_t* stencil;  // nullptr = AlwaysTrue
+    uint8_t* out;            // Find values / Contains flags / InsertAndFind values
+    uint8_t* out2;           // InsertAndFind inserted flags
+    unsigned long long* counters;  // [0] failed / count, [1] assigned
+    uint64_t n;
+    uint64_t empty_key;
+    uint64_t empty_val;
+    uint64_t erased_key;
+    uint32_t cap;
+    uint32_t bucket;
+    uint32_t nbuckets;
+    uint32_t probing;  // 0 linear, 1 double
+    uint32_t hash;
+    uint32_t key_float;
+    uint32_t stencil_size;
+    uint32_t stencil_pred;
+    uint32_t has_tombstone;
+    uint32_t pad;
+};
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// Slot layouts: Pair<K,V> with alignas(min(16, bit_ceil(sK+sV)))
+// (include/pair.h:41, detail/pair/pair.inl:20-25); sets store the key only.
+template <int KS, int VS>
+struct Lay;
+
+template <>
+struct Lay<4, 4> {  // 8-byte slot {k32, v32}
+    static constexpr int SS = 8;
+    using Raw = uint64_t;
+    static __device__ uint64_t key(Raw r) { return (uint32_t)r; }
+    static __device__ uint64_t val(Raw r) { return r >> 32; }
+};
+template <>
+struct Lay<8, 8> {  // 16-byte slot {k64, v64}
+    static constexpr int SS = 16;
+    using Raw = u64x2;
+    static __device__ uint64_t key(Raw r) { return r.x; }
+    static __device__ uint64_t val(Raw r) { return r.y; }
+};
+template <>
+struct Lay<4, 8> {  // 16-byte slot {k32, pad32, v64}
+    static constexpr int SS = 16;
+    using Raw = u64x2;
+    static __device__ uint64_t key(Raw r) { return (uint32_t)r.x; }
+    static __device__ uint64_t val(Raw r) { return r.y; }
+};
+template <>
+struct Lay<8, 4> {  // 16-byte slot {k64, v32, pad32}
+    static constexpr int SS = 16;
+    using Raw = u64x2;
+    static __device__ uint64_t key(Raw r) { return r.x; }
+    static __device__ uint64_t val(Raw r) { return (uint32_t)r.y; }
+};
+template <>
+struct Lay<4, 0> {  // 4-byte set slot
+    static constexpr int SS = 4;
+    using Raw = uint32_t;
+    static __device__ uint64_t key(Raw r) { return r; }
+    static __device__ uint64_t val(Raw r) { return r; }
+};
+template <>
+struct Lay<8, 0> {  // 8-byte set slot
+    static constexpr int SS = 8;
+    using Raw = uint64_t;
+    static __device__ uint64_t key(Raw r) { return r; }
+    static __device__ uint64_t val(Raw r) { return r; }
+};
+
+template <int N>
+__device__ __forceinline__ uint64_t load_bits(const uint8_t* p)
+{
+    if constexpr (N == 4) return *(const uint32_t*)p;
+    else return *(const uint64_t*)p;
+}
+
+template <int N>
+__device__ __forceinline__ void store_bits(uint8_t* p, uint64_t v)
+{
+    if constexpr (N == 4) *(uint32_t*)p = (uint32_t)v;
+    else *(uint64_t*)p = v;
+}
+
+template <int N>
+__device__ __forceinline__ uint64_t cas_bits(uint8_t* p, uint64_t expected, uint64_t desired)
+{
+    if constexpr (N == 4) return atomicCAS((unsigned int*)p, (unsigned int)expected, (unsigned int)desired);
+    else return atomicCAS((unsigned long long*)p, (unsigned long long)expected, (unsigned long long)desired);
+}
+
+template <int N>
+__device__ __forceinline__ uint64_t atomic_load_bits(const uint8_t* p)
+{
+    if constexpr (N == 4)
+        return __hip_atomic_load((const unsigned int*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// KeyEqual = EqualTo<Key> (== on the key type, utility/math_utils.h:67).
+template <int KS>
+__device__ __forceinline__ bool key_eq(uint64_t a, uint64_t b, uint32_t is_float)
+{
+    if (is_float) {
+        if constexpr (KS == 4) return __uint_as_float((uint32_t)a) == __uint_as_float((uint32_t)b);
+        else return __longlong_as_double((long long)a) == __longlong_as_double((long long)b);
+    }
+    return a == b;
+}
+
+__device__ __forceinline__ bool stencil_ok(const OaArgs& a, uint64_t i)
+{
+    if (!a.stencil) return true;
+    int64_t s;
+    if (a.stencil_size == 1) s = (int8_t)a.stencil[i];
+    else if (a.stencil_size == 4) s = ((const int32_t*)a.stencil)[i];
+    else s = ((const int64_t*)a.stencil)[i];
+    switch (a.stencil_pred) {
+    case 1: return s != 0;
+    case 2: return (s & 1) != 0;
+    case 3: return (s & 1) == 0;
+    default: return true;
+    }
+}
+
+// The probe walk of one key.  All fields are uniform across the key's group.
+struct Walk {
+    uint32_t pos;     // next slot in walk order
+    uint32_t bucket;  // DoubleHashing: current bucket start
+    uint32_t init;    // DoubleHashing: first bucket start
+    uint32_t step;    // DoubleHashing: bucket stride
+    uint32_t left;    // LinearProbing: slots not yet scanned
+    bool done;
+};
+
+template <int KS>
+__device__ __forceinline__ Walk make_walk(const OaArgs& a, uint64_t key)
+{
+    Walk w;
+    const uint32_t h1 = probe_hash<KS>(a.hash, key, 0u);
+    const uint32_t start = (h1 % a.nbuckets) * a.bucket;
+    w.pos = start;
+    w.bucket = start;
+    w.init = start;
+    w.left = a.cap;
+    w.done = false;
+    if (a.probing) {
+        const uint32_t h2 = probe_hash<KS>(a.hash, key, 1u);
+        w.step = (h2 % (a.nbuckets - 1u) + 1u) * a.bucket;
+    } else {
+        w.step = a.bucket;
+    }
+    return w;
+}
+
+// Slots of the next window; never crosses a 128-byte line (linear) nor a
+// bucket end (double).
+template <int MAXW>
+__device__ __forceinline__ uint32_t window_len(const OaArgs& a, const Walk& w)
+{
+    if (a.probing == 0) {
+        uint32_t l = MAXW - (w.pos & (MAXW - 1));
+        l = min(l, a.cap - w.pos);
+        return min(l, w.left);
+    }
+    return min(w.bucket + a.bucket - w.pos, (uint32_t)MAXW);
+}
+
+__device__ __forceinline__ void consume(const OaArgs& a, Walk& w, uint32_t c)
+{
+    if (a.probing == 0) {
+        w.left -= c;
+        w.pos += c;
+        if (w.pos == a.cap) w.pos = 0;
+        w.done = (w.left == 0);
+    } else {
+        w.pos += c;
+        if (w.pos == w.bucket + a.bucket) {
+            // ProbingIterator::operator++ (probing_scheme_impl.inl:76-84)
+            const uint32_t rem = a.cap - w.step;
+            w.bucket = w.bucket >= rem ? w.bucket - rem : w.bucket + w.step;
+            w.pos = w.bucket;
+            w.done = (w.bucket == w.init);
+        }
+    }
+}
+
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr int SS = L::SS;
+    constexpr int SPL = 128 / SS;  // slots per 128-byte line
+    constexpr int K = SPL / G;     // slots per lane per window
+    constexpr bool kInsertFamily =
+        OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
+    constexpr int OUTS = VS ? VS : KS;  // Find / InsertAndFind output width
+    constexpr uint64_t kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+
+    const uint32_t lane = __lane_id();
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
+    const uint64_t ekey = a.empty_key;
+
+    unsigned long long cnt0 = 0, cnt1 = 0;
+
+    for (uint64_t i = gid; i < a.n; i += ngroups) {
+        const bool leader = gl == 0;
+        if (!stencil_ok(a, i)) {
+            if (leader) {
+                if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
+                if constexpr (OP == OP_CONTAINS) a.out[i] = 0;
+            }
+            continue;
+        }
+        uint64_t key, pval = 0;
+        if constexpr (kInsertFamily) {
+            key = load_bits<KS>(a.in + i * SS);
+            if constexpr (VS) pval = load_bits<VS>(a.in + i * SS + (SS == 8 ? 4 : 8));
+        } else {
+            key = load_bits<KS>(a.in + i * KS);
+        }
+        if constexpr (kInsertFamily) {
+            // The reserved empty key is never inserted (open_addressing_ref_impl.h:494-496).
+            if (key == ekey) {
+                if (leader) {
+                    if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN) cnt0++;
+                    if constexpr (OP == OP_INSERT_AND_FIND) {
+                        store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
+                        a.out2[i] = 0;
+                    }
+                }
+                continue;
+            }
+        }
+
+        Walk w = make_walk<KS>(a, key);
+        // Outcome: 0 = walk exhausted, 1 = resolved.
+        bool resolved = false;
+        uint64_t prev_line = ~0ull;  // OP_PROBE_STATS: last line counted for this key
+        while (!w.done) {
+            const uint32_t wlen = window_len<SPL>(a, w);
+            Raw s[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t off = gl + k * G;
+                if (off < wlen) s[k] = *(const Raw*)(a.table + (uint64_t)(w.pos + off) * SS);
+            }
+            // First slot (in walk order) of this lane that ends the walk.
+            int fk = K;
+            bool feq = false;
+            Raw ws = s[0];
+#pragma unroll
+            for (int k = K - 1; k >= 0; --k) {
+                const uint32_t off = gl + k * G;
+                if (off < wlen) {
+                    const uint64_t sk = L::key(s[k]);
+                    const bool em = sk == ekey;
+                    const bool eq = !em && key_eq<KS>(sk, key, a.key_float);
+                    bool ev;
+                    if constexpr (OP == OP_ERASE) ev = eq || (em && a.has_tombstone);
+                    else ev = eq || em;
+                    if (ev) {
+                        fk = k;
+                        feq = eq;
+                        ws = s[k];
+                    }
+                }
+            }
+            // Group-wide first event: lowest round, then lowest lane.
+            int wround = K;
+            uint32_t wlane = 0;
+            if constexpr (G == 1) {
+                wround = fk;
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint64_t m = (__ballot(fk == k) >> gbase) & kGroupMask;
+                    if (m) {
+                        wround = k;
+                        wlane = __builtin_ctzll(m);
+                        break;
+                    }
+                }
+            }
+            const uint32_t woff = wlane + wround * G;
+            if constexpr (OP == OP_PROBE_STATS) {
+                // Slots the reference's walk reads in this window: up to and
+                // including the slot that ends it.
+                const uint32_t used = (wround == K) ? wlen : woff + 1;
+                const uint64_t first = ((uint64_t)w.pos * SS) / 128;
+                const uint64_t last = ((uint64_t)(w.pos + used - 1) * SS + SS - 1) / 128;
+                if (leader) {
+                    cnt0 += last - first + 1 - (first == prev_line ? 1 : 0);
+                    cnt1 += used;
+                }
+                prev_line = last;
+            }
+            if (wround == K) {
+                consume(a, w, wlen);
+                continue;
+            }
+            const bool winner = (gl == wlane);
+            uint8_t* wslot = a.table + (uint64_t)(w.pos + woff) * SS;
+            bool weq;
+            if constexpr (G == 1) weq = feq;
+            else weq = ((__ballot(winner && feq) >> gbase) & kGroupMask) != 0;
+
+            if constexpr (OP == OP_FIND || OP == OP_CONTAINS || OP == OP_COUNT || OP == OP_PROBE_STATS) {
+                if (winner) {
+                    if constexpr (OP == OP_FIND) {
+                        store_bits<OUTS>(a.out + i * OUTS, weq ? L::val(ws) : (VS ? a.empty_val : ekey));
+                    } else if constexpr (OP == OP_CONTAINS) {
+                        a.out[i] = weq ? 1 : 0;
+                    } else {
+                        cnt0 += weq ? 1 : 0;
+                    }
+                }
+                resolved = true;
+                break;
+            } else if constexpr (OP == OP_ASSIGN) {
+                // Assign (open_addressing_ref_impl.h:384-410): stop at empty, write payload on match.
+                if (winner && weq) store_bits<VS>(wslot + (SS == 8 ? 4 : 8), pval);
+                resolved = true;
+                break;
+            } else if constexpr (OP == OP_ERASE) {
+                if (!weq) {  // empty slot with a tombstone configured: not present
+                    resolved = false;
+                    w.done = true;
+                    break;
+                }
+                // AttemptInsertStable(slot, content, erasedSlot) (ref_impl.h:359).
+                int r = 0;  // 0 failed (continue), 1 erased, 2 already erased
+                if (winner) {
+                    if constexpr (SS <= 8) {
+                        uint64_t expect, desired;
+                        if constexpr (VS) {
+                            expect = (uint64_t)ws;
+                            desired = (a.erased_key & 0xffffffffull) | (a.empty_val << 32);
+                        } else {
+                            expect = (uint64_t)ws;
+                            desired = a.erased_key;
+                        }
+                        const uint64_t old = cas_bits<SS>(wslot, expect, desired);
+                        if (old == expect) r = 1;
+                        else if (L::key((Raw)old) == a.erased_key) r = 2;
+                    } else {
+                        const uint64_t old = cas_bits<KS>(wslot, L::key(ws), a.erased_key);
+                        if (old == L::key(ws)) {
+                            store_bits<VS>(wslot + 8, a.empty_val);
+                            r = 1;
+                        } else if (old == a.erased_key) {
+                            r = 2;
+                        }
+                    }
+                }
+                uint32_t rr;
+                if constexpr (G == 1) rr = r;
+                else rr = (((__ballot(winner && r == 1) >> gbase) & kGroupMask) ? 1u : 0u) |
+                          (((__ballot(winner && r == 2) >> gbase) & kGroupMask) ? 2u : 0u);
+                if (rr == 1) {
+                    resolved = true;
+                    break;
+                }
+                if (rr == 2) {
+                    resolved = false;
+                    w.done = true;
+                    break;
+                }
+                consume(a, w, woff + 1);
+                continue;
+            } else {
+                // Insert family: EQUAL -> duplicate; AVAILABLE -> CAS attempt
+                // (InsertVerdict, open_addressing_ref_impl.h:490-532).
+                int r = 0;  // 0 lost to another key, 1 inserted, 2 duplicate
+                uint64_t seen = 0;  // stored key (set) / payload (8-byte map slot) on a duplicate
+                if (weq) {
+                    r = 2;
+                    seen = VS ? L::val(ws) : L::key(ws);
+                    if constexpr (OP == OP_INSERT_OR_ASSIGN) {
+                        // AssignPayload (ref_impl.h:210-220)
+                        if (winner) store_bits<VS>(wslot + (SS == 8 ? 4 : 8), pval);
+                    }
+                } else {
+                    if (winner) {
+                        if constexpr (SS <= 8) {
+                            // PackCas / PackCasInsertOrAssign: one CAS of the whole slot
+                            // against (emptyKey, emptyValue) (ref_impl.h:127-147, 234-257).
+                            uint64_t expect, desired;
+                            if constexpr (VS) {
+                                expect = (ekey & 0xffffffffull) | (a.empty_val << 32);
+                                desired = (key & 0xffffffffull) | (pval << 32);
+                            } else {
+                                expect = ekey;
+                                desired = key;
+                            }
+                            const uint64_t old = cas_bits<SS>(wslot, expect, desired);
+                            if (old == expect) {
+                                r = 1;
+                            } else if (key_eq<KS>(L::key((Raw)old), key, a.key_float)) {
+                                r = 2;
+                                seen = VS ? L::val((Raw)old) : L::key((Raw)old);
+                                if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<SS>(wslot, desired);
+                            }
+                        } else {
+                            // CasDependentWrite / BackToBackCasInsertOrAssign: CAS the key,
+                            // then store the payload (ref_impl.h:165-185, 260-279).
+                            const uint64_t old = cas_bits<KS>(wslot, ekey, key);
+                            if (old == ekey) {
+                                store_bits<VS>(wslot + 8, pval);
+                                r = 1;
+                            } else if (key_eq<KS>(old, key, a.key_float)) {
+                                r = 2;
+                                if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<VS>(wslot + 8, pval);
+                            }
+                        }
+                    }
+                    if constexpr (G > 1) {
+                        const uint64_t m1 = (__ballot(winner && r == 1) >> gbase) & kGroupMask;
+                        const uint64_t m2 = (__ballot(winner && r == 2) >> gbase) & kGroupMask;
+                        r = m1 ? 1 : (m2 ? 2 : 0);
+                    }
+                }
+                if (r == 0) {
+                    consume(a, w, woff + 1);
+                    continue;
+                }
+                if (leader) {
+                    if constexpr (OP == OP_INSERT) cnt0 += (r == 2);
+                    if constexpr (OP == OP_INSERT_OR_ASSIGN) cnt1 += (r == 2);
+                }
+                if constexpr (OP == OP_INSERT_AND_FIND) {
+                    if (winner) {
+                        uint64_t v;
+                        if (r == 1) {
+                            v = VS ? pval : key;
+                        } else if constexpr (VS && SS == 16) {
+                            // WaitForPayload (ref_impl.h:474-486): a racing inserter may
+                            // have stored the key but not yet the payload.  Bounded here.
+                            v = atomic_load_bits<VS>(wslot + 8);
+                            for (int spin = 0; spin < (1 << 22) && v == a.empty_val; ++spin)
+                                v = atomic_load_bits<VS>(wslot + 8);
+                        } else {
+                            v = seen;
+                        }
+                        store_bits<OUTS>(a.out + i * OUTS, v);
+                        a.out2[i] = (r == 1) ? 1 : 0;
+                    }
+                }
+                resolved = true;
+                break;
+            }
+        }
+        if (!resolved && leader) {
+            // Walk exhausted (full table / miss after a full cycle).
+            if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
+            if constexpr (OP == OP_CONTAINS) a.out[i] = 0;
+            if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ERASE) cnt0++;
+            if constexpr (OP == OP_INSERT_AND_FIND) {
+                store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
+                a.out2[i] = 0;
+            }
+        }
+    }
+
+    constexpr bool kTwoCounters = OP == OP_INSERT_OR_ASSIGN || OP == OP_PROBE_STATS;
+    if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_COUNT || OP == OP_ERASE ||
+                  OP == OP_PROBE_STATS) {
+        if (a.counters) {
+            // Wave reduction, one atomic per wave.
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                cnt0 += __shfl_xor(cnt0, o);
+                if constexpr (kTwoCounters) cnt1 += __shfl_xor(cnt1, o);
+            }
+            if (lane == 0) {
+                if (cnt0) atomicAdd(&a.counters[0], cnt0);
+                if constexpr (kTwoCounters) {
+                    if (cnt1) atomicAdd(&a.counters[1], cnt1);
+                }
+            }
+        }
+    }
+}
+
+// Sentinel fill (Clear / ClearAsync, open_addressing_impl.h:341-452):
+// streams the 16-byte replicated slot image over the whole slot array.
+static __global__ __launch_bounds__(256) void oa_fill_kernel(uint8_t* table, uint64_t bytes, u64x2 pattern)
+{
+    const uint64_t n16 = bytes / 16;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    u64x2* p = (u64x2*)table;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+        __builtin_nontemporal_store(pattern, p + i);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t* pw = (const uint32_t*)&pattern;
+        for (uint64_t b = n16 * 16; b < bytes; b += 4) *(uint32_t*)(table + b) = pw[(b % 16) / 4];
+    }
+}
+
+static __global__ void zero_counters_kernel(unsigned long long* c, int n)
+{
+    if (threadIdx.x < (unsigned)n) c[threadIdx.x] = 0;
+}
+
+}  // namespace mxco

@@ -1,0 +1,66 @@
+// Per-layout launchers of the open-addressing kernels.  Each slot layout is
+// instantiated in its own translation unit (oa_inst_*.hip) so the kernels
+// compile in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "oa_kernels.h"
+
+namespace mxco {
+
+template <int KS, int VS, int G>
+hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s)
+{
+    const dim3 block(256);
+    switch (op) {
+    case OP_FIND: oa_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a); break;
+    case OP_CONTAINS: oa_kernel<KS, VS, G, OP_CONTAINS><<<grid, block, 0, s>>>(a); break;
+    case OP_COUNT: oa_kernel<KS, VS, G, OP_COUNT><<<grid, block, 0, s>>>(a); break;
+    case OP_INSERT: oa_kernel<KS, VS, G, OP_INSERT><<<grid, block, 0, s>>>(a); break;
+    case OP_INSERT_AND_FIND: oa_kernel<KS, VS, G, OP_INSERT_AND_FIND><<<grid, block, 0, s>>>(a); break;
+    case OP_ERASE: oa_kernel<KS, VS, G, OP_ERASE><<<grid, block, 0, s>>>(a); break;
+    case OP_PROBE_STATS: oa_kernel<KS, VS, G, OP_PROBE_STATS><<<grid, block, 0, s>>>(a); break;
+    case OP_INSERT_OR_ASSIGN:
+        if constexpr (VS == 0) return hipErrorInvalidValue;
+        else oa_kernel<KS, VS, G, OP_INSERT_OR_ASSIGN><<<grid, block, 0, s>>>(a);
+        break;
+    case OP_ASSIGN:
+        if constexpr (VS == 0) return hipErrorInvalidValue;
+        else oa_kernel<KS, VS, G, OP_ASSIGN><<<grid, block, 0, s>>>(a);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int KS, int VS>
+hipError_t launch_oa_layout(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s)
+{
+    switch (g) {
+    case 1: return launch_oa_g<KS, VS, 1>(op, a, grid, s);
+    case 2: return launch_oa_g<KS, VS, 2>(op, a, grid, s);
+    case 4: return launch_oa_g<KS, VS, 4>(op, a, grid, s);
+    case 8: return launch_oa_g<KS, VS, 8>(op, a, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_oa_k4v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_oa_k8v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_oa_k4v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_oa_k8v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_oa_k4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_oa_k8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+
+inline hipError_t launch_oa(int ks, int vs, int g, int op, const OaArgs& a, dim3 grid, hipStream_t s)
+{
+    if (ks == 4 && vs == 4) return launch_oa_k4v4(g, op, a, grid, s);
+    if (ks == 8 && vs == 8) return launch_oa_k8v8(g, op, a, grid, s);
+    if (ks == 4 && vs == 8) return launch_oa_k4v8(g, op, a, grid, s);
+    if (ks == 8 && vs == 4) return launch_oa_k8v4(g, op, a, grid, s);
+    if (ks == 4 && vs == 0) return launch_oa_k4(g, op, a, grid, s);
+    if (ks == 8 && vs == 0) return launch_oa_k8(g, op, a, grid, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mxco

@@ -1,0 +1,340 @@
+"""Python front end of libmxco.so (ctypes over the C ABI in include/mxco.h).
+
+The classes mirror the reference's container API (aclco::StaticMap,
+StaticSet, StaticMultimap, BloomFilter, HyperLogLog): same method names
+(snake_case), same argument meaning, same error behaviour.  Device buffers
+are torch tensors on the GPU; streams default to torch's current stream.
+The library must be built (ops-collections_amd/build.py): importing this
+module without it raises ImportError -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmxco.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libmxco.so is not built (run ops-collections_amd/build.py): %s" % LIB_PATH)
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+vp = ctypes.c_void_p
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int
+pu64 = ctypes.POINTER(ctypes.c_uint64)
+
+OK, INVALID_ARGUMENT, LENGTH_ERROR, OUT_OF_MEMORY, RUNTIME_ERROR, LOGIC_ERROR, UNSUPPORTED = range(7)
+PROBING = {"linear": 0, "double": 1}
+HASH = {"murmurhash3_32": 0, "xxhash_32": 1, "xxhash_64": 2, "murmurhash3_fmix32": 3, "murmurhash3_fmix64": 4}
+PRED = {"always": 0, "nonzero": 1, "odd": 2, "even": 3}
+
+
+class LengthError(ValueError):
+    """std::length_error: batch beyond the kernel index range."""
+
+
+class LogicError(RuntimeError):
+    """std::logic_error: moved-from or uninitialised object."""
+
+
+_EXC = {INVALID_ARGUMENT: ValueError, LENGTH_ERROR: LengthError, OUT_OF_MEMORY: MemoryError,
+        RUNTIME_ERROR: RuntimeError, LOGIC_ERROR: LogicError, UNSUPPORTED: NotImplementedError}
+
+
+class OaDesc(ctypes.Structure):
+    _fields_ = [("key_size", u32), ("value_size", u32), ("key_kind", u32), ("bucket_size", u32),
+                ("probing", u32), ("hash", u32), ("group_size", u32), ("reserved", u32)]
+
+
+class Stencil(ctypes.Structure):
+    _fields_ = [("data", vp), ("elem_size", u32), ("pred", u32)]
+
+
+def _sig(name, args, res=i32):
+    f = getattr(_lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_lib.mxco_last_error.restype = ctypes.c_char_p
+_sig("mxco_valid_extent", [u64, u32, u32, pu64])
+_sig("mxco_device_cu_count", [ctypes.POINTER(i32)])
+_sig("mxco_oa_create", [ctypes.POINTER(OaDesc), u64, vp, vp, vp, ctypes.POINTER(vp)])
+_sig("mxco_oa_destroy", [vp])
+_sig("mxco_oa_capacity", [vp, pu64])
+_sig("mxco_oa_data", [vp, ctypes.POINTER(vp)])
+_sig("mxco_oa_slot_size", [vp, ctypes.POINTER(u32)])
+_sig("mxco_oa_set_erased_key", [vp, vp, vp])
+_sig("mxco_oa_clear", [vp, vp, i32])
+_sig("mxco_oa_insert", [vp, vp, ctypes.POINTER(Stencil), u64, vp, i32, pu64])
+_sig("mxco_oa_insert_or_assign", [vp, vp, u64, vp, i32, pu64, pu64])
+_sig("mxco_oa_assign_if", [vp, vp, ctypes.POINTER(Stencil), u64, vp, i32])
+_sig("mxco_oa_insert_and_find", [vp, vp, vp, vp, u64, vp, i32])
+_sig("mxco_oa_find", [vp, vp, ctypes.POINTER(Stencil), vp, u64, vp, i32])
+_sig("mxco_oa_contains", [vp, vp, ctypes.POINTER(Stencil), vp, u64, vp, i32])
+_sig("mxco_oa_count", [vp, vp, u64, vp, pu64])
+_sig("mxco_oa_erase", [vp, vp, u64, vp, i32, pu64])
+_sig("mxco_oa_probe_stats", [vp, vp, u64, vp, pu64, pu64])
+
+
+def lib():
+    return _lib
+
+
+def check(status):
+    if status != OK:
+        msg = _lib.mxco_last_error().decode()
+        raise _EXC.get(status, RuntimeError)(msg)
+
+
+def _stream(stream):
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, torch.cuda.Stream):
+        return stream.cuda_stream
+    return stream
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _scalar_bytes(value, dtype):
+    t = torch.tensor([value], dtype=dtype)
+    return ctypes.create_string_buffer(t.numpy().tobytes(), t.element_size())
+
+
+def _stencil(stencil, pred):
+    if stencil is None:
+        return None
+    s = Stencil()
+    s.data = _ptr(stencil)
+    s.elem_size = stencil.element_size() if isinstance(stencil, torch.Tensor) else 1
+    s.pred = PRED[pred] if isinstance(pred, str) else int(pred)
+    return ctypes.byref(s)
+
+
+def valid_extent(capacity, bucket_size=5, probing="linear"):
+    out = u64()
+    check(_lib.mxco_valid_extent(capacity, bucket_size, PROBING[probing], ctypes.byref(out)))
+    return out.value
+
+
+def cu_count():
+    n = i32()
+    check(_lib.mxco_device_cu_count(ctypes.byref(n)))
+    return n.value
+
+
+def slot_layout(key_dtype, value_dtype):
+    """(slot size, payload offset) of Pair<Key, T> (include/pair.h:41)."""
+    ks = torch.empty(0, dtype=key_dtype).element_size()
+    if value_dtype is None:
+        return ks, 0
+    vs = torch.empty(0, dtype=value_dtype).element_size()
+    if ks == 4 and vs == 4:
+        return 8, 4
+    return 16, 8
+
+
+def make_pairs(keys, values, key_dtype=None, value_dtype=None):
+    """Packs device keys/values into the slot layout: uint8 tensor (n, slot)."""
+    key_dtype = key_dtype or keys.dtype
+    value_dtype = value_dtype or values.dtype
+    ss, off = slot_layout(key_dtype, value_dtype)
+    n = keys.shape[0]
+    out = torch.zeros((n, ss), dtype=torch.uint8, device=keys.device)
+    ks = torch.empty(0, dtype=key_dtype).element_size()
+    vs = torch.empty(0, dtype=value_dtype).element_size()
+    out[:, :ks] = keys.to(key_dtype).contiguous().view(torch.uint8).view(n, ks)
+    out[:, off:off + vs] = values.to(value_dtype).contiguous().view(torch.uint8).view(n, vs)
+    return out
+
+
+class _OpenAddressing:
+    _set = False
+
+    def __init__(self, capacity, empty_key, empty_value, key_dtype, value_dtype, bucket_size, probing, hash,
+                 group_size, stream):
+        self.key_dtype = key_dtype
+        self.value_dtype = value_dtype
+        ks = torch.empty(0, dtype=key_dtype).element_size()
+        vs = 0 if value_dtype is None else torch.empty(0, dtype=value_dtype).element_size()
+        d = OaDesc(ks, vs, 1 if key_dtype.is_floating_point else 0, bucket_size, PROBING[probing], HASH[hash],
+                   group_size, 0)
+        self._ek = _scalar_bytes(empty_key, key_dtype)
+        self._ev = _scalar_bytes(empty_value, value_dtype) if value_dtype is not None else None
+        h = vp()
+        check(_lib.mxco_oa_create(ctypes.byref(d), capacity, self._ek, self._ev, _stream(stream), ctypes.byref(h)))
+        self._h = h
+        self.empty_key = empty_key
+        self.empty_value = empty_value
+        c, s = u64(), u32()
+        check(_lib.mxco_oa_capacity(self._h, ctypes.byref(c)))
+        check(_lib.mxco_oa_slot_size(self._h, ctypes.byref(s)))
+        self._capacity = c.value
+        self.slot_size = s.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.mxco_oa_destroy(h)
+            self._h = None
+
+    def capacity(self):
+        return self._capacity
+
+    def data(self):
+        p = vp()
+        check(_lib.mxco_oa_data(self._h, ctypes.byref(p)))
+        return p.value
+
+    def set_erased_key(self, erased_key, stream=None):
+        self._erk = _scalar_bytes(erased_key, self.key_dtype)
+        check(_lib.mxco_oa_set_erased_key(self._h, self._erk, _stream(stream)))
+
+    def clear(self, stream=None):
+        check(_lib.mxco_oa_clear(self._h, _stream(stream), 1))
+
+    def clear_async(self, stream=None):
+        check(_lib.mxco_oa_clear(self._h, _stream(stream), 0))
+
+    def insert(self, values, n=None, stream=None):
+        n = values.shape[0] if n is None else n
+        f = u64()
+        check(_lib.mxco_oa_insert(self._h, _ptr(values), None, n, _stream(stream), 1, ctypes.byref(f)))
+        return f.value
+
+    def insert_async(self, values, n=None, stream=None):
+        n = values.shape[0] if n is None else n
+        check(_lib.mxco_oa_insert(self._h, _ptr(values), None, n, _stream(stream), 0, None))
+
+    def insert_if(self, values, stencil, pred, n=None, stream=None):
+        n = values.shape[0] if n is None else n
+        f = u64()
+        check(_lib.mxco_oa_insert(self._h, _ptr(values), _stencil(stencil, pred), n, _stream(stream), 1,
+                                  ctypes.byref(f)))
+        return f.value
+
+    def insert_if_async(self, values, stencil, pred, n=None, stream=None):
+        n = values.shape[0] if n is None else n
+        check(_lib.mxco_oa_insert(self._h, _ptr(values), _stencil(stencil, pred), n, _stream(stream), 0, None))
+
+    def insert_and_find(self, values, out_find=None, out_inserted=None, n=None, stream=None, sync=True):
+        n = values.shape[0] if n is None else n
+        dev = values.device if isinstance(values, torch.Tensor) else "cuda"
+        od = self.value_dtype if self.value_dtype is not None else self.key_dtype
+        if out_find is None:
+            out_find = torch.empty(n, dtype=od, device=dev)
+        if out_inserted is None:
+            out_inserted = torch.empty(n, dtype=torch.bool, device=dev)
+        check(_lib.mxco_oa_insert_and_find(self._h, _ptr(values), _ptr(out_find), _ptr(out_inserted), n,
+                                           _stream(stream), int(sync)))
+        return out_find, out_inserted
+
+    def _out(self, keys, out, n, dtype):
+        if out is None:
+            dev = keys.device if isinstance(keys, torch.Tensor) else "cuda"
+            out = torch.empty(n, dtype=dtype, device=dev)
+        return out
+
+    def find(self, keys, out=None, n=None, stream=None, stencil=None, pred="always", sync=True):
+        n = keys.shape[0] if n is None else n
+        od = self.value_dtype if self.value_dtype is not None else self.key_dtype
+        out = self._out(keys, out, n, od)
+        check(_lib.mxco_oa_find(self._h, _ptr(keys), _stencil(stencil, pred), _ptr(out), n, _stream(stream),
+                                int(sync)))
+        return out
+
+    def find_async(self, keys, out=None, n=None, stream=None, stencil=None, pred="always"):
+        return self.find(keys, out, n, stream, stencil, pred, sync=False)
+
+    def contains(self, keys, out=None, n=None, stream=None, stencil=None, pred="always", sync=True):
+        n = keys.shape[0] if n is None else n
+        out = self._out(keys, out, n, torch.bool)
+        check(_lib.mxco_oa_contains(self._h, _ptr(keys), _stencil(stencil, pred), _ptr(out), n, _stream(stream),
+                                    int(sync)))
+        return out
+
+    def contains_async(self, keys, out=None, n=None, stream=None, stencil=None, pred="always"):
+        return self.contains(keys, out, n, stream, stencil, pred, sync=False)
+
+    def count(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        c = u64()
+        check(_lib.mxco_oa_count(self._h, _ptr(keys), n, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def erase(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        f = u64()
+        check(_lib.mxco_oa_erase(self._h, _ptr(keys), n, _stream(stream), 1, ctypes.byref(f)))
+        return f.value
+
+    def erase_async(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        check(_lib.mxco_oa_erase(self._h, _ptr(keys), n, _stream(stream), 0, None))
+
+    def probe_stats(self, keys, n=None, stream=None):
+        """(lines, slots) the reference's Find walk reads for `keys`."""
+        n = keys.shape[0] if n is None else n
+        ln, sl = u64(), u64()
+        check(_lib.mxco_oa_probe_stats(self._h, _ptr(keys), n, _stream(stream), ctypes.byref(ln), ctypes.byref(sl)))
+        return ln.value, sl.value
+
+    def slots(self):
+        """Device copy of the raw slot array as uint8 (capacity, slot_size)."""
+        n = self._capacity * self.slot_size
+        out = torch.empty(n, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        src = self.data()
+        # Device-to-device copy through the HIP runtime (cudart is hipart here).
+        torch.cuda.cudart().cudaMemcpy(out.data_ptr(), src, n, 3)
+        return out.view(self._capacity, self.slot_size)
+
+
+class StaticMap(_OpenAddressing):
+    """aclco::StaticMap<Key, T> (include/static_map.h:29-35).
+
+    Defaults: LinearProbing<murmurhash3_32<Key>>, Storage<5>."""
+
+    def __init__(self, capacity, empty_key, empty_value, key_dtype=torch.int32, value_dtype=torch.int32,
+                 bucket_size=5, probing="linear", hash="murmurhash3_32", group_size=0, stream=None):
+        super().__init__(capacity, empty_key, empty_value, key_dtype, value_dtype, bucket_size, probing, hash,
+                         group_size, stream)
+
+    def insert_or_assign(self, values, n=None, stream=None):
+        """Returns (failed, assigned)."""
+        n = values.shape[0] if n is None else n
+        f, a = u64(), u64()
+        check(_lib.mxco_oa_insert_or_assign(self._h, _ptr(values), n, _stream(stream), 1, ctypes.byref(f),
+                                            ctypes.byref(a)))
+        return f.value, a.value
+
+    def insert_or_assign_async(self, values, n=None, stream=None):
+        n = values.shape[0] if n is None else n
+        check(_lib.mxco_oa_insert_or_assign(self._h, _ptr(values), n, _stream(stream), 0, None, None))
+
+    def assign_if_async(self, values, stencil, pred, n=None, stream=None):
+        n = values.shape[0] if n is None else n
+        check(_lib.mxco_oa_assign_if(self._h, _ptr(values), _stencil(stencil, pred), n, _stream(stream), 0))
+
+
+class StaticSet(_OpenAddressing):
+    """aclco::StaticSet<Key> (include/static_set.h:27-32).
+
+    Defaults: DoubleHashing<xxhash_32<Key>>, Storage<5>."""
+    _set = True
+
+    def __init__(self, capacity, empty_key, key_dtype=torch.int32, bucket_size=5, probing="double",
+                 hash="xxhash_32", group_size=0, stream=None):
+        super().__init__(capacity, empty_key, None, key_dtype, None, bucket_size, probing, hash, group_size,
+                         stream)

@@ -1,0 +1,346 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Every function of oracle/oracle.c is checked here against something that is
+not a retyping of its own formula: the reference's known-answer tests, the
+published test vectors of the hash algorithms, an independent hash library
+(python-xxhash), the SHA-256 of the reference's prime table, brute force on
+tiny inputs, and closed forms (dict goldens for the map semantics, linear
+counting for HyperLogLog).
+"""
+import hashlib
+import math
+import struct
+
+import numpy as np
+import pytest
+import xxhash
+
+import oracle as O
+
+L = O.lib()
+
+
+# --------------------------------------------------------------------------
+# Hash functions
+# --------------------------------------------------------------------------
+
+def test_reference_hash_kats():
+    # tests/utility/hash_test.cpp:17-46
+    assert L.orc_fmix32(0x41424344, 0xFFFFEEEE) == 0x570DF661
+    assert L.orc_fmix64(0x41424344, 0xFFFFEEEE) == 0x5EE45A8EB612F708
+    assert L.orc_murmur3_32(struct.pack("<I", 0x41424344), 4, 0xFFFFEEEE) == 0xD6BDBBB4
+    assert L.orc_xxh32(struct.pack("<I", 0x75BCD15), 4, 0) == 0xB20A85EE
+    assert L.orc_xxh32(struct.pack("<I", 0x2A), 4, 0) == 0x454235D1
+    # tests/utility/xxhash64_test.cpp:90-103
+    for key, seed, exp in [(0, 0, 4246796580750024372), (0, 42, 3614696996920510707),
+                           (42, 0, 15516826743637085169), (123456789, 0, 9462334144942111946)]:
+        assert L.orc_xxh64(struct.pack("<I", key), 4, seed) == exp
+    for key, seed, exp in [(0, 0, 3803688792395291579), (0, 42, 13194218611613725804),
+                           (42, 0, 13066772586158965587), (123456789, 0, 14662639848940634189)]:
+        assert L.orc_xxh64(struct.pack("<Q", key), 8, seed) == exp
+
+
+def test_murmur3_published_vectors():
+    # SMHasher MurmurHash3_x86_32 verification strings.
+    cases = [(b"", 0, 0), (b"", 1, 0x514E28B7), (b"", 0xFFFFFFFF, 0x81F16F39),
+             (b"\0\0\0\0", 0, 0x2362F9DE), (b"aaaa", 0x9747B28C, 0x5A97808A),
+             (b"Hello, world!", 0x9747B28C, 0x24884CBA),
+             (b"The quick brown fox jumps over the lazy dog", 0x9747B28C, 0x2FA826CD)]
+    for data, seed, exp in cases:
+        assert L.orc_murmur3_32(data, len(data), seed) == exp, (data, seed)
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 7, 8, 15, 16, 31, 32, 33, 64, 100])
+def test_xxhash_against_library(n):
+    rng = np.random.default_rng(n)
+    for _ in range(20):
+        data = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        seed = int(rng.integers(0, 2**32))
+        assert L.orc_xxh32(data, n, seed) == xxhash.xxh32_intdigest(data, seed)
+        assert L.orc_xxh64(data, n, seed) == xxhash.xxh64_intdigest(data, seed)
+
+
+def test_fmix_matches_murmur_finalizer():
+    # fmix32 of the length-xored state is the tail of murmur3 over zero blocks:
+    # murmur3("", seed) = fmix32(seed ^ 0).
+    for seed in [0, 1, 0xDEADBEEF, 0xFFFFFFFF]:
+        assert L.orc_murmur3_32(b"", 0, seed) == L.orc_fmix32(seed, 0)
+
+
+# --------------------------------------------------------------------------
+# Capacity primes and MakeValidExtent
+# --------------------------------------------------------------------------
+
+REF_PRIME_SHA256 = "614c7e6da9ec95170031f33739827f56b33c69b2b9854fd7a5585467c9ac8b43"
+
+
+def test_prime_table_matches_reference_sha256():
+    p = O.primes()
+    assert len(p) == 140741
+    assert p[0] == 2 and p[-1] == 17177758133
+    assert hashlib.sha256(p.astype("<i8").tobytes()).hexdigest() == REF_PRIME_SHA256
+
+
+def test_primes_are_prime_and_sorted():
+    p = O.primes()
+    assert np.all(np.diff(p) > 0)
+    # brute-force primality on the small members
+    small = p[p < 200000]
+    sieve = np.ones(200000, bool)
+    sieve[:2] = False
+    for i in range(2, 448):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    assert sieve[small].all()
+
+
+def test_extent_kats():
+    # tests/utility/extent_test.cpp
+    assert [O.valid_extent(c, 5, O.PROBE_LINEAR) for c in (0, 1, 5, 6, 7, 20)] == [10, 5, 5, 10, 10, 20]
+    assert O.valid_extent(20, 5, O.PROBE_DOUBLE) == 25
+    assert O.valid_extent(8, 1, O.PROBE_DOUBLE) == 11
+    assert O.valid_extent(21, 3, O.PROBE_DOUBLE) == 21
+    # workloads: 2^20 / 2^28 requested slots, bucket 5
+    assert O.valid_extent(1 << 20, 5, O.PROBE_LINEAR) == 1048580
+    assert O.valid_extent(1 << 28, 5, O.PROBE_LINEAR) == 268435460
+    # beyond the largest prime: the reference throws
+    assert O.valid_extent(17177758134 * 5, 5, O.PROBE_DOUBLE) == 0
+
+
+def test_pair_layout():
+    # Pair<F,S> alignas(min(16, bit_ceil(sF+sS))) with natural member alignment.
+    assert O.pair_layout(4, 4) == (8, 4)
+    assert O.pair_layout(8, 8) == (16, 8)
+    assert O.pair_layout(4, 8) == (16, 8)
+    assert O.pair_layout(8, 4) == (16, 8)
+    assert O.pair_layout(4, 0) == (4, 0)
+
+
+# --------------------------------------------------------------------------
+# Probing
+# --------------------------------------------------------------------------
+
+def _probe(t, key):
+    s, st = O.ctypes.c_uint32(), O.ctypes.c_uint32()
+    k = np.array([key], dtype=t.kd)
+    L.orc_oa_probe_start(t.t, O._ptr(k), O.ctypes.byref(s), O.ctypes.byref(st))
+    return s.value, st.value
+
+
+def test_probe_start_closed_form():
+    t = O.OpenAddressing(np.int32, np.int32, 1000, -1, -1, bucket=5)
+    nb = t.capacity // 5
+    for key in [0, 1, 42, 123456789, -7]:
+        h = L.orc_murmur3_32(struct.pack("<i", key), 4, 0)
+        assert _probe(t, key) == ((h % nb) * 5, 5)
+    d = O.OpenAddressing(np.int32, None, 1000, -1, None, bucket=5, probing=O.PROBE_DOUBLE,
+                         hash_family=O.HASH_XXHASH_32)
+    nb = d.capacity // 5
+    for key in [0, 1, 42, 123456789]:
+        h1 = L.orc_xxh32(struct.pack("<i", key), 4, 0)
+        h2 = L.orc_xxh32(struct.pack("<i", key), 4, 1)
+        assert _probe(d, key) == ((h1 % nb) * 5, (h2 % (nb - 1) + 1) * 5)
+
+
+@pytest.mark.parametrize("probing,hashf", [(O.PROBE_LINEAR, O.HASH_MURMUR3_32), (O.PROBE_DOUBLE, O.HASH_XXHASH_32)])
+def test_full_cycle_fills_every_slot(probing, hashf):
+    # tests/utility/probing_scheme_test.cpp: the walk visits every bucket, so
+    # `capacity` distinct keys fill the table and one more fails.
+    t = O.OpenAddressing(np.int32, np.int32, 57, -1, -1, bucket=3, probing=probing, hash_family=hashf)
+    cap = t.capacity
+    keys = np.arange(cap + 1, dtype=np.int32) * 7919
+    assert t.insert(keys[:cap], keys[:cap]) == 0
+    assert t.insert(keys[cap:], keys[cap:]) == 1
+    slots = t.slots().view(np.int32).reshape(cap, 2)
+    assert set(slots[:, 0].tolist()) == set(keys[:cap].tolist())
+
+
+# --------------------------------------------------------------------------
+# Map / set semantics against dict goldens (tests/common/golden.h:22-150)
+# --------------------------------------------------------------------------
+
+def _golden_insert(keys, values, empty_key):
+    g, failed = {}, 0
+    for k, v in zip(keys.tolist(), values.tolist()):
+        if k == empty_key or k in g:
+            failed += 1
+        else:
+            g[k] = v
+    return g, failed
+
+
+@pytest.mark.parametrize("kd,vd", [(np.int32, np.int32), (np.int64, np.int64), (np.int32, np.int64),
+                                   (np.int64, np.int32)])
+@pytest.mark.parametrize("probing", [O.PROBE_LINEAR, O.PROBE_DOUBLE])
+def test_map_semantics_vs_dict(kd, vd, probing):
+    rng = np.random.default_rng(1)
+    keys = rng.integers(0, 3000, 4000).astype(kd)
+    keys[::97] = -1  # reserved empty key
+    vals = rng.integers(-10**6, 10**6, 4000).astype(vd)
+    t = O.OpenAddressing(kd, vd, 5000, -1, -1, bucket=5, probing=probing)
+    g, failed = _golden_insert(keys, vals, -1)
+    assert t.insert(keys, vals) == failed
+    q = np.concatenate([keys, rng.integers(3000, 6000, 1000).astype(kd)])
+    out, hit = t.find(q)
+    exp = np.array([g.get(k, -1) for k in q.tolist()], dtype=vd)
+    np.testing.assert_array_equal(out, exp)
+    np.testing.assert_array_equal(hit, np.array([k in g for k in q.tolist()]))
+
+
+def test_insert_or_assign_and_insert_and_find_vs_dict():
+    rng = np.random.default_rng(2)
+    keys = rng.integers(0, 500, 2000).astype(np.int64)
+    vals = rng.integers(0, 10**9, 2000).astype(np.int64)
+    t = O.OpenAddressing(np.int64, np.int64, 1000, -1, -1)
+    # InsertOrAssign: last writer wins sequentially, assigned counts existing keys
+    g, assigned = {}, 0
+    for k, v in zip(keys.tolist(), vals.tolist()):
+        assigned += k in g
+        g[k] = v
+    f, a = t.insert_or_assign(keys, vals)
+    assert (f, a) == (0, assigned)
+    out, hit = t.find(np.arange(500, dtype=np.int64))
+    np.testing.assert_array_equal(out, np.array([g.get(k, -1) for k in range(500)]))
+    # InsertAndFind: first occurrence inserts and returns its value, later ones see it
+    t2 = O.OpenAddressing(np.int64, np.int64, 1000, -1, -1)
+    found, inserted = t2.insert_and_find(keys, vals)
+    first = {}
+    exp_f, exp_i = [], []
+    for k, v in zip(keys.tolist(), vals.tolist()):
+        if k not in first:
+            first[k] = v
+            exp_i.append(True)
+        else:
+            exp_i.append(False)
+        exp_f.append(first[k])
+    np.testing.assert_array_equal(found, exp_f)
+    np.testing.assert_array_equal(inserted, exp_i)
+
+
+def test_negative_counts():
+    # n > capacity -> fail == n - capacity; reserved key -> failed
+    t = O.OpenAddressing(np.int32, np.int32, 10, -1, -1)
+    cap = t.capacity
+    keys = np.arange(cap + 7, dtype=np.int32)
+    assert t.insert(keys, keys) == 7
+    t2 = O.OpenAddressing(np.int32, np.int32, 10, -1, -1)
+    assert t2.insert(np.array([-1, -1, 3], np.int32), np.array([1, 2, 3], np.int32)) == 2
+
+
+def test_erase_tombstone_and_plain():
+    keys = np.arange(100, dtype=np.int32)
+    t = O.OpenAddressing(np.int32, np.int32, 200, -1, -1)
+    t.insert(keys, keys)
+    assert t.erase(keys[:50], erased_key=-2) == 0
+    out, hit = t.find(keys)
+    assert not hit[:50].any() and hit[50:].all()
+    assert t.erase(keys[:50], erased_key=-2) == 50
+
+
+def test_set_semantics():
+    rng = np.random.default_rng(3)
+    keys = rng.integers(0, 1000, 3000).astype(np.int32)
+    t = O.OpenAddressing(np.int32, None, 2000, -1, None, probing=O.PROBE_DOUBLE, hash_family=O.HASH_XXHASH_32)
+    failed = 3000 - len(set(keys.tolist()))
+    assert t.insert(keys) == failed
+    q = np.arange(-5, 1500, dtype=np.int32)
+    out, hit = t.find(q)
+    s = set(keys.tolist())
+    np.testing.assert_array_equal(hit, [k in s for k in q.tolist()])
+    np.testing.assert_array_equal(out, [k if k in s else -1 for k in q.tolist()])
+
+
+def test_multithreaded_restatement_matches_sequential():
+    rng = np.random.default_rng(4)
+    keys = rng.choice(2**31 - 2, 50000, replace=False).astype(np.int64)
+    vals = keys * 3 + 1
+    a = O.OpenAddressing(np.int64, np.int64, 100000, -1, -1)
+    b = O.OpenAddressing(np.int64, np.int64, 100000, -1, -1)
+    assert a.insert(keys, vals) == b.mt_insert(keys, vals, 4) == 0
+    q = np.concatenate([keys, keys + 2**40])
+    np.testing.assert_array_equal(a.find(q)[0], b.mt_find(q, 4))
+
+
+def test_find_line_count_bounds():
+    # every walk reads at least one line; a hit at LF 0.5 rarely reads more than two
+    rng = np.random.default_rng(5)
+    keys = rng.choice(2**31 - 2, 20000, replace=False).astype(np.int64)
+    t = O.OpenAddressing(np.int64, np.int64, 40000, -1, -1)
+    t.insert(keys, keys)
+    _, _, lines = t.find(keys, lines=True)
+    assert 20000 <= lines < 20000 * 2
+
+
+# --------------------------------------------------------------------------
+# Bloom filter: bit-level restatement with an independent XXH64
+# --------------------------------------------------------------------------
+
+SALTS = [0x47B6137B, 0x44974D91, 0x8824AD5B, 0xA2B7289D, 0x705495C7, 0x2DF1424B, 0x9EFC4947, 0x5C6BFB31]
+
+
+def _py_bloom_add(words, num_blocks, key_bytes):
+    h = xxhash.xxh64_intdigest(key_bytes, 0)
+    blk = ((h >> 32) * num_blocks) >> 32
+    lo = h & 0xFFFFFFFF
+    for w in range(8):
+        bit = ((SALTS[w] * lo) & 0xFFFFFFFF) >> 27
+        words[blk * 8 + w] |= 1 << bit
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint32, np.float32])
+def test_bloom_vs_bitwise_restatement(dtype):
+    rng = np.random.default_rng(6)
+    nb = 37
+    keys = rng.integers(-10**6, 10**6, 3000).astype(dtype)
+    words = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(words, nb, keys)
+    ref = [0] * (nb * 8)
+    for k in keys:
+        _py_bloom_add(ref, nb, k.tobytes())
+    np.testing.assert_array_equal(words, np.array(ref, np.uint32))
+    # no false negatives; contains == all pattern bits present
+    assert O.bloom_contains(words, nb, keys).all()
+
+
+# --------------------------------------------------------------------------
+# HyperLogLog
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_hll_registers_closed_form(dtype):
+    # tests/hyperloglog/add_test.cpp:321-335 GoldenAdd
+    p = 13
+    rng = np.random.default_rng(7)
+    keys = rng.integers(-2**31, 2**31 - 1, 5000).astype(dtype)
+    regs = np.zeros(1 << p, np.uint8)
+    O.hll_add(regs, p, keys)
+    ref = np.zeros(1 << p, np.uint8)
+    for k in keys:
+        h = xxhash.xxh64_intdigest(k.tobytes(), 0)
+        idx = h >> (64 - p)
+        w = ((h << p) & (2**64 - 1)) | (1 << (p - 1))
+        rank = 64 - w.bit_length() + 1
+        ref[idx] = max(ref[idx], rank)
+    np.testing.assert_array_equal(regs, ref)
+
+
+def test_hll_estimate_linear_counting_and_bias_paths():
+    p = 14
+    m = 1 << p
+    tables = np.load(O.os.path.join(O._HERE, "data", "hllpp_p14.npz"))
+    raw, bias, thr = tables["raw"], tables["bias"], float(tables["threshold"])
+    # Linear counting: few keys -> estimate = round(m ln(m/V))
+    regs = np.zeros(m, np.uint8)
+    O.hll_add(regs, p, np.arange(1000, dtype=np.int64))
+    h = O.hll_histogram(regs, p)
+    z = L.orc_hll_z(O._ptr(h))
+    est = L.orc_hll_finalize(p, z, int(h[0]), O._ptr(raw), O._ptr(bias), len(raw), thr)
+    assert est == round(m * math.log(m / h[0]))
+    # Large cardinality (E >= 5m): raw HLL estimate alpha m^2 / z
+    regs[:] = 20
+    h = O.hll_histogram(regs, p)
+    z = L.orc_hll_z(O._ptr(h))
+    est = L.orc_hll_finalize(p, z, int(h[0]), O._ptr(raw), O._ptr(bias), len(raw), thr)
+    alpha = 0.7213 / (1 + 1.079 / m)
+    assert est == round(alpha * m * m / z)
+    assert abs(z - m * 2.0 ** -20) < 1e-12
