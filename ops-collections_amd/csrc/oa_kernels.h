@@ -137,6 +137,19 @@ __device__ __forceinline__ uint64_t atomic_load_bits(const uint8_t* p)
         return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Agent-scope store: visible to agent-scope loads on every XCD during the
+// kernel (a plain store may sit in the writer's XCD-local L2 until the
+// kernel ends).  Used where a reader in the same launch waits for the value.
+template <int N>
+__device__ __forceinline__ void publish_bits(uint8_t* p, uint64_t v)
+{
+    if constexpr (N == 4)
+        __hip_atomic_store((unsigned int*)p, (unsigned int)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        __hip_atomic_store((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // KeyEqual = EqualTo<Key> (== on the key type, utility/math_utils.h:67).
 template <int KS>
 __device__ __forceinline__ bool key_eq(uint64_t a, uint64_t b, uint32_t is_float)
@@ -355,7 +368,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
                         store_bits<OUTS>(a.out + i * OUTS, weq ? L::val(ws) : (VS ? a.empty_val : ekey));
                     } else if constexpr (OP == OP_CONTAINS) {
                         a.out[i] = weq ? 1 : 0;
-                    } else {
+                    } else if constexpr (OP == OP_COUNT) {
                         cnt0 += weq ? 1 : 0;
                     }
                 }
@@ -450,7 +463,8 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
                             // then store the payload (ref_impl.h:165-185, 260-279).
                             const uint64_t old = cas_bits<KS>(wslot, ekey, key);
                             if (old == ekey) {
-                                store_bits<VS>(wslot + 8, pval);
+                                if constexpr (OP == OP_INSERT_AND_FIND) publish_bits<VS>(wslot + 8, pval);
+                                else store_bits<VS>(wslot + 8, pval);
                                 r = 1;
                             } else if (key_eq<KS>(old, key, a.key_float)) {
                                 r = 2;

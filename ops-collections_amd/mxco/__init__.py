@@ -19,6 +19,7 @@ if not os.path.exists(LIB_PATH):
     raise ImportError("libmxco.so is not built (run ops-collections_amd/build.py): %s" % LIB_PATH)
 
 _lib = ctypes.CDLL(LIB_PATH)
+_hip = ctypes.CDLL("libamdhip64.so")  # already loaded as a dependency of libmxco.so
 
 vp = ctypes.c_void_p
 u32 = ctypes.c_uint32
@@ -295,9 +296,8 @@ class _OpenAddressing:
         n = self._capacity * self.slot_size
         out = torch.empty(n, dtype=torch.uint8, device="cuda")
         torch.cuda.synchronize()
-        src = self.data()
-        # Device-to-device copy through the HIP runtime (cudart is hipart here).
-        torch.cuda.cudart().cudaMemcpy(out.data_ptr(), src, n, 3)
+        check(0 if _hip.hipMemcpy(vp(out.data_ptr()), vp(self.data()), ctypes.c_size_t(n), 3) == 0
+              else RUNTIME_ERROR)
         return out.view(self._capacity, self.slot_size)
 
 

@@ -155,6 +155,12 @@ class OpenAddressing:
         buf = (ctypes.c_uint8 * (self.capacity * self.slot_size)).from_address(lib().orc_oa_slots(self.t))
         return np.frombuffer(buf, np.uint8).reshape(self.capacity, self.slot_size).copy()
 
+    def load_slots(self, slots):
+        """Overwrite the slot array with `slots` (same layout, e.g. a GPU table copied to host)."""
+        b = np.ascontiguousarray(slots).view(np.uint8).reshape(-1)
+        assert b.size == self.capacity * self.slot_size
+        ctypes.memmove(lib().orc_oa_slots(self.t), b.ctypes.data, b.size)
+
     def _pairs(self, keys, values):
         return np.ascontiguousarray(pack_pairs(keys.astype(self.kd), None if self.vd is None else
                                                values.astype(self.vd)))

@@ -209,11 +209,34 @@ def test_erase_with_and_without_tombstone(group):
     ref = O.OpenAddressing(np.int32, np.int32, 6000, -1, -1)
     ref.insert(keys, keys)
     assert ref.erase(keys[:1000], erased_key=-2) == 0
-    # plain erase (no tombstone) on a set
+    # Plain erase (no tombstone) empties the slot, which can cut later probe
+    # chains exactly as in the reference; compare with the oracle on the GPU's
+    # own slot layout.
     s = mx().StaticSet(6000, -1, group_size=group)
     s.insert(dev(keys))
-    assert s.erase(dev(keys[:10])) == 0
-    assert s.count(dev(keys)) == len(keys) - 10
+    ref = O.OpenAddressing(np.int32, None, 6000, -1, None, probing=O.PROBE_DOUBLE, hash_family=O.HASH_XXHASH_32)
+    ref.load_slots(s.slots().cpu().numpy())
+    assert s.erase(dev(keys[:300])) == ref.erase(keys[:300]) == 0
+    assert s.contains(dev(keys)).cpu().numpy().tobytes() == ref.find(keys)[1].astype(np.uint8).tobytes()
+    np.testing.assert_array_equal(s.slots().cpu().numpy(), ref.slots())
+
+
+@pytest.mark.parametrize("group", GROUPS)
+@pytest.mark.parametrize("probing", [O.PROBE_LINEAR, O.PROBE_DOUBLE])
+def test_probe_stats_match_oracle_line_count(group, probing):
+    """The roofline's algorithmic bytes use the lines Find's walk reads; the
+    kernel's count equals the oracle's replay on the same slot array."""
+    rng = np.random.default_rng(15)
+    m = mx().StaticMap(200000, -1, -1, torch.int64, torch.int64, probing=PROBE[probing], group_size=group)
+    n = int(0.8 * m.capacity())
+    keys = distinct_keys(rng, n, 2**62, np.int64)
+    m.insert(dev_pairs(keys, keys))
+    ref = O.OpenAddressing(np.int64, np.int64, 200000, -1, -1, probing=probing)
+    ref.load_slots(m.slots().cpu().numpy())
+    q = np.concatenate([keys[:50000], rng.integers(2**62, 2**63 - 1, 50000)])
+    lines, slots = m.probe_stats(dev(q))
+    _, _, exp_lines = ref.find(q, lines=True)
+    assert lines == exp_lines
 
 
 @pytest.mark.parametrize("group", GROUPS)
