@@ -27,6 +27,7 @@ def parse():
     ap.add_argument("--slots-log2", type=int, default=28)
     ap.add_argument("--lf", type=float, default=0.8)
     ap.add_argument("--group", type=int, default=0, help="lanes per key (0 = library default)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent grid blocks per CU (0 = default)")
     ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -159,6 +160,8 @@ def main():
         print("bench.py: this build measures one GPU; --gpus %d is not supported (see DESIGN.md on sharding)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.blocks_per_cu:
+        os.environ["MXCO_OA_BLOCKS_PER_CU"] = str(args.blocks_per_cu)
     import torch
     if args.copy_calibration:
         copy_calibration()
