@@ -185,6 +185,63 @@ MXCO_API mxco_status mxco_oa_erase(mxco_oa* table, const void* keys, uint64_t n,
 MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
                                          uint64_t* lines, uint64_t* slots);
 
+/* ---------------------------------------------------------------------- */
+/* BloomFilter (include/bloom_filter.h:37-133): the default policy        */
+/* BloomFilterPolicy<Key, xxhash_64<Key>, uint32_t, 8 words, 8 bits>      */
+/* (include/bloom_filter_policy.h:54-181).  Keys are 4 or 8 bytes         */
+/* (int32/uint32/float or int64/uint64); the hash covers their bytes.     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_bloom mxco_bloom;
+
+/* BloomFilter(numBlocks, policy, stream) (bloom_filter_lifecycle.inl:15-55):
+ * num_blocks 256-bit blocks, cleared; seed is the policy's xxhash_64 seed. */
+MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, uint64_t seed, hipStream_t stream,
+                                       mxco_bloom** out);
+MXCO_API mxco_status mxco_bloom_destroy(mxco_bloom* filter);
+/* Data() / BlockExtent(): device pointer to the 8*num_blocks words. */
+MXCO_API mxco_status mxco_bloom_data(const mxco_bloom* filter, uint32_t** words, uint64_t* num_blocks);
+/* Clear / ClearAsync. */
+MXCO_API mxco_status mxco_bloom_clear(mxco_bloom* filter, hipStream_t stream, int sync);
+/* Add / AddAsync / AddIf / AddIfAsync (bloom_filter_add.inl): stencil is a
+ * byte per key (non-zero = add) or NULL for all keys. */
+MXCO_API mxco_status mxco_bloom_add(mxco_bloom* filter, const void* keys, const uint8_t* stencil, uint64_t n,
+                                    hipStream_t stream, int sync);
+/* Contains / ContainsAsync / ContainsIf / ContainsIfAsync
+ * (bloom_filter_query.inl): out[i] = 1 byte bool, 0 where the stencil is 0. */
+MXCO_API mxco_status mxco_bloom_contains(const mxco_bloom* filter, const void* keys, const uint8_t* stencil,
+                                         uint8_t* out, uint64_t n, hipStream_t stream, int sync);
+/* Merge (word-wise OR) / Intersect (word-wise AND) (bloom_filter_merge.inl:23-69). */
+MXCO_API mxco_status mxco_bloom_merge(mxco_bloom* dst, const mxco_bloom* src, hipStream_t stream, int sync);
+MXCO_API mxco_status mxco_bloom_intersect(mxco_bloom* dst, const mxco_bloom* src, hipStream_t stream, int sync);
+
+/* ---------------------------------------------------------------------- */
+/* HyperLogLog (include/hyperloglog.h:24-59): int32 or int64 keys,        */
+/* precision p in [13, 18] (2^p one-byte registers).  All calls are       */
+/* synchronous, as in the reference.                                      */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_hll mxco_hll;
+
+/* CreateWithSketchSizeKB / CreateWithStandardDeviation
+ * (hyperloglog.inl:116-140): the precision each factory picks. */
+MXCO_API mxco_status mxco_hll_precision_for_sketch_kb(uint32_t sketch_kb, uint32_t* precision);
+MXCO_API mxco_status mxco_hll_precision_for_stddev(double deviation, uint32_t* precision);
+/* CreateWithPrecision(p) (hyperloglog.inl:124-130); the sketch starts cleared. */
+MXCO_API mxco_status mxco_hll_create(uint32_t precision, uint32_t key_size, hipStream_t stream, mxco_hll** out);
+MXCO_API mxco_status mxco_hll_destroy(mxco_hll* sketch);
+/* Data(): device pointer to the 2^p registers. */
+MXCO_API mxco_status mxco_hll_data(const mxco_hll* sketch, const uint8_t** registers, uint32_t* precision);
+MXCO_API mxco_status mxco_hll_clear(mxco_hll* sketch, hipStream_t stream);
+/* Add(keys, count, stream) (hyperloglog.inl:224-313). */
+MXCO_API mxco_status mxco_hll_add(mxco_hll* sketch, const void* keys, uint64_t n, hipStream_t stream);
+/* Merge(other, stream) (hyperloglog.inl:315-326): register-wise maximum. */
+MXCO_API mxco_status mxco_hll_merge(mxco_hll* dst, const mxco_hll* src, hipStream_t stream);
+/* The 65-bin rank histogram Estimate() is computed from (kernels.h:74-92). */
+MXCO_API mxco_status mxco_hll_histogram(const mxco_hll* sketch, hipStream_t stream, uint32_t* hist65);
+/* Estimate(stream) (hyperloglog.inl:328-345): HLL++ in FP64 on the host. */
+MXCO_API mxco_status mxco_hll_estimate(const mxco_hll* sketch, hipStream_t stream, uint64_t* estimate);
+
 /* Number of compute units of the current device (the persistent grids and
  * the tuner size themselves from it). */
 MXCO_API mxco_status mxco_device_cu_count(int* cus);

@@ -1,0 +1,257 @@
+// BloomFilter (include/bloom_filter.h:37-133) for gfx950: the split-block
+// filter of BloomFilterPolicy's defaults (include/bloom_filter_policy.h:54-181):
+// 256-bit blocks of eight 32-bit words, one pattern bit per word, xxhash_64.
+//
+//   h      = XXH64(key bytes, seed)
+//   block  = (hi32(h) * numBlocks) >> 32              (BlockIndex, :102-107)
+//   word w = 1 << ((salt[w] * lo32(h)) >> 27)        (WordPatternImpl, :157-168)
+//
+// Add reads the 32-byte block once and issues 64-bit atomic ORs only for the
+// word pairs that still miss bits (OR is idempotent and order-free, so the
+// words are the same whatever the schedule).  Contains reads the block with
+// two 16-byte loads.  Merge / Intersect are word-wise OR / AND streams.
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "mxco_common.h"
+#include "mxco_hash.h"
+
+using namespace mxco;
+
+struct mxco_bloom {
+    uint32_t* words;
+    uint64_t num_blocks;
+    uint32_t key_size;
+    uint64_t seed;
+};
+
+namespace {
+
+// Parquet split-block Bloom filter salts (bloom_filter_policy.h:170-178, first eight).
+__constant__ uint32_t kSalts[8] = {0x47b6137bu, 0x44974d91u, 0x8824ad5bu, 0xa2b7289du,
+                                   0x705495c7u, 0x2df1424bu, 0x9efc4947u, 0x5c6bfb31u};
+
+template <int KS>
+__device__ __forceinline__ uint64_t key_at(const uint8_t* keys, uint64_t i)
+{
+    if constexpr (KS == 4) return ((const uint32_t*)keys)[i];
+    else return ((const uint64_t*)keys)[i];
+}
+
+__device__ __forceinline__ void locate(uint64_t h, uint32_t nb, uint32_t& blk, uint64_t m[4])
+{
+    blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * nb) >> 32);
+    const uint32_t lo = (uint32_t)h;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t a = 1u << ((kSalts[2 * p] * lo) >> 27);
+        const uint32_t b = 1u << ((kSalts[2 * p + 1] * lo) >> 27);
+        m[p] = (uint64_t)a | ((uint64_t)b << 32);
+    }
+}
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+template <int KS>
+__global__ __launch_bounds__(256) void bloom_add_kernel(uint32_t* words, uint32_t nb, const uint8_t* keys,
+                                                        const uint8_t* stencil, uint64_t n, uint64_t seed)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (stencil && stencil[i] == 0) continue;
+        const uint64_t h = xxhash64<KS>(key_at<KS>(keys, i), seed);
+        uint32_t blk;
+        uint64_t m[4];
+        locate(h, nb, blk, m);
+        u64x2* b = (u64x2*)(words + (uint64_t)blk * 8);
+        const u64x2 c0 = b[0], c1 = b[1];
+        unsigned long long* w = (unsigned long long*)b;
+        if ((c0.x & m[0]) != m[0]) atomicOr(w + 0, (unsigned long long)m[0]);
+        if ((c0.y & m[1]) != m[1]) atomicOr(w + 1, (unsigned long long)m[1]);
+        if ((c1.x & m[2]) != m[2]) atomicOr(w + 2, (unsigned long long)m[2]);
+        if ((c1.y & m[3]) != m[3]) atomicOr(w + 3, (unsigned long long)m[3]);
+    }
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void bloom_contains_kernel(const uint32_t* words, uint32_t nb, const uint8_t* keys,
+                                                             const uint8_t* stencil, uint8_t* out, uint64_t n,
+                                                             uint64_t seed)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (stencil && stencil[i] == 0) {
+            out[i] = 0;
+            continue;
+        }
+        const uint64_t h = xxhash64<KS>(key_at<KS>(keys, i), seed);
+        uint32_t blk;
+        uint64_t m[4];
+        locate(h, nb, blk, m);
+        const u64x2* b = (const u64x2*)(words + (uint64_t)blk * 8);
+        const u64x2 c0 = b[0], c1 = b[1];
+        const bool hit = ((c0.x & m[0]) == m[0]) & ((c0.y & m[1]) == m[1]) & ((c1.x & m[2]) == m[2]) &
+                         ((c1.y & m[3]) == m[3]);
+        out[i] = hit ? 1 : 0;
+    }
+}
+
+// op 0: dst |= src, op 1: dst &= src, op 2: dst = 0.  Blocks are 32 bytes,
+// so the word count is a multiple of 8 and 16-byte vectors cover it exactly.
+__global__ __launch_bounds__(256) void bloom_words_kernel(uint32_t* dst, const uint32_t* src, uint64_t n16, int op)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    u64x2* d = (u64x2*)dst;
+    const u64x2* s = (const u64x2*)src;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+        if (op == 2) {
+            u64x2 z = {0, 0};
+            d[i] = z;
+        } else {
+            const u64x2 a = d[i], b = s[i];
+            d[i] = op == 0 ? (a | b) : (a & b);
+        }
+    }
+}
+
+mxco_status check_filter(const mxco_bloom* f, const char* fn)
+{
+    if (!f || !f->words) return fail(MXCO_LOGIC_ERROR, std::string("BloomFilter::") + fn + ": filter has no storage");
+    return MXCO_OK;
+}
+
+mxco_status words_op(mxco_bloom* dst, const mxco_bloom* src, int op, hipStream_t stream, int sync, const char* fn)
+{
+    const uint64_t n16 = dst->num_blocks * 2;
+    bloom_words_kernel<<<grid_for(n16, 8), 256, 0, stream>>>(dst->words, src ? src->words : nullptr, n16, op);
+    MXCO_HIP(hipGetLastError(), fn);
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), fn);
+    return MXCO_OK;
+}
+
+mxco_status compatible(const mxco_bloom* a, const mxco_bloom* b, const char* fn)
+{
+    if (!a || !b || !a->words || !b->words)
+        return fail(MXCO_LOGIC_ERROR, std::string("BloomFilter::") + fn + ": a moved-from filter cannot participate");
+    if (a->num_blocks != b->num_blocks)
+        return fail(MXCO_INVALID_ARGUMENT, std::string("BloomFilter::") + fn + ": filters have different block extents");
+    if (a->seed != b->seed || a->key_size != b->key_size)
+        return fail(MXCO_INVALID_ARGUMENT, std::string("BloomFilter::") + fn + ": filters have different policies");
+    return MXCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, uint64_t seed, hipStream_t stream,
+                                       mxco_bloom** out)
+{
+    if (!out) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter: null output");
+    *out = nullptr;
+    if (key_size != 4 && key_size != 8) return fail(MXCO_UNSUPPORTED, "BloomFilter: keys must be 4 or 8 bytes");
+    if (num_blocks == 0) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter: numBlocks must be greater than zero");
+    if (num_blocks > 0xffffffffull)
+        return fail(MXCO_LENGTH_ERROR, "BloomFilter: numBlocks exceeds the supported block-index range");
+    mxco_bloom* f = new (std::nothrow) mxco_bloom();
+    if (!f) return fail(MXCO_OUT_OF_MEMORY, "BloomFilter: host allocation");
+    f->num_blocks = num_blocks;
+    f->key_size = key_size;
+    f->seed = seed;
+    hipError_t e = hipMalloc((void**)&f->words, num_blocks * 32);
+    if (e != hipSuccess) {
+        delete f;
+        return hip_fail(e, "BloomFilter: allocation");
+    }
+    mxco_status st = words_op(f, nullptr, 2, stream, 1, "BloomFilter::Clear");
+    if (st != MXCO_OK) {
+        (void)hipFree(f->words);
+        delete f;
+        return st;
+    }
+    *out = f;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_destroy(mxco_bloom* f)
+{
+    if (!f) return ok();
+    if (f->words) (void)hipFree(f->words);
+    delete f;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_data(const mxco_bloom* f, uint32_t** words, uint64_t* num_blocks)
+{
+    if (mxco_status s = check_filter(f, "Data")) return s;
+    if (words) *words = f->words;
+    if (num_blocks) *num_blocks = f->num_blocks;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_clear(mxco_bloom* f, hipStream_t stream, int sync)
+{
+    if (mxco_status s = check_filter(f, "Clear")) return s;
+    if (mxco_status s = words_op(f, nullptr, 2, stream, sync, "BloomFilter::Clear")) return s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_add(mxco_bloom* f, const void* keys, const uint8_t* stencil, uint64_t n,
+                                    hipStream_t stream, int sync)
+{
+    if (n == 0) return ok();
+    if (mxco_status s = check_filter(f, "Add")) return s;
+    if (!keys) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter::Add: keys must not be null when keyNum is non-zero");
+    const unsigned grid = grid_for(n, 8);
+    if (f->key_size == 4)
+        bloom_add_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                     stencil, n, f->seed);
+    else
+        bloom_add_kernel<8><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                     stencil, n, f->seed);
+    MXCO_HIP(hipGetLastError(), "BloomFilter::Add");
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "BloomFilter::Add");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_contains(const mxco_bloom* f, const void* keys, const uint8_t* stencil, uint8_t* out,
+                                         uint64_t n, hipStream_t stream, int sync)
+{
+    if (n == 0) return ok();
+    if (mxco_status s = check_filter(f, "Contains")) return s;
+    if (!keys)
+        return fail(MXCO_INVALID_ARGUMENT, "BloomFilter::Contains: keys must not be null when keyNum is non-zero");
+    if (!out)
+        return fail(MXCO_INVALID_ARGUMENT,
+                    "BloomFilter::Contains: outputValues must not be null when keyNum is non-zero");
+    const unsigned grid = grid_for(n, 8);
+    if (f->key_size == 4)
+        bloom_contains_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                          stencil, out, n, f->seed);
+    else
+        bloom_contains_kernel<8><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                          stencil, out, n, f->seed);
+    MXCO_HIP(hipGetLastError(), "BloomFilter::Contains");
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "BloomFilter::Contains");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_merge(mxco_bloom* dst, const mxco_bloom* src, hipStream_t stream, int sync)
+{
+    if (mxco_status s = compatible(dst, src, "Merge")) return s;
+    if (dst == src) return ok();
+    if (mxco_status s = words_op(dst, src, 0, stream, sync, "BloomFilter::Merge")) return s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_intersect(mxco_bloom* dst, const mxco_bloom* src, hipStream_t stream, int sync)
+{
+    if (mxco_status s = compatible(dst, src, "Intersect")) return s;
+    if (dst == src) return ok();
+    if (mxco_status s = words_op(dst, src, 1, stream, sync, "BloomFilter::Intersect")) return s;
+    return ok();
+}
+
+}  // extern "C"

@@ -29,14 +29,14 @@ def main():
     thr = arr(text, "threshold_data")
     lines = ["// HLL++ empirical tables (Heule et al., EDBT 2013), precisions 13..18.",
              "// Generated by oracle/data/extract_hllpp.py; data only.",
-             "static const double kHllThreshold[15] = {" + ", ".join(repr(x) for x in thr) + "};"]
+             "static const double kHllThreshold[15] = {" + ", ".join(repr(float(x)) for x in thr) + "};"]
     for p in range(13, 19):
         raw = arr(text, "raw_estimate_data_p%d" % p)
         bias = arr(text, "bias_data_p%d" % p)
         assert len(raw) == len(bias), p
         np.savez(os.path.join(HERE, "hllpp_p%d.npz" % p), raw=raw, bias=bias, threshold=thr[p - 4])
-        lines.append("static const double kHllRaw%d[%d] = {%s};" % (p, len(raw), ", ".join(repr(x) for x in raw)))
-        lines.append("static const double kHllBias%d[%d] = {%s};" % (p, len(bias), ", ".join(repr(x) for x in bias)))
+        lines.append("static const double kHllRaw%d[%d] = {%s};" % (p, len(raw), ", ".join(repr(float(x)) for x in raw)))
+        lines.append("static const double kHllBias%d[%d] = {%s};" % (p, len(bias), ", ".join(repr(float(x)) for x in bias)))
     open(INC, "w").write("\n".join(lines) + "\n")
 
 
