@@ -1,0 +1,181 @@
+// STL-style C++ front end of libmxco.so: the building blocks shared by the
+// container headers (extents, hashers, probing schemes, storage, pairs,
+// predicates and the status -> exception mapping).
+//
+// The names and template shapes follow the reference's public headers so
+// that code written against them compiles after changing the include and
+// namespace (include/extent.h:20-99, include/hash_functions.h:17-30,
+// include/probing_scheme.h:37-122, include/storage.h:16-20,
+// include/pair.h:41, include/utility/math_utils.h:67).  The template
+// arguments select a configuration of the compiled kernels; they carry no
+// device code of their own.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "../mxco.h"
+
+namespace mxco {
+
+static constexpr std::size_t dynamicExtent = static_cast<std::size_t>(-1);
+
+// Extent<SizeType, N> (include/extent.h:20-60): a static or dynamic count.
+template <typename SizeType, std::size_t N = dynamicExtent>
+struct Extent {
+    static_assert(std::is_integral_v<SizeType> && std::is_unsigned_v<SizeType>,
+                  "SizeType must be an unsigned integral type");
+    using ValueType = SizeType;
+    constexpr Extent() = default;
+    constexpr Extent(SizeType) noexcept {}
+    constexpr operator SizeType() const noexcept { return N; }
+};
+
+template <typename SizeType>
+struct Extent<SizeType, dynamicExtent> {
+    static_assert(std::is_integral_v<SizeType> && std::is_unsigned_v<SizeType>,
+                  "SizeType must be an unsigned integral type");
+    using ValueType = SizeType;
+    constexpr Extent(SizeType size) noexcept : size_(size) {}
+    constexpr operator SizeType() const noexcept { return size_; }
+
+private:
+    SizeType size_;
+};
+
+// Storage<B> (include/storage.h:16-20): the bucket size.
+template <int32_t BucketSize>
+struct Storage {
+    static constexpr int32_t bucketSize = BucketSize;
+};
+
+// Hasher tags (include/hash_functions.h:17-30).  Seeds are fixed by the
+// containers, as in the reference (stateful hashers are rejected).
+template <typename Key>
+struct murmurhash3_32 {
+    static constexpr mxco_hash id = MXCO_HASH_MURMUR3_32;
+    constexpr bool operator==(murmurhash3_32 const&) const noexcept { return true; }
+};
+template <typename Key>
+struct xxhash_32 {
+    static constexpr mxco_hash id = MXCO_HASH_XXHASH_32;
+    constexpr bool operator==(xxhash_32 const&) const noexcept { return true; }
+};
+template <typename Key>
+struct xxhash_64 {
+    static constexpr mxco_hash id = MXCO_HASH_XXHASH_64;
+    constexpr bool operator==(xxhash_64 const&) const noexcept { return true; }
+};
+template <typename Key>
+struct murmurhash3_fmix32 {
+    static constexpr mxco_hash id = MXCO_HASH_FMIX_32;
+    constexpr bool operator==(murmurhash3_fmix32 const&) const noexcept { return true; }
+};
+template <typename Key>
+struct murmurhash3_fmix64 {
+    static constexpr mxco_hash id = MXCO_HASH_FMIX_64;
+    constexpr bool operator==(murmurhash3_fmix64 const&) const noexcept { return true; }
+};
+
+// Probing schemes (include/probing_scheme.h:37-122).
+template <typename Hash>
+struct LinearProbing {
+    using Hasher = Hash;
+    static constexpr mxco_probing id = MXCO_PROBING_LINEAR;
+    static constexpr mxco_hash hash = Hash::id;
+    constexpr bool operator==(LinearProbing const&) const noexcept { return true; }
+};
+
+template <typename Hash1, typename Hash2 = Hash1>
+struct DoubleHashing {
+    static_assert(Hash1::id == Hash2::id, "DoubleHashing: both hashes must be of one family (seeds 0 and 1)");
+    static constexpr mxco_probing id = MXCO_PROBING_DOUBLE;
+    static constexpr mxco_hash hash = Hash1::id;
+    constexpr bool operator==(DoubleHashing const&) const noexcept { return true; }
+};
+
+// KeyEqual default (include/utility/math_utils.h:67).
+template <typename T>
+struct EqualTo {
+    constexpr bool operator()(T const& lhs, T const& rhs) const noexcept { return lhs == rhs; }
+    constexpr bool operator==(EqualTo const&) const noexcept { return true; }
+};
+
+// Pair<F, S> (include/pair.h:41): the slot layout of a map.
+namespace detail {
+constexpr std::size_t bit_ceil(std::size_t n)
+{
+    std::size_t x = 1;
+    while (x < n) x <<= 1;
+    return x;
+}
+constexpr std::size_t pair_alignment(std::size_t a, std::size_t b)
+{
+    return bit_ceil(a + b) < 16 ? bit_ceil(a + b) : 16;
+}
+}  // namespace detail
+
+template <typename First, typename Second>
+struct alignas(detail::pair_alignment(sizeof(First), sizeof(Second))) Pair {
+    using FirstType = First;
+    using SecondType = Second;
+    First first;
+    Second second;
+};
+
+// Stencil predicates.  The reference default-constructs a predicate type on
+// the device; the compiled kernels provide these four.
+struct AlwaysTrue {
+    static constexpr mxco_pred id = MXCO_PRED_ALWAYS;
+};
+struct IsNonZero {
+    static constexpr mxco_pred id = MXCO_PRED_NONZERO;
+};
+struct IsOdd {
+    static constexpr mxco_pred id = MXCO_PRED_ODD;
+};
+struct IsEven {
+    static constexpr mxco_pred id = MXCO_PRED_EVEN;
+};
+
+namespace detail {
+
+template <typename P, typename = void>
+struct HasPredId : std::false_type {};
+template <typename P>
+struct HasPredId<P, std::void_t<decltype(P::id)>> : std::true_type {};
+
+// Throws the exception type the reference throws for the status.
+inline void check(mxco_status s)
+{
+    if (s == MXCO_OK) return;
+    const std::string msg = mxco_last_error();
+    switch (s) {
+    case MXCO_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+    case MXCO_LENGTH_ERROR: throw std::length_error(msg);
+    case MXCO_OUT_OF_MEMORY: throw std::bad_alloc();
+    case MXCO_LOGIC_ERROR: throw std::logic_error(msg);
+    default: throw std::runtime_error(msg);
+    }
+}
+
+template <typename StencilT, typename Predicate>
+mxco_stencil make_stencil(StencilT const* stencil)
+{
+    static_assert(HasPredId<Predicate>::value,
+                  "Predicate must be one of mxco::AlwaysTrue, IsNonZero, IsOdd, IsEven");
+    static_assert(sizeof(StencilT) == 1 || sizeof(StencilT) == 4 || sizeof(StencilT) == 8,
+                  "stencil elements must be 1, 4 or 8 bytes");
+    mxco_stencil s;
+    s.data = stencil;
+    s.elem_size = sizeof(StencilT);
+    s.pred = Predicate::id;
+    return s;
+}
+
+}  // namespace detail
+}  // namespace mxco

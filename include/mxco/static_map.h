@@ -1,0 +1,209 @@
+// mxco::StaticMap -- drop-in for aclco::StaticMap (include/static_map.h:29-614).
+//
+// Same template parameters, same member functions and argument order; the
+// stream argument is a hipStream_t.  Every call forwards to the C ABI of
+// libmxco.so (include/mxco.h) and rethrows its status as the reference's
+// exception type.
+#pragma once
+
+#include <memory>
+#include <utility>
+
+#include "common.h"
+
+namespace mxco {
+
+static constexpr std::size_t defaultMapBucketSize = 5;
+
+template <class Key, class T, class Extent = mxco::Extent<std::size_t>, class KeyEqual = mxco::EqualTo<Key>,
+          class ProbingScheme = mxco::LinearProbing<mxco::murmurhash3_32<Key>>,
+          class Storage = mxco::Storage<defaultMapBucketSize>>
+class StaticMap {
+    static_assert(sizeof(Key) == 4 || sizeof(Key) == 8, "keys must be 4 or 8 bytes");
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "payloads must be 4 or 8 bytes");
+    static_assert(std::is_same_v<KeyEqual, mxco::EqualTo<Key>>, "KeyEqual must be mxco::EqualTo<Key>");
+
+public:
+    static constexpr auto bucketSize = Storage::bucketSize;
+    using SizeType = typename Extent::ValueType;
+    using KeyType = Key;
+    using MappedType = T;
+    using ValueType = Pair<Key, T>;
+
+    StaticMap(StaticMap const&) = delete;
+    StaticMap& operator=(StaticMap const&) = delete;
+    StaticMap(StaticMap&&) = default;
+    StaticMap& operator=(StaticMap&&) = default;
+    ~StaticMap() = default;
+
+    StaticMap(Extent capacity, Key emptyKey, T emptyValue, KeyEqual const& = {}, ProbingScheme const& = {},
+              Storage = {}, hipStream_t stream = nullptr)
+        : emptyKey_{emptyKey}, emptyValue_{emptyValue}
+    {
+        mxco_oa_desc d{};
+        d.key_size = sizeof(Key);
+        d.value_size = sizeof(T);
+        d.key_kind = std::is_floating_point_v<Key> ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
+        d.bucket_size = static_cast<uint32_t>(bucketSize);
+        d.probing = ProbingScheme::id;
+        d.hash = ProbingScheme::hash;
+        mxco_oa* h = nullptr;
+        detail::check(mxco_oa_create(&d, static_cast<SizeType>(capacity), &emptyKey, &emptyValue, stream, &h));
+        impl_.reset(h);
+    }
+
+    void SetErasedKey(Key erasedKey, hipStream_t stream) { detail::check(mxco_oa_set_erased_key(get(), &erasedKey, stream)); }
+
+    SizeType Insert(void* values, Extent valueNum, hipStream_t stream)
+    {
+        uint64_t failed = 0;
+        detail::check(mxco_oa_insert(get(), values, nullptr, static_cast<SizeType>(valueNum), stream, 1, &failed));
+        return static_cast<SizeType>(failed);
+    }
+    void InsertAsync(void* values, Extent valueNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_insert(get(), values, nullptr, static_cast<SizeType>(valueNum), stream, 0, nullptr));
+    }
+    SizeType InsertOrAssign(void* values, Extent valueNum, hipStream_t stream)
+    {
+        SizeType assigned = 0;
+        return InsertOrAssign(values, valueNum, stream, assigned);
+    }
+    SizeType InsertOrAssign(void* values, Extent valueNum, hipStream_t stream, SizeType& assignedNum)
+    {
+        uint64_t f = 0, a = 0;
+        detail::check(mxco_oa_insert_or_assign(get(), values, static_cast<SizeType>(valueNum), stream, 1, &f, &a));
+        assignedNum = static_cast<SizeType>(a);
+        return static_cast<SizeType>(f);
+    }
+    void InsertOrAssignAsync(void* values, Extent valueNum, hipStream_t stream)
+    {
+        detail::check(
+            mxco_oa_insert_or_assign(get(), values, static_cast<SizeType>(valueNum), stream, 0, nullptr, nullptr));
+    }
+    template <typename StencilT, typename Predicate>
+    void AssignIfAsync(void* values, StencilT* stencil, Extent valueNum, hipStream_t stream)
+    {
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_assign_if(get(), values, &s, static_cast<SizeType>(valueNum), stream, 0));
+    }
+    template <typename StencilT, typename Predicate>
+    SizeType InsertIf(void* values, StencilT* stencil, Extent valueNum, hipStream_t stream)
+    {
+        if (static_cast<SizeType>(valueNum) != 0 && stencil == nullptr) return static_cast<SizeType>(valueNum);
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        uint64_t failed = 0;
+        detail::check(mxco_oa_insert(get(), values, &s, static_cast<SizeType>(valueNum), stream, 1, &failed));
+        return static_cast<SizeType>(failed);
+    }
+    template <typename StencilT, typename Predicate>
+    void InsertIfAsync(void* values, StencilT* stencil, Extent valueNum, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_insert(get(), values, &s, static_cast<SizeType>(valueNum), stream, 0, nullptr));
+    }
+    void InsertAndFind(void* values, void* outputFind, void* outputInsert, Extent valueNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_insert_and_find(get(), values, outputFind, static_cast<uint8_t*>(outputInsert),
+                                              static_cast<SizeType>(valueNum), stream, 1));
+    }
+    void InsertAndFindAsync(void* values, void* outputFind, void* outputInsert, Extent valueNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_insert_and_find(get(), values, outputFind, static_cast<uint8_t*>(outputInsert),
+                                              static_cast<SizeType>(valueNum), stream, 0));
+    }
+    SizeType Erase(void* keys, Extent keyNum, hipStream_t stream)
+    {
+        uint64_t failed = 0;
+        detail::check(mxco_oa_erase(get(), keys, static_cast<SizeType>(keyNum), stream, 1, &failed));
+        return static_cast<SizeType>(failed);
+    }
+    void EraseAsync(void* keys, Extent keyNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_erase(get(), keys, static_cast<SizeType>(keyNum), stream, 0, nullptr));
+    }
+    void Clear(hipStream_t stream) { detail::check(mxco_oa_clear(get(), stream, 1)); }
+    void ClearAsync(hipStream_t stream) noexcept { (void)mxco_oa_clear(get(), stream, 0); }
+    void Find(void* keys, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_find(get(), keys, nullptr, outputValues, static_cast<SizeType>(keyNum), stream, 1));
+    }
+    void FindAsync(void* keys, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_find(get(), keys, nullptr, outputValues, static_cast<SizeType>(keyNum), stream, 0));
+    }
+    template <typename StencilT, typename Predicate>
+    void FindIf(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_find(get(), keys, &s, outputValues, static_cast<SizeType>(keyNum), stream, 1));
+    }
+    template <typename StencilT, typename Predicate>
+    void FindIfAsync(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_find(get(), keys, &s, outputValues, static_cast<SizeType>(keyNum), stream, 0));
+    }
+    void Contains(void* keys, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_contains(get(), keys, nullptr, static_cast<uint8_t*>(outputValues),
+                                       static_cast<SizeType>(keyNum), stream, 1));
+    }
+    void ContainsAsync(void* keys, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        detail::check(mxco_oa_contains(get(), keys, nullptr, static_cast<uint8_t*>(outputValues),
+                                       static_cast<SizeType>(keyNum), stream, 0));
+    }
+    template <typename StencilT, typename Predicate>
+    void ContainsIf(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_contains(get(), keys, &s, static_cast<uint8_t*>(outputValues),
+                                       static_cast<SizeType>(keyNum), stream, 1));
+    }
+    template <typename StencilT, typename Predicate>
+    void ContainsIfAsync(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_contains(get(), keys, &s, static_cast<uint8_t*>(outputValues),
+                                       static_cast<SizeType>(keyNum), stream, 0));
+    }
+    SizeType Count(void* keys, Extent keyNum, hipStream_t stream)
+    {
+        uint64_t c = 0;
+        detail::check(mxco_oa_count(get(), keys, static_cast<SizeType>(keyNum), stream, &c));
+        return static_cast<SizeType>(c);
+    }
+    auto Capacity() const noexcept
+    {
+        uint64_t c = 0;
+        (void)mxco_oa_capacity(get(), &c);
+        return static_cast<SizeType>(c);
+    }
+    KeyEqual GetKeyEqual() const noexcept { return {}; }
+    ValueType* Data() const
+    {
+        void* p = nullptr;
+        detail::check(mxco_oa_data(get(), &p));
+        return static_cast<ValueType*>(p);
+    }
+    Key EmptyKeySentinel() const noexcept { return emptyKey_; }
+    T EmptyValueSentinel() const noexcept { return emptyValue_; }
+
+private:
+    struct Deleter {
+        void operator()(mxco_oa* h) const noexcept { (void)mxco_oa_destroy(h); }
+    };
+    mxco_oa* get() const noexcept { return impl_.get(); }
+
+    std::unique_ptr<mxco_oa, Deleter> impl_;
+    Key emptyKey_;
+    T emptyValue_;
+};
+
+}  // namespace mxco

@@ -1,0 +1,159 @@
+// mxco::StaticSet -- drop-in for aclco::StaticSet (include/static_set.h:27-525).
+// Defaults: DoubleHashing<xxhash_32<Key>>, Storage<5>.  Find returns the
+// stored key or the empty key.
+#pragma once
+
+#include <memory>
+
+#include "common.h"
+
+namespace mxco {
+
+static constexpr std::size_t defaultSetBucketSize = 5;
+
+template <class Key, class Extent = mxco::Extent<std::size_t>, class KeyEqual = mxco::EqualTo<Key>,
+          class ProbingScheme = mxco::DoubleHashing<mxco::xxhash_32<Key>>,
+          class Storage = mxco::Storage<defaultSetBucketSize>>
+class StaticSet {
+    static_assert(sizeof(Key) == 4 || sizeof(Key) == 8, "keys must be 4 or 8 bytes");
+    static_assert(std::is_same_v<KeyEqual, mxco::EqualTo<Key>>, "KeyEqual must be mxco::EqualTo<Key>");
+
+public:
+    static constexpr auto bucketSize = Storage::bucketSize;
+    using SizeType = typename Extent::ValueType;
+    using KeyType = Key;
+    using ValueType = Key;
+
+    StaticSet(StaticSet const&) = delete;
+    StaticSet& operator=(StaticSet const&) = delete;
+    StaticSet(StaticSet&&) = default;
+    StaticSet& operator=(StaticSet&&) = default;
+
+    StaticSet(Extent capacity, Key emptyKey, KeyEqual const& = {}, ProbingScheme const& = {}, Storage = {},
+              hipStream_t stream = nullptr)
+        : emptyKey_{emptyKey}
+    {
+        mxco_oa_desc d{};
+        d.key_size = sizeof(Key);
+        d.value_size = 0;
+        d.key_kind = std::is_floating_point_v<Key> ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
+        d.bucket_size = static_cast<uint32_t>(bucketSize);
+        d.probing = ProbingScheme::id;
+        d.hash = ProbingScheme::hash;
+        mxco_oa* h = nullptr;
+        detail::check(mxco_oa_create(&d, static_cast<SizeType>(capacity), &emptyKey, nullptr, stream, &h));
+        impl_.reset(h);
+    }
+
+    void SetErasedKey(Key erasedKey, hipStream_t stream) { detail::check(mxco_oa_set_erased_key(get(), &erasedKey, stream)); }
+    SizeType Insert(void* keys, Extent n, hipStream_t stream)
+    {
+        uint64_t f = 0;
+        detail::check(mxco_oa_insert(get(), keys, nullptr, static_cast<SizeType>(n), stream, 1, &f));
+        return static_cast<SizeType>(f);
+    }
+    void InsertAsync(void* keys, Extent n, hipStream_t stream)
+    {
+        detail::check(mxco_oa_insert(get(), keys, nullptr, static_cast<SizeType>(n), stream, 0, nullptr));
+    }
+    template <typename StencilT, typename Predicate>
+    SizeType InsertIf(void* keys, StencilT* stencil, Extent n, hipStream_t stream)
+    {
+        if (static_cast<SizeType>(n) != 0 && stencil == nullptr) return static_cast<SizeType>(n);
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        uint64_t f = 0;
+        detail::check(mxco_oa_insert(get(), keys, &s, static_cast<SizeType>(n), stream, 1, &f));
+        return static_cast<SizeType>(f);
+    }
+    template <typename StencilT, typename Predicate>
+    void InsertIfAsync(void* keys, StencilT* stencil, Extent n, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_insert(get(), keys, &s, static_cast<SizeType>(n), stream, 0, nullptr));
+    }
+    void InsertAndFind(void* keys, void* outputFind, void* outputInsert, Extent n, hipStream_t stream)
+    {
+        detail::check(mxco_oa_insert_and_find(get(), keys, outputFind, static_cast<uint8_t*>(outputInsert),
+                                              static_cast<SizeType>(n), stream, 1));
+    }
+    void InsertAndFindAsync(void* keys, void* outputFind, void* outputInsert, Extent n, hipStream_t stream)
+    {
+        detail::check(mxco_oa_insert_and_find(get(), keys, outputFind, static_cast<uint8_t*>(outputInsert),
+                                              static_cast<SizeType>(n), stream, 0));
+    }
+    SizeType Erase(void* keys, Extent n, hipStream_t stream)
+    {
+        uint64_t f = 0;
+        detail::check(mxco_oa_erase(get(), keys, static_cast<SizeType>(n), stream, 1, &f));
+        return static_cast<SizeType>(f);
+    }
+    void EraseAsync(void* keys, Extent n, hipStream_t stream)
+    {
+        detail::check(mxco_oa_erase(get(), keys, static_cast<SizeType>(n), stream, 0, nullptr));
+    }
+    void Clear(hipStream_t stream) { detail::check(mxco_oa_clear(get(), stream, 1)); }
+    void ClearAsync(hipStream_t stream) noexcept { (void)mxco_oa_clear(get(), stream, 0); }
+    void Find(void* keys, void* out, Extent n, hipStream_t stream)
+    {
+        detail::check(mxco_oa_find(get(), keys, nullptr, out, static_cast<SizeType>(n), stream, 1));
+    }
+    void FindAsync(void* keys, void* out, Extent n, hipStream_t stream)
+    {
+        detail::check(mxco_oa_find(get(), keys, nullptr, out, static_cast<SizeType>(n), stream, 0));
+    }
+    template <typename StencilT, typename Predicate>
+    void FindIf(void* keys, StencilT* stencil, void* out, Extent n, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(mxco_oa_find(get(), keys, &s, out, static_cast<SizeType>(n), stream, 1));
+    }
+    void Contains(void* keys, void* out, Extent n, hipStream_t stream)
+    {
+        detail::check(
+            mxco_oa_contains(get(), keys, nullptr, static_cast<uint8_t*>(out), static_cast<SizeType>(n), stream, 1));
+    }
+    void ContainsAsync(void* keys, void* out, Extent n, hipStream_t stream)
+    {
+        detail::check(
+            mxco_oa_contains(get(), keys, nullptr, static_cast<uint8_t*>(out), static_cast<SizeType>(n), stream, 0));
+    }
+    template <typename StencilT, typename Predicate>
+    void ContainsIf(void* keys, StencilT* stencil, void* out, Extent n, hipStream_t stream)
+    {
+        if (stencil == nullptr) return;
+        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::check(
+            mxco_oa_contains(get(), keys, &s, static_cast<uint8_t*>(out), static_cast<SizeType>(n), stream, 1));
+    }
+    SizeType Count(void* keys, Extent n, hipStream_t stream)
+    {
+        uint64_t c = 0;
+        detail::check(mxco_oa_count(get(), keys, static_cast<SizeType>(n), stream, &c));
+        return static_cast<SizeType>(c);
+    }
+    auto Capacity() const noexcept
+    {
+        uint64_t c = 0;
+        (void)mxco_oa_capacity(get(), &c);
+        return static_cast<SizeType>(c);
+    }
+    Key* Data() const
+    {
+        void* p = nullptr;
+        detail::check(mxco_oa_data(get(), &p));
+        return static_cast<Key*>(p);
+    }
+    Key EmptyKeySentinel() const noexcept { return emptyKey_; }
+
+private:
+    struct Deleter {
+        void operator()(mxco_oa* h) const noexcept { (void)mxco_oa_destroy(h); }
+    };
+    mxco_oa* get() const noexcept { return impl_.get(); }
+    std::unique_ptr<mxco_oa, Deleter> impl_;
+    Key emptyKey_;
+};
+
+}  // namespace mxco

@@ -49,6 +49,15 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    # C++ header-API check program (run by tests/test_header_api_gpu.py).
+    test_src = os.path.join(HERE, "..", "tests", "cpp", "header_api_test.cpp")
+    if os.path.exists(test_src):
+        exe = os.path.join(LIBDIR, "header_api_test")
+        cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), test_src, "-L", LIBDIR,
+               "-lmxco", "-Wl,-rpath,$ORIGIN", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("header API test build failed:\n%s" % r.stderr)
     if verbose:
         print("built", LIB)
     return LIB

@@ -338,3 +338,172 @@ class StaticSet(_OpenAddressing):
                  hash="xxhash_32", group_size=0, stream=None):
         super().__init__(capacity, empty_key, None, key_dtype, None, bucket_size, probing, hash, group_size,
                          stream)
+
+
+# ---------------------------------------------------------------------------
+# BloomFilter / HyperLogLog
+# ---------------------------------------------------------------------------
+
+_sig("mxco_bloom_create", [u64, u32, u64, vp, ctypes.POINTER(vp)])
+_sig("mxco_bloom_destroy", [vp])
+_sig("mxco_bloom_data", [vp, ctypes.POINTER(vp), pu64])
+_sig("mxco_bloom_clear", [vp, vp, i32])
+_sig("mxco_bloom_add", [vp, vp, vp, u64, vp, i32])
+_sig("mxco_bloom_contains", [vp, vp, vp, vp, u64, vp, i32])
+_sig("mxco_bloom_merge", [vp, vp, vp, i32])
+_sig("mxco_bloom_intersect", [vp, vp, vp, i32])
+_sig("mxco_hll_precision_for_sketch_kb", [u32, ctypes.POINTER(u32)])
+_sig("mxco_hll_precision_for_stddev", [ctypes.c_double, ctypes.POINTER(u32)])
+_sig("mxco_hll_create", [u32, u32, vp, ctypes.POINTER(vp)])
+_sig("mxco_hll_destroy", [vp])
+_sig("mxco_hll_data", [vp, ctypes.POINTER(vp), ctypes.POINTER(u32)])
+_sig("mxco_hll_clear", [vp, vp])
+_sig("mxco_hll_add", [vp, vp, u64, vp])
+_sig("mxco_hll_merge", [vp, vp, vp])
+_sig("mxco_hll_histogram", [vp, vp, ctypes.POINTER(u32)])
+_sig("mxco_hll_estimate", [vp, vp, pu64])
+
+
+def _dev_view(ptr, nbytes, dtype=torch.uint8):
+    """Device copy of `nbytes` at device address `ptr`."""
+    out = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    if nbytes:
+        check(0 if _hip.hipMemcpy(vp(out.data_ptr()), vp(ptr), ctypes.c_size_t(nbytes), 3) == 0 else RUNTIME_ERROR)
+    return out.view(dtype)
+
+
+class BloomFilter:
+    """aclco::BloomFilter<Key> with the default policy (include/bloom_filter.h:37)."""
+
+    def __init__(self, num_blocks, key_dtype=torch.int64, seed=0, stream=None):
+        self.key_dtype = key_dtype
+        self.seed = seed
+        h = vp()
+        check(_lib.mxco_bloom_create(num_blocks, torch.empty(0, dtype=key_dtype).element_size(), seed,
+                                     _stream(stream), ctypes.byref(h)))
+        self._h = h
+        self._nb = num_blocks
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.mxco_bloom_destroy(self._h)
+            self._h = None
+
+    def block_extent(self):
+        return self._nb
+
+    def num_words(self):
+        return self._nb * 8
+
+    def size_bytes(self):
+        return self._nb * 32
+
+    def data(self):
+        p, nb = vp(), u64()
+        check(_lib.mxco_bloom_data(self._h, ctypes.byref(p), ctypes.byref(nb)))
+        return p.value
+
+    def words(self):
+        """Device copy of the filter words (uint32 viewed as int32)."""
+        return _dev_view(self.data(), self._nb * 32, torch.int32)
+
+    def clear(self, stream=None, sync=True):
+        check(_lib.mxco_bloom_clear(self._h, _stream(stream), int(sync)))
+
+    def add(self, keys, n=None, stream=None, sync=True):
+        n = keys.shape[0] if n is None else n
+        check(_lib.mxco_bloom_add(self._h, _ptr(keys), None, n, _stream(stream), int(sync)))
+
+    def add_async(self, keys, n=None, stream=None):
+        self.add(keys, n, stream, sync=False)
+
+    def add_if(self, keys, stencil, n=None, stream=None, sync=True):
+        n = keys.shape[0] if n is None else n
+        if n and stencil is None:
+            raise ValueError("BloomFilter::AddIf: stencil must not be null when keyNum is non-zero")
+        check(_lib.mxco_bloom_add(self._h, _ptr(keys), _ptr(stencil), n, _stream(stream), int(sync)))
+
+    def contains(self, keys, out=None, n=None, stream=None, sync=True, stencil=None):
+        n = keys.shape[0] if n is None else n
+        if out is None and n:
+            out = torch.empty(n, dtype=torch.bool, device="cuda")
+        check(_lib.mxco_bloom_contains(self._h, _ptr(keys), _ptr(stencil), _ptr(out), n, _stream(stream),
+                                       int(sync)))
+        return out
+
+    def contains_if(self, keys, stencil, out=None, n=None, stream=None, sync=True):
+        n = keys.shape[0] if n is None else n
+        if n and stencil is None:
+            raise ValueError("BloomFilter::ContainsIfAsync: stencil must not be null when keyNum is non-zero")
+        return self.contains(keys, out, n, stream, sync, stencil)
+
+    def merge(self, other, stream=None, sync=True):
+        check(_lib.mxco_bloom_merge(self._h, other._h, _stream(stream), int(sync)))
+
+    def intersect(self, other, stream=None, sync=True):
+        check(_lib.mxco_bloom_intersect(self._h, other._h, _stream(stream), int(sync)))
+
+
+class HyperLogLog:
+    """aclco::HyperLogLog<Key> (include/hyperloglog.h:24-59)."""
+
+    def __init__(self, precision, key_dtype=torch.int64, stream=None):
+        self.key_dtype = key_dtype
+        h = vp()
+        check(_lib.mxco_hll_create(precision, torch.empty(0, dtype=key_dtype).element_size(), _stream(stream),
+                                   ctypes.byref(h)))
+        self._h = h
+        self.precision = precision
+
+    @classmethod
+    def create_with_precision(cls, p, key_dtype=torch.int64, stream=None):
+        return cls(p, key_dtype, stream)
+
+    @classmethod
+    def create_with_sketch_size_kb(cls, kb, key_dtype=torch.int64, stream=None):
+        p = u32()
+        check(_lib.mxco_hll_precision_for_sketch_kb(kb, ctypes.byref(p)))
+        return cls(p.value, key_dtype, stream)
+
+    @classmethod
+    def create_with_standard_deviation(cls, sd, key_dtype=torch.int64, stream=None):
+        p = u32()
+        check(_lib.mxco_hll_precision_for_stddev(sd, ctypes.byref(p)))
+        return cls(p.value, key_dtype, stream)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.mxco_hll_destroy(self._h)
+            self._h = None
+
+    def sketch_size_kb(self):
+        return (1 << self.precision) // 1024
+
+    def data(self):
+        p, pr = vp(), u32()
+        check(_lib.mxco_hll_data(self._h, ctypes.byref(p), ctypes.byref(pr)))
+        return p.value
+
+    def registers(self):
+        return _dev_view(self.data(), 1 << self.precision)
+
+    def clear(self, stream=None):
+        check(_lib.mxco_hll_clear(self._h, _stream(stream)))
+
+    def add(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        check(_lib.mxco_hll_add(self._h, _ptr(keys), n, _stream(stream)))
+
+    def merge(self, other, stream=None):
+        check(_lib.mxco_hll_merge(self._h, other._h, _stream(stream)))
+
+    def histogram(self, stream=None):
+        h = (u32 * 65)()
+        check(_lib.mxco_hll_histogram(self._h, _stream(stream), h))
+        return list(h)
+
+    def estimate(self, stream=None):
+        e = u64()
+        check(_lib.mxco_hll_estimate(self._h, _stream(stream), ctypes.byref(e)))
+        return e.value

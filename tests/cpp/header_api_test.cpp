@@ -1,0 +1,156 @@
+// Exercises the C++ drop-in headers (include/mxco/*.h) the way reference
+// callers use aclco::StaticMap / StaticSet / BloomFilter / HyperLogLog:
+// device buffers, void* arguments, a stream, exceptions on misuse.
+// Prints "OK" and exits 0 when every check holds.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <vector>
+
+#include "mxco/bloom_filter.h"
+#include "mxco/hyperloglog.h"
+#include "mxco/static_map.h"
+#include "mxco/static_set.h"
+
+#define CHECK(c)                                                             \
+    do {                                                                     \
+        if (!(c)) {                                                          \
+            std::fprintf(stderr, "check failed: %s (line %d)\n", #c, __LINE__); \
+            std::exit(1);                                                    \
+        }                                                                    \
+    } while (0)
+
+template <typename T>
+T* to_device(std::vector<T> const& v)
+{
+    T* p = nullptr;
+    CHECK(hipMalloc(&p, v.size() * sizeof(T) + 16) == hipSuccess);
+    CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess);
+    return p;
+}
+
+template <typename T>
+std::vector<T> to_host(T const* p, size_t n)
+{
+    std::vector<T> v(n);
+    CHECK(hipMemcpy(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess);
+    return v;
+}
+
+int main()
+{
+    hipStream_t s;
+    CHECK(hipStreamCreate(&s) == hipSuccess);
+    const int n = 10000;
+
+    {  // StaticMap<int32, int32> with the reference's defaults
+        mxco::StaticMap<int32_t, int32_t> map(mxco::Extent<size_t>{2 * n}, -1, -1);
+        CHECK(map.Capacity() == 20000);
+        std::vector<mxco::Pair<int32_t, int32_t>> pairs(n + 10);
+        for (int i = 0; i < n; ++i) pairs[i] = {i * 3, i * 7 + 1};
+        for (int i = 0; i < 10; ++i) pairs[n + i] = {i * 3, 5};  // duplicates
+        auto* dp = to_device(pairs);
+        CHECK(map.Insert(dp, pairs.size(), s) == 10);
+        std::vector<int32_t> keys(2 * n);
+        for (int i = 0; i < 2 * n; ++i) keys[i] = i * 3;
+        auto* dk = to_device(keys);
+        int32_t* dout;
+        bool* dcon;
+        CHECK(hipMalloc(&dout, 2 * n * sizeof(int32_t)) == hipSuccess);
+        CHECK(hipMalloc(&dcon, 2 * n) == hipSuccess);
+        map.Find(dk, dout, keys.size(), s);
+        map.Contains(dk, dcon, keys.size(), s);
+        auto out = to_host(dout, 2 * n);
+        auto con = to_host(reinterpret_cast<uint8_t*>(dcon), 2 * n);
+        for (int i = 0; i < 2 * n; ++i) {
+            CHECK(out[i] == (i < n ? i * 7 + 1 : -1));
+            CHECK(con[i] == (i < n));
+        }
+        CHECK(map.Count(dk, keys.size(), s) == static_cast<size_t>(n));
+        size_t assigned = 0;
+        CHECK(map.InsertOrAssign(dp, 100, s, assigned) == 0 && assigned == 100);
+        CHECK(map.Insert(nullptr, 7, s) == 7);
+        bool threw = false;
+        try {
+            map.Find(dk, dout, size_t{1} << 32, s);
+        } catch (std::length_error const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        // InsertIf with the IsOdd predicate over an int32 stencil
+        mxco::StaticMap<int64_t, int64_t> m64(mxco::Extent<size_t>{1000}, -1, -1);
+        std::vector<mxco::Pair<int64_t, int64_t>> p64(100);
+        std::vector<int32_t> st(100);
+        for (int i = 0; i < 100; ++i) {
+            p64[i] = {i + 1000000000000ll, i};
+            st[i] = i;
+        }
+        auto* dp64 = to_device(p64);
+        auto* dst = to_device(st);
+        CHECK((m64.InsertIf<int32_t, mxco::IsOdd>(dp64, dst, 100, s)) == 0);
+        std::vector<int64_t> k64(100);
+        for (int i = 0; i < 100; ++i) k64[i] = i + 1000000000000ll;
+        auto* dk64 = to_device(k64);
+        CHECK(m64.Count(dk64, 100, s) == 50);
+        CHECK(hipFree(dp) == hipSuccess && hipFree(dk) == hipSuccess && hipFree(dout) == hipSuccess);
+        CHECK(hipFree(dcon) == hipSuccess && hipFree(dp64) == hipSuccess && hipFree(dst) == hipSuccess);
+        CHECK(hipFree(dk64) == hipSuccess);
+    }
+    {  // StaticSet<int64> (DoubleHashing<xxhash_32>, prime capacity)
+        mxco::StaticSet<int64_t> set(mxco::Extent<size_t>{20}, -1);
+        CHECK(set.Capacity() == 25);
+        std::vector<int64_t> keys = {5, 6, 7, 5, -1};
+        auto* dk = to_device(keys);
+        CHECK(set.Insert(dk, keys.size(), s) == 2);
+        int64_t* dout;
+        CHECK(hipMalloc(&dout, 5 * sizeof(int64_t)) == hipSuccess);
+        set.Find(dk, dout, keys.size(), s);
+        auto out = to_host(dout, 5);
+        CHECK(out[0] == 5 && out[1] == 6 && out[2] == 7 && out[3] == 5 && out[4] == -1);
+        CHECK(hipFree(dk) == hipSuccess && hipFree(dout) == hipSuccess);
+    }
+    {  // BloomFilter<int64>
+        mxco::BloomFilter<int64_t> f(mxco::Extent<size_t>{64});
+        CHECK(f.NumWords() == 512 && f.SizeBytes() == 2048);
+        std::vector<int64_t> keys(1000);
+        for (int i = 0; i < 1000; ++i) keys[i] = i * 11;
+        auto* dk = to_device(keys);
+        f.Add(dk, keys.size(), s);
+        bool* dc;
+        CHECK(hipMalloc(&dc, 1000) == hipSuccess);
+        f.Contains(dk, keys.size(), dc, s);
+        auto c = to_host(reinterpret_cast<uint8_t*>(dc), 1000);
+        for (uint8_t b : c) CHECK(b == 1);
+        bool threw = false;
+        try {
+            mxco::BloomFilter<int64_t> bad(mxco::Extent<size_t>{0});
+        } catch (std::invalid_argument const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        CHECK(hipFree(dk) == hipSuccess && hipFree(dc) == hipSuccess);
+    }
+    {  // HyperLogLog<int64>
+        auto h = mxco::HyperLogLog<int64_t>::CreateWithSketchSizeKB(32);
+        CHECK(h.SketchSizeKB() == 32);
+        std::vector<int64_t> keys(100000);
+        for (int i = 0; i < 100000; ++i) keys[i] = i;
+        auto* dk = to_device(keys);
+        h.Add(dk, keys.size(), s);
+        const double e = static_cast<double>(h.Estimate(s));
+        CHECK(e > 100000 * 0.97 && e < 100000 * 1.03);
+        bool threw = false;
+        try {
+            (void)mxco::HyperLogLog<int64_t>::CreateWithPrecision(12);
+        } catch (std::invalid_argument const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        CHECK(hipFree(dk) == hipSuccess);
+    }
+    CHECK(hipStreamDestroy(s) == hipSuccess);
+    std::printf("OK\n");
+    return 0;
+}

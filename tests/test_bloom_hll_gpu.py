@@ -1,0 +1,179 @@
+"""BloomFilter and HyperLogLog on the GPU through the C ABI, against the
+oracle: filter words and HLL registers bit-exact, Contains outputs exact,
+estimates equal to the FP64 finalizer applied to the same registers."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TD = {np.int32: torch.int32, np.int64: torch.int64, np.float32: torch.float32, np.uint32: torch.int32,
+      np.uint64: torch.int64}
+
+
+def mx():
+    import mxco
+    return mxco
+
+
+def dev(a):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    if a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a).cuda()
+
+
+def words_of(f):
+    return f.words().cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.uint32, np.float32, np.int64, np.uint64])
+@pytest.mark.parametrize("num_blocks,n", [(1, 100), (37, 5000), (4096, 200000)])
+@pytest.mark.parametrize("seed", [0, 42])
+def test_bloom_words_and_contains_bit_exact(dtype, num_blocks, n, seed):
+    rng = np.random.default_rng(num_blocks + n)
+    keys = rng.integers(-2**31, 2**31 - 1, n).astype(dtype)
+    keys[: n // 10] = keys[n // 10: 2 * (n // 10)]  # duplicates
+    f = mx().BloomFilter(num_blocks, TD[dtype], seed=seed)
+    f.add(dev(keys))
+    ref = np.zeros(num_blocks * 8, np.uint32)
+    O.bloom_add(ref, num_blocks, keys, seed)
+    np.testing.assert_array_equal(words_of(f), ref)
+    q = np.concatenate([keys[:1000], rng.integers(-2**31, 2**31 - 1, 3000).astype(dtype)])
+    got = f.contains(dev(q)).cpu().numpy()
+    np.testing.assert_array_equal(got, O.bloom_contains(ref, num_blocks, q, seed))
+    assert got[:min(1000, n)].all()  # no false negatives
+
+
+def test_bloom_add_if_contains_if_merge_intersect():
+    rng = np.random.default_rng(3)
+    nb = 1024
+    a_keys = rng.integers(0, 2**62, 20000).astype(np.int64)
+    b_keys = rng.integers(0, 2**62, 20000).astype(np.int64)
+    st = (rng.integers(0, 3, 20000) == 0).astype(np.uint8)
+    a = mx().BloomFilter(nb)
+    a.add_if(dev(a_keys), dev(st))
+    ra = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(ra, nb, a_keys[st != 0])
+    np.testing.assert_array_equal(words_of(a), ra)
+    c = a.contains_if(dev(a_keys), dev(st)).cpu().numpy()
+    assert c[st != 0].all() and not c[st == 0].any()
+    b = mx().BloomFilter(nb)
+    b.add(dev(b_keys))
+    rb = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(rb, nb, b_keys)
+    m = mx().BloomFilter(nb)
+    m.merge(a)
+    m.merge(b)
+    np.testing.assert_array_equal(words_of(m), ra | rb)
+    m.intersect(b)
+    np.testing.assert_array_equal(words_of(m), (ra | rb) & rb)
+    m.clear()
+    assert not words_of(m).any()
+
+
+def test_bloom_errors():
+    import mxco
+    with pytest.raises(ValueError):
+        mxco.BloomFilter(0)
+    f = mxco.BloomFilter(8)
+    with pytest.raises(ValueError):
+        f.add(0, n=4)
+    with pytest.raises(ValueError):
+        f.merge(mxco.BloomFilter(16))
+    with pytest.raises(ValueError):
+        f.merge(mxco.BloomFilter(8, seed=1))
+    f.add(0, n=0)  # zero keys: no-op
+
+
+def test_bloom_large_fpr():
+    """2^22 blocks (128 MiB) with 2^24 keys: words bit-exact, FPR as expected."""
+    nb = 1 << 22
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 2**62, 1 << 24).astype(np.int64)
+    f = mx().BloomFilter(nb)
+    f.add(dev(keys))
+    ref = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(ref, nb, keys)
+    np.testing.assert_array_equal(words_of(f), ref)
+    absent = rng.integers(2**62, 2**63 - 1, 1 << 20).astype(np.int64)
+    fpr = f.contains(dev(absent)).float().mean().item()
+    assert fpr == O.bloom_contains(ref, nb, absent).mean()
+    assert fpr < 0.01  # 4 keys per block on average
+
+
+# --------------------------------------------------------------------------
+# HyperLogLog
+# --------------------------------------------------------------------------
+
+def hll_tables(p):
+    d = np.load(O.os.path.join(O._HERE, "data", "hllpp_p%d.npz" % p))
+    return d["raw"], d["bias"], float(d["threshold"])
+
+
+def oracle_estimate(regs, p):
+    h = O.hll_histogram(regs, p)
+    raw, bias, thr = hll_tables(p)
+    z = O.lib().orc_hll_z(O._ptr(h))
+    return int(O.lib().orc_hll_finalize(p, z, int(h[0]), O._ptr(raw), O._ptr(bias), len(raw), thr)), h
+
+
+@pytest.mark.parametrize("p", [13, 14, 15, 16, 17, 18])
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_hll_registers_and_estimate(p, dtype):
+    rng = np.random.default_rng(p)
+    s = mx().HyperLogLog(p, TD[dtype])
+    ref = np.zeros(1 << p, np.uint8)
+    # sizes on both sides of the direct / LDS-private dispatch boundary
+    for n in (1000, 1 << (p + 3), 3 * (1 << 18) + 13):
+        keys = rng.integers(-2**31, 2**31 - 1, n).astype(dtype)
+        s.add(dev(keys))
+        O.hll_add(ref, p, keys)
+        regs = s.registers().cpu().numpy()
+        np.testing.assert_array_equal(regs, ref)
+        est, hist = oracle_estimate(ref, p)
+        assert s.histogram() == hist.tolist()
+        assert s.estimate() == est
+
+
+def test_hll_merge_and_accuracy():
+    p = 15
+    rng = np.random.default_rng(9)
+    parts = [rng.integers(0, 2**62, 200000).astype(np.int64) for _ in range(4)]
+    sk = [mx().HyperLogLog(p) for _ in parts]
+    for s, k in zip(sk, parts):
+        s.add(dev(k))
+    total = mx().HyperLogLog(p)
+    for s in sk:
+        total.merge(s)
+    ref = np.zeros(1 << p, np.uint8)
+    for k in parts:
+        O.hll_add(ref, p, k)
+    np.testing.assert_array_equal(total.registers().cpu().numpy(), ref)
+    true = len(np.unique(np.concatenate(parts)))
+    est = total.estimate()
+    assert est == oracle_estimate(ref, p)[0]
+    assert abs(est - true) / true < 3 * 1.04 / math.sqrt(1 << p)
+
+
+def test_hll_factories_and_errors():
+    import mxco
+    assert mxco.HyperLogLog.create_with_sketch_size_kb(32).precision == 15
+    assert mxco.HyperLogLog.create_with_standard_deviation(0.01).precision == 14
+    with pytest.raises(ValueError):
+        mxco.HyperLogLog(12)
+    with pytest.raises(ValueError):
+        mxco.HyperLogLog.create_with_sketch_size_kb(12)
+    with pytest.raises(ValueError):
+        mxco.HyperLogLog(14).merge(mxco.HyperLogLog(15))
+    s = mxco.HyperLogLog(13)
+    with pytest.raises(ValueError):
+        s.add(0, n=5)
+    s.add(0, n=0)
+    assert s.estimate() == 0
