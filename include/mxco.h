@@ -186,6 +186,49 @@ MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint6
                                          uint64_t* lines, uint64_t* slots);
 
 /* ---------------------------------------------------------------------- */
+/* StaticMultimap (include/static_multimap.h:88-556): signed int32/int64  */
+/* keys and values, duplicates kept.  All calls are synchronous, as in    */
+/* the reference (docs/static_multimap_API文档和使用示例.md:23).            */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_mm mxco_mm;
+
+/* StaticMultimap(capacity, emptyKey, emptyValue, stream): capacity > 0 pairs
+ * (Capacity() returns it unchanged); the table starts empty. */
+MXCO_API mxco_status mxco_mm_create(uint32_t key_size, uint32_t value_size, uint64_t capacity, const void* empty_key,
+                                    const void* empty_value, hipStream_t stream, mxco_mm** out);
+MXCO_API mxco_status mxco_mm_destroy(mxco_mm* map);
+/* Capacity() / Size(stream) / Clear(stream) (static_multimap.h:397-410). */
+MXCO_API mxco_status mxco_mm_capacity(const mxco_mm* map, uint64_t* capacity);
+MXCO_API mxco_status mxco_mm_size(mxco_mm* map, hipStream_t stream, uint64_t* size);
+MXCO_API mxco_status mxco_mm_clear(mxco_mm* map, hipStream_t stream);
+/* Insert / InsertIf<Stencil, Predicate> (static_multimap.h:323-364, 437-445):
+ * checks every selected pair for the reserved key/value first
+ * (MXCO_INVALID_ARGUMENT, nothing inserted), then admits the input-order
+ * prefix of the selected pairs that fits; *num_failed = selected - admitted. */
+MXCO_API mxco_status mxco_mm_insert(mxco_mm* map, const void* pairs, const mxco_stencil* stencil, uint64_t n,
+                                    hipStream_t stream, uint64_t* num_failed);
+/* Contains / ContainsIf (static_multimap.h:446-455): 1-byte bool per key. */
+MXCO_API mxco_status mxco_mm_contains(mxco_mm* map, const void* keys, const mxco_stencil* stencil, uint8_t* out,
+                                      uint64_t n, hipStream_t stream);
+/* Find / FindIf (static_multimap.h:456-465): smallest value of the key, or
+ * the empty value on a miss or a false predicate. */
+MXCO_API mxco_status mxco_mm_find(mxco_mm* map, const void* keys, const mxco_stencil* stencil, void* out_values,
+                                  uint64_t n, hipStream_t stream);
+/* Count(keys, n, stream) (static_multimap.h:466-490): total matches. */
+MXCO_API mxco_status mxco_mm_count(mxco_mm* map, const void* keys, uint64_t n, hipStream_t stream, uint64_t* count);
+/* Retrieve(keys, n, probeOut, valueOut, outputCapacity, stream)
+ * (static_multimap.h:491-505): every match grouped by query index, values
+ * of one query ascending, probe_out holding the query key; MXCO_LENGTH_ERROR
+ * (nothing written) when the matches exceed output_capacity. */
+MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* map, const void* keys, uint64_t n, void* probe_out, void* value_out,
+                                      uint64_t output_capacity, hipStream_t stream, uint64_t* num_out);
+/* RetrieveAll(keyOut, valueOut, outputCapacity, stream) (static_multimap.h:506-540):
+ * the stored pairs as a multiset (order not part of the contract). */
+MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* map, void* key_out, void* value_out, uint64_t output_capacity,
+                                          hipStream_t stream, uint64_t* num_out);
+
+/* ---------------------------------------------------------------------- */
 /* BloomFilter (include/bloom_filter.h:37-133): the default policy        */
 /* BloomFilterPolicy<Key, xxhash_64<Key>, uint32_t, 8 words, 8 bits>      */
 /* (include/bloom_filter_policy.h:54-181).  Keys are 4 or 8 bytes         */

@@ -1,0 +1,627 @@
+// StaticMultimap (include/static_multimap.h:88-556) for gfx950.
+//
+// The reference's public contract (docs/static_multimap_API文档和使用示例.md:21-45)
+// fixes results, not layout:
+//   - Insert admits the input-order prefix of the selected pairs that fits
+//     the remaining capacity, after checking every selected pair for the
+//     reserved key/value (std::invalid_argument, nothing modified);
+//   - Find returns the minimum value of the key, Contains a bool, Count the
+//     total number of matches (repeated queries count again);
+//   - Retrieve writes every match grouped by query index with the values of
+//     each query ascending; RetrieveAll writes the stored multiset.
+//
+// Layout here: a power-of-two table of P >= 2 * capacity Pair<K,V> slots
+// (load <= 1/2), linear probing from fmix64(key).  Every pair takes its own
+// slot (first empty slot on its walk, 64-bit CAS on the key word), so all
+// duplicates of a key lie on one contiguous run that ends at an empty slot.
+// A walk reads the rest of a 128-byte line per round trip.  Scans and the
+// per-query value sort use hipCUB.
+#include <string.h>
+
+#include <hipcub/hipcub.hpp>
+#include <new>
+#include <string>
+
+#include "mxco_common.h"
+#include "oa_kernels.h"
+
+using namespace mxco;
+
+struct mxco_mm {
+    uint32_t ks, vs, ss;
+    uint64_t capacity;
+    uint64_t size;
+    uint64_t slots_pow2;  // P
+    uint8_t* table;
+    uint64_t empty_key, empty_val;
+    // scratch, grown on demand
+    uint32_t* status;
+    uint64_t* host;  // pinned, 4 words
+    void* buf[6];
+    size_t buf_size[6];
+};
+
+namespace {
+
+template <int KS>
+__device__ __forceinline__ uint64_t sext(uint64_t k)
+{
+    if constexpr (KS == 4) return (uint64_t)(int64_t)(int32_t)(uint32_t)k;
+    else return k;
+}
+
+template <int KS, int VS>
+__device__ __forceinline__ uint64_t mm_start(uint64_t key, uint64_t mask)
+{
+    return fmix64(sext<KS>(key)) & mask;
+}
+
+template <int VS>
+__device__ __forceinline__ int64_t sval(uint64_t v)
+{
+    if constexpr (VS == 4) return (int64_t)(int32_t)(uint32_t)v;
+    else return (int64_t)v;
+}
+
+__device__ __forceinline__ bool selected(const uint8_t* stencil, uint32_t esz, uint32_t pred, uint64_t i)
+{
+    if (!stencil) return true;
+    int64_t s;
+    if (esz == 1) s = (int8_t)stencil[i];
+    else if (esz == 4) s = ((const int32_t*)stencil)[i];
+    else s = ((const int64_t*)stencil)[i];
+    switch (pred) {
+    case MXCO_PRED_NONZERO: return s != 0;
+    case MXCO_PRED_ODD: return (s & 1) != 0;
+    case MXCO_PRED_EVEN: return (s & 1) == 0;
+    default: return true;
+    }
+}
+
+// Selection flags and reserved-value check (the reference's Validate kernel).
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void mm_validate_kernel(const uint8_t* pairs, const uint8_t* stencil, uint32_t esz,
+                                                          uint32_t pred, uint64_t n, uint64_t ek, uint64_t ev,
+                                                          uint64_t* flags, uint32_t* status)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const bool sel = selected(stencil, esz, pred, i);
+        if (flags) flags[i] = sel ? 1 : 0;
+        if (sel) {
+            const uint64_t k = load_bits<KS>(pairs + i * SS);
+            const uint64_t v = load_bits<VS>(pairs + i * SS + (SS == 8 ? 4 : 8));
+            bad |= (k == ek) | (v == ev);
+        }
+    }
+    if (__ballot(bad) != 0 && __lane_id() == 0) atomicOr(status, 1u);
+}
+
+// Insert: input i goes in when it is selected and its selection rank is
+// below `accepted` (rank = i for unconditional inserts).
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void mm_insert_kernel(uint8_t* table, uint64_t mask, const uint8_t* pairs,
+                                                        const uint8_t* stencil, uint32_t esz, uint32_t pred,
+                                                        const uint64_t* ranks, uint64_t n, uint64_t accepted,
+                                                        uint64_t ek, uint64_t ev, uint32_t* status)
+{
+    using L = Lay<KS, VS>;
+    constexpr int SS = L::SS;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (ranks) {
+            if (!selected(stencil, esz, pred, i) || ranks[i] >= accepted) continue;
+        } else if (i >= accepted) {
+            continue;
+        }
+        const uint64_t key = load_bits<KS>(pairs + i * SS);
+        const uint64_t val = load_bits<VS>(pairs + i * SS + (SS == 8 ? 4 : 8));
+        uint64_t pos = mm_start<KS, VS>(key, mask);
+        bool done = false;
+        for (uint64_t scanned = 0; scanned <= mask && !done; ++scanned, pos = (pos + 1) & mask) {
+            uint8_t* slot = table + pos * SS;
+            if (load_bits<KS>(slot) != ek) continue;
+            if constexpr (SS == 8) {
+                const uint64_t expect = (ek & 0xffffffffull) | (ev << 32);
+                const uint64_t want = (key & 0xffffffffull) | (val << 32);
+                done = atomicCAS((unsigned long long*)slot, expect, want) == expect;
+            } else {
+                if (cas_bits<KS>(slot, ek, key) == ek) {
+                    store_bits<VS>(slot + 8, val);
+                    done = true;
+                }
+            }
+        }
+        if (!done) atomicOr(status, 2u);
+    }
+}
+
+enum { Q_CONTAINS = 0, Q_FIND = 1, Q_COUNT = 2, Q_GATHER = 3 };
+
+// One lane per query; the walk covers the run of occupied slots from the
+// key's start slot, one line-remainder window per round trip.
+template <int KS, int VS, int MODE>
+__global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uint64_t mask, const uint8_t* keys,
+                                                       const uint8_t* stencil, uint32_t esz, uint32_t pred,
+                                                       uint64_t n, uint64_t ek, uint64_t ev, uint8_t* out,
+                                                       uint64_t* counts, const uint64_t* offsets, uint8_t* probe_out,
+                                                       uint8_t* value_out, unsigned long long* total)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr int SS = L::SS;
+    constexpr int SPL = 128 / SS;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long local = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t key = load_bits<KS>(keys + i * KS);
+        const bool sel = selected(stencil, esz, pred, i);
+        uint64_t matches = 0;
+        int64_t best = 0;
+        bool any = false;
+        uint64_t wpos = (MODE == Q_GATHER) ? offsets[i] : 0;
+        if (sel && key != ek) {
+            uint64_t pos = mm_start<KS, VS>(key, mask);
+            uint64_t left = mask + 1;
+            bool stop = false;
+            while (!stop && left) {
+                uint32_t wlen = SPL - (uint32_t)(pos & (SPL - 1));
+                if (wlen > left) wlen = (uint32_t)left;
+                if ((mask + 1) - pos < wlen) wlen = (uint32_t)((mask + 1) - pos);
+                Raw s[SPL];
+#pragma unroll
+                for (int k = 0; k < SPL; ++k)
+                    if ((uint32_t)k < wlen) s[k] = *(const Raw*)(table + (pos + k) * SS);
+#pragma unroll
+                for (int k = 0; k < SPL; ++k) {
+                    if ((uint32_t)k >= wlen || stop) continue;
+                    const uint64_t sk = L::key(s[k]);
+                    if (sk == ek) {
+                        stop = true;
+                    } else if (sk == key) {
+                        const int64_t v = sval<VS>(L::val(s[k]));
+                        if (MODE == Q_CONTAINS) stop = true;
+                        if (MODE == Q_GATHER) {
+                            store_bits<KS>(probe_out + wpos * KS, key);
+                            store_bits<VS>(value_out + wpos * VS, (uint64_t)v);
+                            ++wpos;
+                        }
+                        best = (!any || v < best) ? v : best;
+                        any = true;
+                        ++matches;
+                    }
+                }
+                left -= wlen;
+                pos = (pos + wlen) & mask;
+            }
+        }
+        if constexpr (MODE == Q_CONTAINS) out[i] = any ? 1 : 0;
+        if constexpr (MODE == Q_FIND) store_bits<VS>(out + i * VS, any ? (uint64_t)best : ev);
+        if constexpr (MODE == Q_COUNT) {
+            if (counts) counts[i] = matches;
+            local += matches;
+        }
+    }
+    if constexpr (MODE == Q_COUNT) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+        if (__lane_id() == 0 && local) atomicAdd(total, local);
+    }
+}
+
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void mm_clear_kernel(uint8_t* table, uint64_t slots, uint64_t ek, uint64_t ev)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += stride) {
+        if constexpr (SS == 8) {
+            *(uint64_t*)(table + i * 8) = (ek & 0xffffffffull) | (ev << 32);
+        } else {
+            u64x2 v;
+            v.x = KS == 4 ? (ek & 0xffffffffull) : ek;
+            v.y = VS == 4 ? (ev & 0xffffffffull) : ev;
+            *(u64x2*)(table + i * 16) = v;
+        }
+    }
+}
+
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void mm_occupied_kernel(const uint8_t* table, uint64_t slots, uint64_t ek,
+                                                          uint64_t* flags)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += stride)
+        flags[i] = load_bits<KS>(table + i * SS) != ek;
+}
+
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void mm_export_kernel(const uint8_t* table, uint64_t slots, uint64_t ek,
+                                                        const uint64_t* offsets, uint8_t* keys_out,
+                                                        uint8_t* values_out)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += stride) {
+        const uint64_t k = load_bits<KS>(table + i * SS);
+        if (k == ek) continue;
+        const uint64_t o = offsets[i];
+        store_bits<KS>(keys_out + o * KS, k);
+        store_bits<VS>(values_out + o * VS, load_bits<VS>(table + i * SS + (SS == 8 ? 4 : 8)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+mxco_status reserve(mxco_mm* m, int which, size_t bytes)
+{
+    if (m->buf_size[which] >= bytes) return MXCO_OK;
+    if (m->buf[which]) (void)hipFree(m->buf[which]);
+    m->buf[which] = nullptr;
+    m->buf_size[which] = 0;
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    MXCO_HIP(hipMalloc(&m->buf[which], want), "StaticMultimap: scratch");
+    m->buf_size[which] = want;
+    return MXCO_OK;
+}
+
+template <typename T>
+T* sbuf(mxco_mm* m, int which)
+{
+    return (T*)m->buf[which];
+}
+
+// out[0..n] = exclusive prefix sums of in[0..n-1]; out[n] (the total) is also
+// returned in *total.  in[n] is read but does not affect the result.
+mxco_status exclusive_sum(mxco_mm* m, const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s, uint64_t* total)
+{
+    size_t temp = 0;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, in, out, (int64_t)n + 1, s), "scan");
+    if (mxco_status st = reserve(m, 3, temp)) return st;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(m->buf[3], temp, in, out, (int64_t)n + 1, s), "scan");
+    MXCO_HIP(hipMemcpyAsync(m->host, out + n, 8, hipMemcpyDeviceToHost, s), "scan total");
+    MXCO_HIP(hipStreamSynchronize(s), "scan");
+    *total = m->host[0];
+    return MXCO_OK;
+}
+
+#define MM_DISPATCH(m, CALL)                              \
+    do {                                                  \
+        if ((m)->ks == 4 && (m)->vs == 4) { CALL(4, 4); } \
+        else if ((m)->ks == 4) { CALL(4, 8); }            \
+        else if ((m)->vs == 4) { CALL(8, 4); }            \
+        else { CALL(8, 8); }                              \
+    } while (0)
+
+mxco_status check_mm(const mxco_mm* m)
+{
+    if (!m || !m->table) return fail(MXCO_LOGIC_ERROR, "StaticMultimap: moved-from object");
+    return MXCO_OK;
+}
+
+mxco_status clear_table(mxco_mm* m, hipStream_t s)
+{
+    const unsigned grid = grid_for(m->slots_pow2, 8);
+#define CLR(K, V) mm_clear_kernel<K, V><<<grid, 256, 0, s>>>(m->table, m->slots_pow2, m->empty_key, m->empty_val)
+    MM_DISPATCH(m, CLR);
+#undef CLR
+    MXCO_HIP(hipGetLastError(), "StaticMultimap: clear");
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: clear");
+    m->size = 0;
+    return MXCO_OK;
+}
+
+mxco_status set_pred(const mxco_stencil* st, const uint8_t** data, uint32_t* esz, uint32_t* pred)
+{
+    *data = nullptr;
+    *esz = 1;
+    *pred = MXCO_PRED_ALWAYS;
+    if (!st) return MXCO_OK;
+    if (!st->data) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    if (st->elem_size != 1 && st->elem_size != 4 && st->elem_size != 8)
+        return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: stencil elements must be 1, 4 or 8 bytes");
+    *data = (const uint8_t*)st->data;
+    *esz = st->elem_size;
+    *pred = st->pred;
+    return MXCO_OK;
+}
+
+mxco_status query(mxco_mm* m, int mode, const void* keys, const mxco_stencil* st, void* out, uint64_t n,
+                  uint64_t* counts, const uint64_t* offsets, void* probe_out, void* value_out,
+                  unsigned long long* total, hipStream_t s)
+{
+    const uint8_t* sd;
+    uint32_t esz, pred;
+    if (mxco_status e = set_pred(st, &sd, &esz, &pred)) return e;
+    const unsigned grid = grid_for(n, 16);
+    const uint64_t mask = m->slots_pow2 - 1;
+#define QL(K, V)                                                                                                  \
+    switch (mode) {                                                                                               \
+    case Q_CONTAINS:                                                                                              \
+        mm_query_kernel<K, V, Q_CONTAINS><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz,   \
+                                                               pred, n, m->empty_key, m->empty_val,               \
+                                                               (uint8_t*)out, nullptr, nullptr, nullptr, nullptr, \
+                                                               nullptr);                                          \
+        break;                                                                                                    \
+    case Q_FIND:                                                                                                  \
+        mm_query_kernel<K, V, Q_FIND><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz, pred,  \
+                                                           n, m->empty_key, m->empty_val, (uint8_t*)out, nullptr, \
+                                                           nullptr, nullptr, nullptr, nullptr);                   \
+        break;                                                                                                    \
+    case Q_COUNT:                                                                                                 \
+        mm_query_kernel<K, V, Q_COUNT><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz, pred, \
+                                                            n, m->empty_key, m->empty_val, nullptr, counts,       \
+                                                            nullptr, nullptr, nullptr, total);                    \
+        break;                                                                                                    \
+    default:                                                                                                      \
+        mm_query_kernel<K, V, Q_GATHER><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz,      \
+                                                             pred, n, m->empty_key, m->empty_val, nullptr,        \
+                                                             nullptr, offsets, (uint8_t*)probe_out,              \
+                                                             (uint8_t*)value_out, nullptr);                       \
+    }
+    MM_DISPATCH(m, QL);
+#undef QL
+    MXCO_HIP(hipGetLastError(), "StaticMultimap: query");
+    return MXCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MXCO_API mxco_status mxco_mm_create(uint32_t key_size, uint32_t value_size, uint64_t capacity, const void* empty_key,
+                                    const void* empty_value, hipStream_t stream, mxco_mm** out)
+{
+    if (!out || !empty_key || !empty_value) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null argument");
+    *out = nullptr;
+    if ((key_size != 4 && key_size != 8) || (value_size != 4 && value_size != 8))
+        return fail(MXCO_UNSUPPORTED, "StaticMultimap supports signed I32/I64 keys and values");
+    if (capacity == 0) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: capacity must be positive");
+    if (capacity > (1ull << 40)) return fail(MXCO_LENGTH_ERROR, "StaticMultimap: capacity too large");
+    mxco_mm* m = new (std::nothrow) mxco_mm();
+    if (!m) return fail(MXCO_OUT_OF_MEMORY, "StaticMultimap: host allocation");
+    m->ks = key_size;
+    m->vs = value_size;
+    m->ss = (key_size == 4 && value_size == 4) ? 8 : 16;
+    m->capacity = capacity;
+    uint64_t p = 64;
+    while (p < 2 * capacity) p <<= 1;
+    m->slots_pow2 = p;
+    memcpy(&m->empty_key, empty_key, key_size);
+    memcpy(&m->empty_val, empty_value, value_size);
+    hipError_t e = hipMalloc((void**)&m->table, p * m->ss);
+    if (e == hipSuccess) e = hipMalloc((void**)&m->status, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->host, 4 * sizeof(uint64_t), 0);
+    mxco_status st = e == hipSuccess ? clear_table(m, stream) : hip_fail(e, "StaticMultimap: allocation");
+    if (st != MXCO_OK) {
+        if (m->table) (void)hipFree(m->table);
+        if (m->status) (void)hipFree(m->status);
+        if (m->host) (void)hipHostFree(m->host);
+        delete m;
+        return st;
+    }
+    *out = m;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_destroy(mxco_mm* m)
+{
+    if (!m) return ok();
+    if (m->table) (void)hipFree(m->table);
+    if (m->status) (void)hipFree(m->status);
+    if (m->host) (void)hipHostFree(m->host);
+    for (int i = 0; i < 6; ++i)
+        if (m->buf[i]) (void)hipFree(m->buf[i]);
+    delete m;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_capacity(const mxco_mm* m, uint64_t* capacity)
+{
+    if (!capacity) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null output");
+    *capacity = m && m->table ? m->capacity : 0;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_size(mxco_mm* m, hipStream_t stream, uint64_t* size)
+{
+    if (mxco_status s = check_mm(m)) return s;
+    if (!size) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null output");
+    MXCO_HIP(hipStreamSynchronize(stream), "StaticMultimap: size");
+    *size = m->size;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_clear(mxco_mm* m, hipStream_t stream)
+{
+    if (mxco_status s = check_mm(m)) return s;
+    if (mxco_status s = clear_table(m, stream)) return s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_insert(mxco_mm* m, const void* pairs, const mxco_stencil* stencil, uint64_t n,
+                                    hipStream_t s, uint64_t* num_failed)
+{
+    if (mxco_status e = check_mm(m)) return e;
+    if (num_failed) *num_failed = 0;
+    if (n == 0) return ok();
+    if (!pairs) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    const uint8_t* sd;
+    uint32_t esz, pred;
+    if (mxco_status e = set_pred(stencil, &sd, &esz, &pred)) return e;
+    const bool conditional = sd != nullptr && pred != MXCO_PRED_ALWAYS;
+    if (!conditional) sd = nullptr;
+    uint64_t* flags = nullptr;
+    if (conditional) {
+        if (mxco_status e = reserve(m, 0, (n + 1) * sizeof(uint64_t))) return e;
+        flags = sbuf<uint64_t>(m, 0);
+    }
+    MXCO_HIP(hipMemsetAsync(m->status, 0, sizeof(uint32_t), s), "StaticMultimap: clear status");
+    const unsigned grid = grid_for(n, 8);
+#define VAL(K, V)                                                                                                  \
+    mm_validate_kernel<K, V><<<grid, 256, 0, s>>>((const uint8_t*)pairs, sd, esz, pred, n, m->empty_key,          \
+                                                  m->empty_val, flags, m->status)
+    MM_DISPATCH(m, VAL);
+#undef VAL
+    MXCO_HIP(hipGetLastError(), "StaticMultimap: validate");
+    uint64_t selected = n;
+    uint64_t* ranks = nullptr;
+    if (conditional) {
+        if (mxco_status e = reserve(m, 4, (n + 1) * sizeof(uint64_t))) return e;
+        ranks = sbuf<uint64_t>(m, 4);
+        if (mxco_status e = exclusive_sum(m, flags, ranks, n, s, &selected)) return e;
+    }
+    MXCO_HIP(hipMemcpyAsync(m->host + 1, m->status, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "status");
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: validate");
+    if (((uint32_t*)(m->host + 1))[0] & 1u)
+        return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: selected input contains a reserved key or value");
+    const uint64_t room = m->capacity - m->size;
+    const uint64_t accepted = selected < room ? selected : room;
+    if (accepted) {
+#define INS(K, V)                                                                                                  \
+    mm_insert_kernel<K, V><<<grid, 256, 0, s>>>(m->table, m->slots_pow2 - 1, (const uint8_t*)pairs, sd, esz, pred, \
+                                                ranks, n, accepted, m->empty_key, m->empty_val, m->status)
+        MM_DISPATCH(m, INS);
+#undef INS
+        MXCO_HIP(hipGetLastError(), "StaticMultimap: insert");
+        MXCO_HIP(hipMemcpyAsync(m->host + 1, m->status, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "status");
+        MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: insert");
+        if (((uint32_t*)(m->host + 1))[0] & 2u)
+            return fail(MXCO_RUNTIME_ERROR, "StaticMultimap: bounded insertion failed despite available capacity");
+        m->size += accepted;
+    }
+    if (num_failed) *num_failed = selected - accepted;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_contains(mxco_mm* m, const void* keys, const mxco_stencil* stencil, uint8_t* out,
+                                      uint64_t n, hipStream_t s)
+{
+    if (mxco_status e = check_mm(m)) return e;
+    if (n == 0) return ok();
+    if (!keys || !out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    if (mxco_status e = query(m, Q_CONTAINS, keys, stencil, out, n, nullptr, nullptr, nullptr, nullptr, nullptr, s))
+        return e;
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: contains");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_find(mxco_mm* m, const void* keys, const mxco_stencil* stencil, void* out, uint64_t n,
+                                  hipStream_t s)
+{
+    if (mxco_status e = check_mm(m)) return e;
+    if (n == 0) return ok();
+    if (!keys || !out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    if (mxco_status e = query(m, Q_FIND, keys, stencil, out, n, nullptr, nullptr, nullptr, nullptr, nullptr, s))
+        return e;
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: find");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_count(mxco_mm* m, const void* keys, uint64_t n, hipStream_t s, uint64_t* count)
+{
+    if (mxco_status e = check_mm(m)) return e;
+    if (!count) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null output");
+    *count = 0;
+    if (n == 0) return ok();
+    if (!keys) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    if (mxco_status e = reserve(m, 2, sizeof(unsigned long long))) return e;
+    unsigned long long* total = sbuf<unsigned long long>(m, 2);
+    MXCO_HIP(hipMemsetAsync(total, 0, 8, s), "StaticMultimap: count");
+    if (mxco_status e = query(m, Q_COUNT, keys, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, total, s))
+        return e;
+    MXCO_HIP(hipMemcpyAsync(m->host, total, 8, hipMemcpyDeviceToHost, s), "StaticMultimap: count");
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: count");
+    *count = m->host[0];
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* m, const void* keys, uint64_t n, void* probe_out, void* value_out,
+                                      uint64_t output_capacity, hipStream_t s, uint64_t* num_out)
+{
+    if (mxco_status e = check_mm(m)) return e;
+    if (!num_out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null output");
+    *num_out = 0;
+    if (n == 0) return ok();
+    if (!keys) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    // counts -> exclusive offsets (n + 1 entries)
+    if (mxco_status e = reserve(m, 0, (n + 1) * sizeof(uint64_t))) return e;
+    if (mxco_status e = reserve(m, 4, (n + 1) * sizeof(uint64_t))) return e;
+    if (mxco_status e = reserve(m, 2, sizeof(unsigned long long))) return e;
+    uint64_t* counts = sbuf<uint64_t>(m, 0);
+    uint64_t* offsets = sbuf<uint64_t>(m, 4);
+    unsigned long long* total_dev = sbuf<unsigned long long>(m, 2);
+    MXCO_HIP(hipMemsetAsync(total_dev, 0, 8, s), "StaticMultimap: retrieve");
+    if (mxco_status e = query(m, Q_COUNT, keys, nullptr, nullptr, n, counts, nullptr, nullptr, nullptr, total_dev, s))
+        return e;
+    uint64_t total = 0;
+    if (mxco_status e = exclusive_sum(m, counts, offsets, n, s, &total)) return e;
+    if (total > output_capacity) return fail(MXCO_LENGTH_ERROR, "StaticMultimap: output capacity is insufficient");
+    if (total == 0) return ok();
+    if (!probe_out || !value_out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    // gather unsorted values into scratch, then sort every query's segment into value_out
+    if (mxco_status e = reserve(m, 1, total * m->vs)) return e;
+    if (mxco_status e = query(m, Q_GATHER, keys, nullptr, nullptr, n, nullptr, offsets, probe_out, m->buf[1],
+                              nullptr, s))
+        return e;
+    size_t temp = 0;
+    hipError_t he;
+    if (m->vs == 4) {
+        he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int32_t*)m->buf[1], (int32_t*)value_out,
+                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
+    } else {
+        he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int64_t*)m->buf[1], (int64_t*)value_out,
+                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
+    }
+    MXCO_HIP(he, "StaticMultimap: sort");
+    if (mxco_status e = reserve(m, 3, temp)) return e;
+    if (m->vs == 4) {
+        he = hipcub::DeviceSegmentedSort::SortKeys(m->buf[3], temp, (const int32_t*)m->buf[1], (int32_t*)value_out,
+                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
+    } else {
+        he = hipcub::DeviceSegmentedSort::SortKeys(m->buf[3], temp, (const int64_t*)m->buf[1], (int64_t*)value_out,
+                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
+    }
+    MXCO_HIP(he, "StaticMultimap: sort");
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: retrieve");
+    *num_out = total;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* m, void* key_out, void* value_out, uint64_t output_capacity,
+                                          hipStream_t s, uint64_t* num_out)
+{
+    if (mxco_status e = check_mm(m)) return e;
+    if (!num_out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null output");
+    *num_out = 0;
+    if (m->size > output_capacity) return fail(MXCO_LENGTH_ERROR, "StaticMultimap: output capacity is insufficient");
+    if (m->size == 0) return ok();
+    if (!key_out || !value_out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+    const uint64_t P = m->slots_pow2;
+    if (mxco_status e = reserve(m, 0, (P + 1) * sizeof(uint64_t))) return e;
+    if (mxco_status e = reserve(m, 4, (P + 1) * sizeof(uint64_t))) return e;
+    uint64_t* flags = sbuf<uint64_t>(m, 0);
+    uint64_t* offsets = sbuf<uint64_t>(m, 4);
+    const unsigned grid = grid_for(P, 8);
+#define OCC(K, V) mm_occupied_kernel<K, V><<<grid, 256, 0, s>>>(m->table, P, m->empty_key, flags)
+    MM_DISPATCH(m, OCC);
+#undef OCC
+    MXCO_HIP(hipGetLastError(), "StaticMultimap: retrieve all");
+    uint64_t total = 0;
+    if (mxco_status e = exclusive_sum(m, flags, offsets, P, s, &total)) return e;
+#define EXP(K, V) \
+    mm_export_kernel<K, V><<<grid, 256, 0, s>>>(m->table, P, m->empty_key, offsets, (uint8_t*)key_out, (uint8_t*)value_out)
+    MM_DISPATCH(m, EXP);
+#undef EXP
+    MXCO_HIP(hipGetLastError(), "StaticMultimap: retrieve all");
+    MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: retrieve all");
+    *num_out = total;
+    return ok();
+}
+
+}  // extern "C"

@@ -507,3 +507,104 @@ class HyperLogLog:
         e = u64()
         check(_lib.mxco_hll_estimate(self._h, _stream(stream), ctypes.byref(e)))
         return e.value
+
+
+# ---------------------------------------------------------------------------
+# StaticMultimap
+# ---------------------------------------------------------------------------
+
+_sig("mxco_mm_create", [u32, u32, u64, vp, vp, vp, ctypes.POINTER(vp)])
+_sig("mxco_mm_destroy", [vp])
+_sig("mxco_mm_capacity", [vp, pu64])
+_sig("mxco_mm_size", [vp, vp, pu64])
+_sig("mxco_mm_clear", [vp, vp])
+_sig("mxco_mm_insert", [vp, vp, ctypes.POINTER(Stencil), u64, vp, pu64])
+_sig("mxco_mm_contains", [vp, vp, ctypes.POINTER(Stencil), vp, u64, vp])
+_sig("mxco_mm_find", [vp, vp, ctypes.POINTER(Stencil), vp, u64, vp])
+_sig("mxco_mm_count", [vp, vp, u64, vp, pu64])
+_sig("mxco_mm_retrieve", [vp, vp, u64, vp, vp, u64, vp, pu64])
+_sig("mxco_mm_retrieve_all", [vp, vp, vp, u64, vp, pu64])
+
+
+class StaticMultimap:
+    """aclco::StaticMultimap<Key, Value> (include/static_multimap.h:88)."""
+
+    def __init__(self, capacity, empty_key, empty_value, key_dtype=torch.int64, value_dtype=torch.int64,
+                 stream=None):
+        self.key_dtype, self.value_dtype = key_dtype, value_dtype
+        self._ek = _scalar_bytes(empty_key, key_dtype)
+        self._ev = _scalar_bytes(empty_value, value_dtype)
+        self.empty_key, self.empty_value = empty_key, empty_value
+        h = vp()
+        check(_lib.mxco_mm_create(torch.empty(0, dtype=key_dtype).element_size(),
+                                  torch.empty(0, dtype=value_dtype).element_size(), capacity, self._ek, self._ev,
+                                  _stream(stream), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.mxco_mm_destroy(self._h)
+            self._h = None
+
+    def capacity(self):
+        c = u64()
+        check(_lib.mxco_mm_capacity(self._h, ctypes.byref(c)))
+        return c.value
+
+    def size(self, stream=None):
+        c = u64()
+        check(_lib.mxco_mm_size(self._h, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def clear(self, stream=None):
+        check(_lib.mxco_mm_clear(self._h, _stream(stream)))
+
+    def insert(self, pairs, n=None, stream=None):
+        n = pairs.shape[0] if n is None else n
+        f = u64()
+        check(_lib.mxco_mm_insert(self._h, _ptr(pairs), None, n, _stream(stream), ctypes.byref(f)))
+        return f.value
+
+    def insert_if(self, pairs, stencil, pred, n=None, stream=None):
+        n = pairs.shape[0] if n is None else n
+        f = u64()
+        if n and stencil is None:
+            raise ValueError("StaticMultimap: null nonempty buffer")
+        check(_lib.mxco_mm_insert(self._h, _ptr(pairs), _stencil(stencil, pred), n, _stream(stream),
+                                  ctypes.byref(f)))
+        return f.value
+
+    def contains(self, keys, out=None, n=None, stream=None, stencil=None, pred="always"):
+        n = keys.shape[0] if n is None else n
+        if out is None and n:
+            out = torch.empty(n, dtype=torch.bool, device="cuda")
+        check(_lib.mxco_mm_contains(self._h, _ptr(keys), _stencil(stencil, pred), _ptr(out), n, _stream(stream)))
+        return out
+
+    def find(self, keys, out=None, n=None, stream=None, stencil=None, pred="always"):
+        n = keys.shape[0] if n is None else n
+        if out is None and n:
+            out = torch.empty(n, dtype=self.value_dtype, device="cuda")
+        check(_lib.mxco_mm_find(self._h, _ptr(keys), _stencil(stencil, pred), _ptr(out), n, _stream(stream)))
+        return out
+
+    def count(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        c = u64()
+        check(_lib.mxco_mm_count(self._h, _ptr(keys), n, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def retrieve(self, keys, probe_out, value_out, output_capacity=None, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        cap = value_out.shape[0] if output_capacity is None else output_capacity
+        c = u64()
+        check(_lib.mxco_mm_retrieve(self._h, _ptr(keys), n, _ptr(probe_out), _ptr(value_out), cap, _stream(stream),
+                                    ctypes.byref(c)))
+        return c.value
+
+    def retrieve_all(self, key_out, value_out, output_capacity=None, stream=None):
+        cap = value_out.shape[0] if output_capacity is None else output_capacity
+        c = u64()
+        check(_lib.mxco_mm_retrieve_all(self._h, _ptr(key_out), _ptr(value_out), cap, _stream(stream),
+                                        ctypes.byref(c)))
+        return c.value

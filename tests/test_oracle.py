@@ -344,3 +344,35 @@ def test_hll_estimate_linear_counting_and_bias_paths():
     alpha = 0.7213 / (1 + 1.079 / m)
     assert est == round(alpha * m * m / z)
     assert abs(z - m * 2.0 ** -20) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# StaticMultimap restatement against a dict-of-lists brute force
+# --------------------------------------------------------------------------
+
+def test_multimap_oracle_vs_brute_force():
+    from multimap import MultimapOracle
+    rng = np.random.default_rng(8)
+    cap = 300
+    mm = MultimapOracle(cap, -1, -1)
+    ref, size = {}, 0
+    for batch in range(4):
+        k = rng.integers(0, 50, 120)
+        v = rng.integers(0, 1000, 120)
+        sel = rng.integers(0, 2, 120).astype(bool)
+        failed = mm.insert(k, v, sel)
+        chosen = [(a, b) for a, b, s in zip(k.tolist(), v.tolist(), sel.tolist()) if s]
+        room = cap - size
+        for a, b in chosen[:room]:
+            ref.setdefault(a, []).append(b)
+        size += min(room, len(chosen))
+        assert failed == len(chosen) - min(room, len(chosen))
+    q = np.concatenate([np.arange(-1, 60), np.array([3, 3, 7])])
+    assert mm.contains(q).tolist() == [k in ref for k in q.tolist()]
+    assert mm.find(q).tolist() == [min(ref[k]) if k in ref else -1 for k in q.tolist()]
+    assert mm.count(q) == sum(len(ref.get(k, [])) for k in q.tolist())
+    probe, vals = mm.retrieve(q)
+    exp = [(k, x) for k in q.tolist() for x in sorted(ref.get(k, []))]
+    assert list(zip(probe.tolist(), vals.tolist())) == exp
+    with pytest.raises(ValueError):
+        mm.insert(np.array([1, -1]), np.array([2, 3]))
