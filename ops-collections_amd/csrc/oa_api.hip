@@ -32,6 +32,7 @@ struct mxco_oa {
     int device;
     int group;
     unsigned blocks_per_cu;
+    uint32_t store_mode;
 };
 
 namespace {
@@ -85,6 +86,7 @@ OaArgs base_args(const mxco_oa* t)
     a.empty_val = t->empty_val;
     a.erased_key = t->has_tomb ? t->erased_key : t->empty_key;
     a.has_tombstone = t->has_tomb;
+    a.store_mode = t->store_mode;
     return a;
 }
 
@@ -198,6 +200,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     t->has_tomb = false;
     t->group = d.group_size ? (int)d.group_size : default_group(t->ss);
     t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 16);
+    t->store_mode = (uint32_t)env_int("MXCO_OA_STORE_MODE", 0);
     (void)hipGetDevice(&t->device);
 
     hipError_t e = hipMalloc((void**)&t->slots, cap * t->ss);

@@ -54,7 +54,7 @@ struct OaArgs {
     uint32_t stencil_size;
     uint32_t stencil_pred;
     uint32_t has_tombstone;
-    uint32_t pad;
+    uint32_t store_mode;  // payload store after a key CAS (see store_payload)
 };
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
@@ -238,6 +238,21 @@ __device__ __forceinline__ void consume(const OaArgs& a, Walk& w, uint32_t c)
     }
 }
 
+// Payload store after a won key CAS on a 16-byte slot.  mode 0: plain
+// store (written back from the XCD's L2 at the end of the launch); 1: agent-
+// scope atomic store (visible to every XCD at once); 2: non-temporal store.
+template <int N>
+__device__ __forceinline__ void store_payload(uint8_t* p, uint64_t v, uint32_t mode)
+{
+    if (mode == 1) publish_bits<N>(p, v);
+    else if (mode == 2) {
+        if constexpr (N == 4) __builtin_nontemporal_store((uint32_t)v, (uint32_t*)p);
+        else __builtin_nontemporal_store(v, (uint64_t*)p);
+    } else {
+        store_bits<N>(p, v);
+    }
+}
+
 template <int KS, int VS, int G, int OP>
 __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 {
@@ -249,52 +264,78 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     constexpr bool kInsertFamily =
         OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
     constexpr int OUTS = VS ? VS : KS;  // Find / InsertAndFind output width
+    constexpr int VOFF = SS == 8 ? 4 : 8;
     constexpr uint64_t kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
 
     const uint32_t lane = __lane_id();
     const uint32_t gl = lane & (G - 1);
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
-    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const bool leader = gl == 0;
     const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
     const uint64_t ekey = a.empty_key;
+    const uint64_t miss_out = VS ? a.empty_val : ekey;
 
     unsigned long long cnt0 = 0, cnt1 = 0;
 
-    for (uint64_t i = gid; i < a.n; i += ngroups) {
-        const bool leader = gl == 0;
-        if (!stencil_ok(a, i)) {
+    // Per-group state: the key being probed and its walk.  The loop below
+    // advances every group by one window per iteration; a group whose key
+    // is resolved takes its next key at once instead of waiting for the
+    // slowest walk of the wavefront.
+    uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    uint64_t key = 0, pval = 0;
+    uint64_t prev_line = ~0ull;  // OP_PROBE_STATS: last line counted for this key
+    Walk w;
+    w.done = true;
+
+    // Sets up key `idx`.  Returns false when its outcome needs no walk
+    // (false predicate, reserved key) -- the outcome is written here.
+    auto begin = [&](uint64_t idx) -> bool {
+        if (!stencil_ok(a, idx)) {
             if (leader) {
-                if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
-                if constexpr (OP == OP_CONTAINS) a.out[i] = 0;
+                if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + idx * OUTS, miss_out);
+                if constexpr (OP == OP_CONTAINS) a.out[idx] = 0;
             }
-            continue;
-        }
-        uint64_t key, pval = 0;
-        if constexpr (kInsertFamily) {
-            key = load_bits<KS>(a.in + i * SS);
-            if constexpr (VS) pval = load_bits<VS>(a.in + i * SS + (SS == 8 ? 4 : 8));
-        } else {
-            key = load_bits<KS>(a.in + i * KS);
+            return false;
         }
         if constexpr (kInsertFamily) {
+            key = load_bits<KS>(a.in + idx * SS);
+            if constexpr (VS) pval = load_bits<VS>(a.in + idx * SS + VOFF);
             // The reserved empty key is never inserted (open_addressing_ref_impl.h:494-496).
             if (key == ekey) {
                 if (leader) {
                     if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN) cnt0++;
                     if constexpr (OP == OP_INSERT_AND_FIND) {
-                        store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
-                        a.out2[i] = 0;
+                        store_bits<OUTS>(a.out + idx * OUTS, miss_out);
+                        a.out2[idx] = 0;
                     }
                 }
-                continue;
+                return false;
             }
+        } else {
+            key = load_bits<KS>(a.in + idx * KS);
         }
+        w = make_walk<KS>(a, key);
+        prev_line = ~0ull;
+        return true;
+    };
 
-        Walk w = make_walk<KS>(a, key);
-        // Outcome: 0 = walk exhausted, 1 = resolved.
-        bool resolved = false;
-        uint64_t prev_line = ~0ull;  // OP_PROBE_STATS: last line counted for this key
-        while (!w.done) {
+    while (i < a.n && !begin(i)) i += ngroups;
+
+    while (i < a.n) {
+        bool key_done = false;
+        if (w.done) {
+            // Walk exhausted: full table / miss after a full cycle.
+            if (leader) {
+                if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + i * OUTS, miss_out);
+                if constexpr (OP == OP_CONTAINS) a.out[i] = 0;
+                if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ERASE) cnt0++;
+                if constexpr (OP == OP_INSERT_AND_FIND) {
+                    store_bits<OUTS>(a.out + i * OUTS, miss_out);
+                    a.out2[i] = 0;
+                }
+            }
+            key_done = true;
+        } else {
             const uint32_t wlen = window_len<SPL>(a, w);
             Raw s[K];
 #pragma unroll
@@ -332,10 +373,9 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const uint64_t m = (__ballot(fk == k) >> gbase) & kGroupMask;
-                    if (m) {
+                    if (m && wround == K) {
                         wround = k;
                         wlane = __builtin_ctzll(m);
-                        break;
                     }
                 }
             }
@@ -354,169 +394,147 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
             }
             if (wround == K) {
                 consume(a, w, wlen);
-                continue;
-            }
-            const bool winner = (gl == wlane);
-            uint8_t* wslot = a.table + (uint64_t)(w.pos + woff) * SS;
-            bool weq;
-            if constexpr (G == 1) weq = feq;
-            else weq = ((__ballot(winner && feq) >> gbase) & kGroupMask) != 0;
-
-            if constexpr (OP == OP_FIND || OP == OP_CONTAINS || OP == OP_COUNT || OP == OP_PROBE_STATS) {
-                if (winner) {
-                    if constexpr (OP == OP_FIND) {
-                        store_bits<OUTS>(a.out + i * OUTS, weq ? L::val(ws) : (VS ? a.empty_val : ekey));
-                    } else if constexpr (OP == OP_CONTAINS) {
-                        a.out[i] = weq ? 1 : 0;
-                    } else if constexpr (OP == OP_COUNT) {
-                        cnt0 += weq ? 1 : 0;
-                    }
-                }
-                resolved = true;
-                break;
-            } else if constexpr (OP == OP_ASSIGN) {
-                // Assign (open_addressing_ref_impl.h:384-410): stop at empty, write payload on match.
-                if (winner && weq) store_bits<VS>(wslot + (SS == 8 ? 4 : 8), pval);
-                resolved = true;
-                break;
-            } else if constexpr (OP == OP_ERASE) {
-                if (!weq) {  // empty slot with a tombstone configured: not present
-                    resolved = false;
-                    w.done = true;
-                    break;
-                }
-                // AttemptInsertStable(slot, content, erasedSlot) (ref_impl.h:359).
-                int r = 0;  // 0 failed (continue), 1 erased, 2 already erased
-                if (winner) {
-                    if constexpr (SS <= 8) {
-                        uint64_t expect, desired;
-                        if constexpr (VS) {
-                            expect = (uint64_t)ws;
-                            desired = (a.erased_key & 0xffffffffull) | (a.empty_val << 32);
-                        } else {
-                            expect = (uint64_t)ws;
-                            desired = a.erased_key;
-                        }
-                        const uint64_t old = cas_bits<SS>(wslot, expect, desired);
-                        if (old == expect) r = 1;
-                        else if (L::key((Raw)old) == a.erased_key) r = 2;
-                    } else {
-                        const uint64_t old = cas_bits<KS>(wslot, L::key(ws), a.erased_key);
-                        if (old == L::key(ws)) {
-                            store_bits<VS>(wslot + 8, a.empty_val);
-                            r = 1;
-                        } else if (old == a.erased_key) {
-                            r = 2;
-                        }
-                    }
-                }
-                uint32_t rr;
-                if constexpr (G == 1) rr = r;
-                else rr = (((__ballot(winner && r == 1) >> gbase) & kGroupMask) ? 1u : 0u) |
-                          (((__ballot(winner && r == 2) >> gbase) & kGroupMask) ? 2u : 0u);
-                if (rr == 1) {
-                    resolved = true;
-                    break;
-                }
-                if (rr == 2) {
-                    resolved = false;
-                    w.done = true;
-                    break;
-                }
-                consume(a, w, woff + 1);
-                continue;
             } else {
-                // Insert family: EQUAL -> duplicate; AVAILABLE -> CAS attempt
-                // (InsertVerdict, open_addressing_ref_impl.h:490-532).
-                int r = 0;  // 0 lost to another key, 1 inserted, 2 duplicate
-                uint64_t seen = 0;  // stored key (set) / payload (8-byte map slot) on a duplicate
-                if (weq) {
-                    r = 2;
-                    seen = VS ? L::val(ws) : L::key(ws);
-                    if constexpr (OP == OP_INSERT_OR_ASSIGN) {
-                        // AssignPayload (ref_impl.h:210-220)
-                        if (winner) store_bits<VS>(wslot + (SS == 8 ? 4 : 8), pval);
+                const bool winner = (gl == wlane);
+                uint8_t* wslot = a.table + (uint64_t)(w.pos + woff) * SS;
+                bool weq;
+                if constexpr (G == 1) weq = feq;
+                else weq = ((__ballot(winner && feq) >> gbase) & kGroupMask) != 0;
+
+                if constexpr (OP == OP_FIND || OP == OP_CONTAINS || OP == OP_COUNT || OP == OP_PROBE_STATS) {
+                    if (winner) {
+                        if constexpr (OP == OP_FIND) {
+                            store_bits<OUTS>(a.out + i * OUTS, weq ? L::val(ws) : miss_out);
+                        } else if constexpr (OP == OP_CONTAINS) {
+                            a.out[i] = weq ? 1 : 0;
+                        } else if constexpr (OP == OP_COUNT) {
+                            cnt0 += weq ? 1 : 0;
+                        }
+                    }
+                    key_done = true;
+                } else if constexpr (OP == OP_ASSIGN) {
+                    // Assign (open_addressing_ref_impl.h:384-410): stop at empty, write payload on match.
+                    if (winner && weq) store_bits<VS>(wslot + VOFF, pval);
+                    key_done = true;
+                } else if constexpr (OP == OP_ERASE) {
+                    if (!weq) {
+                        // empty slot with a tombstone configured: not present
+                        w.done = true;
+                    } else {
+                        // AttemptInsertStable(slot, content, erasedSlot) (ref_impl.h:359).
+                        int r = 0;  // 0 failed (continue), 1 erased, 2 already erased
+                        if (winner) {
+                            if constexpr (SS <= 8) {
+                                const uint64_t expect = (uint64_t)ws;
+                                uint64_t desired;
+                                if constexpr (VS) desired = (a.erased_key & 0xffffffffull) | (a.empty_val << 32);
+                                else desired = a.erased_key;
+                                const uint64_t old = cas_bits<SS>(wslot, expect, desired);
+                                if (old == expect) r = 1;
+                                else if (L::key((Raw)old) == a.erased_key) r = 2;
+                            } else {
+                                const uint64_t old = cas_bits<KS>(wslot, L::key(ws), a.erased_key);
+                                if (old == L::key(ws)) {
+                                    store_bits<VS>(wslot + 8, a.empty_val);
+                                    r = 1;
+                                } else if (old == a.erased_key) {
+                                    r = 2;
+                                }
+                            }
+                        }
+                        if constexpr (G > 1)
+                            r = ((__ballot(winner && r == 1) >> gbase) & kGroupMask) ? 1
+                                : (((__ballot(winner && r == 2) >> gbase) & kGroupMask) ? 2 : 0);
+                        if (r == 1) key_done = true;
+                        else if (r == 2) w.done = true;  // counted as not erased next iteration
+                        else consume(a, w, woff + 1);
                     }
                 } else {
-                    if (winner) {
-                        if constexpr (SS <= 8) {
-                            // PackCas / PackCasInsertOrAssign: one CAS of the whole slot
-                            // against (emptyKey, emptyValue) (ref_impl.h:127-147, 234-257).
-                            uint64_t expect, desired;
-                            if constexpr (VS) {
-                                expect = (ekey & 0xffffffffull) | (a.empty_val << 32);
-                                desired = (key & 0xffffffffull) | (pval << 32);
+                    // Insert family: EQUAL -> duplicate; AVAILABLE -> CAS attempt
+                    // (InsertVerdict, open_addressing_ref_impl.h:490-532).
+                    int r = 0;          // 0 lost to another key, 1 inserted, 2 duplicate
+                    uint64_t seen = 0;  // stored key (set) / payload (8-byte map slot) on a duplicate
+                    if (weq) {
+                        r = 2;
+                        seen = VS ? L::val(ws) : L::key(ws);
+                        if constexpr (OP == OP_INSERT_OR_ASSIGN) {
+                            // AssignPayload (ref_impl.h:210-220)
+                            if (winner) store_bits<VS>(wslot + VOFF, pval);
+                        }
+                    } else {
+                        if (winner) {
+                            if constexpr (SS <= 8) {
+                                // PackCas / PackCasInsertOrAssign: one CAS of the whole slot
+                                // against (emptyKey, emptyValue) (ref_impl.h:127-147, 234-257).
+                                uint64_t expect, desired;
+                                if constexpr (VS) {
+                                    expect = (ekey & 0xffffffffull) | (a.empty_val << 32);
+                                    desired = (key & 0xffffffffull) | (pval << 32);
+                                } else {
+                                    expect = ekey;
+                                    desired = key;
+                                }
+                                const uint64_t old = cas_bits<SS>(wslot, expect, desired);
+                                if (old == expect) {
+                                    r = 1;
+                                } else if (key_eq<KS>(L::key((Raw)old), key, a.key_float)) {
+                                    r = 2;
+                                    seen = VS ? L::val((Raw)old) : L::key((Raw)old);
+                                    if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<SS>(wslot, desired);
+                                }
                             } else {
-                                expect = ekey;
-                                desired = key;
-                            }
-                            const uint64_t old = cas_bits<SS>(wslot, expect, desired);
-                            if (old == expect) {
-                                r = 1;
-                            } else if (key_eq<KS>(L::key((Raw)old), key, a.key_float)) {
-                                r = 2;
-                                seen = VS ? L::val((Raw)old) : L::key((Raw)old);
-                                if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<SS>(wslot, desired);
-                            }
-                        } else {
-                            // CasDependentWrite / BackToBackCasInsertOrAssign: CAS the key,
-                            // then store the payload (ref_impl.h:165-185, 260-279).
-                            const uint64_t old = cas_bits<KS>(wslot, ekey, key);
-                            if (old == ekey) {
-                                if constexpr (OP == OP_INSERT_AND_FIND) publish_bits<VS>(wslot + 8, pval);
-                                else store_bits<VS>(wslot + 8, pval);
-                                r = 1;
-                            } else if (key_eq<KS>(old, key, a.key_float)) {
-                                r = 2;
-                                if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<VS>(wslot + 8, pval);
+                                // CasDependentWrite / BackToBackCasInsertOrAssign: CAS the key,
+                                // then store the payload (ref_impl.h:165-185, 260-279).
+                                const uint64_t old = cas_bits<KS>(wslot, ekey, key);
+                                if (old == ekey) {
+                                    if constexpr (OP == OP_INSERT_AND_FIND) publish_bits<VS>(wslot + 8, pval);
+                                    else store_payload<VS>(wslot + 8, pval, a.store_mode);
+                                    r = 1;
+                                } else if (key_eq<KS>(old, key, a.key_float)) {
+                                    r = 2;
+                                    if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<VS>(wslot + 8, pval);
+                                }
                             }
                         }
-                    }
-                    if constexpr (G > 1) {
-                        const uint64_t m1 = (__ballot(winner && r == 1) >> gbase) & kGroupMask;
-                        const uint64_t m2 = (__ballot(winner && r == 2) >> gbase) & kGroupMask;
-                        r = m1 ? 1 : (m2 ? 2 : 0);
-                    }
-                }
-                if (r == 0) {
-                    consume(a, w, woff + 1);
-                    continue;
-                }
-                if (leader) {
-                    if constexpr (OP == OP_INSERT) cnt0 += (r == 2);
-                    if constexpr (OP == OP_INSERT_OR_ASSIGN) cnt1 += (r == 2);
-                }
-                if constexpr (OP == OP_INSERT_AND_FIND) {
-                    if (winner) {
-                        uint64_t v;
-                        if (r == 1) {
-                            v = VS ? pval : key;
-                        } else if constexpr (VS && SS == 16) {
-                            // WaitForPayload (ref_impl.h:474-486): a racing inserter may
-                            // have stored the key but not yet the payload.  Bounded here.
-                            v = atomic_load_bits<VS>(wslot + 8);
-                            for (int spin = 0; spin < (1 << 22) && v == a.empty_val; ++spin)
-                                v = atomic_load_bits<VS>(wslot + 8);
-                        } else {
-                            v = seen;
+                        if constexpr (G > 1) {
+                            const uint64_t m1 = (__ballot(winner && r == 1) >> gbase) & kGroupMask;
+                            const uint64_t m2 = (__ballot(winner && r == 2) >> gbase) & kGroupMask;
+                            r = m1 ? 1 : (m2 ? 2 : 0);
                         }
-                        store_bits<OUTS>(a.out + i * OUTS, v);
-                        a.out2[i] = (r == 1) ? 1 : 0;
+                    }
+                    if (r == 0) {
+                        consume(a, w, woff + 1);
+                    } else {
+                        if (leader) {
+                            if constexpr (OP == OP_INSERT) cnt0 += (r == 2);
+                            if constexpr (OP == OP_INSERT_OR_ASSIGN) cnt1 += (r == 2);
+                        }
+                        if constexpr (OP == OP_INSERT_AND_FIND) {
+                            if (winner) {
+                                uint64_t v;
+                                if (r == 1) {
+                                    v = VS ? pval : key;
+                                } else if constexpr (VS && SS == 16) {
+                                    // WaitForPayload (ref_impl.h:474-486): a racing inserter may
+                                    // have stored the key but not yet the payload.  Bounded here.
+                                    v = atomic_load_bits<VS>(wslot + 8);
+                                    for (int spin = 0; spin < (1 << 22) && v == a.empty_val; ++spin)
+                                        v = atomic_load_bits<VS>(wslot + 8);
+                                } else {
+                                    v = seen;
+                                }
+                                store_bits<OUTS>(a.out + i * OUTS, v);
+                                a.out2[i] = (r == 1) ? 1 : 0;
+                            }
+                        }
+                        key_done = true;
                     }
                 }
-                resolved = true;
-                break;
             }
         }
-        if (!resolved && leader) {
-            // Walk exhausted (full table / miss after a full cycle).
-            if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
-            if constexpr (OP == OP_CONTAINS) a.out[i] = 0;
-            if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ERASE) cnt0++;
-            if constexpr (OP == OP_INSERT_AND_FIND) {
-                store_bits<OUTS>(a.out + i * OUTS, VS ? a.empty_val : ekey);
-                a.out2[i] = 0;
-            }
+        if (key_done) {
+            i += ngroups;
+            while (i < a.n && !begin(i)) i += ngroups;
         }
     }
 

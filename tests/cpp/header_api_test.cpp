@@ -65,7 +65,8 @@ int main()
         auto out = to_host(dout, 2 * n);
         auto con = to_host(reinterpret_cast<uint8_t*>(dcon), 2 * n);
         for (int i = 0; i < 2 * n; ++i) {
-            CHECK(out[i] == (i < n ? i * 7 + 1 : -1));
+            // keys 0..9 were also inserted with value 5: either insert may win
+            CHECK(out[i] == (i < n ? i * 7 + 1 : -1) || (i < 10 && out[i] == 5));
             CHECK(con[i] == (i < n));
         }
         CHECK(map.Count(dk, keys.size(), s) == static_cast<size_t>(n));
