@@ -33,6 +33,7 @@ struct mxco_oa {
     int group;
     unsigned blocks_per_cu;
     uint32_t store_mode;
+    int inflight;
 };
 
 namespace {
@@ -118,7 +119,7 @@ mxco_status set_stencil(OaArgs& a, const mxco_stencil* st, const char* fn)
 mxco_status launch(const mxco_oa* t, int op, OaArgs& a, hipStream_t stream, const char* fn)
 {
     const unsigned grid = grid_for(a.n * (uint64_t)t->group, t->blocks_per_cu);
-    hipError_t e = launch_oa(t->d.key_size, t->d.value_size, t->group, op, a, dim3(grid), stream);
+    hipError_t e = launch_oa(t->d.key_size, t->d.value_size, t->group, op, a, dim3(grid), stream, t->inflight);
     if (e != hipSuccess) return hip_fail(e, fn);
     return MXCO_OK;
 }
@@ -201,6 +202,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     t->group = d.group_size ? (int)d.group_size : default_group(t->ss);
     t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 16);
     t->store_mode = (uint32_t)env_int("MXCO_OA_STORE_MODE", 0);
+    t->inflight = env_int("MXCO_OA_INFLIGHT", 1) == 2 ? 2 : 1;
     (void)hipGetDevice(&t->device);
 
     hipError_t e = hipMalloc((void**)&t->slots, cap * t->ss);

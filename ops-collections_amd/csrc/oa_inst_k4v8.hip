@@ -2,8 +2,8 @@
 #include "oa_launch.h"
 
 namespace mxco {
-hipError_t launch_oa_k4v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s)
+hipError_t launch_oa_k4v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight)
 {
-    return launch_oa_layout<4, 8>(g, op, a, grid, s);
+    return launch_oa_layout<4, 8>(g, op, a, grid, s, inflight);
 }
 }  // namespace mxco

@@ -253,7 +253,7 @@ __device__ __forceinline__ void store_payload(uint8_t* p, uint64_t v, uint32_t m
     }
 }
 
-template <int KS, int VS, int G, int OP>
+template <int KS, int VS, int G, int OP, int U = 1>
 __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 {
     using L = Lay<KS, VS>;
@@ -281,15 +281,21 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     // advances every group by one window per iteration; a group whose key
     // is resolved takes its next key at once instead of waiting for the
     // slowest walk of the wavefront.
-    uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    uint64_t key = 0, pval = 0;
-    uint64_t prev_line = ~0ull;  // OP_PROBE_STATS: last line counted for this key
-    Walk w;
-    w.done = true;
+    //
+    // Each group keeps U keys in flight: slot u walks keys
+    // gid + u*ngroups, gid + (u+U)*ngroups, ...  The window loads of all U
+    // walks are issued before any of them is examined.
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    uint64_t I[U], KEY[U], PVAL[U], PREV[U];
+    Walk W[U];
+    uint32_t WLEN[U];
+    Raw S[U][K];
 
     // Sets up key `idx`.  Returns false when its outcome needs no walk
     // (false predicate, reserved key) -- the outcome is written here.
-    auto begin = [&](uint64_t idx) -> bool {
+    auto begin = [&](int u, uint64_t idx) -> bool {
+        uint64_t& key = KEY[u];
+        uint64_t& pval = PVAL[u];
         if (!stencil_ok(a, idx)) {
             if (leader) {
                 if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + idx * OUTS, miss_out);
@@ -314,14 +320,42 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
         } else {
             key = load_bits<KS>(a.in + idx * KS);
         }
-        w = make_walk<KS>(a, key);
-        prev_line = ~0ull;
+        W[u] = make_walk<KS>(a, key);
+        PREV[u] = ~0ull;
         return true;
     };
+    auto advance = [&](int u, uint64_t from) {
+        uint64_t idx = from;
+        while (idx < a.n && !begin(u, idx)) idx += (uint64_t)U * ngroups;
+        I[u] = idx;
+    };
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        W[u].done = true;
+        advance(u, gid + (uint64_t)u * ngroups);
+    }
 
-    while (i < a.n && !begin(i)) i += ngroups;
+    // Window loads of slot u (nothing when its key is exhausted or finished).
+    auto load = [&](int u) {
+        if (I[u] >= a.n || W[u].done) return;
+        const uint32_t wlen = window_len<SPL>(a, W[u]);
+        WLEN[u] = wlen;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t off = gl + k * G;
+            if (off < wlen) S[u][k] = *(const Raw*)(a.table + (uint64_t)(W[u].pos + off) * SS);
+        }
+    };
 
-    while (i < a.n) {
+    // One step of slot u's walk on the loaded window.
+    auto step = [&](int u) {
+        if (I[u] >= a.n) return;
+        const uint64_t i = I[u];
+        const uint64_t key = KEY[u];
+        const uint64_t pval = PVAL[u];
+        Walk& w = W[u];
+        uint64_t& prev_line = PREV[u];
+        Raw (&s)[K] = S[u];
         bool key_done = false;
         if (w.done) {
             // Walk exhausted: full table / miss after a full cycle.
@@ -336,13 +370,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
             }
             key_done = true;
         } else {
-            const uint32_t wlen = window_len<SPL>(a, w);
-            Raw s[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t off = gl + k * G;
-                if (off < wlen) s[k] = *(const Raw*)(a.table + (uint64_t)(w.pos + off) * SS);
-            }
+            const uint32_t wlen = WLEN[u];
             // First slot (in walk order) of this lane that ends the walk.
             int fk = K;
             bool feq = false;
@@ -532,10 +560,18 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
                 }
             }
         }
-        if (key_done) {
-            i += ngroups;
-            while (i < a.n && !begin(i)) i += ngroups;
-        }
+        if (key_done) advance(u, i + (uint64_t)U * ngroups);
+    };
+
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) any |= I[u] < a.n;
+        if (!any) break;
+#pragma unroll
+        for (int u = 0; u < U; ++u) load(u);
+#pragma unroll
+        for (int u = 0; u < U; ++u) step(u);
     }
 
     constexpr bool kTwoCounters = OP == OP_INSERT_OR_ASSIGN || OP == OP_PROBE_STATS;

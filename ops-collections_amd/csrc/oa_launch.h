@@ -9,9 +9,18 @@
 namespace mxco {
 
 template <int KS, int VS, int G>
-hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s)
+hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight)
 {
     const dim3 block(256);
+    if (inflight == 2) {
+        // two keys in flight per group: the bulk Find / Contains / Insert paths
+        switch (op) {
+        case OP_FIND: oa_kernel<KS, VS, G, OP_FIND, 2><<<grid, block, 0, s>>>(a); return hipGetLastError();
+        case OP_CONTAINS: oa_kernel<KS, VS, G, OP_CONTAINS, 2><<<grid, block, 0, s>>>(a); return hipGetLastError();
+        case OP_INSERT: oa_kernel<KS, VS, G, OP_INSERT, 2><<<grid, block, 0, s>>>(a); return hipGetLastError();
+        default: break;
+        }
+    }
     switch (op) {
     case OP_FIND: oa_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a); break;
     case OP_CONTAINS: oa_kernel<KS, VS, G, OP_CONTAINS><<<grid, block, 0, s>>>(a); break;
@@ -34,32 +43,33 @@ hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s)
 }
 
 template <int KS, int VS>
-hipError_t launch_oa_layout(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s)
+hipError_t launch_oa_layout(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight)
 {
     switch (g) {
-    case 1: return launch_oa_g<KS, VS, 1>(op, a, grid, s);
-    case 2: return launch_oa_g<KS, VS, 2>(op, a, grid, s);
-    case 4: return launch_oa_g<KS, VS, 4>(op, a, grid, s);
-    case 8: return launch_oa_g<KS, VS, 8>(op, a, grid, s);
+    case 1: return launch_oa_g<KS, VS, 1>(op, a, grid, s, inflight);
+    case 2: return launch_oa_g<KS, VS, 2>(op, a, grid, s, inflight);
+    case 4: return launch_oa_g<KS, VS, 4>(op, a, grid, s, inflight);
+    case 8: return launch_oa_g<KS, VS, 8>(op, a, grid, s, inflight);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_oa_k4v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_oa_k8v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_oa_k4v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_oa_k8v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_oa_k4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_oa_k8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_oa_k4v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+hipError_t launch_oa_k8v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+hipError_t launch_oa_k4v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+hipError_t launch_oa_k8v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+hipError_t launch_oa_k4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+hipError_t launch_oa_k8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
 
-inline hipError_t launch_oa(int ks, int vs, int g, int op, const OaArgs& a, dim3 grid, hipStream_t s)
+inline hipError_t launch_oa(int ks, int vs, int g, int op, const OaArgs& a, dim3 grid, hipStream_t s,
+                            int inflight = 1)
 {
-    if (ks == 4 && vs == 4) return launch_oa_k4v4(g, op, a, grid, s);
-    if (ks == 8 && vs == 8) return launch_oa_k8v8(g, op, a, grid, s);
-    if (ks == 4 && vs == 8) return launch_oa_k4v8(g, op, a, grid, s);
-    if (ks == 8 && vs == 4) return launch_oa_k8v4(g, op, a, grid, s);
-    if (ks == 4 && vs == 0) return launch_oa_k4(g, op, a, grid, s);
-    if (ks == 8 && vs == 0) return launch_oa_k8(g, op, a, grid, s);
+    if (ks == 4 && vs == 4) return launch_oa_k4v4(g, op, a, grid, s, inflight);
+    if (ks == 8 && vs == 8) return launch_oa_k8v8(g, op, a, grid, s, inflight);
+    if (ks == 4 && vs == 8) return launch_oa_k4v8(g, op, a, grid, s, inflight);
+    if (ks == 8 && vs == 4) return launch_oa_k8v4(g, op, a, grid, s, inflight);
+    if (ks == 4 && vs == 0) return launch_oa_k4(g, op, a, grid, s, inflight);
+    if (ks == 8 && vs == 0) return launch_oa_k8(g, op, a, grid, s, inflight);
     return hipErrorInvalidValue;
 }
 
