@@ -29,6 +29,7 @@ def parse():
     ap.add_argument("--group", type=int, default=0, help="lanes per key (0 = library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent grid blocks per CU (0 = default)")
     ap.add_argument("--inflight", type=int, default=0, help="keys in flight per lane group (1 or 2; 0 = default)")
+    ap.add_argument("--probe-flags", type=int, default=-1, help="bit0 sector-sized first window, bit1 nt loads")
     ap.add_argument("--store-mode", type=int, default=-1, help="payload store after a key CAS (0 plain, 1 atomic, 2 nt)")
     ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -166,6 +167,8 @@ def main():
         os.environ["MXCO_OA_BLOCKS_PER_CU"] = str(args.blocks_per_cu)
     if args.inflight:
         os.environ["MXCO_OA_INFLIGHT"] = str(args.inflight)
+    if args.probe_flags >= 0:
+        os.environ["MXCO_OA_PROBE_FLAGS"] = str(args.probe_flags)
     if args.store_mode >= 0:
         os.environ["MXCO_OA_STORE_MODE"] = str(args.store_mode)
     import torch

@@ -34,6 +34,7 @@ struct mxco_oa {
     unsigned blocks_per_cu;
     uint32_t store_mode;
     int inflight;
+    uint32_t probe_flags;
 };
 
 namespace {
@@ -88,6 +89,7 @@ OaArgs base_args(const mxco_oa* t)
     a.erased_key = t->has_tomb ? t->erased_key : t->empty_key;
     a.has_tombstone = t->has_tomb;
     a.store_mode = t->store_mode;
+    a.probe_flags = t->probe_flags;
     return a;
 }
 
@@ -203,6 +205,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 16);
     t->store_mode = (uint32_t)env_int("MXCO_OA_STORE_MODE", 0);
     t->inflight = env_int("MXCO_OA_INFLIGHT", 1) == 2 ? 2 : 1;
+    t->probe_flags = (uint32_t)env_int("MXCO_OA_PROBE_FLAGS", 0);
     (void)hipGetDevice(&t->device);
 
     hipError_t e = hipMalloc((void**)&t->slots, cap * t->ss);

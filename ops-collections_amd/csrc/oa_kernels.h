@@ -55,6 +55,8 @@ struct OaArgs {
     uint32_t stencil_pred;
     uint32_t has_tombstone;
     uint32_t store_mode;  // payload store after a key CAS (see store_payload)
+    uint32_t probe_flags;  // bit 0: first linear window ends at a 64-byte sector; bit 1: non-temporal probe loads
+    uint32_t pad;
 };
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
@@ -212,7 +214,10 @@ template <int MAXW>
 __device__ __forceinline__ uint32_t window_len(const OaArgs& a, const Walk& w)
 {
     if (a.probing == 0) {
-        uint32_t l = MAXW - (w.pos & (MAXW - 1));
+        // Optionally stop the first window at the end of its 64-byte sector:
+        // most hits end there, and the sector is the unit HBM delivers.
+        const uint32_t span = ((a.probe_flags & 1u) && w.left == a.cap && MAXW >= 2) ? MAXW / 2 : MAXW;
+        uint32_t l = span - (w.pos & (span - 1));
         l = min(l, a.cap - w.pos);
         return min(l, w.left);
     }
@@ -343,7 +348,10 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t off = gl + k * G;
-            if (off < wlen) S[u][k] = *(const Raw*)(a.table + (uint64_t)(W[u].pos + off) * SS);
+            if (off < wlen) {
+                const Raw* src = (const Raw*)(a.table + (uint64_t)(W[u].pos + off) * SS);
+                S[u][k] = (a.probe_flags & 2u) ? __builtin_nontemporal_load(src) : *src;
+            }
         }
     };
 
