@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--store-mode", type=int, default=-1, help="payload store after a key CAS (0 plain, 1 atomic, 2 nt)")
     ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
@@ -121,6 +122,74 @@ def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
     del m
     torch.cuda.empty_cache()
     return res
+
+
+def run_batched(args, torch, mxco, steps=3, warmup=1):
+    """1024 StaticMap int32 tables, capacities log-uniform 2^10..2^22, LF
+    U[0.5,0.85], finds log-uniform 2^6..2^20 at 50 % hit: one graph replay
+    (Clear + one persistent Insert launch + one persistent Find launch)
+    against 1024 plain per-table Clear/Insert/Find calls."""
+    from mxco import workloads
+    w = workloads.batched_tables(1024, (10, 22), (0.5, 0.85), (6, 20), seed=4,
+                                 valid_extent=lambda c: mxco.valid_extent(c, 5, "linear"))
+    T = len(w["caps"])
+    pairs, q = w["pairs"], w["queries"]
+    b = mxco.BatchedStaticMap(w["caps"], -1, -1)
+    ins = b.plan(mxco.BATCH_INSERT, [(t, w["ins_off"][t], w["n_ins"][t]) for t in range(T)])
+    fnd = b.plan(mxco.BATCH_FIND, [(t, w["find_off"][t], w["n_find"][t]) for t in range(T)])
+    out_b = torch.empty_like(q)
+    tuned = {"insert": ins.tune(pairs), "find": fnd.tune(q, out_b)}
+
+    def record(s):
+        b.clear(s)
+        ins.run(pairs, stream=s)
+        fnd.run(q, out_b, stream=s)
+
+    g = mxco.Graph(record)
+    for _ in range(warmup):
+        g.launch()
+    g.stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+    g.stream.synchronize()
+    t_batched = (time.perf_counter() - t0) / steps
+    ok_b = bool(torch.equal(out_b, w["expected"])) and ins.failures(g.stream) == [0] * T
+    del g
+    # plain: one StaticMap per table, the reference's synchronous calls
+    maps = [mxco.StaticMap(w["caps"][t], -1, -1) for t in range(T)]
+    out_p = torch.empty_like(q)
+    views = [(pairs[w["ins_off"][t]:w["ins_off"][t] + w["n_ins"][t]],
+              q[w["find_off"][t]:w["find_off"][t] + w["n_find"][t]],
+              out_p[w["find_off"][t]:w["find_off"][t] + w["n_find"][t]]) for t in range(T)]
+
+    def plain_step():
+        failed = 0
+        for m, (pv, qv, ov) in zip(maps, views):
+            m.clear_async()
+            failed += m.insert(pv)
+            m.find(qv, out=ov)
+        return failed
+
+    for _ in range(warmup):
+        plain_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    failed = 0
+    for _ in range(steps):
+        failed += plain_step()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / steps
+    ok_p = failed == 0 and bool(torch.equal(out_p, w["expected"]))
+    keys = sum(w["n_ins"]) + sum(w["n_find"])
+    del maps, b
+    torch.cuda.empty_cache()
+    return {"tables": T, "slots": sum(w["caps"]), "inserts": sum(w["n_ins"]), "finds": sum(w["n_find"]),
+            "batched_ms": round(t_batched * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+            "batched_gkeys_s": round(keys / t_batched / 1e9, 4), "plain_gkeys_s": round(keys / t_plain / 1e9, 4),
+            "speedup": round(t_plain / t_batched, 3), "identical_results": ok_b and ok_p,
+            "tuned_config": {"insert": ins.config(), "find": fnd.config()},
+            "tune_ms": {k: round(v, 3) for k, v in tuned.items()}}
 
 
 def cpu_baseline(args):
@@ -231,6 +300,10 @@ def main():
                            "find_gkeys_s": round(x["n_find"] / x["find_ms"] / 1e6, 3)})
                 log("sweep %s" % json.dumps(sw[-1]))
         line["sweep"] = sw
+    if not args.no_batched:
+        log("bench: batched 1024 tables")
+        line["batched"] = run_batched(args, torch, mxco)
+        log("bench: batched %s" % json.dumps(line["batched"]))
     if not args.no_cpu:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(args)

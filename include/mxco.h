@@ -186,6 +186,54 @@ MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint6
                                          uint64_t* lines, uint64_t* slots);
 
 /* ---------------------------------------------------------------------- */
+/* Batched tables (no single reference counterpart: the batched form of   */
+/* StaticMap::Insert / Find / Contains over many tables, each with the    */
+/* reference's per-table results).                                        */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_batch mxco_batch;
+typedef struct mxco_batch_plan mxco_batch_plan;
+
+/* One work item: `count` inputs of table `table`, starting at element
+ * `offset` of the plan's input (and output) arrays. */
+typedef struct mxco_batch_item {
+    uint32_t table;
+    uint32_t reserved;
+    uint64_t offset;
+    uint64_t count;
+} mxco_batch_item;
+
+enum { MXCO_BATCH_INSERT = 0, MXCO_BATCH_FIND = 1, MXCO_BATCH_CONTAINS = 2 };
+
+/* num_tables tables of one descriptor, table t with requested capacity
+ * capacities[t] (rounded as mxco_valid_extent), all in one slot arena. */
+MXCO_API mxco_status mxco_batch_create(const mxco_oa_desc* desc, uint32_t num_tables, const uint64_t* capacities,
+                                       const void* empty_key, const void* empty_value, hipStream_t stream,
+                                       mxco_batch** out);
+MXCO_API mxco_status mxco_batch_destroy(mxco_batch* batch);
+/* Capacity and slot array of one table (the layout of StaticMap::Data()). */
+MXCO_API mxco_status mxco_batch_table(const mxco_batch* batch, uint32_t table, uint64_t* capacity, void** slots);
+/* Clears every table with one launch (capture-safe). */
+MXCO_API mxco_status mxco_batch_clear(mxco_batch* batch, hipStream_t stream);
+/* A device work list for one operation over host items; group_size 0 = default. */
+MXCO_API mxco_status mxco_batch_plan_create(mxco_batch* batch, int op, const mxco_batch_item* items,
+                                            uint32_t num_items, uint32_t group_size, mxco_batch_plan** out);
+MXCO_API mxco_status mxco_batch_plan_destroy(mxco_batch_plan* plan);
+/* One persistent-kernel launch of the plan (asynchronous, capture-safe).
+ * Insert reads slot-layout pairs; Find writes payloads (or keys for sets);
+ * Contains writes 1-byte flags. */
+MXCO_API mxco_status mxco_batch_run(mxco_batch_plan* plan, const void* in, void* out, uint8_t* out2,
+                                    hipStream_t stream);
+/* Per-item failure counts of the last insert run (synchronous). */
+MXCO_API mxco_status mxco_batch_failures(mxco_batch_plan* plan, hipStream_t stream, uint64_t* failed_per_item);
+/* Autotuner: times the plan under every lane-group width x grid size and
+ * keeps the fastest (results do not depend on the choice). */
+MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* plan, const void* in, void* out, uint8_t* out2,
+                                          hipStream_t stream, float* best_ms);
+MXCO_API mxco_status mxco_batch_plan_config(mxco_batch_plan* plan, uint32_t* group_size, uint32_t* blocks_per_cu);
+MXCO_API mxco_status mxco_batch_plan_set_config(mxco_batch_plan* plan, uint32_t group_size, uint32_t blocks_per_cu);
+
+/* ---------------------------------------------------------------------- */
 /* StaticMultimap (include/static_multimap.h:88-556): signed int32/int64  */
 /* keys and values, duplicates kept.  All calls are synchronous, as in    */
 /* the reference (docs/static_multimap_API文档和使用示例.md:23).            */

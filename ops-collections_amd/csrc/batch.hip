@@ -1,0 +1,427 @@
+// Batched form: thousands of independent StaticMap / StaticSet tables of
+// uneven sizes in one slot arena, built and probed by one persistent kernel
+// per phase from a device work list.
+//
+// A plan turns host work items (table, input offset, count) into a device
+// chunk list; the persistent kernel's workgroups take chunks in grid-stride
+// order, load the chunk's table descriptor and run the same probe state
+// machine as the single-table kernels (oa_process), so every table gets the
+// reference's results (open_addressing_ref_impl.h:490-605).  Every launch is
+// capture-safe (no host copies, no memset nodes): a caller can record Clear
+// + Insert + Find into one hipGraph and replay it.  A built-in tuner times
+// the plan under each lane-group width and grid size and keeps the fastest;
+// results do not depend on that choice.
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mxco_common.h"
+#include "oa_kernels.h"
+
+using namespace mxco;
+
+namespace {
+
+struct BatchTable {
+    uint8_t* slots;
+    uint32_t cap;
+    uint32_t nbuckets;
+};
+
+struct BatchChunk {
+    uint32_t item;
+    uint32_t len;
+    uint64_t start;  // first key of the chunk, relative to the item
+};
+
+constexpr uint32_t kChunk = 2048;  // keys per chunk
+
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) void oa_batch_kernel(OaArgs base, const BatchTable* tables,
+                                                       const mxco_batch_item* items, const BatchChunk* chunks,
+                                                       uint32_t nchunks, unsigned long long* item_counters)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    constexpr bool kInsertFamily =
+        OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
+    constexpr int IN_STRIDE = kInsertFamily ? SS : KS;
+    constexpr int OUT_STRIDE = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
+    const uint32_t gpb = blockDim.x / G;
+    const uint32_t gj = threadIdx.x / G;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const BatchChunk ch = chunks[c];
+        const mxco_batch_item it = items[ch.item];
+        const BatchTable t = tables[it.table];
+        OaArgs a = base;
+        a.table = t.slots;
+        a.cap = t.cap;
+        a.nbuckets = t.nbuckets;
+        a.in = base.in + it.offset * IN_STRIDE;
+        if (base.out) a.out = base.out + it.offset * OUT_STRIDE;
+        if (base.out2) a.out2 = base.out2 + it.offset;
+        a.n = it.count;
+        unsigned long long c0 = 0, c1 = 0;
+        oa_process<KS, VS, G, OP>(a, ch.start + gj, ch.start + ch.len, gpb, c0, c1);
+        if (item_counters) flush_counters<OP>(item_counters + 2ull * ch.item, c0, c1);
+    }
+}
+
+template <int KS, int VS, int G>
+hipError_t launch_batch_g(int op, const OaArgs& a, const BatchTable* tables, const mxco_batch_item* items,
+                          const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters, unsigned grid,
+                          hipStream_t s)
+{
+    const dim3 block(256);
+    switch (op) {
+    case OP_INSERT:
+        oa_batch_kernel<KS, VS, G, OP_INSERT><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
+        break;
+    case OP_FIND:
+        oa_batch_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
+        break;
+    case OP_CONTAINS:
+        oa_batch_kernel<KS, VS, G, OP_CONTAINS><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int KS, int VS>
+hipError_t launch_batch(int g, int op, const OaArgs& a, const BatchTable* tables, const mxco_batch_item* items,
+                        const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters, unsigned grid,
+                        hipStream_t s)
+{
+    switch (g) {
+    case 1: return launch_batch_g<KS, VS, 1>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 2: return launch_batch_g<KS, VS, 2>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 4: return launch_batch_g<KS, VS, 4>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 8: return launch_batch_g<KS, VS, 8>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void arena_fill_kernel(uint8_t* p, uint64_t bytes, u64x2 pattern)
+{
+    const uint64_t n16 = bytes / 16;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(pattern, (u64x2*)p + i);
+}
+
+}  // namespace
+
+struct mxco_batch {
+    mxco_oa_desc d;
+    uint32_t ss;
+    uint32_t ntables;
+    uint8_t* arena;
+    uint64_t arena_bytes;
+    BatchTable* tables_dev;
+    std::vector<BatchTable> tables;
+    uint64_t empty_key, empty_val;
+};
+
+struct mxco_batch_plan {
+    mxco_batch* batch;
+    int op;
+    uint32_t nitems;
+    uint32_t nchunks;
+    mxco_batch_item* items_dev;
+    BatchChunk* chunks_dev;
+    unsigned long long* counters;  // 2 per item
+    unsigned long long* host_counters;
+    int group;
+    unsigned blocks_per_cu;
+};
+
+namespace {
+
+OaArgs batch_args(const mxco_batch* b)
+{
+    OaArgs a;
+    memset(&a, 0, sizeof(a));
+    a.bucket = b->d.bucket_size;
+    a.probing = b->d.probing;
+    a.hash = b->d.hash;
+    a.key_float = b->d.key_kind == MXCO_KEY_FLOAT;
+    a.empty_key = b->empty_key;
+    a.empty_val = b->empty_val;
+    a.erased_key = b->empty_key;
+    return a;
+}
+
+hipError_t dispatch(const mxco_batch* b, int g, int op, const OaArgs& a, const mxco_batch_plan* p, unsigned grid,
+                    hipStream_t s)
+{
+    const int ks = b->d.key_size, vs = b->d.value_size;
+#define LB(K, V) return launch_batch<K, V>(g, op, a, b->tables_dev, p->items_dev, p->chunks_dev, p->nchunks, \
+                                           op == OP_INSERT ? p->counters : nullptr, grid, s)
+    if (ks == 4 && vs == 4) LB(4, 4);
+    if (ks == 8 && vs == 8) LB(8, 8);
+    if (ks == 4 && vs == 8) LB(4, 8);
+    if (ks == 8 && vs == 4) LB(8, 4);
+    if (ks == 4 && vs == 0) LB(4, 0);
+    if (ks == 8 && vs == 0) LB(8, 0);
+#undef LB
+    return hipErrorInvalidValue;
+}
+
+mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, int g,
+                     unsigned bpc)
+{
+    if (p->op == OP_INSERT) {
+        zero_u64_kernel<<<grid_for(2ull * p->nitems, 1), 256, 0, s>>>(p->counters, 2ull * p->nitems);
+        MXCO_HIP(hipGetLastError(), "batch: zero counters");
+    }
+    OaArgs a = batch_args(p->batch);
+    a.in = (const uint8_t*)in;
+    a.out = (uint8_t*)out;
+    a.out2 = out2;
+    const unsigned grid = std::max(1u, std::min<unsigned>(p->nchunks, (unsigned)cu_count() * bpc));
+    MXCO_HIP(dispatch(p->batch, g, p->op, a, p, grid, s), "batch: launch");
+    return MXCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MXCO_API mxco_status mxco_batch_create(const mxco_oa_desc* desc, uint32_t num_tables, const uint64_t* capacities,
+                                       const void* empty_key, const void* empty_value, hipStream_t stream,
+                                       mxco_batch** out)
+{
+    if (!desc || !capacities || !empty_key || !out || num_tables == 0)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_create: bad arguments");
+    *out = nullptr;
+    const mxco_oa_desc& d = *desc;
+    if ((d.key_size != 4 && d.key_size != 8) || (d.value_size != 0 && d.value_size != 4 && d.value_size != 8))
+        return fail(MXCO_UNSUPPORTED, "mxco_batch_create: keys of 4/8 bytes, payloads of 0/4/8 bytes");
+    if (d.value_size && !empty_value) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_create: null empty value");
+    if (d.bucket_size < 1 || d.bucket_size > 1024 || d.probing > 1 || d.hash > MXCO_HASH_FMIX_64)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_create: bad descriptor");
+    mxco_batch* b = new (std::nothrow) mxco_batch();
+    if (!b) return fail(MXCO_OUT_OF_MEMORY, "mxco_batch_create: host allocation");
+    b->d = d;
+    b->ss = d.value_size == 0 ? d.key_size : ((d.key_size == 4 && d.value_size == 4) ? 8 : 16);
+    b->ntables = num_tables;
+    memcpy(&b->empty_key, empty_key, d.key_size);
+    if (d.value_size) memcpy(&b->empty_val, empty_value, d.value_size);
+    b->tables.resize(num_tables);
+    uint64_t off = 0;
+    std::vector<uint64_t> offs(num_tables);
+    for (uint32_t t = 0; t < num_tables; ++t) {
+        const uint64_t cap = valid_extent(capacities[t], d.bucket_size, d.probing);
+        if (cap == 0 || cap > 0xffffffffull) {
+            delete b;
+            return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_create: invalid capacity for table " + std::to_string(t));
+        }
+        offs[t] = off;
+        b->tables[t].cap = (uint32_t)cap;
+        b->tables[t].nbuckets = (uint32_t)(cap / d.bucket_size);
+        off += (cap * b->ss + 255) & ~255ull;  // each table starts on a 256-byte boundary
+    }
+    b->arena_bytes = off;
+    hipError_t e = hipMalloc((void**)&b->arena, off);
+    if (e == hipSuccess) {
+        for (uint32_t t = 0; t < num_tables; ++t) b->tables[t].slots = b->arena + offs[t];
+        e = hipMalloc((void**)&b->tables_dev, num_tables * sizeof(BatchTable));
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(b->tables_dev, b->tables.data(), num_tables * sizeof(BatchTable), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (b->arena) (void)hipFree(b->arena);
+        if (b->tables_dev) (void)hipFree(b->tables_dev);
+        delete b;
+        return hip_fail(e, "mxco_batch_create: allocation");
+    }
+    *out = b;
+    mxco_status st = mxco_batch_clear(b, stream);
+    if (st == MXCO_OK) st = hipStreamSynchronize(stream) == hipSuccess ? MXCO_OK : MXCO_RUNTIME_ERROR;
+    return st == MXCO_OK ? ok() : st;
+}
+
+MXCO_API mxco_status mxco_batch_destroy(mxco_batch* b)
+{
+    if (!b) return ok();
+    if (b->arena) (void)hipFree(b->arena);
+    if (b->tables_dev) (void)hipFree(b->tables_dev);
+    delete b;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_table(const mxco_batch* b, uint32_t table, uint64_t* capacity, void** slots)
+{
+    if (!b || table >= b->ntables) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_table: bad table");
+    if (capacity) *capacity = b->tables[table].cap;
+    if (slots) *slots = b->tables[table].slots;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_clear(mxco_batch* b, hipStream_t stream)
+{
+    if (!b) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_clear: null");
+    uint8_t img[16];
+    memset(img, 0, sizeof(img));
+    const uint32_t voff = (b->ss == 8 && b->d.value_size) ? 4 : 8;
+    for (uint32_t o = 0; o < 16; o += b->ss) {
+        memcpy(img + o, &b->empty_key, b->d.key_size);
+        if (b->d.value_size) memcpy(img + o + voff, &b->empty_val, b->d.value_size);
+    }
+    u64x2 pat;
+    memcpy(&pat, img, 16);
+    arena_fill_kernel<<<grid_for(b->arena_bytes / 16, 8), 256, 0, stream>>>(b->arena, b->arena_bytes, pat);
+    MXCO_HIP(hipGetLastError(), "mxco_batch_clear");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_plan_create(mxco_batch* b, int op, const mxco_batch_item* items, uint32_t num_items,
+                                            uint32_t group_size, mxco_batch_plan** out)
+{
+    if (!b || !items || !out || num_items == 0) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: bad args");
+    if (op != MXCO_BATCH_INSERT && op != MXCO_BATCH_FIND && op != MXCO_BATCH_CONTAINS)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: unknown operation");
+    if (group_size != 0 && group_size != 1 && group_size != 2 && group_size != 4 && group_size != 8)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: group size must be 0, 1, 2, 4 or 8");
+    *out = nullptr;
+    std::vector<BatchChunk> chunks;
+    for (uint32_t i = 0; i < num_items; ++i) {
+        if (items[i].table >= b->ntables)
+            return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: item names an unknown table");
+        for (uint64_t s = 0; s < items[i].count; s += kChunk)
+            chunks.push_back({i, (uint32_t)std::min<uint64_t>(kChunk, items[i].count - s), s});
+    }
+    // Chunks of one item stay adjacent, so neighbouring workgroups share a
+    // table's lines in L2.
+    mxco_batch_plan* p = new (std::nothrow) mxco_batch_plan();
+    if (!p) return fail(MXCO_OUT_OF_MEMORY, "mxco_batch_plan_create: host allocation");
+    p->batch = b;
+    p->op = op == MXCO_BATCH_INSERT ? OP_INSERT : (op == MXCO_BATCH_FIND ? OP_FIND : OP_CONTAINS);
+    p->nitems = num_items;
+    p->nchunks = (uint32_t)chunks.size();
+    p->group = group_size ? (int)group_size : (b->ss == 16 ? 4 : 4);
+    p->blocks_per_cu = 16;
+    hipError_t e = hipMalloc((void**)&p->items_dev, num_items * sizeof(mxco_batch_item));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->chunks_dev, std::max<size_t>(1, chunks.size()) * sizeof(BatchChunk));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->counters, 2ull * num_items * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&p->host_counters, 2ull * num_items * sizeof(unsigned long long), 0);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->items_dev, items, num_items * sizeof(mxco_batch_item), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !chunks.empty())
+        e = hipMemcpy(p->chunks_dev, chunks.data(), chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)mxco_batch_plan_destroy(p);
+        return hip_fail(e, "mxco_batch_plan_create: allocation");
+    }
+    *out = p;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_plan_destroy(mxco_batch_plan* p)
+{
+    if (!p) return ok();
+    if (p->items_dev) (void)hipFree(p->items_dev);
+    if (p->chunks_dev) (void)hipFree(p->chunks_dev);
+    if (p->counters) (void)hipFree(p->counters);
+    if (p->host_counters) (void)hipHostFree(p->host_counters);
+    delete p;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_run(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t stream)
+{
+    if (!p || !in) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_run: null argument");
+    if (p->op != OP_INSERT && !out) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_run: null output");
+    if (mxco_status s = run_plan(p, in, out, out2, stream, p->group, p->blocks_per_cu)) return s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_failures(mxco_batch_plan* p, hipStream_t stream, uint64_t* failed_per_item)
+{
+    if (!p || !failed_per_item || p->op != OP_INSERT)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_failures: an insert plan and an output are required");
+    MXCO_HIP(hipMemcpyAsync(p->host_counters, p->counters, 2ull * p->nitems * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stream),
+             "mxco_batch_failures");
+    MXCO_HIP(hipStreamSynchronize(stream), "mxco_batch_failures");
+    for (uint32_t i = 0; i < p->nitems; ++i) failed_per_item[i] = p->host_counters[2ull * i];
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_plan_config(mxco_batch_plan* p, uint32_t* group_size, uint32_t* blocks_per_cu)
+{
+    if (!p) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_config: null plan");
+    if (group_size) *group_size = (uint32_t)p->group;
+    if (blocks_per_cu) *blocks_per_cu = p->blocks_per_cu;
+    return ok();
+}
+
+// Times the plan under every lane-group width and grid size and keeps the
+// fastest.  The run must be repeatable on the given buffers: a Find or
+// Contains plan always is; an insert plan is timed on a cleared table
+// arena each time and left cleared.
+MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2,
+                                          hipStream_t stream, float* best_ms)
+{
+    if (!p || !in) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_tune: null argument");
+    static const int groups[] = {1, 2, 4, 8};
+    static const unsigned bpcs[] = {4, 8, 16, 32};
+    hipEvent_t e0, e1;
+    MXCO_HIP(hipEventCreate(&e0), "tune");
+    MXCO_HIP(hipEventCreate(&e1), "tune");
+    float best = 1e30f;
+    int bg = p->group;
+    unsigned bb = p->blocks_per_cu;
+    mxco_status st = MXCO_OK;
+    for (int g : groups) {
+        for (unsigned bpc : bpcs) {
+            float total = 0;
+            for (int rep = 0; rep < 3 && st == MXCO_OK; ++rep) {
+                if (p->op == OP_INSERT) st = mxco_batch_clear(p->batch, stream);
+                if (st != MXCO_OK) break;
+                if (hipEventRecord(e0, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune: event");
+                if (st == MXCO_OK) st = run_plan(p, in, out, out2, stream, g, bpc);
+                if (st == MXCO_OK && hipEventRecord(e1, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
+                if (st == MXCO_OK && hipEventSynchronize(e1) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
+                float ms = 0;
+                if (st == MXCO_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+                if (rep > 0) total += ms;  // first repetition warms up
+            }
+            if (st != MXCO_OK) break;
+            if (total < best) {
+                best = total;
+                bg = g;
+                bb = bpc;
+            }
+        }
+        if (st != MXCO_OK) break;
+    }
+    if (p->op == OP_INSERT && st == MXCO_OK) st = mxco_batch_clear(p->batch, stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != MXCO_OK) return st;
+    MXCO_HIP(hipStreamSynchronize(stream), "tune");
+    p->group = bg;
+    p->blocks_per_cu = bb;
+    if (best_ms) *best_ms = best / 2;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_plan_set_config(mxco_batch_plan* p, uint32_t group_size, uint32_t blocks_per_cu)
+{
+    if (!p || (group_size != 1 && group_size != 2 && group_size != 4 && group_size != 8) || blocks_per_cu == 0)
+        return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_set_config: bad arguments");
+    p->group = (int)group_size;
+    p->blocks_per_cu = blocks_per_cu;
+    return ok();
+}
+
+}  // extern "C"

@@ -70,7 +70,7 @@ int default_group(uint32_t ss)
 {
     int g = env_int("MXCO_OA_GROUP", 0);
     if (g == 1 || g == 2 || g == 4 || g == 8) return g;
-    return ss == 16 ? 2 : 4;
+    return 4;
 }
 
 OaArgs base_args(const mxco_oa* t)
@@ -202,7 +202,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     t->erased_key = t->empty_key;
     t->has_tomb = false;
     t->group = d.group_size ? (int)d.group_size : default_group(t->ss);
-    t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 16);
+    t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 256);
     t->store_mode = (uint32_t)env_int("MXCO_OA_STORE_MODE", 0);
     t->inflight = env_int("MXCO_OA_INFLIGHT", 1) == 2 ? 2 : 1;
     t->probe_flags = (uint32_t)env_int("MXCO_OA_PROBE_FLAGS", 0);

@@ -258,8 +258,12 @@ __device__ __forceinline__ void store_payload(uint8_t* p, uint64_t v, uint32_t m
     }
 }
 
+// Probes keys first, first + stride, ... < end of one table for the calling
+// lane group and accumulates the per-lane counters.  Every lane group of the
+// wavefront calls it with its own `first`; `end` and `stride` are uniform.
 template <int KS, int VS, int G, int OP, int U = 1>
-__global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
+__device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint64_t end, uint64_t stride,
+                                           unsigned long long& cnt0, unsigned long long& cnt1)
 {
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
@@ -276,11 +280,9 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     const uint32_t gl = lane & (G - 1);
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
     const bool leader = gl == 0;
-    const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
+    const uint64_t ngroups = stride;
     const uint64_t ekey = a.empty_key;
     const uint64_t miss_out = VS ? a.empty_val : ekey;
-
-    unsigned long long cnt0 = 0, cnt1 = 0;
 
     // Per-group state: the key being probed and its walk.  The loop below
     // advances every group by one window per iteration; a group whose key
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     // Each group keeps U keys in flight: slot u walks keys
     // gid + u*ngroups, gid + (u+U)*ngroups, ...  The window loads of all U
     // walks are issued before any of them is examined.
-    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const uint64_t gid = first;
     uint64_t I[U], KEY[U], PVAL[U], PREV[U];
     Walk W[U];
     uint32_t WLEN[U];
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     };
     auto advance = [&](int u, uint64_t from) {
         uint64_t idx = from;
-        while (idx < a.n && !begin(u, idx)) idx += (uint64_t)U * ngroups;
+        while (idx < end && !begin(u, idx)) idx += (uint64_t)U * ngroups;
         I[u] = idx;
     };
 #pragma unroll
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 
     // Window loads of slot u (nothing when its key is exhausted or finished).
     auto load = [&](int u) {
-        if (I[u] >= a.n || W[u].done) return;
+        if (I[u] >= end || W[u].done) return;
         const uint32_t wlen = window_len<SPL>(a, W[u]);
         WLEN[u] = wlen;
 #pragma unroll
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 
     // One step of slot u's walk on the loaded window.
     auto step = [&](int u) {
-        if (I[u] >= a.n) return;
+        if (I[u] >= end) return;
         const uint64_t i = I[u];
         const uint64_t key = KEY[u];
         const uint64_t pval = PVAL[u];
@@ -574,7 +576,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     for (;;) {
         bool any = false;
 #pragma unroll
-        for (int u = 0; u < U; ++u) any |= I[u] < a.n;
+        for (int u = 0; u < U; ++u) any |= I[u] < end;
         if (!any) break;
 #pragma unroll
         for (int u = 0; u < U; ++u) load(u);
@@ -582,24 +584,38 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
         for (int u = 0; u < U; ++u) step(u);
     }
 
+}
+
+// Wave reduction of the two counters, one atomic per wavefront.
+template <int OP>
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long cnt0,
+                                               unsigned long long cnt1)
+{
     constexpr bool kTwoCounters = OP == OP_INSERT_OR_ASSIGN || OP == OP_PROBE_STATS;
     if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_COUNT || OP == OP_ERASE ||
                   OP == OP_PROBE_STATS) {
-        if (a.counters) {
-            // Wave reduction, one atomic per wave.
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                cnt0 += __shfl_xor(cnt0, o);
-                if constexpr (kTwoCounters) cnt1 += __shfl_xor(cnt1, o);
-            }
-            if (lane == 0) {
-                if (cnt0) atomicAdd(&a.counters[0], cnt0);
-                if constexpr (kTwoCounters) {
-                    if (cnt1) atomicAdd(&a.counters[1], cnt1);
-                }
+        for (int o = 32; o > 0; o >>= 1) {
+            cnt0 += __shfl_xor(cnt0, o);
+            if constexpr (kTwoCounters) cnt1 += __shfl_xor(cnt1, o);
+        }
+        if (__lane_id() == 0) {
+            if (cnt0) atomicAdd(&counters[0], cnt0);
+            if constexpr (kTwoCounters) {
+                if (cnt1) atomicAdd(&counters[1], cnt1);
             }
         }
     }
+}
+
+template <int KS, int VS, int G, int OP, int U = 1>
+__global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
+{
+    unsigned long long cnt0 = 0, cnt1 = 0;
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
+    oa_process<KS, VS, G, OP, U>(a, gid, a.n, ngroups, cnt0, cnt1);
+    if (a.counters) flush_counters<OP>(a.counters, cnt0, cnt1);
 }
 
 // Sentinel fill (Clear / ClearAsync, open_addressing_impl.h:341-452):

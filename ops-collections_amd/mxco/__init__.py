@@ -608,3 +608,142 @@ class StaticMultimap:
         check(_lib.mxco_mm_retrieve_all(self._h, _ptr(key_out), _ptr(value_out), cap, _stream(stream),
                                         ctypes.byref(c)))
         return c.value
+
+
+# ---------------------------------------------------------------------------
+# Batched tables, graphs
+# ---------------------------------------------------------------------------
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("table", u32), ("reserved", u32), ("offset", u64), ("count", u64)]
+
+
+BATCH_INSERT, BATCH_FIND, BATCH_CONTAINS = 0, 1, 2
+_sig("mxco_batch_create", [ctypes.POINTER(OaDesc), u32, ctypes.POINTER(u64), vp, vp, vp, ctypes.POINTER(vp)])
+_sig("mxco_batch_destroy", [vp])
+_sig("mxco_batch_table", [vp, u32, pu64, ctypes.POINTER(vp)])
+_sig("mxco_batch_clear", [vp, vp])
+_sig("mxco_batch_plan_create", [vp, i32, ctypes.POINTER(BatchItem), u32, u32, ctypes.POINTER(vp)])
+_sig("mxco_batch_plan_destroy", [vp])
+_sig("mxco_batch_run", [vp, vp, vp, vp, vp])
+_sig("mxco_batch_failures", [vp, vp, pu64])
+_sig("mxco_batch_plan_tune", [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_float)])
+_sig("mxco_batch_plan_config", [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)])
+_sig("mxco_batch_plan_set_config", [vp, u32, u32])
+
+
+class BatchPlan:
+    def __init__(self, batch, op, items, group_size=0):
+        self.batch = batch
+        self.op = op
+        arr = (BatchItem * len(items))()
+        for j, (t, off, cnt) in enumerate(items):
+            arr[j].table, arr[j].offset, arr[j].count = t, off, cnt
+        self.nitems = len(items)
+        h = vp()
+        check(_lib.mxco_batch_plan_create(batch._h, op, arr, len(items), group_size, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.mxco_batch_plan_destroy(self._h)
+            self._h = None
+
+    def run(self, inp, out=None, out2=None, stream=None):
+        check(_lib.mxco_batch_run(self._h, _ptr(inp), _ptr(out), _ptr(out2), _stream(stream)))
+
+    def failures(self, stream=None):
+        f = (u64 * self.nitems)()
+        check(_lib.mxco_batch_failures(self._h, _stream(stream), f))
+        return list(f)
+
+    def tune(self, inp, out=None, out2=None, stream=None):
+        ms = ctypes.c_float()
+        check(_lib.mxco_batch_plan_tune(self._h, _ptr(inp), _ptr(out), _ptr(out2), _stream(stream),
+                                        ctypes.byref(ms)))
+        return ms.value
+
+    def config(self):
+        g, b = u32(), u32()
+        check(_lib.mxco_batch_plan_config(self._h, ctypes.byref(g), ctypes.byref(b)))
+        return g.value, b.value
+
+    def set_config(self, group_size, blocks_per_cu):
+        check(_lib.mxco_batch_plan_set_config(self._h, group_size, blocks_per_cu))
+
+
+class BatchedStaticMap:
+    """Many independent StaticMap tables in one arena, driven by plans."""
+
+    def __init__(self, capacities, empty_key, empty_value, key_dtype=torch.int32, value_dtype=torch.int32,
+                 bucket_size=5, probing="linear", hash="murmurhash3_32", stream=None):
+        self.key_dtype, self.value_dtype = key_dtype, value_dtype
+        ks = torch.empty(0, dtype=key_dtype).element_size()
+        vs = 0 if value_dtype is None else torch.empty(0, dtype=value_dtype).element_size()
+        d = OaDesc(ks, vs, 1 if key_dtype.is_floating_point else 0, bucket_size, PROBING[probing], HASH[hash], 0, 0)
+        self._ek = _scalar_bytes(empty_key, key_dtype)
+        self._ev = _scalar_bytes(empty_value, value_dtype) if value_dtype is not None else None
+        caps = (u64 * len(capacities))(*capacities)
+        h = vp()
+        check(_lib.mxco_batch_create(ctypes.byref(d), len(capacities), caps, self._ek, self._ev, _stream(stream),
+                                     ctypes.byref(h)))
+        self._h = h
+        self.num_tables = len(capacities)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.mxco_batch_destroy(self._h)
+            self._h = None
+
+    def capacity(self, t):
+        c = u64()
+        check(_lib.mxco_batch_table(self._h, t, ctypes.byref(c), None))
+        return c.value
+
+    def clear(self, stream=None):
+        check(_lib.mxco_batch_clear(self._h, _stream(stream)))
+
+    def plan(self, op, items, group_size=0):
+        return BatchPlan(self, op, items, group_size)
+
+
+_hip.hipStreamBeginCapture.argtypes = [vp, i32]
+_hip.hipStreamEndCapture.argtypes = [vp, ctypes.POINTER(vp)]
+_hip.hipGraphInstantiate.argtypes = [ctypes.POINTER(vp), vp, vp, vp, ctypes.c_size_t]
+_hip.hipGraphLaunch.argtypes = [vp, vp]
+_hip.hipGraphExecDestroy.argtypes = [vp]
+_hip.hipGraphDestroy.argtypes = [vp]
+
+
+class Graph:
+    """Records the library calls made by `fn(stream)` on a private stream into
+    a hipGraph (thread-local capture) and replays them with launch()."""
+
+    def __init__(self, fn):
+        self.stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        s = vp(self.stream.cuda_stream)
+        if _hip.hipStreamBeginCapture(s, 1) != 0:
+            raise RuntimeError("hipStreamBeginCapture failed")
+        try:
+            fn(self.stream)
+        finally:
+            g = vp()
+            rc = _hip.hipStreamEndCapture(s, ctypes.byref(g))
+        if rc != 0:
+            raise RuntimeError("hipStreamEndCapture failed")
+        self._graph = g
+        e = vp()
+        if _hip.hipGraphInstantiate(ctypes.byref(e), g, None, None, 0) != 0:
+            raise RuntimeError("hipGraphInstantiate failed")
+        self._exec = e
+
+    def launch(self, stream=None):
+        if _hip.hipGraphLaunch(self._exec, vp(_stream(stream if stream is not None else self.stream))) != 0:
+            raise RuntimeError("hipGraphLaunch failed")
+
+    def __del__(self):
+        if getattr(self, "_exec", None):
+            _hip.hipGraphExecDestroy(self._exec)
+            _hip.hipGraphDestroy(self._graph)
+            self._exec = None
