@@ -373,7 +373,7 @@ MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, vo
 {
     if (!p || !in) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_tune: null argument");
     static const int groups[] = {1, 2, 4, 8};
-    static const unsigned bpcs[] = {4, 8, 16, 32};
+    static const unsigned bpcs[] = {8, 32, 128, 256};
     hipEvent_t e0, e1;
     MXCO_HIP(hipEventCreate(&e0), "tune");
     MXCO_HIP(hipEventCreate(&e1), "tune");
