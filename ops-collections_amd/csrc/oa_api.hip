@@ -204,7 +204,10 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     t->group = d.group_size ? (int)d.group_size : default_group(t->ss);
     t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 256);
     t->store_mode = (uint32_t)env_int("MXCO_OA_STORE_MODE", 0);
-    t->inflight = env_int("MXCO_OA_INFLIGHT", 1) == 2 ? 2 : 1;
+    {
+        const int m = env_int("MXCO_OA_INFLIGHT", 1);
+        t->inflight = (m == 2 || m == 3) ? m : 1;
+    }
     t->probe_flags = (uint32_t)env_int("MXCO_OA_PROBE_FLAGS", 0);
     (void)hipGetDevice(&t->device);
 

@@ -618,6 +618,58 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     if (a.counters) flush_counters<OP>(a.counters, cnt0, cnt1);
 }
 
+// Chunk-staged variant for the streaming bulk operations.  A workgroup takes
+// chunks of kStageKeys consecutive inputs, copies the chunk's inputs into
+// LDS with coalesced non-temporal loads, lets its lane groups probe from LDS
+// and write results to LDS, then streams the results out with coalesced
+// non-temporal stores.  Without it, lane groups that have drifted apart
+// write single results into many lines at different times and the L2 writes
+// each line back several times.
+constexpr uint32_t kStageKeys = 1024;
+
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) void oa_staged_kernel(OaArgs a)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    constexpr bool kInsertFamily =
+        OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
+    constexpr int IN = kInsertFamily ? SS : KS;
+    constexpr int OUT = (OP == OP_FIND) ? (VS ? VS : KS) : (OP == OP_CONTAINS ? 1 : 0);
+    __shared__ __align__(16) uint8_t sin[kStageKeys * IN];
+    __shared__ __align__(16) uint8_t sout[OUT ? kStageKeys * OUT : 16];
+    unsigned long long cnt0 = 0, cnt1 = 0;
+    const uint32_t gpb = blockDim.x / G;
+    const uint32_t gj = threadIdx.x / G;
+    const uint64_t nchunks = (a.n + kStageKeys - 1) / kStageKeys;
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint64_t base = c * kStageKeys;
+        const uint32_t len = (uint32_t)min((uint64_t)kStageKeys, a.n - base);
+        const uint32_t in_bytes = len * IN;
+        // 4-byte granules: inputs of 4, 8 and 16 bytes are all multiples of 4
+        for (uint32_t t = threadIdx.x; t < in_bytes / 4; t += blockDim.x)
+            ((uint32_t*)sin)[t] = __builtin_nontemporal_load((const uint32_t*)(a.in + base * IN) + t);
+        __syncthreads();
+        OaArgs la = a;
+        la.in = sin;
+        la.out = OUT ? sout : nullptr;
+        la.n = len;
+        if (a.stencil) la.stencil = a.stencil + base * a.stencil_size;
+        oa_process<KS, VS, G, OP, 1>(la, gj, len, gpb, cnt0, cnt1);
+        __syncthreads();
+        if constexpr (OUT == 8) {
+            for (uint32_t t = threadIdx.x; t < len; t += blockDim.x)
+                __builtin_nontemporal_store(((const uint64_t*)sout)[t], (uint64_t*)(a.out + base * 8) + t);
+        } else if constexpr (OUT == 4) {
+            for (uint32_t t = threadIdx.x; t < len; t += blockDim.x)
+                __builtin_nontemporal_store(((const uint32_t*)sout)[t], (uint32_t*)(a.out + base * 4) + t);
+        } else if constexpr (OUT == 1) {
+            for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) a.out[base + t] = sout[t];
+        }
+        if constexpr (OUT != 0) __syncthreads();
+    }
+    if (a.counters) flush_counters<OP>(a.counters, cnt0, cnt1);
+}
+
 // Sentinel fill (Clear / ClearAsync, open_addressing_impl.h:341-452):
 // streams the 16-byte replicated slot image over the whole slot array.
 static __global__ __launch_bounds__(256) void oa_fill_kernel(uint8_t* table, uint64_t bytes, u64x2 pattern)
