@@ -277,6 +277,75 @@ MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* map, void* key_out, void* val
                                           hipStream_t stream, uint64_t* num_out);
 
 /* ---------------------------------------------------------------------- */
+/* Serving engine (no reference counterpart: a request loop over resident */
+/* StaticMap / StaticMultimap tables with continuous batching).  Every    */
+/* request's result equals running the table's requests one by one in    */
+/* arrival order with the calls above.                                    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_server mxco_server;
+
+typedef struct mxco_server_config {
+    uint64_t hbm_reserve_bytes; /* admission keeps at least this much HBM free */
+    uint64_t hbm_budget_bytes;  /* 0 = no cap on the tables' total bytes */
+    uint64_t max_tick_keys;     /* keys folded into one tick (0 = 2^24) */
+    uint32_t group_size;        /* lanes per key of the batched probes (0 = 4) */
+    uint32_t blocks_per_cu;     /* persistent grid (0 = 64) */
+    uint8_t empty_key[8];       /* StaticMap sentinels, little-endian */
+    uint8_t empty_value[8];
+} mxco_server_config;
+
+enum {
+    MXCO_REQ_INSERT = 0,      /* StaticMap::Insert: in = pairs, result = failed count */
+    MXCO_REQ_FIND = 1,        /* StaticMap::Find: in = keys, out = values */
+    MXCO_REQ_CONTAINS = 2,    /* StaticMap::Contains: out = 1-byte flags */
+    MXCO_REQ_MM_INSERT = 3,   /* StaticMultimap::Insert: result = failed count */
+    MXCO_REQ_MM_FIND = 4,     /* StaticMultimap::Find: out = smallest values */
+    MXCO_REQ_MM_COUNT = 5,    /* StaticMultimap::Count: result = matches */
+    MXCO_REQ_MM_RETRIEVE = 6  /* StaticMultimap::Retrieve: out = probe keys, out2 = values, result = matches */
+};
+
+typedef struct mxco_request {
+    uint32_t table;
+    uint32_t op;
+    const void* in;
+    uint64_t n;
+    void* out;
+    void* out2;
+    uint64_t out_capacity;
+} mxco_request;
+
+/* All StaticMap tables of a server share map_desc (key/value sizes, bucket,
+ * probing, hash); multimap tables are int64 -> int64. */
+MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxco_oa_desc* map_desc,
+                                        mxco_server** out);
+MXCO_API mxco_status mxco_server_destroy(mxco_server* server);
+/* Admits a resident table when free HBM (and the budget) allow it,
+ * MXCO_OUT_OF_MEMORY otherwise. */
+MXCO_API mxco_status mxco_server_add_map(mxco_server* server, uint64_t capacity, uint32_t* table);
+MXCO_API mxco_status mxco_server_add_multimap(mxco_server* server, uint64_t capacity, const void* empty_key,
+                                              const void* empty_value, uint32_t* table);
+/* Waits for outstanding requests, then frees the table's HBM. */
+MXCO_API mxco_status mxco_server_drop_table(mxco_server* server, uint32_t table);
+/* Queues a request and returns at once. */
+MXCO_API mxco_status mxco_server_submit(mxco_server* server, const mxco_request* request, uint64_t* ticket);
+/* Waits for a ticket: its result and submit-to-completion latency. */
+MXCO_API mxco_status mxco_server_wait(mxco_server* server, uint64_t ticket, uint64_t* result, double* latency_us);
+/* Waits until every submitted request has completed. */
+MXCO_API mxco_status mxco_server_drain(mxco_server* server);
+/* Open-loop replay: submits request i at arrival_us[i] after the call starts,
+ * then waits for all; fills per-request results and latencies. */
+MXCO_API mxco_status mxco_server_replay(mxco_server* server, const mxco_request* requests, const double* arrival_us,
+                                        uint32_t n, uint64_t* results, double* latency_us);
+/* The same open-loop replay served one request at a time by the plain
+ * synchronous calls (the unbatched baseline). */
+MXCO_API mxco_status mxco_server_replay_plain(mxco_server* server, const mxco_request* requests,
+                                              const double* arrival_us, uint32_t n, uint64_t* results,
+                                              double* latency_us);
+MXCO_API mxco_status mxco_server_stats(mxco_server* server, uint64_t* ticks, uint64_t* tick_requests);
+MXCO_API mxco_status mxco_server_table(mxco_server* server, uint32_t table, mxco_oa** map, mxco_mm** multimap);
+
+/* ---------------------------------------------------------------------- */
 /* BloomFilter (include/bloom_filter.h:37-133): the default policy        */
 /* BloomFilterPolicy<Key, xxhash_64<Key>, uint32_t, 8 words, 8 bits>      */
 /* (include/bloom_filter_policy.h:54-181).  Keys are 4 or 8 bytes         */

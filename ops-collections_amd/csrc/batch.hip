@@ -19,90 +19,72 @@
 #include <vector>
 
 #include "mxco_common.h"
+#include "batch_kernels.h"
 #include "oa_kernels.h"
 
 using namespace mxco;
 
-namespace {
+namespace mxco {
 
-struct BatchTable {
-    uint8_t* slots;
-    uint32_t cap;
-    uint32_t nbuckets;
-};
-
-struct BatchChunk {
-    uint32_t item;
-    uint32_t len;
-    uint64_t start;  // first key of the chunk, relative to the item
-};
-
-constexpr uint32_t kChunk = 2048;  // keys per chunk
-
-template <int KS, int VS, int G, int OP>
-__global__ __launch_bounds__(256) void oa_batch_kernel(OaArgs base, const BatchTable* tables,
-                                                       const mxco_batch_item* items, const BatchChunk* chunks,
-                                                       uint32_t nchunks, unsigned long long* item_counters)
-{
-    constexpr int SS = Lay<KS, VS>::SS;
-    constexpr bool kInsertFamily =
-        OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
-    constexpr int IN_STRIDE = kInsertFamily ? SS : KS;
-    constexpr int OUT_STRIDE = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
-    const uint32_t gpb = blockDim.x / G;
-    const uint32_t gj = threadIdx.x / G;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const BatchChunk ch = chunks[c];
-        const mxco_batch_item it = items[ch.item];
-        const BatchTable t = tables[it.table];
-        OaArgs a = base;
-        a.table = t.slots;
-        a.cap = t.cap;
-        a.nbuckets = t.nbuckets;
-        a.in = base.in + it.offset * IN_STRIDE;
-        if (base.out) a.out = base.out + it.offset * OUT_STRIDE;
-        if (base.out2) a.out2 = base.out2 + it.offset;
-        a.n = it.count;
-        unsigned long long c0 = 0, c1 = 0;
-        oa_process<KS, VS, G, OP>(a, ch.start + gj, ch.start + ch.len, gpb, c0, c1);
-        if (item_counters) flush_counters<OP>(item_counters + 2ull * ch.item, c0, c1);
-    }
-}
-
-template <int KS, int VS, int G>
-hipError_t launch_batch_g(int op, const OaArgs& a, const BatchTable* tables, const mxco_batch_item* items,
-                          const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters, unsigned grid,
-                          hipStream_t s)
+template <int KS, int VS, int G, typename Item>
+static hipError_t launch_batch_g(int op, const OaArgs& a, const BatchTable* tables, const Item* items,
+                                 const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters,
+                                 unsigned grid, hipStream_t s)
 {
     const dim3 block(256);
     switch (op) {
     case OP_INSERT:
-        oa_batch_kernel<KS, VS, G, OP_INSERT><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
+        oa_batch_kernel<KS, VS, G, OP_INSERT, Item><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks,
+                                                                          counters);
         break;
     case OP_FIND:
-        oa_batch_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
+        oa_batch_kernel<KS, VS, G, OP_FIND, Item><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
         break;
     case OP_CONTAINS:
-        oa_batch_kernel<KS, VS, G, OP_CONTAINS><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks, counters);
+        oa_batch_kernel<KS, VS, G, OP_CONTAINS, Item><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks,
+                                                                            counters);
         break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int KS, int VS>
-hipError_t launch_batch(int g, int op, const OaArgs& a, const BatchTable* tables, const mxco_batch_item* items,
-                        const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters, unsigned grid,
-                        hipStream_t s)
+template <int KS, int VS, typename Item>
+static hipError_t launch_batch_l(int g, int op, const OaArgs& a, const BatchTable* tables, const Item* items,
+                                 const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters,
+                                 unsigned grid, hipStream_t s)
 {
     switch (g) {
-    case 1: return launch_batch_g<KS, VS, 1>(op, a, tables, items, chunks, nchunks, counters, grid, s);
-    case 2: return launch_batch_g<KS, VS, 2>(op, a, tables, items, chunks, nchunks, counters, grid, s);
-    case 4: return launch_batch_g<KS, VS, 4>(op, a, tables, items, chunks, nchunks, counters, grid, s);
-    case 8: return launch_batch_g<KS, VS, 8>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 1: return launch_batch_g<KS, VS, 1, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 2: return launch_batch_g<KS, VS, 2, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 4: return launch_batch_g<KS, VS, 4, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 8: return launch_batch_g<KS, VS, 8, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
+
+hipError_t launch_batch_any(int ks, int vs, int g, int op, bool serve, const OaArgs& a, const BatchTable* tables,
+                            const void* items, const BatchChunk* chunks, uint32_t nchunks,
+                            unsigned long long* counters, unsigned grid, hipStream_t s)
+{
+#define LB(K, V)                                                                                              \
+    return serve ? launch_batch_l<K, V, ServeItem>(g, op, a, tables, (const ServeItem*)items, chunks, nchunks, \
+                                                   counters, grid, s)                                         \
+                 : launch_batch_l<K, V, mxco_batch_item>(g, op, a, tables, (const mxco_batch_item*)items,     \
+                                                         chunks, nchunks, counters, grid, s)
+    if (ks == 4 && vs == 4) LB(4, 4);
+    if (ks == 8 && vs == 8) LB(8, 8);
+    if (ks == 4 && vs == 8) LB(4, 8);
+    if (ks == 8 && vs == 4) LB(8, 4);
+    if (ks == 4 && vs == 0) LB(4, 0);
+    if (ks == 8 && vs == 0) LB(8, 0);
+#undef LB
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mxco
+
+namespace {
 
 __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n)
 {
@@ -162,17 +144,8 @@ OaArgs batch_args(const mxco_batch* b)
 hipError_t dispatch(const mxco_batch* b, int g, int op, const OaArgs& a, const mxco_batch_plan* p, unsigned grid,
                     hipStream_t s)
 {
-    const int ks = b->d.key_size, vs = b->d.value_size;
-#define LB(K, V) return launch_batch<K, V>(g, op, a, b->tables_dev, p->items_dev, p->chunks_dev, p->nchunks, \
-                                           op == OP_INSERT ? p->counters : nullptr, grid, s)
-    if (ks == 4 && vs == 4) LB(4, 4);
-    if (ks == 8 && vs == 8) LB(8, 8);
-    if (ks == 4 && vs == 8) LB(4, 8);
-    if (ks == 8 && vs == 4) LB(8, 4);
-    if (ks == 4 && vs == 0) LB(4, 0);
-    if (ks == 8 && vs == 0) LB(8, 0);
-#undef LB
-    return hipErrorInvalidValue;
+    return launch_batch_any(b->d.key_size, b->d.value_size, g, op, false, a, b->tables_dev, p->items_dev,
+                            p->chunks_dev, p->nchunks, op == OP_INSERT ? p->counters : nullptr, grid, s);
 }
 
 mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, int g,

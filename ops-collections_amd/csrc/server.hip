@@ -1,0 +1,676 @@
+// Serving engine: resident StaticMap / StaticMultimap tables in HBM and a
+// request loop with continuous batching.
+//
+//  - submit() appends a request (table, op, device input/output, count) to
+//    an arrival-ordered queue and returns a ticket at once;
+//  - the dispatcher thread drains the queue into *ticks*: every StaticMap
+//    request that can join without breaking its table's arrival order goes
+//    into the tick, and the tick runs as one persistent insert launch then
+//    one find and one contains launch over all its requests (the batched
+//    probe kernel), queued on the map stream without waiting for earlier
+//    ticks -- requests that arrive while a tick runs are admitted into the
+//    next one as tick buffers free up;
+//  - StaticMultimap requests run in arrival order on a second stream by a
+//    worker thread, overlapping the map ticks on the device;
+//  - a completion thread retires ticks in order (event wait), stamps each
+//    request's completion time and publishes its result.
+// Per table, requests take effect in arrival order: a tick takes a table's
+// requests while they are inserts followed by reads, so every result equals
+// the result of the same requests run one by one.  New tables are admitted
+// while free HBM stays above the configured reserve.
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_kernels.h"
+#include "mxco_common.h"
+
+using namespace mxco;
+
+namespace {
+
+double now_us()
+{
+    using namespace std::chrono;
+    return duration<double, std::micro>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+    mxco_request r;
+    uint64_t ticket;
+    double t_submit = 0, t_done = 0;
+    uint64_t result = 0;
+    mxco_status status = MXCO_OK;
+    bool done = false;
+};
+
+struct Table {
+    int kind;  // 0 map, 1 multimap
+    mxco_oa* map = nullptr;
+    mxco_mm* mm = nullptr;
+    uint64_t bytes = 0;
+    bool live = false;
+};
+
+constexpr int kTickSlots = 4;
+
+struct Tick {
+    ServeItem* items_host = nullptr;
+    ServeItem* items_dev = nullptr;
+    BatchChunk* chunks_host = nullptr;
+    BatchChunk* chunks_dev = nullptr;
+    unsigned long long* counters_dev = nullptr;
+    unsigned long long* counters_host = nullptr;
+    uint32_t item_cap = 0, chunk_cap = 0;
+    hipEvent_t done = nullptr;
+    std::vector<Req*> reqs;
+    std::vector<uint32_t> insert_items;  // request index of each insert item
+    bool busy = false;
+};
+
+__global__ void zero_counters_u64(unsigned long long* p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+}  // namespace
+
+struct mxco_server {
+    mxco_server_config cfg;
+    mxco_oa_desc map_desc;
+    int device = 0;
+    std::vector<Table> tables;
+    BatchTable* tables_dev = nullptr;
+    uint32_t tables_cap = 0;
+    std::mutex mu;
+    std::condition_variable cv_pending, cv_done, cv_ticks, cv_mm;
+    std::deque<Req*> pending;
+    std::deque<Req*> mm_pending;
+    std::unordered_map<uint64_t, Req*> reqs;
+    uint64_t next_ticket = 1;
+    Tick ticks[kTickSlots];
+    std::deque<Tick*> inflight;
+    hipStream_t map_stream = nullptr, mm_stream = nullptr;
+    std::thread dispatcher, completer, mm_worker;
+    bool stop = false;
+    std::string worker_error;
+    // statistics
+    uint64_t n_ticks = 0, n_tick_requests = 0;
+};
+
+namespace {
+
+mxco_status grow_tick(Tick& t, uint32_t items, uint32_t chunks)
+{
+    if (items > t.item_cap) {
+        if (t.items_host) (void)hipHostFree(t.items_host);
+        if (t.items_dev) (void)hipFree(t.items_dev);
+        if (t.counters_dev) (void)hipFree(t.counters_dev);
+        if (t.counters_host) (void)hipHostFree(t.counters_host);
+        uint32_t cap = std::max(items, 2 * t.item_cap);
+        MXCO_HIP(hipHostMalloc((void**)&t.items_host, cap * sizeof(ServeItem), 0), "server: tick buffers");
+        MXCO_HIP(hipMalloc((void**)&t.items_dev, cap * sizeof(ServeItem)), "server: tick buffers");
+        MXCO_HIP(hipMalloc((void**)&t.counters_dev, 2ull * cap * sizeof(unsigned long long)), "server: tick buffers");
+        MXCO_HIP(hipHostMalloc((void**)&t.counters_host, 2ull * cap * sizeof(unsigned long long), 0),
+                 "server: tick buffers");
+        t.item_cap = cap;
+    }
+    if (chunks > t.chunk_cap) {
+        if (t.chunks_host) (void)hipHostFree(t.chunks_host);
+        if (t.chunks_dev) (void)hipFree(t.chunks_dev);
+        uint32_t cap = std::max(chunks, 2 * t.chunk_cap);
+        MXCO_HIP(hipHostMalloc((void**)&t.chunks_host, cap * sizeof(BatchChunk), 0), "server: tick buffers");
+        MXCO_HIP(hipMalloc((void**)&t.chunks_dev, cap * sizeof(BatchChunk)), "server: tick buffers");
+        t.chunk_cap = cap;
+    }
+    return MXCO_OK;
+}
+
+OaArgs map_args(const mxco_server* s)
+{
+    OaArgs a;
+    memset(&a, 0, sizeof(a));
+    a.bucket = s->map_desc.bucket_size;
+    a.probing = s->map_desc.probing;
+    a.hash = s->map_desc.hash;
+    a.key_float = s->map_desc.key_kind == MXCO_KEY_FLOAT;
+    memcpy(&a.empty_key, s->cfg.empty_key, s->map_desc.key_size);
+    memcpy(&a.empty_val, s->cfg.empty_value, s->map_desc.value_size);
+    a.erased_key = a.empty_key;
+    return a;
+}
+
+bool is_map_op(uint32_t op) { return op == MXCO_REQ_INSERT || op == MXCO_REQ_FIND || op == MXCO_REQ_CONTAINS; }
+
+// Runs one tick's requests (already chosen) on the map stream.
+mxco_status launch_tick(mxco_server* s, Tick& t)
+{
+    // items grouped by op: inserts first, then finds, then contains
+    std::vector<uint32_t> order[3];
+    for (uint32_t i = 0; i < t.reqs.size(); ++i) {
+        const uint32_t op = t.reqs[i]->r.op;
+        order[op == MXCO_REQ_INSERT ? 0 : (op == MXCO_REQ_FIND ? 1 : 2)].push_back(i);
+    }
+    uint32_t nitems = (uint32_t)t.reqs.size();
+    uint32_t nchunks = 0;
+    for (Req* r : t.reqs) nchunks += (uint32_t)((r->r.n + kChunk - 1) / kChunk);
+    if (mxco_status e = grow_tick(t, nitems, nchunks)) return e;
+    uint32_t item = 0, chunk = 0;
+    uint32_t first_chunk[3], n_chunks[3];
+    t.insert_items.clear();
+    for (int k = 0; k < 3; ++k) {
+        first_chunk[k] = chunk;
+        for (uint32_t ri : order[k]) {
+            const mxco_request& r = t.reqs[ri]->r;
+            t.items_host[item] = {r.table, 0, (const uint8_t*)r.in, (uint8_t*)r.out, r.n};
+            if (k == 0) t.insert_items.push_back(ri);
+            for (uint64_t st = 0; st < r.n; st += kChunk)
+                t.chunks_host[chunk++] = {item, (uint32_t)std::min<uint64_t>(kChunk, r.n - st), st};
+            ++item;
+        }
+        n_chunks[k] = chunk - first_chunk[k];
+    }
+    hipStream_t st = s->map_stream;
+    MXCO_HIP(hipMemcpyAsync(t.items_dev, t.items_host, nitems * sizeof(ServeItem), hipMemcpyHostToDevice, st),
+             "server: upload items");
+    MXCO_HIP(hipMemcpyAsync(t.chunks_dev, t.chunks_host, chunk * sizeof(BatchChunk), hipMemcpyHostToDevice, st),
+             "server: upload chunks");
+    const uint32_t ninsert = (uint32_t)order[0].size();
+    if (ninsert) {
+        zero_counters_u64<<<1, 256, 0, st>>>(t.counters_dev, 2 * ninsert);
+        MXCO_HIP(hipGetLastError(), "server: zero counters");
+    }
+    const OaArgs a = map_args(s);
+    const int ops[3] = {OP_INSERT, OP_FIND, OP_CONTAINS};
+    for (int k = 0; k < 3; ++k) {
+        if (!n_chunks[k]) continue;
+        const unsigned grid = std::max(1u, std::min<unsigned>(n_chunks[k], (unsigned)cu_count() * s->cfg.blocks_per_cu));
+        MXCO_HIP(launch_batch_any(s->map_desc.key_size, s->map_desc.value_size, s->cfg.group_size, ops[k], true, a,
+                                  s->tables_dev, t.items_dev, t.chunks_dev + first_chunk[k], n_chunks[k],
+                                  k == 0 ? t.counters_dev : nullptr, grid, st),
+                 "server: launch");
+    }
+    if (ninsert)
+        MXCO_HIP(hipMemcpyAsync(t.counters_host, t.counters_dev, 2ull * ninsert * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, st),
+                 "server: read counters");
+    MXCO_HIP(hipEventRecord(t.done, st), "server: record");
+    return MXCO_OK;
+}
+
+void finish(mxco_server* s, Req* r, uint64_t result, mxco_status st)
+{
+    r->result = result;
+    r->status = st;
+    r->t_done = now_us();
+    r->done = true;
+}
+
+void dispatcher_loop(mxco_server* s)
+{
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_pending.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+        if (s->stop && s->pending.empty()) break;
+        // a free tick slot
+        Tick* t = nullptr;
+        s->cv_ticks.wait(lk, [&] {
+            for (Tick& x : s->ticks)
+                if (!x.busy) return true;
+            return s->stop;
+        });
+        for (Tick& x : s->ticks)
+            if (!x.busy) {
+                t = &x;
+                break;
+            }
+        if (!t) break;
+        // Choose the tick's requests: per table, inserts followed by reads.
+        std::unordered_map<uint32_t, int> phase;  // 1 inserting, 2 reading, 3 blocked
+        std::deque<Req*> keep;
+        uint64_t keys = 0;
+        t->reqs.clear();
+        for (Req* r : s->pending) {
+            const uint32_t tb = r->r.table;
+            if (!is_map_op(r->r.op)) {
+                s->mm_pending.push_back(r);
+                continue;
+            }
+            int& ph = phase[tb];
+            const bool insert = r->r.op == MXCO_REQ_INSERT;
+            const bool fits = keys + r->r.n <= s->cfg.max_tick_keys || t->reqs.empty();
+            if (ph == 3 || !fits || (insert && ph == 2)) {
+                ph = 3;
+                keep.push_back(r);
+                continue;
+            }
+            ph = insert ? 1 : 2;
+            keys += r->r.n;
+            t->reqs.push_back(r);
+        }
+        s->pending.swap(keep);
+        s->cv_mm.notify_one();
+        if (t->reqs.empty()) continue;
+        t->busy = true;
+        ++s->n_ticks;
+        s->n_tick_requests += t->reqs.size();
+        lk.unlock();
+        mxco_status st = launch_tick(s, *t);
+        lk.lock();
+        if (st != MXCO_OK) {
+            for (Req* r : t->reqs) finish(s, r, 0, st);
+            s->worker_error = mxco_last_error();
+            t->busy = false;
+            s->cv_done.notify_all();
+            continue;
+        }
+        s->inflight.push_back(t);
+        s->cv_done.notify_all();  // wakes the completer
+    }
+}
+
+void completer_loop(mxco_server* s)
+{
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_done.wait(lk, [&] { return s->stop || !s->inflight.empty(); });
+        if (s->inflight.empty()) {
+            if (s->stop) break;
+            continue;
+        }
+        Tick* t = s->inflight.front();
+        lk.unlock();
+        const hipError_t e = hipEventSynchronize(t->done);
+        lk.lock();
+        s->inflight.pop_front();
+        const double tnow = now_us();
+        for (Req* r : t->reqs) {
+            r->t_done = tnow;
+            r->status = e == hipSuccess ? MXCO_OK : MXCO_RUNTIME_ERROR;
+            r->done = true;
+        }
+        for (uint32_t k = 0; k < t->insert_items.size(); ++k)
+            t->reqs[t->insert_items[k]]->result = t->counters_host[2 * k];
+        t->reqs.clear();
+        t->busy = false;
+        s->cv_ticks.notify_all();
+        s->cv_done.notify_all();
+    }
+}
+
+// Multimap requests, in arrival order, on the second stream.
+void mm_loop(mxco_server* s)
+{
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_mm.wait(lk, [&] { return s->stop || !s->mm_pending.empty(); });
+        if (s->mm_pending.empty()) {
+            if (s->stop) break;
+            continue;
+        }
+        Req* r = s->mm_pending.front();
+        s->mm_pending.pop_front();
+        mxco_mm* mm = s->tables[r->r.table].mm;
+        lk.unlock();
+        uint64_t result = 0;
+        mxco_status st = MXCO_OK;
+        switch (r->r.op) {
+        case MXCO_REQ_MM_INSERT: st = mxco_mm_insert(mm, r->r.in, nullptr, r->r.n, s->mm_stream, &result); break;
+        case MXCO_REQ_MM_FIND: st = mxco_mm_find(mm, r->r.in, nullptr, r->r.out, r->r.n, s->mm_stream); break;
+        case MXCO_REQ_MM_COUNT: st = mxco_mm_count(mm, r->r.in, r->r.n, s->mm_stream, &result); break;
+        case MXCO_REQ_MM_RETRIEVE:
+            st = mxco_mm_retrieve(mm, r->r.in, r->r.n, r->r.out, r->r.out2, r->r.out_capacity, s->mm_stream, &result);
+            break;
+        default: st = fail(MXCO_INVALID_ARGUMENT, "server: unknown request");
+        }
+        lk.lock();
+        finish(s, r, result, st);
+        s->cv_done.notify_all();
+    }
+}
+
+mxco_status validate(mxco_server* s, const mxco_request* r)
+{
+    if (r->table >= s->tables.size() || !s->tables[r->table].live)
+        return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
+    const int kind = s->tables[r->table].kind;
+    if (kind == 0 && !is_map_op(r->op)) return fail(MXCO_INVALID_ARGUMENT, "server: not a StaticMap operation");
+    if (kind == 1 && is_map_op(r->op)) return fail(MXCO_INVALID_ARGUMENT, "server: not a StaticMultimap operation");
+    if (r->n && !r->in) return fail(MXCO_INVALID_ARGUMENT, "server: null input");
+    if (r->n && r->op != MXCO_REQ_INSERT && r->op != MXCO_REQ_MM_INSERT && r->op != MXCO_REQ_MM_COUNT && !r->out)
+        return fail(MXCO_INVALID_ARGUMENT, "server: null output");
+    return MXCO_OK;
+}
+
+mxco_status admit(mxco_server* s, uint64_t bytes)
+{
+    size_t free_b = 0, total_b = 0;
+    MXCO_HIP(hipMemGetInfo(&free_b, &total_b), "server: hipMemGetInfo");
+    uint64_t used = 0;
+    for (const Table& t : s->tables)
+        if (t.live) used += t.bytes;
+    if (s->cfg.hbm_budget_bytes && used + bytes > s->cfg.hbm_budget_bytes)
+        return fail(MXCO_OUT_OF_MEMORY, "server: admission refused (table budget)");
+    if ((uint64_t)free_b < bytes + s->cfg.hbm_reserve_bytes)
+        return fail(MXCO_OUT_OF_MEMORY, "server: admission refused (free HBM)");
+    return MXCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxco_oa_desc* map_desc,
+                                        mxco_server** out)
+{
+    if (!cfg || !map_desc || !out) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    *out = nullptr;
+    if (map_desc->value_size == 0) return fail(MXCO_UNSUPPORTED, "server: StaticMap tables need a payload");
+    mxco_server* s = new (std::nothrow) mxco_server();
+    if (!s) return fail(MXCO_OUT_OF_MEMORY, "server: host allocation");
+    s->cfg = *cfg;
+    s->map_desc = *map_desc;
+    if (s->map_desc.group_size == 0) s->map_desc.group_size = 4;
+    if (!s->cfg.group_size) s->cfg.group_size = s->map_desc.group_size;
+    if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 64;
+    if (!s->cfg.max_tick_keys) s->cfg.max_tick_keys = 1ull << 24;
+    (void)hipGetDevice(&s->device);
+    s->tables_cap = 4096;
+    hipError_t e = hipMalloc((void**)&s->tables_dev, s->tables_cap * sizeof(BatchTable));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->map_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->mm_stream, hipStreamNonBlocking);
+    for (int i = 0; i < kTickSlots && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&s->ticks[i].done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)mxco_server_destroy(s);
+        return hip_fail(e, "server: create");
+    }
+    s->dispatcher = std::thread(dispatcher_loop, s);
+    s->completer = std::thread(completer_loop, s);
+    s->mm_worker = std::thread(mm_loop, s);
+    *out = s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
+{
+    if (!s) return ok();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_pending.notify_all();
+    s->cv_done.notify_all();
+    s->cv_ticks.notify_all();
+    s->cv_mm.notify_all();
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    if (s->mm_worker.joinable()) s->mm_worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_done.notify_all();
+    if (s->completer.joinable()) s->completer.join();
+    if (s->map_stream) (void)hipStreamSynchronize(s->map_stream);
+    if (s->mm_stream) (void)hipStreamSynchronize(s->mm_stream);
+    for (Table& t : s->tables) {
+        if (t.map) (void)mxco_oa_destroy(t.map);
+        if (t.mm) (void)mxco_mm_destroy(t.mm);
+    }
+    for (Tick& t : s->ticks) {
+        if (t.items_host) (void)hipHostFree(t.items_host);
+        if (t.items_dev) (void)hipFree(t.items_dev);
+        if (t.chunks_host) (void)hipHostFree(t.chunks_host);
+        if (t.chunks_dev) (void)hipFree(t.chunks_dev);
+        if (t.counters_dev) (void)hipFree(t.counters_dev);
+        if (t.counters_host) (void)hipHostFree(t.counters_host);
+        if (t.done) (void)hipEventDestroy(t.done);
+    }
+    for (auto& kv : s->reqs) delete kv.second;
+    if (s->tables_dev) (void)hipFree(s->tables_dev);
+    if (s->map_stream) (void)hipStreamDestroy(s->map_stream);
+    if (s->mm_stream) (void)hipStreamDestroy(s->mm_stream);
+    delete s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_add_map(mxco_server* s, uint64_t capacity, uint32_t* table)
+{
+    if (!s || !table) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    const uint64_t cap = valid_extent(capacity, s->map_desc.bucket_size, s->map_desc.probing);
+    const uint32_t ss = (s->map_desc.key_size == 4 && s->map_desc.value_size == 4) ? 8 : 16;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->tables.size() >= s->tables_cap) return fail(MXCO_LENGTH_ERROR, "server: too many tables");
+    if (mxco_status e = admit(s, cap * ss)) return e;
+    mxco_oa* m = nullptr;
+    if (mxco_status e = mxco_oa_create(&s->map_desc, capacity, s->cfg.empty_key, s->cfg.empty_value, s->map_stream, &m))
+        return e;
+    BatchTable bt;
+    void* slots = nullptr;
+    uint64_t c = 0;
+    (void)mxco_oa_data(m, &slots);
+    (void)mxco_oa_capacity(m, &c);
+    bt.slots = (uint8_t*)slots;
+    bt.cap = (uint32_t)c;
+    bt.nbuckets = (uint32_t)(c / s->map_desc.bucket_size);
+    const uint32_t id = (uint32_t)s->tables.size();
+    // a new descriptor entry: no running kernel reads this index yet
+    MXCO_HIP(hipMemcpy(s->tables_dev + id, &bt, sizeof(bt), hipMemcpyHostToDevice), "server: add table");
+    Table t;
+    t.kind = 0;
+    t.map = m;
+    t.bytes = c * ss;
+    t.live = true;
+    s->tables.push_back(t);
+    *table = id;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_add_multimap(mxco_server* s, uint64_t capacity, const void* empty_key,
+                                              const void* empty_value, uint32_t* table)
+{
+    if (!s || !table) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->tables.size() >= s->tables_cap) return fail(MXCO_LENGTH_ERROR, "server: too many tables");
+    uint64_t p = 64;
+    while (p < 2 * capacity) p <<= 1;
+    if (mxco_status e = admit(s, p * 16)) return e;
+    mxco_mm* mm = nullptr;
+    if (mxco_status e = mxco_mm_create(8, 8, capacity, empty_key, empty_value, s->mm_stream, &mm)) return e;
+    Table t;
+    t.kind = 1;
+    t.mm = mm;
+    t.bytes = p * 16;
+    t.live = true;
+    s->tables.push_back(t);
+    *table = (uint32_t)(s->tables.size() - 1);
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_drop_table(mxco_server* s, uint32_t table)
+{
+    if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    (void)mxco_server_drain(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (table >= s->tables.size() || !s->tables[table].live) return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
+    Table& t = s->tables[table];
+    if (t.map) (void)mxco_oa_destroy(t.map);
+    if (t.mm) (void)mxco_mm_destroy(t.mm);
+    t.map = nullptr;
+    t.mm = nullptr;
+    t.live = false;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_submit(mxco_server* s, const mxco_request* req, uint64_t* ticket)
+{
+    if (!s || !req || !ticket) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    Req* r = new (std::nothrow) Req();
+    if (!r) return fail(MXCO_OUT_OF_MEMORY, "server: host allocation");
+    r->r = *req;
+    r->t_submit = now_us();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (mxco_status e = validate(s, req)) {
+            delete r;
+            return e;
+        }
+        r->ticket = s->next_ticket++;
+        s->reqs[r->ticket] = r;
+        if (r->r.n == 0) {
+            finish(s, r, 0, MXCO_OK);
+        } else {
+            s->pending.push_back(r);
+        }
+    }
+    s->cv_pending.notify_one();
+    *ticket = r->ticket;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_wait(mxco_server* s, uint64_t ticket, uint64_t* result, double* latency_us)
+{
+    if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(MXCO_INVALID_ARGUMENT, "server: unknown ticket");
+    Req* r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (result) *result = r->result;
+    if (latency_us) *latency_us = r->t_done - r->t_submit;
+    const mxco_status st = r->status;
+    s->reqs.erase(it);
+    delete r;
+    if (st != MXCO_OK) return fail(st, "server: request failed: " + s->worker_error);
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_drain(mxco_server* s)
+{
+    if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] {
+        if (!s->pending.empty() || !s->mm_pending.empty() || !s->inflight.empty()) return false;
+        for (auto& kv : s->reqs)
+            if (!kv.second->done) return false;
+        for (Tick& t : s->ticks)
+            if (t.busy) return false;
+        return true;
+    });
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_replay(mxco_server* s, const mxco_request* reqs, const double* arrival_us,
+                                        uint32_t n, uint64_t* results, double* latency_us)
+{
+    if (!s || (n && (!reqs || !arrival_us))) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::vector<uint64_t> tickets(n);
+    mxco_status first_err = MXCO_OK;
+    std::string err_msg;
+    const double t0 = now_us();
+    for (uint32_t i = 0; i < n; ++i) {
+        const double due = t0 + arrival_us[i];
+        double t = now_us();
+        while (t < due) {  // open loop: submit at the scheduled arrival time
+            if (due - t > 200) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(due - t - 100)));
+            t = now_us();
+        }
+        if (mxco_status e = mxco_server_submit(s, &reqs[i], &tickets[i])) {
+            tickets[i] = 0;
+            if (first_err == MXCO_OK) {
+                first_err = e;
+                err_msg = mxco_last_error();
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t res = 0;
+        double lat = 0;
+        if (tickets[i]) {
+            mxco_status e = mxco_server_wait(s, tickets[i], &res, &lat);
+            if (e != MXCO_OK && first_err == MXCO_OK) {
+                first_err = e;
+                err_msg = mxco_last_error();
+            }
+        }
+        if (results) results[i] = res;
+        if (latency_us) latency_us[i] = lat;
+    }
+    if (first_err != MXCO_OK) return fail(first_err, err_msg);
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_stats(mxco_server* s, uint64_t* ticks, uint64_t* tick_requests)
+{
+    if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (ticks) *ticks = s->n_ticks;
+    if (tick_requests) *tick_requests = s->n_tick_requests;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_table(mxco_server* s, uint32_t table, mxco_oa** map, mxco_mm** multimap)
+{
+    if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (table >= s->tables.size() || !s->tables[table].live) return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
+    if (map) *map = s->tables[table].map;
+    if (multimap) *multimap = s->tables[table].mm;
+    return ok();
+}
+
+// The same open-loop replay, served one request at a time with the plain
+// synchronous calls (the unbatched baseline): request i starts at
+// max(its arrival, the previous completion).
+MXCO_API mxco_status mxco_server_replay_plain(mxco_server* s, const mxco_request* reqs, const double* arrival_us,
+                                              uint32_t n, uint64_t* results, double* latency_us)
+{
+    if (!s || (n && (!reqs || !arrival_us))) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    if (mxco_status e = mxco_server_drain(s)) return e;
+    hipStream_t st = s->map_stream;
+    const double t0 = now_us();
+    for (uint32_t i = 0; i < n; ++i) {
+        const double due = t0 + arrival_us[i];
+        double t = now_us();
+        while (t < due) {
+            if (due - t > 200) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(due - t - 100)));
+            t = now_us();
+        }
+        const mxco_request& r = reqs[i];
+        if (r.table >= s->tables.size() || !s->tables[r.table].live)
+            return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
+        Table& tb = s->tables[r.table];
+        uint64_t res = 0;
+        mxco_status e = MXCO_OK;
+        switch (r.op) {
+        case MXCO_REQ_INSERT: e = mxco_oa_insert(tb.map, r.in, nullptr, r.n, st, 1, &res); break;
+        case MXCO_REQ_FIND: e = mxco_oa_find(tb.map, r.in, nullptr, r.out, r.n, st, 1); break;
+        case MXCO_REQ_CONTAINS: e = mxco_oa_contains(tb.map, r.in, nullptr, (uint8_t*)r.out, r.n, st, 1); break;
+        case MXCO_REQ_MM_INSERT: e = mxco_mm_insert(tb.mm, r.in, nullptr, r.n, st, &res); break;
+        case MXCO_REQ_MM_FIND: e = mxco_mm_find(tb.mm, r.in, nullptr, r.out, r.n, st); break;
+        case MXCO_REQ_MM_COUNT: e = mxco_mm_count(tb.mm, r.in, r.n, st, &res); break;
+        case MXCO_REQ_MM_RETRIEVE:
+            e = mxco_mm_retrieve(tb.mm, r.in, r.n, r.out, r.out2, r.out_capacity, st, &res);
+            break;
+        default: e = fail(MXCO_INVALID_ARGUMENT, "server: unknown request");
+        }
+        if (e != MXCO_OK) return e;
+        if (results) results[i] = res;
+        if (latency_us) latency_us[i] = now_us() - due;
+    }
+    return ok();
+}
+
+}  // extern "C"

@@ -747,3 +747,119 @@ class Graph:
             _hip.hipGraphExecDestroy(self._exec)
             _hip.hipGraphDestroy(self._graph)
             self._exec = None
+
+
+# ---------------------------------------------------------------------------
+# Serving engine
+# ---------------------------------------------------------------------------
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_reserve_bytes", u64), ("hbm_budget_bytes", u64), ("max_tick_keys", u64),
+                ("group_size", u32), ("blocks_per_cu", u32), ("empty_key", ctypes.c_uint8 * 8),
+                ("empty_value", ctypes.c_uint8 * 8)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("table", u32), ("op", u32), ("input", vp), ("n", u64), ("out", vp), ("out2", vp),
+                ("out_capacity", u64)]
+
+
+REQ_INSERT, REQ_FIND, REQ_CONTAINS, REQ_MM_INSERT, REQ_MM_FIND, REQ_MM_COUNT, REQ_MM_RETRIEVE = range(7)
+_sig("mxco_server_create", [ctypes.POINTER(ServerConfig), ctypes.POINTER(OaDesc), ctypes.POINTER(vp)])
+_sig("mxco_server_destroy", [vp])
+_sig("mxco_server_add_map", [vp, u64, ctypes.POINTER(u32)])
+_sig("mxco_server_add_multimap", [vp, u64, vp, vp, ctypes.POINTER(u32)])
+_sig("mxco_server_drop_table", [vp, u32])
+_sig("mxco_server_submit", [vp, ctypes.POINTER(Request), pu64])
+_sig("mxco_server_wait", [vp, u64, pu64, ctypes.POINTER(ctypes.c_double)])
+_sig("mxco_server_drain", [vp])
+_sig("mxco_server_replay", [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), u32, pu64,
+                            ctypes.POINTER(ctypes.c_double)])
+_sig("mxco_server_replay_plain", [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), u32, pu64,
+                                  ctypes.POINTER(ctypes.c_double)])
+_sig("mxco_server_stats", [vp, pu64, pu64])
+
+
+class Server:
+    """Resident int64 StaticMap / StaticMultimap tables behind a continuously
+    batching request loop (native dispatcher, completion and multimap
+    threads)."""
+
+    def __init__(self, empty_key=-1, empty_value=-1, key_dtype=torch.int64, value_dtype=torch.int64,
+                 hbm_reserve_bytes=1 << 30, hbm_budget_bytes=0, max_tick_keys=0, group_size=0, blocks_per_cu=0):
+        cfg = ServerConfig()
+        cfg.hbm_reserve_bytes, cfg.hbm_budget_bytes, cfg.max_tick_keys = hbm_reserve_bytes, hbm_budget_bytes, \
+            max_tick_keys
+        cfg.group_size, cfg.blocks_per_cu = group_size, blocks_per_cu
+        ek = _scalar_bytes(empty_key, key_dtype).raw
+        ev = _scalar_bytes(empty_value, value_dtype).raw
+        for j in range(len(ek)):
+            cfg.empty_key[j] = ek[j]
+        for j in range(len(ev)):
+            cfg.empty_value[j] = ev[j]
+        ks = torch.empty(0, dtype=key_dtype).element_size()
+        vs = torch.empty(0, dtype=value_dtype).element_size()
+        d = OaDesc(ks, vs, 0, 5, 0, 0, 0, 0)
+        h = vp()
+        check(_lib.mxco_server_create(ctypes.byref(cfg), ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._mm_ek = _scalar_bytes(empty_key, torch.int64)
+        self._mm_ev = _scalar_bytes(empty_value, torch.int64)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(_lib.mxco_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_map(self, capacity):
+        t = u32()
+        check(_lib.mxco_server_add_map(self._h, capacity, ctypes.byref(t)))
+        return t.value
+
+    def add_multimap(self, capacity):
+        t = u32()
+        check(_lib.mxco_server_add_multimap(self._h, capacity, self._mm_ek, self._mm_ev, ctypes.byref(t)))
+        return t.value
+
+    def drop_table(self, t):
+        check(_lib.mxco_server_drop_table(self._h, t))
+
+    def submit(self, table, op, inp, n=None, out=None, out2=None, out_capacity=0):
+        r = Request(table, op, _ptr(inp), inp.shape[0] if n is None else n, _ptr(out), _ptr(out2), out_capacity)
+        tk = u64()
+        check(_lib.mxco_server_submit(self._h, ctypes.byref(r), ctypes.byref(tk)))
+        return tk.value
+
+    def wait(self, ticket):
+        res, lat = u64(), ctypes.c_double()
+        check(_lib.mxco_server_wait(self._h, ticket, ctypes.byref(res), ctypes.byref(lat)))
+        return res.value, lat.value
+
+    def drain(self):
+        check(_lib.mxco_server_drain(self._h))
+
+    @staticmethod
+    def _requests(reqs):
+        arr = (Request * len(reqs))()
+        for j, r in enumerate(reqs):
+            arr[j] = Request(*r)
+        return arr
+
+    def replay(self, reqs, arrivals_us, plain=False):
+        """reqs: list of (table, op, in_ptr, n, out_ptr, out2_ptr, out_capacity)."""
+        n = len(reqs)
+        arr = self._requests(reqs)
+        arrivals = (ctypes.c_double * n)(*arrivals_us)
+        res = (u64 * n)()
+        lat = (ctypes.c_double * n)()
+        fn = _lib.mxco_server_replay_plain if plain else _lib.mxco_server_replay
+        check(fn(self._h, arr, arrivals, n, res, lat))
+        return list(res), list(lat)
+
+    def stats(self):
+        a, b = u64(), u64()
+        check(_lib.mxco_server_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

@@ -55,3 +55,65 @@ def batched_tables(num_tables=1024, cap_log2=(10, 22), lf=(0.5, 0.85), find_log2
     return {"caps": caps, "n_ins": n_ins, "n_find": n_find, "ins_off": [int(x) for x in ins_off_l],
             "find_off": [int(x) for x in find_off_l], "pairs": pairs, "queries": queries,
             "expected": torch.where(is_hit & (ni[qt] > 0), queries * 7 + 1, torch.full_like(queries, -1))}
+
+
+def serving(num_maps=48, num_multimaps=16, slots_log2=(20, 26), fill=0.5, num_requests=2000, qps=20000.0,
+            batch_log2=(0, 16), hit=0.7, seed=5):
+    """Resident tables and an open-loop request stream: Poisson arrivals at
+    `qps`; 80 % Find/Contains, 15 % Insert on the StaticMaps, 5 % Retrieve on
+    the multimaps; batch sizes log-uniform 1..2^16 keys at `hit` hit rate.
+
+    Keys of table t are the bijection of (t, j) used by batched_tables; map
+    tables start `fill` full, multimap keys repeat 1..8 times (value = key
+    * 16 + copy).  Returns host plans and device buffers for every request."""
+    rng = np.random.default_rng(seed)
+    dev = "cuda"
+    T = num_maps + num_multimaps
+    caps = [int(2.0 ** rng.uniform(*slots_log2)) for _ in range(T)]
+    salts = rng.integers(0, 2**31, T)
+
+    def keys_of(t, j):
+        return (((j * MULT) & 0x7FFFFFFF) ^ int(salts[t])) + (int(t) << 32)
+
+    init = []  # per table: device pairs to pre-insert
+    n_present = []
+    for t in range(T):
+        if t < num_maps:
+            n = int(fill * caps[t])
+            j = torch.arange(n, device=dev, dtype=torch.int64)
+            k = keys_of(t, j)
+            init.append(torch.stack([k, k * 7 + 1], 1).contiguous())
+            n_present.append(n)
+        else:
+            n_keys = max(1, int(fill * caps[t] / 4.5))
+            rep = torch.from_numpy(rng.integers(1, 9, n_keys)).to(dev)
+            j = torch.repeat_interleave(torch.arange(n_keys, device=dev, dtype=torch.int64), rep)
+            k = keys_of(t, j)
+            copy = torch.arange(j.numel(), device=dev) - torch.repeat_interleave(torch.cumsum(rep, 0) - rep, rep)
+            init.append(torch.stack([k, k * 16 + copy], 1).contiguous())
+            n_present.append(n_keys)
+    gaps = rng.exponential(1e6 / qps, num_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    next_new = list(n_present)
+    for r in range(num_requests):
+        u = rng.random()
+        n = int(2.0 ** rng.uniform(*batch_log2))
+        if u < 0.05:
+            t = int(rng.integers(num_maps, T))
+            op = "retrieve"
+        else:
+            t = int(rng.integers(0, num_maps))
+            op = "find" if u < 0.45 else ("contains" if u < 0.85 else "insert")
+        if op == "insert":
+            j = torch.arange(next_new[t], next_new[t] + n, device=dev, dtype=torch.int64)
+            next_new[t] += n
+            k = keys_of(t, j)
+            inp = torch.stack([k, k * 7 + 1], 1).contiguous()
+        else:
+            hitm = torch.from_numpy(rng.random(n) < hit).to(dev)
+            jh = torch.from_numpy(rng.integers(0, max(1, n_present[t]), n)).to(dev)
+            jm = torch.from_numpy(rng.integers(2**30, 2**31 - 1, n)).to(dev)
+            inp = keys_of(t, torch.where(hitm, jh, jm)).contiguous()
+        reqs.append({"table": t, "op": op, "input": inp, "n": n})
+    return {"caps": caps, "num_maps": num_maps, "init": init, "requests": reqs, "arrivals_us": arrivals.tolist()}

@@ -1,0 +1,113 @@
+"""Serving engine: a continuously batched open-loop request stream over
+resident tables gives exactly the results of the same requests served one
+at a time with the plain synchronous calls; admission refuses tables beyond
+the HBM budget."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def mx():
+    import mxco
+    return mxco
+
+
+OPS = {"insert": 0, "find": 1, "contains": 2, "retrieve": 6}
+
+
+def build(w, **cfg):
+    import mxco
+    s = mxco.Server(**cfg)
+    tables = []
+    for t, cap in enumerate(w["caps"]):
+        tables.append(s.add_map(cap) if t < w["num_maps"] else s.add_multimap(cap))
+    for t, pairs in enumerate(w["init"]):
+        op = mxco.REQ_INSERT if t < w["num_maps"] else mxco.REQ_MM_INSERT
+        res, _ = s.wait(s.submit(tables[t], op, pairs))
+        assert res == 0
+    return s, tables
+
+
+def outputs(w):
+    outs = []
+    for r in w["requests"]:
+        if r["op"] == "find":
+            outs.append((torch.full((r["n"],), 7, dtype=torch.int64, device="cuda"), None))
+        elif r["op"] == "contains":
+            outs.append((torch.full((r["n"],), 7, dtype=torch.uint8, device="cuda"), None))
+        elif r["op"] == "retrieve":
+            cap = 8 * r["n"] + 8
+            outs.append((torch.zeros(cap, dtype=torch.int64, device="cuda"),
+                         torch.zeros(cap, dtype=torch.int64, device="cuda")))
+        else:
+            outs.append((None, None))
+    return outs
+
+
+def plan(w, tables, outs):
+    reqs = []
+    for r, (o1, o2) in zip(w["requests"], outs):
+        cap = o1.numel() if r["op"] == "retrieve" else 0
+        reqs.append((tables[r["table"]], OPS[r["op"]] + (0 if r["table"] < w["num_maps"] else 0),
+                     r["input"].data_ptr(), r["n"], o1.data_ptr() if o1 is not None else None,
+                     o2.data_ptr() if o2 is not None else None, cap))
+    return reqs
+
+
+@pytest.mark.parametrize("qps", [2000.0, 200000.0])
+def test_batched_serving_equals_one_by_one(qps):
+    from mxco import workloads
+    w = workloads.serving(num_maps=6, num_multimaps=2, slots_log2=(12, 16), num_requests=300, qps=qps,
+                          batch_log2=(0, 10), seed=11)
+    s1, t1 = build(w, max_tick_keys=1 << 14)
+    s2, t2 = build(w)
+    o1, o2 = outputs(w), outputs(w)
+    r1, lat1 = s1.replay(plan(w, t1, o1), w["arrivals_us"])
+    r2, lat2 = s2.replay(plan(w, t2, o2), w["arrivals_us"], plain=True)
+    assert r1 == r2
+    for req, a, b in zip(w["requests"], o1, o2):
+        if a[0] is not None:
+            assert torch.equal(a[0], b[0]), req["op"]
+        if a[1] is not None:
+            assert torch.equal(a[1], b[1])
+    ticks, batched = s1.stats()
+    assert ticks >= 1 and batched >= 1
+    if qps > 100000:
+        assert batched > ticks  # requests were folded together
+    assert all(x >= 0 for x in lat1)
+    s1.close()
+    s2.close()
+
+
+def test_find_results_against_construction():
+    """Map finds return key*7+1 for present keys and -1 for absent ones."""
+    import mxco
+    from mxco import workloads
+    w = workloads.serving(num_maps=3, num_multimaps=1, slots_log2=(12, 13), num_requests=60, qps=100000.0,
+                          batch_log2=(0, 9), seed=3)
+    w["requests"] = [r for r in w["requests"] if r["op"] in ("find", "contains")]
+    w["arrivals_us"] = w["arrivals_us"][:len(w["requests"])]
+    s, t = build(w)
+    o = outputs(w)
+    s.replay(plan(w, t, o), w["arrivals_us"])
+    for req, (a, _) in zip(w["requests"], o):
+        ref = mxco.StaticMap(w["caps"][req["table"]], -1, -1, torch.int64, torch.int64)
+        ref.insert(w["init"][req["table"]])
+        if req["op"] == "find":
+            assert torch.equal(a, ref.find(req["input"]))
+        else:
+            assert torch.equal(a.bool(), ref.contains(req["input"]))
+    s.close()
+
+
+def test_admission_by_budget():
+    import mxco
+    s = mxco.Server(hbm_budget_bytes=64 << 20)
+    s.add_map(1 << 20)  # 16 MiB of slots
+    with pytest.raises(MemoryError):
+        s.add_map(1 << 23)  # 128 MiB: over the budget
+    t = s.add_map(1 << 20)
+    s.drop_table(t)
+    s.close()
