@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
@@ -192,6 +193,69 @@ def run_batched(args, torch, mxco, steps=3, warmup=1):
             "tune_ms": {k: round(v, 3) for k, v in tuned.items()}}
 
 
+def run_serving(args, torch, mxco, qps_list=(5000.0, 50000.0), num_requests=2000):
+    """64 resident tables (48 StaticMap + 16 StaticMultimap int64, 2^20..2^26
+    slots, half full), open-loop Poisson request stream (80 % Find/Contains,
+    15 % Insert, 5 % Retrieve; batches log-uniform 1..2^16 keys, 70 % hit):
+    the continuously batching server against one-at-a-time plain calls."""
+    import numpy as np
+    from mxco import workloads
+    ops = {"insert": mxco.REQ_INSERT, "find": mxco.REQ_FIND, "contains": mxco.REQ_CONTAINS,
+           "retrieve": mxco.REQ_MM_RETRIEVE}
+    out = {}
+    for qps in qps_list:
+        w = workloads.serving(num_requests=num_requests, qps=qps, seed=5)
+        res = {}
+        outs_by_mode = {}
+        for mode in ("batched", "plain"):
+            srv = mxco.Server()
+            tabs = [srv.add_map(c) if t < w["num_maps"] else srv.add_multimap(c) for t, c in enumerate(w["caps"])]
+            for t, pairs in enumerate(w["init"]):
+                op = mxco.REQ_INSERT if t < w["num_maps"] else mxco.REQ_MM_INSERT
+                srv.wait(srv.submit(tabs[t], op, pairs))
+            outs, reqs = [], []
+            for r in w["requests"]:
+                n = r["n"]
+                if r["op"] == "retrieve":
+                    o1 = torch.empty(9 * n, dtype=torch.int64, device="cuda")
+                    o2 = torch.empty(9 * n, dtype=torch.int64, device="cuda")
+                elif r["op"] == "find":
+                    o1, o2 = torch.empty(n, dtype=torch.int64, device="cuda"), None
+                elif r["op"] == "contains":
+                    o1, o2 = torch.empty(n, dtype=torch.uint8, device="cuda"), None
+                else:
+                    o1 = o2 = None
+                outs.append((o1, o2))
+                reqs.append((tabs[r["table"]], ops[r["op"]], r["input"].data_ptr(), n,
+                             o1.data_ptr() if o1 is not None else None, o2.data_ptr() if o2 is not None else None,
+                             o1.numel() if r["op"] == "retrieve" else 0))
+            torch.cuda.synchronize()
+            results, lat = srv.replay(reqs, w["arrivals_us"], plain=(mode == "plain"))
+            lat = np.array(lat) / 1e3
+            done = np.array(w["arrivals_us"]) / 1e3 + lat
+            keys = sum(r["n"] for r in w["requests"])
+            ticks, folded = srv.stats()
+            res[mode] = {"p50_ms": round(float(np.percentile(lat, 50)), 4),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                         "gkeys_s": round(keys / (done.max() * 1e-3) / 1e9, 4),
+                         "ticks": ticks, "requests_per_tick": round(folded / max(ticks, 1), 2)}
+            outs_by_mode[mode] = (results, outs)
+            srv.close()
+            del srv
+        same = outs_by_mode["batched"][0] == outs_by_mode["plain"][0]
+        for (a1, a2), (b1, b2) in zip(outs_by_mode["batched"][1], outs_by_mode["plain"][1]):
+            if a1 is not None:
+                same &= bool(torch.equal(a1, b1))
+            if a2 is not None:
+                same &= bool(torch.equal(a2, b2))
+        res["identical_results"] = bool(same)
+        res["keys"] = sum(r["n"] for r in w["requests"])
+        out["qps_%d" % int(qps)] = res
+        del w, outs_by_mode
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(args):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -304,6 +368,10 @@ def main():
         log("bench: batched 1024 tables")
         line["batched"] = run_batched(args, torch, mxco)
         log("bench: batched %s" % json.dumps(line["batched"]))
+    if not args.no_serving:
+        log("bench: serving")
+        line["serving"] = run_serving(args, torch, mxco)
+        log("bench: serving %s" % json.dumps(line["serving"]))
     if not args.no_cpu:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(args)
