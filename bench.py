@@ -193,7 +193,7 @@ def run_batched(args, torch, mxco, steps=3, warmup=1):
             "tune_ms": {k: round(v, 3) for k, v in tuned.items()}}
 
 
-def run_serving(args, torch, mxco, qps_list=(5000.0, 50000.0), num_requests=2000):
+def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_requests=3000):
     """64 resident tables (48 StaticMap + 16 StaticMultimap int64, 2^20..2^26
     slots, half full), open-loop Poisson request stream (80 % Find/Contains,
     15 % Insert, 5 % Retrieve; batches log-uniform 1..2^16 keys, 70 % hit):
@@ -217,8 +217,8 @@ def run_serving(args, torch, mxco, qps_list=(5000.0, 50000.0), num_requests=2000
             for r in w["requests"]:
                 n = r["n"]
                 if r["op"] == "retrieve":
-                    o1 = torch.empty(9 * n, dtype=torch.int64, device="cuda")
-                    o2 = torch.empty(9 * n, dtype=torch.int64, device="cuda")
+                    o1 = torch.zeros(9 * n, dtype=torch.int64, device="cuda")
+                    o2 = torch.zeros(9 * n, dtype=torch.int64, device="cuda")
                 elif r["op"] == "find":
                     o1, o2 = torch.empty(n, dtype=torch.int64, device="cuda"), None
                 elif r["op"] == "contains":

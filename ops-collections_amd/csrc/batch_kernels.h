@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/mxco.h"
 #include "oa_kernels.h"
 
@@ -26,7 +28,7 @@ constexpr uint32_t kChunk = 2048;  // keys per chunk
 // Work item of the serving engine: explicit device pointers per request.
 struct ServeItem {
     uint32_t table;
-    uint32_t reserved;
+    uint32_t op;  // read launches: MXCO_REQ_FIND or MXCO_REQ_CONTAINS
     const uint8_t* in;
     uint8_t* out;
     uint64_t count;
@@ -74,7 +76,15 @@ __global__ __launch_bounds__(256) void oa_batch_kernel(OaArgs base, const BatchT
         item_io<Item>(it, base, IN_STRIDE, OUT_STRIDE, a);
         a.n = it.count;
         unsigned long long c0 = 0, c1 = 0;
-        oa_process<KS, VS, G, OP>(a, ch.start + gj, ch.start + ch.len, gpb, c0, c1);
+        if constexpr (std::is_same<Item, ServeItem>::value && OP == OP_FIND) {
+            // one read launch serves Find and Contains items (uniform per chunk)
+            if (it.op == MXCO_REQ_CONTAINS)
+                oa_process<KS, VS, G, OP_CONTAINS>(a, ch.start + gj, ch.start + ch.len, gpb, c0, c1);
+            else
+                oa_process<KS, VS, G, OP_FIND>(a, ch.start + gj, ch.start + ch.len, gpb, c0, c1);
+        } else {
+            oa_process<KS, VS, G, OP>(a, ch.start + gj, ch.start + ch.len, gpb, c0, c1);
+        }
         if (item_counters) flush_counters<OP>(item_counters + 2ull * ch.item, c0, c1);
     }
 }

@@ -65,23 +65,17 @@ struct Table {
 constexpr int kTickSlots = 4;
 
 struct Tick {
-    ServeItem* items_host = nullptr;
-    ServeItem* items_dev = nullptr;
-    BatchChunk* chunks_host = nullptr;
-    BatchChunk* chunks_dev = nullptr;
-    unsigned long long* counters_dev = nullptr;
-    unsigned long long* counters_host = nullptr;
-    uint32_t item_cap = 0, chunk_cap = 0;
+    // One upload per tick: [items | chunks | zeroed insert counters].
+    uint8_t* blob_host = nullptr;  // pinned
+    uint8_t* blob_dev = nullptr;
+    size_t blob_cap = 0;
+    unsigned long long* counters_host = nullptr;  // pinned read-back of the insert counters
+    uint32_t counters_cap = 0;
     hipEvent_t done = nullptr;
     std::vector<Req*> reqs;
     std::vector<uint32_t> insert_items;  // request index of each insert item
     bool busy = false;
 };
-
-__global__ void zero_counters_u64(unsigned long long* p, uint32_t n)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
 
 }  // namespace
 
@@ -110,28 +104,24 @@ struct mxco_server {
 
 namespace {
 
-mxco_status grow_tick(Tick& t, uint32_t items, uint32_t chunks)
+mxco_status grow_tick(Tick& t, size_t bytes, uint32_t counters)
 {
-    if (items > t.item_cap) {
-        if (t.items_host) (void)hipHostFree(t.items_host);
-        if (t.items_dev) (void)hipFree(t.items_dev);
-        if (t.counters_dev) (void)hipFree(t.counters_dev);
-        if (t.counters_host) (void)hipHostFree(t.counters_host);
-        uint32_t cap = std::max(items, 2 * t.item_cap);
-        MXCO_HIP(hipHostMalloc((void**)&t.items_host, cap * sizeof(ServeItem), 0), "server: tick buffers");
-        MXCO_HIP(hipMalloc((void**)&t.items_dev, cap * sizeof(ServeItem)), "server: tick buffers");
-        MXCO_HIP(hipMalloc((void**)&t.counters_dev, 2ull * cap * sizeof(unsigned long long)), "server: tick buffers");
-        MXCO_HIP(hipHostMalloc((void**)&t.counters_host, 2ull * cap * sizeof(unsigned long long), 0),
-                 "server: tick buffers");
-        t.item_cap = cap;
+    if (bytes > t.blob_cap) {
+        if (t.blob_host) (void)hipHostFree(t.blob_host);
+        if (t.blob_dev) (void)hipFree(t.blob_dev);
+        t.blob_host = nullptr;
+        t.blob_dev = nullptr;
+        const size_t cap = std::max(bytes, 2 * t.blob_cap);
+        MXCO_HIP(hipHostMalloc((void**)&t.blob_host, cap, 0), "server: tick buffers");
+        MXCO_HIP(hipMalloc((void**)&t.blob_dev, cap), "server: tick buffers");
+        t.blob_cap = cap;
     }
-    if (chunks > t.chunk_cap) {
-        if (t.chunks_host) (void)hipHostFree(t.chunks_host);
-        if (t.chunks_dev) (void)hipFree(t.chunks_dev);
-        uint32_t cap = std::max(chunks, 2 * t.chunk_cap);
-        MXCO_HIP(hipHostMalloc((void**)&t.chunks_host, cap * sizeof(BatchChunk), 0), "server: tick buffers");
-        MXCO_HIP(hipMalloc((void**)&t.chunks_dev, cap * sizeof(BatchChunk)), "server: tick buffers");
-        t.chunk_cap = cap;
+    if (counters > t.counters_cap) {
+        if (t.counters_host) (void)hipHostFree(t.counters_host);
+        t.counters_host = nullptr;
+        const uint32_t cap = std::max(counters, 2 * t.counters_cap);
+        MXCO_HIP(hipHostMalloc((void**)&t.counters_host, cap * sizeof(unsigned long long), 0), "server: counters");
+        t.counters_cap = cap;
     }
     return MXCO_OK;
 }
@@ -152,56 +142,71 @@ OaArgs map_args(const mxco_server* s)
 
 bool is_map_op(uint32_t op) { return op == MXCO_REQ_INSERT || op == MXCO_REQ_FIND || op == MXCO_REQ_CONTAINS; }
 
-// Runs one tick's requests (already chosen) on the map stream.
+// Keys per chunk: small ticks are cut finer so that they still spread over
+// many workgroups; large ticks use the batched path's chunk.
+uint32_t tick_chunk(uint64_t keys)
+{
+    const uint64_t target = keys / (4ull * (uint64_t)cu_count());
+    uint32_t c = 64;
+    while (c < kChunk && c * 2 <= target) c *= 2;
+    return c;
+}
+
+// Runs one tick's requests (already chosen) on the map stream: one upload,
+// one insert launch, one read launch (Find and Contains), one read-back.
 mxco_status launch_tick(mxco_server* s, Tick& t)
 {
-    // items grouped by op: inserts first, then finds, then contains
-    std::vector<uint32_t> order[3];
+    std::vector<uint32_t> ins, rd;
+    uint64_t keys = 0;
     for (uint32_t i = 0; i < t.reqs.size(); ++i) {
-        const uint32_t op = t.reqs[i]->r.op;
-        order[op == MXCO_REQ_INSERT ? 0 : (op == MXCO_REQ_FIND ? 1 : 2)].push_back(i);
+        (t.reqs[i]->r.op == MXCO_REQ_INSERT ? ins : rd).push_back(i);
+        keys += t.reqs[i]->r.n;
     }
-    uint32_t nitems = (uint32_t)t.reqs.size();
+    const uint32_t csz = tick_chunk(keys);
+    const uint32_t nitems = (uint32_t)t.reqs.size();
     uint32_t nchunks = 0;
-    for (Req* r : t.reqs) nchunks += (uint32_t)((r->r.n + kChunk - 1) / kChunk);
-    if (mxco_status e = grow_tick(t, nitems, nchunks)) return e;
-    uint32_t item = 0, chunk = 0;
-    uint32_t first_chunk[3], n_chunks[3];
+    for (Req* r : t.reqs) nchunks += (uint32_t)((r->r.n + csz - 1) / csz);
+    const uint32_t ninsert = (uint32_t)ins.size();
+    const size_t off_chunks = (nitems * sizeof(ServeItem) + 15) & ~size_t(15);
+    const size_t off_counters = (off_chunks + nchunks * sizeof(BatchChunk) + 15) & ~size_t(15);
+    const size_t bytes = off_counters + 2ull * ninsert * sizeof(unsigned long long);
+    if (mxco_status e = grow_tick(t, bytes, 2 * ninsert + 2)) return e;
+    ServeItem* items = (ServeItem*)t.blob_host;
+    BatchChunk* chunks = (BatchChunk*)(t.blob_host + off_chunks);
+    memset(t.blob_host + off_counters, 0, 2ull * ninsert * sizeof(unsigned long long));
+    uint32_t item = 0, chunk = 0, first_read = 0;
     t.insert_items.clear();
-    for (int k = 0; k < 3; ++k) {
-        first_chunk[k] = chunk;
-        for (uint32_t ri : order[k]) {
+    for (int k = 0; k < 2; ++k) {
+        if (k == 1) first_read = chunk;
+        for (uint32_t ri : (k == 0 ? ins : rd)) {
             const mxco_request& r = t.reqs[ri]->r;
-            t.items_host[item] = {r.table, 0, (const uint8_t*)r.in, (uint8_t*)r.out, r.n};
+            items[item] = {r.table, r.op, (const uint8_t*)r.in, (uint8_t*)r.out, r.n};
             if (k == 0) t.insert_items.push_back(ri);
-            for (uint64_t st = 0; st < r.n; st += kChunk)
-                t.chunks_host[chunk++] = {item, (uint32_t)std::min<uint64_t>(kChunk, r.n - st), st};
+            for (uint64_t st = 0; st < r.n; st += csz)
+                chunks[chunk++] = {item, (uint32_t)std::min<uint64_t>(csz, r.n - st), st};
             ++item;
         }
-        n_chunks[k] = chunk - first_chunk[k];
     }
     hipStream_t st = s->map_stream;
-    MXCO_HIP(hipMemcpyAsync(t.items_dev, t.items_host, nitems * sizeof(ServeItem), hipMemcpyHostToDevice, st),
-             "server: upload items");
-    MXCO_HIP(hipMemcpyAsync(t.chunks_dev, t.chunks_host, chunk * sizeof(BatchChunk), hipMemcpyHostToDevice, st),
-             "server: upload chunks");
-    const uint32_t ninsert = (uint32_t)order[0].size();
-    if (ninsert) {
-        zero_counters_u64<<<1, 256, 0, st>>>(t.counters_dev, 2 * ninsert);
-        MXCO_HIP(hipGetLastError(), "server: zero counters");
-    }
+    MXCO_HIP(hipMemcpyAsync(t.blob_dev, t.blob_host, bytes, hipMemcpyHostToDevice, st), "server: upload");
     const OaArgs a = map_args(s);
-    const int ops[3] = {OP_INSERT, OP_FIND, OP_CONTAINS};
-    for (int k = 0; k < 3; ++k) {
-        if (!n_chunks[k]) continue;
-        const unsigned grid = std::max(1u, std::min<unsigned>(n_chunks[k], (unsigned)cu_count() * s->cfg.blocks_per_cu));
-        MXCO_HIP(launch_batch_any(s->map_desc.key_size, s->map_desc.value_size, s->cfg.group_size, ops[k], true, a,
-                                  s->tables_dev, t.items_dev, t.chunks_dev + first_chunk[k], n_chunks[k],
-                                  k == 0 ? t.counters_dev : nullptr, grid, st),
-                 "server: launch");
-    }
+    const ServeItem* items_dev = (const ServeItem*)t.blob_dev;
+    const BatchChunk* chunks_dev = (const BatchChunk*)(t.blob_dev + off_chunks);
+    unsigned long long* counters_dev = (unsigned long long*)(t.blob_dev + off_counters);
+    const uint32_t n_ins_chunks = first_read, n_rd_chunks = chunk - first_read;
+    const unsigned cap = (unsigned)cu_count() * s->cfg.blocks_per_cu;
+    if (n_ins_chunks)
+        MXCO_HIP(launch_batch_any(s->map_desc.key_size, s->map_desc.value_size, s->cfg.group_size, OP_INSERT, true,
+                                  a, s->tables_dev, items_dev, chunks_dev, n_ins_chunks, counters_dev,
+                                  std::min(n_ins_chunks, cap), st),
+                 "server: insert launch");
+    if (n_rd_chunks)
+        MXCO_HIP(launch_batch_any(s->map_desc.key_size, s->map_desc.value_size, s->cfg.group_size, OP_FIND, true, a,
+                                  s->tables_dev, items_dev, chunks_dev + first_read, n_rd_chunks, nullptr,
+                                  std::min(n_rd_chunks, cap), st),
+                 "server: read launch");
     if (ninsert)
-        MXCO_HIP(hipMemcpyAsync(t.counters_host, t.counters_dev, 2ull * ninsert * sizeof(unsigned long long),
+        MXCO_HIP(hipMemcpyAsync(t.counters_host, counters_dev, 2ull * ninsert * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, st),
                  "server: read counters");
     MXCO_HIP(hipEventRecord(t.done, st), "server: record");
@@ -292,7 +297,9 @@ void completer_loop(mxco_server* s)
         }
         Tick* t = s->inflight.front();
         lk.unlock();
-        const hipError_t e = hipEventSynchronize(t->done);
+        hipError_t e;
+        // poll: completion is noticed within microseconds of the event
+        while ((e = hipEventQuery(t->done)) == hipErrorNotReady) std::this_thread::yield();
         lk.lock();
         s->inflight.pop_front();
         const double tnow = now_us();
@@ -430,11 +437,8 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
         if (t.mm) (void)mxco_mm_destroy(t.mm);
     }
     for (Tick& t : s->ticks) {
-        if (t.items_host) (void)hipHostFree(t.items_host);
-        if (t.items_dev) (void)hipFree(t.items_dev);
-        if (t.chunks_host) (void)hipHostFree(t.chunks_host);
-        if (t.chunks_dev) (void)hipFree(t.chunks_dev);
-        if (t.counters_dev) (void)hipFree(t.counters_dev);
+        if (t.blob_host) (void)hipHostFree(t.blob_host);
+        if (t.blob_dev) (void)hipFree(t.blob_dev);
         if (t.counters_host) (void)hipHostFree(t.counters_host);
         if (t.done) (void)hipEventDestroy(t.done);
     }
