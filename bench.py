@@ -9,6 +9,7 @@ device-resident before the timed region.  Prints one JSON line.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -34,6 +35,9 @@ def parse():
     ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--no-streams", action="store_true", help="skip the Zipf duplicate-heavy insert stream")
+    ap.add_argument("--no-filters", action="store_true", help="skip the BloomFilter and HyperLogLog legs")
+    ap.add_argument("--hll-sketches", type=int, default=64)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -120,9 +124,160 @@ def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
         ln_f, sl_f = m.probe_stats(q)
         ln_i, sl_i = m.probe_stats(keys)
         res.update({"find_lines": ln_f, "find_slots": sl_f, "insert_lines": ln_i, "insert_slots": sl_i})
+        res["insert_or_assign"] = run_insert_or_assign(torch, mxco, m, keys, steps)
     del m
     torch.cuda.empty_cache()
     return res
+
+
+def _timed(torch, fn, steps):
+    """Mean milliseconds of fn() over `steps` calls, bracketed by events on the current stream."""
+    s = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(s)
+        fn()
+        b.record(s)
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / steps
+
+
+def run_insert_or_assign(torch, mxco, m, keys, steps):
+    """InsertOrAssign of 2^26 distinct keys already in the table, new value key*3+5."""
+    n = min(1 << 26, keys.numel())
+    k = keys[:n]
+    pairs = mxco.make_pairs(k, k * 3 + 5)
+    m.insert_or_assign_async(pairs)
+    ms = _timed(torch, lambda: m.insert_or_assign_async(pairs), steps)
+    failed, assigned = m.insert_or_assign(pairs)
+    ok = failed == 0 and assigned == n and bool(torch.equal(m.find(k), k * 3 + 5))
+    return {"n": n, "ms": round(ms, 4), "gkeys_s": round(n / ms / 1e6, 4), "failed": failed, "assigned": assigned,
+            "verified": ok}
+
+
+def run_zipf_insert(args, torch, mxco, steps=3):
+    """Duplicate-heavy insert stream: 2^27 draws of Zipf(s=1.1) over 2^24
+    distinct keys into an empty table of the same 2^28 slots (value = key*7+1,
+    so first-writer-wins races leave the same table)."""
+    from mxco.workloads import Zipf
+    n, distinct_max = 1 << 27, 1 << 24
+    z = Zipf(distinct_max, 1.1)
+    ranks = z.draw(n, seed=11)
+    seen = torch.zeros(distinct_max, dtype=torch.bool, device="cuda")
+    seen[ranks] = True
+    distinct = int(seen.sum().item())
+    keys = Zipf.keys(ranks)
+    del ranks, z
+    pairs = mxco.make_pairs(keys, keys * 7 + 1)
+    m = mxco.StaticMap(1 << args.slots_log2, -1, -1, torch.int64, torch.int64)
+
+    def step():
+        m.clear_async()
+        m.insert_async(pairs)
+
+    step()
+    s = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+    for e in ev:
+        m.clear_async()
+        e[0].record(s)
+        m.insert_async(pairs)
+        e[1].record(s)
+    torch.cuda.synchronize()
+    ms = sum(a.elapsed_time(b) for a, b in ev) / steps
+    m.clear()
+    not_inserted = m.insert(pairs)
+    uk = Zipf.keys(torch.nonzero(seen).flatten())
+    ok = not_inserted == n - distinct and bool(torch.equal(m.find(uk), uk * 7 + 1))
+    del m, pairs, keys
+    torch.cuda.empty_cache()
+    return {"draws": n, "distinct": distinct, "s": 1.1, "ms": round(ms, 4), "gkeys_s": round(n / ms / 1e6, 4),
+            "duplicates_reported": not_inserted, "verified": ok}
+
+
+def run_bloom(torch, mxco, steps=5):
+    """BloomFilter of 2^22 blocks x 32 B (2^30 bits): Add 2^27 uniform uint64
+    keys, Contains 2^27 queries (first half added, second half absent)."""
+    nb, n = 1 << 22, 1 << 27
+    g = torch.Generator(device="cuda")
+    g.manual_seed(21)
+    keys = torch.randint(-(1 << 63), (1 << 63) - 1, (n,), dtype=torch.int64, device="cuda", generator=g)
+    absent = torch.randint(-(1 << 63), (1 << 63) - 1, (n - n // 2,), dtype=torch.int64, device="cuda", generator=g)
+    q = torch.cat([keys[: n // 2], absent])
+    out = torch.empty(n, dtype=torch.bool, device="cuda")
+    bf = mxco.BloomFilter(nb, torch.int64)
+    bf.add(keys)
+    s = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for e in ev:
+        bf.clear(sync=False)
+        e[0].record(s)
+        bf.add_async(keys)
+        e[1].record(s)
+        bf.contains(q, out=out, sync=False)
+        e[2].record(s)
+    torch.cuda.synchronize()
+    add_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    con_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    no_false_neg = bool(out[: n // 2].all().item())
+    fpr = out[n // 2:].float().mean().item()
+    add_gbs = n * (32 + 32 + 8) / (add_ms * 1e-3) / 1e9
+    con_gbs = n * (32 + 8 + 1) / (con_ms * 1e-3) / 1e9
+    del bf
+    return {"blocks": nb, "keys": n, "add_ms": round(add_ms, 4), "contains_ms": round(con_ms, 4),
+            "add_gkeys_s": round(n / add_ms / 1e6, 3), "contains_gkeys_s": round(n / con_ms / 1e6, 3),
+            "fpr": round(fpr, 5), "no_false_negatives": no_false_neg,
+            "roofline_add": {"bound": "hbm", "achieved": round(add_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(add_gbs / PEAK_HBM_GBS, 4), "bytes_per_key": 72},
+            "roofline_contains": {"bound": "hbm", "achieved": round(con_gbs, 1), "peak": PEAK_HBM_GBS,
+                                  "unit": "GB/s", "frac": round(con_gbs / PEAK_HBM_GBS, 4), "bytes_per_key": 41}}
+
+
+def run_hll(torch, mxco, sketches=64, n=1 << 28):
+    """HyperLogLog with a 32 KiB sketch (p = 15): `sketches` sketches of n
+    int64 keys each, Zipf(s=1.0) over 1e8 distinct values with disjoint seeds,
+    merged into one; the estimate is compared with the exact distinct count."""
+    from mxco.workloads import Zipf
+    universe = 10 ** 8
+    z = Zipf(universe, 1.0)
+    seen = torch.zeros(universe, dtype=torch.bool, device="cuda")
+    total = mxco.HyperLogLog.create_with_sketch_size_kb(32, torch.int64)
+    h = mxco.HyperLogLog(total.precision, torch.int64)
+    regmax = torch.zeros(1 << total.precision, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream()
+    add_ms, merge_ms = [], []
+    for k in range(sketches):
+        r = z.draw(n, seed=1000 + k)
+        seen[r] = True
+        keys = Zipf.keys(r)
+        del r
+        h.clear()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(s)
+        h.add(keys)
+        e[1].record(s)
+        total.merge(h)
+        e[2].record(s)
+        torch.cuda.synchronize()
+        regmax = torch.maximum(regmax, h.registers().view(torch.uint8))
+        add_ms.append(e[0].elapsed_time(e[1]))
+        merge_ms.append(e[1].elapsed_time(e[2]))
+        del keys
+    est = total.estimate()
+    exact = int(seen.sum().item())
+    merged_ok = bool(torch.equal(total.registers().view(torch.uint8), regmax))
+    ms = sum(add_ms[1:]) / max(1, len(add_ms) - 1)  # first add includes one-time set-up
+    gbs = n * 8 / (ms * 1e-3) / 1e9
+    rel = abs(est - exact) / exact
+    tol = 3 * 1.04 / math.sqrt(1 << total.precision)
+    del z, seen
+    torch.cuda.empty_cache()
+    return {"precision": total.precision, "sketches": sketches, "keys_per_sketch": n, "add_ms": round(ms, 4),
+            "add_gkeys_s": round(n / ms / 1e6, 3), "merge_ms": round(sum(merge_ms) / len(merge_ms), 4),
+            "estimate": est, "exact_distinct": exact, "rel_error": round(rel, 5), "tolerance": round(tol, 5),
+            "merge_is_register_max": merged_ok, "within_tolerance": rel <= tol,
+            "roofline_add": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_key": 8}}
 
 
 def run_batched(args, torch, mxco, steps=3, warmup=1):
@@ -293,7 +448,7 @@ def cpu_baseline(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (see DESIGN.md on sharding)"
+        print("bench.py: one GPU only; --gpus %d is not supported (the tables shard by key hash across GPUs, not built here)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.blocks_per_cu:
@@ -364,6 +519,18 @@ def main():
                            "find_gkeys_s": round(x["n_find"] / x["find_ms"] / 1e6, 3)})
                 log("sweep %s" % json.dumps(sw[-1]))
         line["sweep"] = sw
+    if not args.no_streams:
+        log("bench: zipf insert stream")
+        line["zipf_insert"] = run_zipf_insert(args, torch, mxco)
+        log("bench: zipf %s" % json.dumps(line["zipf_insert"]))
+    line["insert_or_assign"] = r["insert_or_assign"]
+    if not args.no_filters:
+        log("bench: bloom filter")
+        line["bloom"] = run_bloom(torch, mxco)
+        log("bench: bloom %s" % json.dumps(line["bloom"]))
+        log("bench: hyperloglog")
+        line["hll"] = run_hll(torch, mxco, sketches=args.hll_sketches)
+        log("bench: hll %s" % json.dumps(line["hll"]))
     if not args.no_batched:
         log("bench: batched 1024 tables")
         line["batched"] = run_batched(args, torch, mxco)

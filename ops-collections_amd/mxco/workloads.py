@@ -117,3 +117,37 @@ def serving(num_maps=48, num_multimaps=16, slots_log2=(20, 26), fill=0.5, num_re
             inp = keys_of(t, torch.where(hitm, jh, jm)).contiguous()
         reqs.append({"table": t, "op": op, "input": inp, "n": n})
     return {"caps": caps, "num_maps": num_maps, "init": init, "requests": reqs, "arrivals_us": arrivals.tolist()}
+
+
+# Odd 64-bit multiplier (golden ratio): rank -> rank * ZIPF_MULT mod 2^64 is a
+# bijection, so distinct ranks give distinct keys, and the ranks drawn here
+# (< 2^40) never map to the sentinel -1 (that needs rank = -1/ZIPF_MULT mod 2^64).
+ZIPF_MULT = 0x9E3779B97F4A7C15 - (1 << 64)
+
+
+class Zipf:
+    """Exact bounded Zipf(s) over ranks 1..n by inverse CDF: the cumulative
+    weights k^-s (FP64, on the device) are searched with uniform FP64 draws.
+
+    draw(count, seed) returns rank-1 in [0, n) as int64; keys(...) maps ranks to
+    64-bit keys by the bijection above."""
+
+    def __init__(self, n, s, device="cuda"):
+        w = torch.arange(1, n + 1, dtype=torch.float64, device=device).pow_(-s)
+        self.cdf = torch.cumsum(w, 0)
+        del w
+        self.cdf /= self.cdf[-1].clone()
+        self.n = n
+        self.device = device
+
+    def draw(self, count, seed):
+        g = torch.Generator(device=self.device)
+        g.manual_seed(seed)
+        u = torch.rand(count, dtype=torch.float64, device=self.device, generator=g)
+        r = torch.searchsorted(self.cdf, u)
+        del u
+        return r.clamp_(max=self.n - 1)
+
+    @staticmethod
+    def keys(ranks):
+        return ranks * ZIPF_MULT

@@ -291,3 +291,24 @@ def test_large_table_properties(group):
     absent = keys + 2**62  # disjoint from the inserted range
     assert m.count(absent) == 0
     assert m.count(keys) == n
+
+
+@pytest.mark.parametrize("group", GROUPS)
+def test_zipf_duplicate_heavy_insert(group):
+    """Duplicate-heavy stream (2^20 draws of Zipf(s=1.1) over 2^14 values):
+    the not-inserted count and every Find equal the oracle's sequential
+    insert of the same stream (value = f(key), so racing duplicates agree)."""
+    from mxco.workloads import Zipf
+    z = Zipf(1 << 14, 1.1)
+    keys_t = Zipf.keys(z.draw(1 << 20, seed=3))
+    keys = keys_t.cpu().numpy()
+    vals = keys * 7 + 1
+    ref = O.OpenAddressing(np.int64, np.int64, 1 << 16, -1, -1)
+    ref_failed = ref.insert(keys, vals)
+    assert ref_failed == len(keys) - len(np.unique(keys))
+    m = mx().StaticMap(1 << 16, -1, -1, torch.int64, torch.int64, group_size=group)
+    assert m.insert(dev_pairs(keys, vals)) == ref_failed
+    q = np.concatenate([np.unique(keys), Zipf.keys(torch.arange(1 << 14, 1 << 15)).numpy()])
+    ref_out, ref_con = ref.find(q)
+    assert np.array_equal(m.find(dev(q)).cpu().numpy(), ref_out)
+    assert np.array_equal(m.contains(dev(q)).cpu().numpy(), ref_con)
