@@ -6,14 +6,26 @@
 //   block  = (hi32(h) * numBlocks) >> 32              (BlockIndex, :102-107)
 //   word w = 1 << ((salt[w] * lo32(h)) >> 27)        (WordPatternImpl, :157-168)
 //
-// Add reads the 32-byte block once and issues 64-bit atomic ORs only for the
-// word pairs that still miss bits (OR is idempotent and order-free, so the
-// words are the same whatever the schedule).  Contains reads the block with
-// two 16-byte loads.  Merge / Intersect are word-wise OR / AND streams.
+// Add has two paths; OR is idempotent and order-free, so both leave the same
+// words whatever the schedule.
+//  * direct (small batches): read the 32-byte block once, then 64-bit atomic
+//    ORs only for the word pairs that still miss bits;
+//  * partitioned (at least half a key per block): the filter is cut into
+//    ranges of 2048 blocks (64 KiB).  A count pass builds per-segment range
+//    histograms, a scan turns them into offsets, a scatter pass writes each
+//    key as (block within range, low hash word) into its range's bin, and an
+//    apply pass loads each touched range into LDS, ORs its bin there and
+//    writes the range back once.  Per key this streams 32 bytes instead of
+//    four random global atomics.
+// Contains reads the block with two 16-byte loads.  Merge / Intersect are
+// word-wise OR / AND streams.
+#include <stdlib.h>
 #include <string.h>
 
 #include <new>
 #include <string>
+
+#include <hipcub/hipcub.hpp>
 
 #include "mxco_common.h"
 #include "mxco_hash.h"
@@ -25,6 +37,8 @@ struct mxco_bloom {
     uint64_t num_blocks;
     uint32_t key_size;
     uint64_t seed;
+    uint8_t* ws;  // partitioned-Add workspace (grown on demand)
+    size_t ws_bytes;
 };
 
 namespace {
@@ -98,6 +112,106 @@ __global__ __launch_bounds__(256) void bloom_contains_kernel(const uint32_t* wor
     }
 }
 
+constexpr uint32_t kRangeShift = 11;  // 2048 blocks x 32 B = 64 KiB of LDS per range
+constexpr uint32_t kRangeBlocks = 1u << kRangeShift;
+constexpr uint32_t kMaxRanges = 8192;             // range histogram in LDS: 32 KiB
+constexpr uint64_t kPassKeys = 1ull << 27;        // keys per partition pass (1 GiB of bin entries)
+constexpr int kUnroll = 4;                        // key loads in flight per lane
+
+// Visits the keys of this workgroup's contiguous segment that pass the
+// stencil, kUnroll loads in flight per lane: f(block, low hash word).
+template <int KS, typename F>
+__device__ __forceinline__ void for_segment(const uint8_t* keys, const uint8_t* stencil, uint64_t n, uint32_t nb,
+                                            uint64_t seed, F&& f)
+{
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t begin = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(n, begin + per);
+    for (uint64_t i0 = begin + threadIdx.x; i0 < end; i0 += (uint64_t)kUnroll * blockDim.x) {
+        uint64_t k[kUnroll];
+        bool live[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+            live[u] = i < end;
+            k[u] = live[u] ? key_at<KS>(keys, i) : 0;
+            if (live[u] && stencil) live[u] = stencil[i] != 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (!live[u]) continue;
+            const uint64_t h = xxhash64<KS>(k[u], seed);
+            f((uint32_t)(((uint64_t)(uint32_t)(h >> 32) * nb) >> 32), (uint32_t)h);
+        }
+    }
+}
+
+// counts[r * segments + segment] = keys of the segment that fall in range r.
+template <int KS>
+__global__ __launch_bounds__(256) void bloom_count_kernel(uint32_t nb, const uint8_t* keys, const uint8_t* stencil,
+                                                          uint64_t n, uint64_t seed, uint32_t nranges,
+                                                          uint32_t* counts)
+{
+    extern __shared__ uint32_t hist[];
+    for (uint32_t j = threadIdx.x; j < nranges; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    for_segment<KS>(keys, stencil, n, nb, seed,
+                    [&](uint32_t blk, uint32_t) { atomicAdd(&hist[blk >> kRangeShift], 1u); });
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nranges; j += blockDim.x) counts[(uint64_t)j * gridDim.x + blockIdx.x] = hist[j];
+}
+
+// Writes each key's (block within range, low hash word) at its range's cursor.
+template <int KS>
+__global__ __launch_bounds__(256) void bloom_scatter_kernel(uint32_t nb, const uint8_t* keys, const uint8_t* stencil,
+                                                            uint64_t n, uint64_t seed, uint32_t nranges,
+                                                            const uint32_t* offsets, uint64_t* entries)
+{
+    extern __shared__ uint32_t cur[];
+    for (uint32_t j = threadIdx.x; j < nranges; j += blockDim.x) cur[j] = offsets[(uint64_t)j * gridDim.x + blockIdx.x];
+    __syncthreads();
+    for_segment<KS>(keys, stencil, n, nb, seed, [&](uint32_t blk, uint32_t lo) {
+        const uint32_t pos = atomicAdd(&cur[blk >> kRangeShift], 1u);
+        entries[pos] = ((uint64_t)(blk & (kRangeBlocks - 1)) << 32) | lo;
+    });
+}
+
+// One range at a time: load its words into LDS, OR its bin, write it back.
+// offsets[r * segments] is the first entry of range r; offsets[nranges *
+// segments] is the total.
+__global__ __launch_bounds__(1024) void bloom_apply_kernel(uint32_t* words, uint32_t nb, uint32_t nranges,
+                                                           uint32_t segments, const uint32_t* offsets,
+                                                           const uint64_t* entries)
+{
+    __shared__ uint32_t lw[kRangeBlocks * 8];
+    for (uint32_t r = blockIdx.x; r < nranges; r += gridDim.x) {
+        const uint32_t s = offsets[(uint64_t)r * segments];
+        const uint32_t e = offsets[(uint64_t)(r + 1) * segments];
+        if (s == e) continue;
+        const uint64_t b0 = (uint64_t)r << kRangeShift;
+        const uint32_t n16 = (uint32_t)min<uint64_t>(kRangeBlocks, nb - b0) * 2;  // 16-byte vectors
+        u64x2* g = (u64x2*)(words + b0 * 8);
+        u64x2* l = (u64x2*)lw;
+        for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) l[j] = g[j];
+        __syncthreads();
+        for (uint32_t k = s + threadIdx.x; k < e; k += blockDim.x) {
+            const uint64_t v = entries[k];
+            const uint32_t lo = (uint32_t)v;
+            uint32_t* w = lw + (uint32_t)(v >> 32) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) atomicOr(w + q, 1u << ((kSalts[q] * lo) >> 27));
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) g[j] = l[j];
+        __syncthreads();
+    }
+}
+
+__global__ void words_zero_tail(uint32_t* p)
+{
+    *p = 0;
+}
+
 // op 0: dst |= src, op 1: dst &= src, op 2: dst = 0.  Blocks are 32 bytes,
 // so the word count is a multiple of 8 and 16-byte vectors cover it exactly.
 __global__ __launch_bounds__(256) void bloom_words_kernel(uint32_t* dst, const uint32_t* src, uint64_t n16, int op)
@@ -114,6 +228,74 @@ __global__ __launch_bounds__(256) void bloom_words_kernel(uint32_t* dst, const u
             d[i] = op == 0 ? (a | b) : (a & b);
         }
     }
+}
+
+// 1: partitioned Add (at least half a key per block and 2^16 keys), 0: direct.
+// MXCO_BLOOM_ADD_PATH=0/1 forces a path.
+int add_path(uint64_t n, uint64_t num_blocks)
+{
+    static const int forced = [] {
+        const char* e = getenv("MXCO_BLOOM_ADD_PATH");
+        return e ? atoi(e) : -1;
+    }();
+    if (forced == 0 || forced == 1) return forced;
+    return (n >= (1u << 16) && 2 * n >= num_blocks) ? 1 : 0;
+}
+
+uint32_t segments_for(uint64_t n)
+{
+    return (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, n / 16384));
+}
+
+mxco_status ensure_ws(mxco_bloom* f, size_t bytes)
+{
+    if (bytes <= f->ws_bytes) return MXCO_OK;
+    if (f->ws) MXCO_HIP(hipFree(f->ws), "BloomFilter::Add: workspace");
+    f->ws = nullptr;
+    f->ws_bytes = 0;
+    MXCO_HIP(hipMalloc((void**)&f->ws, bytes), "BloomFilter::Add: workspace");
+    f->ws_bytes = bytes;
+    return MXCO_OK;
+}
+
+// Partitioned Add of n <= kPassKeys keys (see the file comment).
+mxco_status add_partitioned(mxco_bloom* f, const uint8_t* keys, const uint8_t* stencil, uint64_t n,
+                            hipStream_t stream)
+{
+    const uint32_t nb = (uint32_t)f->num_blocks;
+    const uint32_t nranges = (uint32_t)((f->num_blocks + kRangeBlocks - 1) >> kRangeShift);
+    const uint32_t segs = segments_for(n);
+    const uint64_t ncnt = (uint64_t)nranges * segs + 1;  // + the total
+    size_t temp = 0;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (int64_t)ncnt, stream),
+             "BloomFilter::Add: scan size");
+    const size_t off_counts = ((n * 8) + 255) & ~size_t(255);
+    const size_t off_offsets = off_counts + ((ncnt * 4 + 255) & ~size_t(255));
+    const size_t off_temp = off_offsets + ((ncnt * 4 + 255) & ~size_t(255));
+    if (mxco_status e = ensure_ws(f, off_temp + temp)) return e;
+    uint64_t* entries = (uint64_t*)f->ws;
+    uint32_t* counts = (uint32_t*)(f->ws + off_counts);
+    uint32_t* offsets = (uint32_t*)(f->ws + off_offsets);
+    const size_t lds = (size_t)nranges * 4;
+    // the slot after the last count must read 0 so the scan's last output is the total
+    words_zero_tail<<<1, 1, 0, stream>>>(counts + ncnt - 1);
+    if (f->key_size == 4)
+        bloom_count_kernel<4><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, counts);
+    else
+        bloom_count_kernel<8><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, counts);
+    MXCO_HIP(hipGetLastError(), "BloomFilter::Add: count");
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(f->ws + off_temp, temp, counts, offsets, (int64_t)ncnt, stream),
+             "BloomFilter::Add: scan");
+    if (f->key_size == 4)
+        bloom_scatter_kernel<4><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, offsets, entries);
+    else
+        bloom_scatter_kernel<8><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, offsets, entries);
+    MXCO_HIP(hipGetLastError(), "BloomFilter::Add: scatter");
+    const unsigned grid = std::min<unsigned>(nranges, (unsigned)cu_count() * 2);
+    bloom_apply_kernel<<<grid, 1024, 0, stream>>>(f->words, nb, nranges, segs, offsets, entries);
+    MXCO_HIP(hipGetLastError(), "BloomFilter::Add: apply");
+    return MXCO_OK;
 }
 
 mxco_status check_filter(const mxco_bloom* f, const char* fn)
@@ -179,6 +361,7 @@ MXCO_API mxco_status mxco_bloom_destroy(mxco_bloom* f)
 {
     if (!f) return ok();
     if (f->words) (void)hipFree(f->words);
+    if (f->ws) (void)hipFree(f->ws);
     delete f;
     return ok();
 }
@@ -204,14 +387,24 @@ MXCO_API mxco_status mxco_bloom_add(mxco_bloom* f, const void* keys, const uint8
     if (n == 0) return ok();
     if (mxco_status s = check_filter(f, "Add")) return s;
     if (!keys) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter::Add: keys must not be null when keyNum is non-zero");
-    const unsigned grid = grid_for(n, 8);
-    if (f->key_size == 4)
-        bloom_add_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
-                                                     stencil, n, f->seed);
-    else
-        bloom_add_kernel<8><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
-                                                     stencil, n, f->seed);
-    MXCO_HIP(hipGetLastError(), "BloomFilter::Add");
+    const uint64_t nranges = (f->num_blocks + kRangeBlocks - 1) >> kRangeShift;
+    if (add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
+        for (uint64_t done = 0; done < n; done += kPassKeys) {
+            const uint64_t m = std::min(kPassKeys, n - done);
+            if (mxco_status e = add_partitioned(f, (const uint8_t*)keys + done * f->key_size,
+                                                stencil ? stencil + done : nullptr, m, stream))
+                return e;
+        }
+    } else {
+        const unsigned grid = grid_for(n, 8);
+        if (f->key_size == 4)
+            bloom_add_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                         stencil, n, f->seed);
+        else
+            bloom_add_kernel<8><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                         stencil, n, f->seed);
+        MXCO_HIP(hipGetLastError(), "BloomFilter::Add");
+    }
     if (sync) MXCO_HIP(hipStreamSynchronize(stream), "BloomFilter::Add");
     return ok();
 }

@@ -93,10 +93,21 @@ __global__ __launch_bounds__(kThreads) void hll_private_add_kernel(const uint8_t
     const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
     const uint64_t begin = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(n, begin + per);
-    for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        uint32_t idx, rank;
-        hll_index_rank<KS>(key_at<KS>(keys, i), p, idx, rank);
-        raise_byte(&sk[idx >> 2], (idx & 3u) * 8u, rank);
+    constexpr int U = 4;  // key loads in flight per lane
+    for (uint64_t i0 = begin + threadIdx.x; i0 < end; i0 += (uint64_t)U * blockDim.x) {
+        uint64_t k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+            k[u] = i < end ? key_at<KS>(keys, i) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + (uint64_t)u * blockDim.x >= end) break;
+            uint32_t idx, rank;
+            hll_index_rank<KS>(k[u], p, idx, rank);
+            raise_byte(&sk[idx >> 2], (idx & 3u) * 8u, rank);
+        }
     }
     __syncthreads();
     uint32_t* out = (uint32_t*)(workspace + (uint64_t)blockIdx.x * (1u << p));
@@ -289,7 +300,8 @@ MXCO_API mxco_status mxco_hll_create(uint32_t precision, uint32_t key_size, hipS
     h->p = precision;
     h->key_size = key_size;
     (void)hipGetDevice(&h->device);
-    h->ws_blocks = precision <= 17 ? (uint32_t)cu_count() : 0;
+    // private sketches: two workgroups per CU while two sketches fit in LDS
+    h->ws_blocks = precision <= 17 ? (uint32_t)cu_count() * (precision <= 16 ? 2u : 1u) : 0;
     const size_t sketch = (size_t)1 << precision;
     hipError_t e = hipMalloc((void**)&h->words, sketch);
     if (e == hipSuccess) e = hipMalloc((void**)&h->hist, kBins * sizeof(uint32_t));

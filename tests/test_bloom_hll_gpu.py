@@ -92,6 +92,51 @@ def test_bloom_errors():
     f.add(0, n=0)  # zero keys: no-op
 
 
+def test_bloom_partitioned_add_ragged_ranges_stencil_duplicates():
+    """Batches of at least 2^16 keys and half a key per block take the
+    partitioned Add: a partial last range (3 ranges of 2048 blocks + 77), a
+    stencil, and 50 distinct keys repeated 2000 times each must still give the
+    oracle's words."""
+    nb = 3 * 2048 + 77
+    rng = np.random.default_rng(8)
+    keys = np.repeat(rng.integers(0, 2**62, 50).astype(np.int64), 2000)
+    rng.shuffle(keys)
+    st = (rng.integers(0, 3, len(keys)) != 0).astype(np.uint8)
+    f = mx().BloomFilter(nb)
+    f.add_if(dev(keys), dev(st))
+    ref = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(ref, nb, keys[st != 0])
+    np.testing.assert_array_equal(words_of(f), ref)
+    wide = rng.integers(0, 2**62, 300000).astype(np.int64)
+    f.add(dev(wide))
+    O.bloom_add(ref, nb, wide)
+    np.testing.assert_array_equal(words_of(f), ref)
+
+
+def test_bloom_multi_pass_add_equals_split_adds():
+    """2^27 + 2^20 keys take two partition passes; the words equal those of
+    two separate Adds of the same halves (OR is order-free), and the first
+    2^20 keys' blocks match the oracle."""
+    nb = 1 << 22
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    n = (1 << 27) + (1 << 20)
+    keys = torch.randint(0, 2**62, (n,), dtype=torch.int64, device="cuda", generator=g)
+    a = mx().BloomFilter(nb)
+    a.add(keys)
+    b = mx().BloomFilter(nb)
+    b.add(keys[: 1 << 27])
+    b.add(keys[1 << 27:])
+    assert torch.equal(a.words(), b.words())
+    c = mx().BloomFilter(nb)
+    head = keys[: 1 << 20]
+    c.add(head)
+    ref = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(ref, nb, head.cpu().numpy())
+    np.testing.assert_array_equal(words_of(c), ref)
+    assert a.contains(head).all().item()
+
+
 def test_bloom_large_fpr():
     """2^22 blocks (128 MiB) with 2^24 keys: words bit-exact, FPR as expected."""
     nb = 1 << 22
