@@ -118,7 +118,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
                                     const void* empty_value, hipStream_t stream, mxco_oa** out);
 /* ~StaticMap (frees the slot array). */
 MXCO_API mxco_status mxco_oa_destroy(mxco_oa* table);
-/* Capacity() static_map.h:585, Data() static_map.h:604. */
+/* Capacity() static_map.h:584, Data() static_map.h:604. */
 MXCO_API mxco_status mxco_oa_capacity(const mxco_oa* table, uint64_t* capacity);
 MXCO_API mxco_status mxco_oa_data(const mxco_oa* table, void** slots);
 MXCO_API mxco_status mxco_oa_slot_size(const mxco_oa* table, uint32_t* slot_size);
@@ -126,11 +126,11 @@ MXCO_API mxco_status mxco_oa_slot_size(const mxco_oa* table, uint32_t* slot_size
 /* SetErasedKey(erasedKey, stream) static_map.h:97. */
 MXCO_API mxco_status mxco_oa_set_erased_key(mxco_oa* table, const void* erased_key, hipStream_t stream);
 
-/* Clear / ClearAsync static_map.h:288-292 (open_addressing_impl.h:432-452). */
+/* Clear / ClearAsync static_map.h:318, 331 (open_addressing_impl.h:432-452). */
 MXCO_API mxco_status mxco_oa_clear(mxco_oa* table, hipStream_t stream, int sync);
 
 /* Insert / InsertAsync / InsertIf / InsertIfAsync
- * (static_map.h:116,131,216,232; open_addressing_impl.h:102-221).
+ * (static_map.h:116,131,216,237; open_addressing_impl.h:102-221).
  * pairs: n slot-layout elements.  stencil == NULL means AlwaysTrue.
  * sync != 0: waits and stores the number of inputs not inserted
  * (duplicates, the reserved empty key, table full) in *num_failed.
@@ -144,26 +144,26 @@ MXCO_API mxco_status mxco_oa_insert(mxco_oa* table, const void* pairs, const mxc
 MXCO_API mxco_status mxco_oa_insert_or_assign(mxco_oa* table, const void* pairs, uint64_t n, hipStream_t stream,
                                               int sync, uint64_t* num_failed, uint64_t* num_assigned);
 
-/* AssignIfAsync<StencilT, Predicate> (static_map.h:190; kernels.h:185-210):
+/* AssignIfAsync<StencilT, Predicate> (static_map.h:189; kernels.h:185-210):
  * overwrites the payload of keys already present, never inserts.  Maps only. */
 MXCO_API mxco_status mxco_oa_assign_if(mxco_oa* table, const void* pairs, const mxco_stencil* stencil, uint64_t n,
                                        hipStream_t stream, int sync);
 
-/* InsertAndFind / InsertAndFindAsync (static_map.h:256-271;
+/* InsertAndFind / InsertAndFindAsync (static_map.h:256, 273;
  * open_addressing_ref_impl.h:302-330): out_found[i] = payload now stored
  * for the key (map) or the stored key (set), out_inserted[i] = 1 when this
  * input inserted it. */
 MXCO_API mxco_status mxco_oa_insert_and_find(mxco_oa* table, const void* pairs, void* out_found,
                                              uint8_t* out_inserted, uint64_t n, hipStream_t stream, int sync);
 
-/* Find / FindAsync / FindIf / FindIfAsync (static_map.h:350-400;
+/* Find / FindAsync / FindIf / FindIfAsync (static_map.h:350, 369, 395, 419;
  * kernels.h:314-347): out[i] = payload (map) or stored key (set) of
  * keys[i], or the empty value/key on a miss or a false predicate. */
 MXCO_API mxco_status mxco_oa_find(mxco_oa* table, const void* keys, const mxco_stencil* stencil, void* out,
                                   uint64_t n, hipStream_t stream, int sync);
 
 /* Contains / ContainsAsync / ContainsIf / ContainsIfAsync
- * (static_map.h:439-490; kernels.h:350-377): out[i] = 1 byte bool. */
+ * (static_map.h:439, 459, 485, 509; kernels.h:350-377): out[i] = 1 byte bool. */
 MXCO_API mxco_status mxco_oa_contains(mxco_oa* table, const void* keys, const mxco_stencil* stencil, uint8_t* out,
                                       uint64_t n, hipStream_t stream, int sync);
 
@@ -172,7 +172,7 @@ MXCO_API mxco_status mxco_oa_contains(mxco_oa* table, const void* keys, const mx
 MXCO_API mxco_status mxco_oa_count(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
                                    uint64_t* count);
 
-/* Erase / EraseAsync (static_map.h:278-286; open_addressing_ref_impl.h:341-379):
+/* Erase / EraseAsync (static_map.h:292, 307; open_addressing_ref_impl.h:341-379):
  * writes the tombstone set by mxco_oa_set_erased_key, or an empty slot when
  * none is set.  sync != 0 stores the number of keys not erased. */
 MXCO_API mxco_status mxco_oa_erase(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream, int sync,
@@ -246,32 +246,32 @@ typedef struct mxco_mm mxco_mm;
 MXCO_API mxco_status mxco_mm_create(uint32_t key_size, uint32_t value_size, uint64_t capacity, const void* empty_key,
                                     const void* empty_value, hipStream_t stream, mxco_mm** out);
 MXCO_API mxco_status mxco_mm_destroy(mxco_mm* map);
-/* Capacity() / Size(stream) / Clear(stream) (static_multimap.h:397-410). */
+/* Capacity() / Size(stream) / Clear(stream) (static_multimap.h:411-428). */
 MXCO_API mxco_status mxco_mm_capacity(const mxco_mm* map, uint64_t* capacity);
 MXCO_API mxco_status mxco_mm_size(mxco_mm* map, hipStream_t stream, uint64_t* size);
 MXCO_API mxco_status mxco_mm_clear(mxco_mm* map, hipStream_t stream);
-/* Insert / InsertIf<Stencil, Predicate> (static_multimap.h:323-364, 437-445):
+/* Insert / InsertIf<Stencil, Predicate> (static_multimap.h:323-364, 437-451):
  * checks every selected pair for the reserved key/value first
  * (MXCO_INVALID_ARGUMENT, nothing inserted), then admits the input-order
  * prefix of the selected pairs that fits; *num_failed = selected - admitted. */
 MXCO_API mxco_status mxco_mm_insert(mxco_mm* map, const void* pairs, const mxco_stencil* stencil, uint64_t n,
                                     hipStream_t stream, uint64_t* num_failed);
-/* Contains / ContainsIf (static_multimap.h:446-455): 1-byte bool per key. */
+/* Contains / ContainsIf (static_multimap.h:452-460): 1-byte bool per key. */
 MXCO_API mxco_status mxco_mm_contains(mxco_mm* map, const void* keys, const mxco_stencil* stencil, uint8_t* out,
                                       uint64_t n, hipStream_t stream);
-/* Find / FindIf (static_multimap.h:456-465): smallest value of the key, or
+/* Find / FindIf (static_multimap.h:461-470): smallest value of the key, or
  * the empty value on a miss or a false predicate. */
 MXCO_API mxco_status mxco_mm_find(mxco_mm* map, const void* keys, const mxco_stencil* stencil, void* out_values,
                                   uint64_t n, hipStream_t stream);
-/* Count(keys, n, stream) (static_multimap.h:466-490): total matches. */
+/* Count(keys, n, stream) (static_multimap.h:475-495): total matches. */
 MXCO_API mxco_status mxco_mm_count(mxco_mm* map, const void* keys, uint64_t n, hipStream_t stream, uint64_t* count);
 /* Retrieve(keys, n, probeOut, valueOut, outputCapacity, stream)
- * (static_multimap.h:491-505): every match grouped by query index, values
+ * (static_multimap.h:500-513): every match grouped by query index, values
  * of one query ascending, probe_out holding the query key; MXCO_LENGTH_ERROR
  * (nothing written) when the matches exceed output_capacity. */
 MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* map, const void* keys, uint64_t n, void* probe_out, void* value_out,
                                       uint64_t output_capacity, hipStream_t stream, uint64_t* num_out);
-/* RetrieveAll(keyOut, valueOut, outputCapacity, stream) (static_multimap.h:506-540):
+/* RetrieveAll(keyOut, valueOut, outputCapacity, stream) (static_multimap.h:523-555):
  * the stored pairs as a multiset (order not part of the contract). */
 MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* map, void* key_out, void* value_out, uint64_t output_capacity,
                                           hipStream_t stream, uint64_t* num_out);

@@ -37,6 +37,15 @@
 
 #define ORC_API __attribute__((visibility("default")))
 
+/* Planted-mistake builds for tests/test_oracle_mutations.py: compiling with
+ * -DORC_PLANT=n (n >= 1) injects one plausible mistake, and that test checks
+ * that the pinning tests above reject every such build.  The normal build
+ * (ORC_PLANT = 0) contains none of them. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(n) (ORC_PLANT == (n))
+
 /* ------------------------------------------------------------------------ */
 /* Hash functions.                                                          */
 /* ------------------------------------------------------------------------ */
@@ -61,7 +70,7 @@ ORC_API uint32_t orc_fmix32(uint32_t key, uint32_t seed)
 ORC_API uint64_t orc_fmix64(uint64_t key, uint64_t seed)
 {
     uint64_t h = key ^ seed;
-    h ^= h >> 33; h *= 0xff51afd7ed558ccdull;
+    h ^= h >> (PLANT(1) ? 32 : 33); h *= 0xff51afd7ed558ccdull;
     h ^= h >> 33; h *= 0xc4ceb9fe1a85ec53ull;
     h ^= h >> 33;
     return h;
@@ -88,7 +97,7 @@ ORC_API uint32_t orc_murmur3_32(const void* data, size_t len, uint32_t seed)
     case 1: k ^= (uint32_t)tail[0];
             k *= c1; k = rotl32(k, 15); k *= c2; h ^= k;
     }
-    h ^= (uint32_t)len;
+    if (!PLANT(3)) h ^= (uint32_t)len;
     h ^= h >> 16; h *= 0x85ebca6bu;
     h ^= h >> 13; h *= 0xc2b2ae35u;
     h ^= h >> 16;
@@ -109,7 +118,7 @@ ORC_API uint32_t orc_xxh32(const void* data, size_t len, uint32_t seed)
     if (len >= 16) {
         uint32_t v1 = seed + XP1 + XP2, v2 = seed + XP2, v3 = seed, v4 = seed - XP1;
         while (off + 16 <= len) {
-            v1 = rotl32(v1 + load32(p + off) * XP2, 13) * XP1;
+            v1 = rotl32(v1 + load32(p + off) * XP2, PLANT(12) ? 12 : 13) * XP1;
             v2 = rotl32(v2 + load32(p + off + 4) * XP2, 13) * XP1;
             v3 = rotl32(v3 + load32(p + off + 8) * XP2, 13) * XP1;
             v4 = rotl32(v4 + load32(p + off + 12) * XP2, 13) * XP1;
@@ -173,7 +182,7 @@ ORC_API uint64_t orc_xxh64(const void* data, size_t len, uint64_t seed)
     } else {
         h = seed + YP5;
     }
-    h += (uint64_t)len;
+    if (!PLANT(2)) h += (uint64_t)len;
     while (off + 8 <= len) {
         h ^= xxh64_round(0, load64(p + off));
         h = rotl64(h, 27) * YP1 + YP4;
@@ -312,6 +321,7 @@ ORC_API uint64_t orc_valid_extent(uint64_t cap, uint32_t bucket, int probing)
 {
     uint64_t c = cap ? cap : 1;
     uint64_t size = c / bucket + (c % bucket != 0);
+    if (PLANT(9) && probing == 0) size = c / bucket;
     if (probing == 0) {
         return (size + (cap == 0)) * bucket;
     }
@@ -350,7 +360,7 @@ static void pair_layout(uint32_t ks, uint32_t vs, uint32_t* ss, uint32_t* voff)
     while (a < ks + vs) a <<= 1;
     if (a > 16) a = 16;
     uint32_t off = ks;
-    if (off % vs) off += vs - off % vs;        /* natural alignment of second */
+    if (off % vs && !PLANT(10)) off += vs - off % vs;  /* natural alignment of second */
     uint32_t size = off + vs;
     if (size % a) size += a - size % a;
     *ss = size;
@@ -415,12 +425,12 @@ static probe_it make_iter(const orc_oa* t, const uint8_t* key)
     probe_it it;
     uint32_t nb = t->cap / t->bucket;
     uint32_t h1 = probe_hash(t->hash, key, t->ks, 0);
-    it.idx = (h1 % nb) * t->bucket;
+    it.idx = ((PLANT(5) ? h1 + 1 : h1) % nb) * t->bucket;
     if (t->probing == 0) {
         it.step = t->bucket;
     } else {
         uint32_t h2 = probe_hash(t->hash, key, t->ks, 1);
-        it.step = (h2 % (nb - 1) + 1) * t->bucket;
+        it.step = (PLANT(4) ? h2 % nb + 1 : h2 % (nb - 1) + 1) * t->bucket;
     }
     it.cap = t->cap;
     return it;
@@ -478,7 +488,7 @@ static int insert_verdict(orc_oa* t, const uint8_t* pair, int assign, line_acc* 
                 return R_SUCCESS;
             }
             if (key_equal(t, key, slot)) {
-                if (assign && t->vs) memcpy(slot + t->voff, pair + t->voff, t->vs);
+                if ((assign || PLANT(6)) && t->vs) memcpy(slot + t->voff, pair + t->voff, t->vs);
                 return R_DUPLICATE;
             }
         }
@@ -765,7 +775,7 @@ static inline void bloom_locate(uint64_t h, uint64_t num_blocks, uint32_t* blk, 
 {
     *blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * (uint32_t)num_blocks) >> 32);
     uint32_t lo = (uint32_t)h;
-    for (int w = 0; w < 8; ++w) pat[w] = 1u << ((k_salts[w] * lo) >> 27);
+    for (int w = 0; w < 8; ++w) pat[w] = 1u << ((k_salts[PLANT(7) ? (w ^ 1) : w] * lo) >> 27);
 }
 
 ORC_API void orc_bloom_add(uint32_t* words, uint64_t num_blocks, const void* keys, uint32_t ks, uint64_t n,
@@ -804,7 +814,7 @@ ORC_API void orc_hll_add(uint8_t* regs, uint32_t p, const void* keys, uint32_t k
         uint64_t h = orc_xxh64(k + i * ks, ks, 0);
         uint32_t idx = (uint32_t)(h >> (64 - p));
         uint64_t tail = (h << p) | (1ull << (p - 1));
-        uint8_t rank = (uint8_t)(__builtin_clzll(tail) + 1);
+        uint8_t rank = (uint8_t)(__builtin_clzll(tail) + (PLANT(8) ? 0 : 1));
         if (regs[idx] < rank) regs[idx] = rank;
     }
 }
@@ -856,7 +866,7 @@ ORC_API uint64_t orc_hll_finalize(uint32_t p, double z, int v, const double* raw
     double m = (double)(1u << p);
     double e = hll_alpha_mm(m) / z;
     if (v > 0) {
-        double h = m * log(m / v);
+        double h = m * log(m / (PLANT(11) ? v + 1 : v));
         if (e <= 2.5 * m) return (uint64_t)round(h);
         e = (h <= threshold) ? h : ((e < 5.0 * m) ? e - hll_bias(e, raw, bias, n) : e);
     } else {

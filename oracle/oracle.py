@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# MXCO_ORACLE_LIB selects another build of oracle.c (the planted-mistake
+# builds of tests/test_oracle_mutations.py).
+_LIB_PATH = os.environ.get("MXCO_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 HASH_MURMUR3_32, HASH_XXHASH_32, HASH_XXHASH_64, HASH_FMIX_32, HASH_FMIX_64 = range(5)

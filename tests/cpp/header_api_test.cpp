@@ -1,5 +1,6 @@
 // Exercises the C++ drop-in headers (include/mxco/*.h) the way reference
-// callers use aclco::StaticMap / StaticSet / BloomFilter / HyperLogLog:
+// callers use aclco::StaticMap / StaticSet / StaticMultimap / BloomFilter /
+// HyperLogLog (through the aclco alias header):
 // device buffers, void* arguments, a stream, exceptions on misuse.
 // Prints "OK" and exits 0 when every check holds.
 #include <hip/hip_runtime.h>
@@ -9,10 +10,9 @@
 #include <stdexcept>
 #include <vector>
 
-#include "mxco/bloom_filter.h"
-#include "mxco/hyperloglog.h"
-#include "mxco/static_map.h"
-#include "mxco/static_set.h"
+#include <algorithm>
+
+#include "mxco/aclco.h"
 
 #define CHECK(c)                                                             \
     do {                                                                     \
@@ -150,6 +150,46 @@ int main()
         }
         CHECK(threw);
         CHECK(hipFree(dk) == hipSuccess);
+    }
+    {  // StaticMultimap<int64, int64> through the aclco spelling
+        aclco::StaticMultimap<int64_t, int64_t> mm(aclco::Extent<size_t>{64}, -1, -1, s);
+        CHECK(mm.Capacity() == 64);
+        std::vector<int64_t> pairs;  // key k (1..5) with values 10k+j, j < k
+        for (int64_t k = 1; k <= 5; ++k)
+            for (int64_t j = k - 1; j >= 0; --j) {
+                pairs.push_back(k);
+                pairs.push_back(10 * k + j);
+            }
+        const size_t n = pairs.size() / 2;
+        auto* dp = to_device(pairs);
+        CHECK(mm.Insert(dp, n, s) == 0);
+        CHECK(mm.Size(s) == n);
+        std::vector<int64_t> q = {3, 7, 1};
+        auto* dq = to_device(q);
+        int64_t* dv = nullptr;
+        CHECK(hipMalloc(&dv, 64 * sizeof(int64_t)) == hipSuccess);
+        mm.Find(dq, dv, q.size(), s);
+        auto f = to_host(dv, 3);
+        CHECK(f[0] == 30 && f[1] == -1 && f[2] == 10);
+        CHECK(mm.Count(dq, q.size(), s) == 4);
+        int64_t* dprobe = nullptr;
+        CHECK(hipMalloc(&dprobe, 64 * sizeof(int64_t)) == hipSuccess);
+        CHECK(mm.Retrieve(dq, q.size(), dprobe, dv, 64, s) == 4);
+        auto vals = to_host(dv, 4);
+        auto probes = to_host(dprobe, 4);
+        CHECK(vals[0] == 30 && vals[1] == 31 && vals[2] == 32 && vals[3] == 10);
+        CHECK(probes[0] == 3 && probes[3] == 1);
+        bool threw = false;
+        try {
+            (void)mm.Retrieve(dq, q.size(), dprobe, dv, 2, s);
+        } catch (std::length_error const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        CHECK(hipFree(dp) == hipSuccess);
+        CHECK(hipFree(dq) == hipSuccess);
+        CHECK(hipFree(dv) == hipSuccess);
+        CHECK(hipFree(dprobe) == hipSuccess);
     }
     CHECK(hipStreamDestroy(s) == hipSuccess);
     std::printf("OK\n");
