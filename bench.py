@@ -29,7 +29,9 @@ def parse():
     ap.add_argument("--lf", type=float, default=0.8)
     ap.add_argument("--group", type=int, default=0, help="lanes per key (0 = library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent grid blocks per CU (0 = default)")
-    ap.add_argument("--inflight", type=int, default=0, help="keys in flight per lane group (1 or 2; 0 = default)")
+    ap.add_argument("--inflight", type=int, default=0,
+                    help="probe kernel variant: 1 plain, 2 two keys per group, 3 block-staged chunks, "
+                         "4 wave-staged Find/Contains outputs (0 = library default, 4)")
     ap.add_argument("--probe-flags", type=int, default=-1, help="bit0 sector-sized first window, bit1 nt loads")
     ap.add_argument("--store-mode", type=int, default=-1, help="payload store after a key CAS (0 plain, 1 atomic, 2 nt)")
     ap.add_argument("--sweep", action="store_true", help="also sweep LF 0.5/0.7/0.8/0.9 and group widths")
@@ -481,7 +483,7 @@ def main():
     if os.path.exists(tf):
         traffic = json.load(open(tf))
     dominant_insert = r["insert_ms"] >= r["find_ms"]
-    roof_find = {"kernel": "oa_kernel<8,8,G,OP_FIND>", "bound": "hbm", "achieved": round(find_gbs, 1),
+    roof_find = {"kernel": "oa_wave_staged_kernel<8,8,G,OP_FIND>", "bound": "hbm", "achieved": round(find_gbs, 1),
                  "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(find_gbs / PEAK_HBM_GBS, 4),
                  "traffic": traffic.get("find") if traffic else None,
                  "lines_per_query": r["find_lines"] / r["n_find"], "launch_ms": r["find_ms"],

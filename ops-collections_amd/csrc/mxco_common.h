@@ -41,4 +41,7 @@ uint64_t max_batch();
 // Persistent-grid size for a launch of `threads` threads in blocks of 256.
 unsigned grid_for(uint64_t threads, unsigned blocks_per_cu);
 
+// Integer tuning knob from the environment (dflt when unset or empty).
+int env_int(const char* name, int dflt);
+
 }  // namespace mxco

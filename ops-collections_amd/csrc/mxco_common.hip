@@ -1,4 +1,5 @@
 // Error state, device queries and capacity primes of libmxco.so.
+#include <stdlib.h>
 #include "mxco_common.h"
 
 #include <mutex>
@@ -149,6 +150,12 @@ uint64_t valid_extent(uint64_t requested, uint32_t bucket, uint32_t probing)
     if (probing == MXCO_PROBING_LINEAR) return (size + (requested == 0)) * bucket;
     const uint64_t p = prime_lower_bound(size);
     return p ? p * bucket : 0;
+}
+
+int env_int(const char* name, int dflt)
+{
+    const char* s = getenv(name);
+    return s && *s ? atoi(s) : dflt;
 }
 
 }  // namespace mxco

@@ -57,15 +57,9 @@ void pair_layout(uint32_t ks, uint32_t vs, uint32_t* ss, uint32_t* voff)
     *voff = (ks == 4 && vs == 4) ? 4 : 8;
 }
 
-int env_int(const char* name, int dflt)
-{
-    const char* s = getenv(name);
-    return s && *s ? atoi(s) : dflt;
-}
-
-// Lanes per key when the descriptor leaves it to the library.  Chosen by
-// the tuner's sweep (bench.py --tune) on MI355X: see the group-width table
-// in the documentation.
+// Lanes per key when the descriptor leaves it to the library: 4, the best
+// width for both Find and Insert in the group-width sweep on MI355X
+// (bench.py --sweep).
 int default_group(uint32_t ss)
 {
     int g = env_int("MXCO_OA_GROUP", 0);
@@ -205,8 +199,11 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     t->blocks_per_cu = (unsigned)env_int("MXCO_OA_BLOCKS_PER_CU", 256);
     t->store_mode = (uint32_t)env_int("MXCO_OA_STORE_MODE", 0);
     {
-        const int m = env_int("MXCO_OA_INFLIGHT", 1);
-        t->inflight = (m == 2 || m == 3) ? m : 1;
+        // Probe kernel variant: 4 (default) stages Find / Contains outputs per
+        // wavefront; 1 plain; 2 two keys in flight per group; 3 block-staged
+        // chunks.  Operations a variant does not cover use the plain kernel.
+        const int m = env_int("MXCO_OA_INFLIGHT", 4);
+        t->inflight = (m == 2 || m == 3 || m == 4) ? m : 1;
     }
     t->probe_flags = (uint32_t)env_int("MXCO_OA_PROBE_FLAGS", 0);
     (void)hipGetDevice(&t->device);

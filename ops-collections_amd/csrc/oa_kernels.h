@@ -670,6 +670,197 @@ __global__ __launch_bounds__(256) void oa_staged_kernel(OaArgs a)
     if (a.counters) flush_counters<OP>(a.counters, cnt0, cnt1);
 }
 
+// Find / Contains with wave-staged outputs.  Each wavefront owns chunks of
+// kWaveChunk consecutive keys (two chunks open at a time) and hands their
+// keys to its lane groups as groups free up, so the flat per-group walk is
+// kept.  Results go to the chunk's LDS buffer; when every key of a chunk is
+// resolved the wavefront writes the chunk's outputs as whole lines.  With
+// plain per-key stores, groups that drifted apart write one result at a
+// time into the same output line and the L2 writes that line back several
+// times.
+constexpr uint32_t kWaveChunk = 128;
+
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
+{
+    static_assert(OP == OP_FIND || OP == OP_CONTAINS, "Find / Contains only");
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr int SS = L::SS;
+    constexpr int SPL = 128 / SS;
+    constexpr int K = SPL / G;
+    constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
+    constexpr uint64_t kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    __shared__ __align__(16) uint8_t stage_all[4][2][kWaveChunk * OUTB];
+
+    const uint32_t lane = __lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    uint8_t* st0 = stage_all[wib][0];
+    uint8_t* st1 = stage_all[wib][1];
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const bool leader = gl == 0;
+    const uint64_t n = a.n;
+    const uint64_t ekey = a.empty_key;
+    const uint64_t miss_out = OP == OP_CONTAINS ? 0 : (VS ? a.empty_val : ekey);
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    uint64_t next_chunk = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+
+    // Wave-uniform state of the two open chunks: base key, length, keys
+    // handed out, keys resolved.  `old` is the chunk opened first.
+    uint64_t cb0, cb1;
+    uint32_t cl0, cl1, ch0 = 0, ch1 = 0, cd0 = 0, cd1 = 0;
+    auto open_chunk = [&](uint64_t& cb, uint32_t& cl, uint32_t& ch, uint32_t& cd) {
+        cb = next_chunk * kWaveChunk;
+        cl = cb < n ? (uint32_t)min((uint64_t)kWaveChunk, n - cb) : 0u;
+        ch = 0;
+        cd = 0;
+        next_chunk += nwaves;
+    };
+    open_chunk(cb0, cl0, ch0, cd0);
+    open_chunk(cb1, cl1, ch1, cd1);
+    int old = 0;
+
+    // Per-group state (uniform within the group).
+    bool active = false;
+    int B = 0;  // chunk buffer of the group's key
+    uint64_t I = 0, key = 0;
+    Walk w;
+    w.done = true;
+    Raw s[K];
+
+    auto put = [&](uint64_t v) {
+        uint8_t* d = (B ? st1 : st0) + (uint32_t)(I - (B ? cb1 : cb0)) * OUTB;
+        if constexpr (OUTB == 1) *d = (uint8_t)v;
+        else if constexpr (OUTB == 4) *(uint32_t*)d = (uint32_t)v;
+        else *(uint64_t*)d = v;
+    };
+    auto flush = [&](const uint8_t* st, uint64_t cb, uint32_t cl) {
+        for (uint32_t j = lane; j < cl; j += 64) {
+            if constexpr (OUTB == 1) a.out[cb + j] = st[j];
+            else if constexpr (OUTB == 4) ((uint32_t*)a.out)[cb + j] = ((const uint32_t*)st)[j];
+            else ((uint64_t*)a.out)[cb + j] = ((const uint64_t*)st)[j];
+        }
+    };
+
+    for (;;) {
+        if (cl0 == 0 && cl1 == 0) break;  // every chunk resolved and written
+        bool key_done = false;
+
+        // Hand the open chunks' next keys to the groups without one, oldest chunk first.
+        const uint64_t need = __ballot(leader && !active);
+        const uint32_t av_old = old ? cl1 - ch1 : cl0 - ch0;
+        const uint32_t av_new = old ? cl0 - ch0 : cl1 - ch1;
+        if (need != 0 && (av_old | av_new) != 0) {
+            const uint32_t nneed = (uint32_t)__popcll(need);
+            if (!active) {
+                const uint32_t rank = (uint32_t)__popcll(need & ((1ull << gbase) - 1ull));
+                bool fresh = false;
+                if (rank < av_old) {
+                    B = old;
+                    I = (old ? cb1 + ch1 : cb0 + ch0) + rank;
+                    fresh = true;
+                } else if (rank - av_old < av_new) {
+                    B = old ^ 1;
+                    I = (old ? cb0 + ch0 : cb1 + ch1) + (rank - av_old);
+                    fresh = true;
+                }
+                if (fresh) {
+                    active = true;
+                    if (!stencil_ok(a, I)) {
+                        if (leader) put(miss_out);
+                        key_done = true;
+                    } else {
+                        key = load_bits<KS>(a.in + I * KS);
+                        w = make_walk<KS>(a, key);
+                    }
+                }
+            }
+            const uint32_t t_old = min(nneed, av_old);
+            const uint32_t t_new = min(nneed - t_old, av_new);
+            if (old) {
+                ch1 += t_old;
+                ch0 += t_new;
+            } else {
+                ch0 += t_old;
+                ch1 += t_new;
+            }
+        }
+
+        if (active && !key_done) {
+            if (w.done) {
+                // walk exhausted after a full cycle: a miss
+                if (leader) put(miss_out);
+                key_done = true;
+            } else {
+                const uint32_t wlen = window_len<SPL>(a, w);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t off = gl + k * G;
+                    if (off < wlen) s[k] = *(const Raw*)(a.table + (uint64_t)(w.pos + off) * SS);
+                }
+                int fk = K;
+                bool feq = false;
+                Raw ws = s[0];
+#pragma unroll
+                for (int k = K - 1; k >= 0; --k) {
+                    const uint32_t off = gl + k * G;
+                    if (off < wlen) {
+                        const uint64_t sk = L::key(s[k]);
+                        const bool em = sk == ekey;
+                        const bool eq = !em && key_eq<KS>(sk, key, a.key_float);
+                        if (eq || em) {
+                            fk = k;
+                            feq = eq;
+                            ws = s[k];
+                        }
+                    }
+                }
+                int wround = K;
+                uint32_t wlane = 0;
+                if constexpr (G == 1) {
+                    wround = fk;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const uint64_t m = (__ballot(fk == k) >> gbase) & kGroupMask;
+                        if (m && wround == K) {
+                            wround = k;
+                            wlane = __builtin_ctzll(m);
+                        }
+                    }
+                }
+                if (wround == K) {
+                    consume(a, w, wlen);
+                } else {
+                    if (gl == wlane) {
+                        if constexpr (OP == OP_FIND) put(feq ? L::val(ws) : miss_out);
+                        else put(feq ? 1 : 0);
+                    }
+                    key_done = true;
+                }
+            }
+        }
+
+        const uint32_t d0 = (uint32_t)__popcll(__ballot(leader && key_done && B == 0));
+        const uint32_t d1 = (uint32_t)__popcll(__ballot(leader && key_done && B == 1));
+        cd0 += d0;
+        cd1 += d1;
+        if (key_done) active = false;
+
+        if (cl0 != 0 && cd0 == cl0) {
+            flush(st0, cb0, cl0);
+            open_chunk(cb0, cl0, ch0, cd0);
+            if (old == 0) old = 1;
+        }
+        if (cl1 != 0 && cd1 == cl1) {
+            flush(st1, cb1, cl1);
+            open_chunk(cb1, cl1, ch1, cd1);
+            if (old == 1) old = 0;
+        }
+    }
+}
+
 // Sentinel fill (Clear / ClearAsync, open_addressing_impl.h:341-452):
 // streams the 16-byte replicated slot image over the whole slot array.
 static __global__ __launch_bounds__(256) void oa_fill_kernel(uint8_t* table, uint64_t bytes, u64x2 pattern)

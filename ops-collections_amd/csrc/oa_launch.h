@@ -12,6 +12,16 @@ template <int KS, int VS, int G>
 hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight)
 {
     const dim3 block(256);
+    if (inflight == 4) {
+        // wave-staged outputs: the bulk Find / Contains paths
+        switch (op) {
+        case OP_FIND: oa_wave_staged_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a); return hipGetLastError();
+        case OP_CONTAINS:
+            oa_wave_staged_kernel<KS, VS, G, OP_CONTAINS><<<grid, block, 0, s>>>(a);
+            return hipGetLastError();
+        default: break;
+        }
+    }
     if (inflight == 3) {
         // LDS-staged chunks: the streaming Find / Contains / Insert / Count paths
         switch (op) {
