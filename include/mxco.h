@@ -227,7 +227,10 @@ MXCO_API mxco_status mxco_batch_run(mxco_batch_plan* plan, const void* in, void*
 /* Per-item failure counts of the last insert run (synchronous). */
 MXCO_API mxco_status mxco_batch_failures(mxco_batch_plan* plan, hipStream_t stream, uint64_t* failed_per_item);
 /* Autotuner: times the plan under every lane-group width x grid size and
- * keeps the fastest (results do not depend on the choice). */
+ * keeps the fastest (results do not depend on the choice).  It runs the plan
+ * several times; an insert plan's tables are cleared before each timing and
+ * are left as one run leaves them, so tune an insert plan before the find
+ * plans that read its tables. */
 MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* plan, const void* in, void* out, uint8_t* out2,
                                           hipStream_t stream, float* best_ms);
 MXCO_API mxco_status mxco_batch_plan_config(mxco_batch_plan* plan, uint32_t* group_size, uint32_t* blocks_per_cu);

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <new>
 #include <string>
 #include <vector>
@@ -32,6 +33,23 @@ static hipError_t launch_batch_g(int op, const OaArgs& a, const BatchTable* tabl
                                  unsigned grid, hipStream_t s)
 {
     const dim3 block(256);
+    if constexpr (std::is_same<Item, mxco_batch_item>::value) {
+        // plans: Find / Contains with wave-staged outputs (MXCO_BATCH_WAVE_STAGED=0 selects the plain kernel)
+        static const int staged = env_int("MXCO_BATCH_WAVE_STAGED", 1);
+        if (staged && (op == OP_FIND || op == OP_CONTAINS)) {
+            // a chunk is 16 wave sub-chunks, 4 wavefronts per workgroup: a
+            // grid capped below the chunk count stays as tuned, a smaller one
+            // grows to one workgroup per 4 sub-chunks
+            const unsigned sg = grid >= nchunks ? nchunks * (kChunk / kWaveChunk / 4) : grid;
+            if (op == OP_FIND)
+                oa_batch_wave_staged_kernel<KS, VS, G, OP_FIND><<<sg, block, 0, s>>>(a, tables, items, chunks,
+                                                                                     nchunks);
+            else
+                oa_batch_wave_staged_kernel<KS, VS, G, OP_CONTAINS><<<sg, block, 0, s>>>(a, tables, items, chunks,
+                                                                                         nchunks);
+            return hipGetLastError();
+        }
+    }
     switch (op) {
     case OP_INSERT:
         oa_batch_kernel<KS, VS, G, OP_INSERT, Item><<<grid, block, 0, s>>>(a, tables, items, chunks, nchunks,
@@ -377,7 +395,11 @@ MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, vo
         }
         if (st != MXCO_OK) break;
     }
+    // Leave the tables as one run() of the plan leaves them (an insert plan
+    // tuned above ran several times), so a find plan tuned next sees the
+    // filled tables.
     if (p->op == OP_INSERT && st == MXCO_OK) st = mxco_batch_clear(p->batch, stream);
+    if (p->op == OP_INSERT && st == MXCO_OK) st = run_plan(p, in, out, out2, stream, bg, bb);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != MXCO_OK) return st;

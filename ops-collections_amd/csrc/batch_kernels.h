@@ -89,6 +89,59 @@ __global__ __launch_bounds__(256) void oa_batch_kernel(OaArgs base, const BatchT
     }
 }
 
+// Wave-staged Find / Contains over a plan's chunk list: every 2048-key chunk
+// is cut into kWaveChunk-key sub-chunks, and wavefront w takes sub-chunks
+// w, w + waves, ... (skipping the empty tails of short chunks).
+template <int KS, int OUTB>
+struct BatchChunks {
+    const OaArgs* base;
+    const BatchTable* tables;
+    const mxco_batch_item* items;
+    const BatchChunk* chunks;
+    uint32_t nchunks;
+    uint64_t next, step;
+    __device__ __forceinline__ WsChunk open()
+    {
+        constexpr uint32_t kSub = kChunk / kWaveChunk;
+        WsChunk c;
+        c.len = 0;
+        for (;;) {
+            const uint64_t sc = next;
+            next += step;
+            const uint64_t ci = sc / kSub;
+            if (ci >= nchunks) return c;
+            const BatchChunk ch = chunks[ci];
+            const uint32_t off = (uint32_t)(sc % kSub) * kWaveChunk;
+            if (off >= ch.len) continue;
+            const mxco_batch_item it = items[ch.item];
+            const BatchTable t = tables[it.table];
+            const uint64_t first = it.offset + ch.start + off;
+            c.table = t.slots;
+            c.cap = t.cap;
+            c.nbuckets = t.nbuckets;
+            c.in = base->in + first * KS;
+            c.out = base->out + first * OUTB;
+            c.stencil = nullptr;
+            c.len = min(kWaveChunk, ch.len - off);
+            return c;
+        }
+    }
+};
+
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) void oa_batch_wave_staged_kernel(OaArgs base, const BatchTable* tables,
+                                                                   const mxco_batch_item* items,
+                                                                   const BatchChunk* chunks, uint32_t nchunks)
+{
+    static_assert(kChunk % kWaveChunk == 0, "chunks split into whole sub-chunks");
+    constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
+    __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
+    const uint32_t wib = threadIdx.x >> 6;
+    BatchChunks<KS, OUTB> src{&base, tables, items, chunks, nchunks,
+                              (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib, (uint64_t)gridDim.x * (blockDim.x >> 6)};
+    wave_staged_find<KS, VS, G, OP>(base, src, stage[wib][0], stage[wib][1]);
+}
+
 // Launches the persistent multi-table kernel (defined in batch.hip).
 // `serve` selects ServeItem items instead of mxco_batch_item.
 hipError_t launch_batch_any(int ks, int vs, int g, int op, bool serve, const OaArgs& a, const BatchTable* tables,

@@ -671,17 +671,50 @@ __global__ __launch_bounds__(256) void oa_staged_kernel(OaArgs a)
 }
 
 // Find / Contains with wave-staged outputs.  Each wavefront owns chunks of
-// kWaveChunk consecutive keys (two chunks open at a time) and hands their
-// keys to its lane groups as groups free up, so the flat per-group walk is
-// kept.  Results go to the chunk's LDS buffer; when every key of a chunk is
-// resolved the wavefront writes the chunk's outputs as whole lines.  With
-// plain per-key stores, groups that drifted apart write one result at a
-// time into the same output line and the L2 writes that line back several
-// times.
+// up to kWaveChunk consecutive keys of one table (two chunks open at a time)
+// and hands their keys to its lane groups as groups free up, so the flat
+// per-group walk is kept.  Results go to the chunk's LDS buffer; when every
+// key of a chunk is resolved the wavefront writes the chunk's outputs as
+// whole lines.  With plain per-key stores, groups that drifted apart write
+// one result at a time into the same output line and the L2 writes that
+// line back several times.
 constexpr uint32_t kWaveChunk = 128;
 
-template <int KS, int VS, int G, int OP>
-__global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
+// One chunk: its table, and its input / output / stencil already offset to
+// the chunk's first key.  len == 0: the source is exhausted.
+struct WsChunk {
+    uint8_t* table;
+    const uint8_t* in;
+    uint8_t* out;
+    const uint8_t* stencil;
+    uint32_t cap;
+    uint32_t nbuckets;
+    uint32_t len;
+};
+
+// Chunks of one table: wavefront w takes chunks w, w + waves, ...
+template <int KS, int OUTB>
+struct PlainChunks {
+    const OaArgs* a;
+    uint64_t next, step;
+    __device__ __forceinline__ WsChunk open()
+    {
+        WsChunk c;
+        const uint64_t base = next * kWaveChunk;
+        next += step;
+        c.table = a->table;
+        c.cap = a->cap;
+        c.nbuckets = a->nbuckets;
+        c.len = base < a->n ? (uint32_t)min((uint64_t)kWaveChunk, a->n - base) : 0u;
+        c.in = a->in + base * KS;
+        c.out = a->out + base * OUTB;
+        c.stencil = a->stencil ? a->stencil + base * a->stencil_size : nullptr;
+        return c;
+    }
+};
+
+template <int KS, int VS, int G, int OP, typename Src>
+__device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint8_t* st0, uint8_t* st1)
 {
     static_assert(OP == OP_FIND || OP == OP_CONTAINS, "Find / Contains only");
     using L = Lay<KS, VS>;
@@ -691,66 +724,53 @@ __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
     constexpr int K = SPL / G;
     constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
     constexpr uint64_t kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
-    __shared__ __align__(16) uint8_t stage_all[4][2][kWaveChunk * OUTB];
 
     const uint32_t lane = __lane_id();
-    const uint32_t wib = threadIdx.x >> 6;
-    uint8_t* st0 = stage_all[wib][0];
-    uint8_t* st1 = stage_all[wib][1];
     const uint32_t gl = lane & (G - 1);
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
     const bool leader = gl == 0;
-    const uint64_t n = a.n;
     const uint64_t ekey = a.empty_key;
     const uint64_t miss_out = OP == OP_CONTAINS ? 0 : (VS ? a.empty_val : ekey);
-    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-    uint64_t next_chunk = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib;
 
-    // Wave-uniform state of the two open chunks: base key, length, keys
-    // handed out, keys resolved.  `old` is the chunk opened first.
-    uint64_t cb0, cb1;
-    uint32_t cl0, cl1, ch0 = 0, ch1 = 0, cd0 = 0, cd1 = 0;
-    auto open_chunk = [&](uint64_t& cb, uint32_t& cl, uint32_t& ch, uint32_t& cd) {
-        cb = next_chunk * kWaveChunk;
-        cl = cb < n ? (uint32_t)min((uint64_t)kWaveChunk, n - cb) : 0u;
-        ch = 0;
-        cd = 0;
-        next_chunk += nwaves;
-    };
-    open_chunk(cb0, cl0, ch0, cd0);
-    open_chunk(cb1, cl1, ch1, cd1);
+    // Wave-uniform state of the two open chunks: keys handed out and keys
+    // resolved.  `old` is the chunk opened first.
+    WsChunk c0 = src.open();
+    WsChunk c1 = src.open();
+    uint32_t ch0 = 0, ch1 = 0, cd0 = 0, cd1 = 0;
     int old = 0;
 
     // Per-group state (uniform within the group).
     bool active = false;
-    int B = 0;  // chunk buffer of the group's key
-    uint64_t I = 0, key = 0;
+    int B = 0;       // chunk buffer of the group's key
+    uint32_t I = 0;  // key index within the chunk
+    uint64_t key = 0;
+    OaArgs ga = a;   // the key's table
     Walk w;
     w.done = true;
     Raw s[K];
 
     auto put = [&](uint64_t v) {
-        uint8_t* d = (B ? st1 : st0) + (uint32_t)(I - (B ? cb1 : cb0)) * OUTB;
+        uint8_t* d = (B ? st1 : st0) + I * OUTB;
         if constexpr (OUTB == 1) *d = (uint8_t)v;
         else if constexpr (OUTB == 4) *(uint32_t*)d = (uint32_t)v;
         else *(uint64_t*)d = v;
     };
-    auto flush = [&](const uint8_t* st, uint64_t cb, uint32_t cl) {
-        for (uint32_t j = lane; j < cl; j += 64) {
-            if constexpr (OUTB == 1) a.out[cb + j] = st[j];
-            else if constexpr (OUTB == 4) ((uint32_t*)a.out)[cb + j] = ((const uint32_t*)st)[j];
-            else ((uint64_t*)a.out)[cb + j] = ((const uint64_t*)st)[j];
+    auto flush = [&](const uint8_t* st, const WsChunk& c) {
+        for (uint32_t j = lane; j < c.len; j += 64) {
+            if constexpr (OUTB == 1) c.out[j] = st[j];
+            else if constexpr (OUTB == 4) ((uint32_t*)c.out)[j] = ((const uint32_t*)st)[j];
+            else ((uint64_t*)c.out)[j] = ((const uint64_t*)st)[j];
         }
     };
 
     for (;;) {
-        if (cl0 == 0 && cl1 == 0) break;  // every chunk resolved and written
+        if (c0.len == 0 && c1.len == 0) break;  // every chunk resolved and written
         bool key_done = false;
 
         // Hand the open chunks' next keys to the groups without one, oldest chunk first.
         const uint64_t need = __ballot(leader && !active);
-        const uint32_t av_old = old ? cl1 - ch1 : cl0 - ch0;
-        const uint32_t av_new = old ? cl0 - ch0 : cl1 - ch1;
+        const uint32_t av_old = old ? c1.len - ch1 : c0.len - ch0;
+        const uint32_t av_new = old ? c0.len - ch0 : c1.len - ch1;
         if (need != 0 && (av_old | av_new) != 0) {
             const uint32_t nneed = (uint32_t)__popcll(need);
             if (!active) {
@@ -758,21 +778,25 @@ __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
                 bool fresh = false;
                 if (rank < av_old) {
                     B = old;
-                    I = (old ? cb1 + ch1 : cb0 + ch0) + rank;
+                    I = (old ? ch1 : ch0) + rank;
                     fresh = true;
                 } else if (rank - av_old < av_new) {
                     B = old ^ 1;
-                    I = (old ? cb0 + ch0 : cb1 + ch1) + (rank - av_old);
+                    I = (old ? ch0 : ch1) + (rank - av_old);
                     fresh = true;
                 }
                 if (fresh) {
                     active = true;
-                    if (!stencil_ok(a, I)) {
+                    ga.table = B ? c1.table : c0.table;
+                    ga.cap = B ? c1.cap : c0.cap;
+                    ga.nbuckets = B ? c1.nbuckets : c0.nbuckets;
+                    ga.stencil = B ? c1.stencil : c0.stencil;
+                    if (!stencil_ok(ga, I)) {
                         if (leader) put(miss_out);
                         key_done = true;
                     } else {
-                        key = load_bits<KS>(a.in + I * KS);
-                        w = make_walk<KS>(a, key);
+                        key = load_bits<KS>((B ? c1.in : c0.in) + (uint64_t)I * KS);
+                        w = make_walk<KS>(ga, key);
                     }
                 }
             }
@@ -793,11 +817,11 @@ __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
                 if (leader) put(miss_out);
                 key_done = true;
             } else {
-                const uint32_t wlen = window_len<SPL>(a, w);
+                const uint32_t wlen = window_len<SPL>(ga, w);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const uint32_t off = gl + k * G;
-                    if (off < wlen) s[k] = *(const Raw*)(a.table + (uint64_t)(w.pos + off) * SS);
+                    if (off < wlen) s[k] = *(const Raw*)(ga.table + (uint64_t)(w.pos + off) * SS);
                 }
                 int fk = K;
                 bool feq = false;
@@ -831,7 +855,7 @@ __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
                     }
                 }
                 if (wround == K) {
-                    consume(a, w, wlen);
+                    consume(ga, w, wlen);
                 } else {
                     if (gl == wlane) {
                         if constexpr (OP == OP_FIND) put(feq ? L::val(ws) : miss_out);
@@ -842,23 +866,36 @@ __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
             }
         }
 
-        const uint32_t d0 = (uint32_t)__popcll(__ballot(leader && key_done && B == 0));
-        const uint32_t d1 = (uint32_t)__popcll(__ballot(leader && key_done && B == 1));
-        cd0 += d0;
-        cd1 += d1;
+        cd0 += (uint32_t)__popcll(__ballot(leader && key_done && B == 0));
+        cd1 += (uint32_t)__popcll(__ballot(leader && key_done && B == 1));
         if (key_done) active = false;
 
-        if (cl0 != 0 && cd0 == cl0) {
-            flush(st0, cb0, cl0);
-            open_chunk(cb0, cl0, ch0, cd0);
+        if (c0.len != 0 && cd0 == c0.len) {
+            flush(st0, c0);
+            c0 = src.open();
+            ch0 = 0;
+            cd0 = 0;
             if (old == 0) old = 1;
         }
-        if (cl1 != 0 && cd1 == cl1) {
-            flush(st1, cb1, cl1);
-            open_chunk(cb1, cl1, ch1, cd1);
+        if (c1.len != 0 && cd1 == c1.len) {
+            flush(st1, c1);
+            c1 = src.open();
+            ch1 = 0;
+            cd1 = 0;
             if (old == 1) old = 0;
         }
     }
+}
+
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
+{
+    constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
+    __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
+    const uint32_t wib = threadIdx.x >> 6;
+    PlainChunks<KS, OUTB> src{&a, (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib,
+                              (uint64_t)gridDim.x * (blockDim.x >> 6)};
+    wave_staged_find<KS, VS, G, OP>(a, src, stage[wib][0], stage[wib][1]);
 }
 
 // Sentinel fill (Clear / ClearAsync, open_addressing_impl.h:341-452):

@@ -33,6 +33,15 @@ HASH = {"murmurhash3_32": 0, "xxhash_32": 1, "xxhash_64": 2, "murmurhash3_fmix32
 PRED = {"always": 0, "nonzero": 1, "odd": 2, "even": 3}
 
 
+def _release(obj, destroy):
+    """Frees obj._h with the named C-ABI destructor (no-op at interpreter exit,
+    when the module globals are already gone)."""
+    h = getattr(obj, "_h", None)
+    if h and _lib is not None:
+        getattr(_lib, destroy)(h)
+    obj._h = None
+
+
 class LengthError(ValueError):
     """std::length_error: batch beyond the kernel index range."""
 
@@ -185,10 +194,7 @@ class _OpenAddressing:
         self.slot_size = s.value
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            _lib.mxco_oa_destroy(h)
-            self._h = None
+        _release(self, "mxco_oa_destroy")
 
     def capacity(self):
         return self._capacity
@@ -386,9 +392,7 @@ class BloomFilter:
         self._nb = num_blocks
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.mxco_bloom_destroy(self._h)
-            self._h = None
+        _release(self, "mxco_bloom_destroy")
 
     def block_extent(self):
         return self._nb
@@ -473,9 +477,7 @@ class HyperLogLog:
         return cls(p.value, key_dtype, stream)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.mxco_hll_destroy(self._h)
-            self._h = None
+        _release(self, "mxco_hll_destroy")
 
     def sketch_size_kb(self):
         return (1 << self.precision) // 1024
@@ -542,9 +544,7 @@ class StaticMultimap:
         self._h = h
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.mxco_mm_destroy(self._h)
-            self._h = None
+        _release(self, "mxco_mm_destroy")
 
     def capacity(self):
         c = u64()
@@ -645,9 +645,7 @@ class BatchPlan:
         self._h = h
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.mxco_batch_plan_destroy(self._h)
-            self._h = None
+        _release(self, "mxco_batch_plan_destroy")
 
     def run(self, inp, out=None, out2=None, stream=None):
         check(_lib.mxco_batch_run(self._h, _ptr(inp), _ptr(out), _ptr(out2), _stream(stream)))
@@ -691,9 +689,7 @@ class BatchedStaticMap:
         self.num_tables = len(capacities)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.mxco_batch_destroy(self._h)
-            self._h = None
+        _release(self, "mxco_batch_destroy")
 
     def capacity(self, t):
         c = u64()

@@ -76,7 +76,7 @@ LAYOUTS = [(np.int32, np.int32), (np.int64, np.int64), (np.int32, np.int64), (np
 
 @pytest.mark.parametrize("kd,vd", LAYOUTS, ids=lambda x: getattr(x, "__name__", str(x)))
 @pytest.mark.parametrize("probing", [O.PROBE_LINEAR, O.PROBE_DOUBLE])
-@pytest.mark.parametrize("bucket", [1, 5, 8])
+@pytest.mark.parametrize("bucket", [1, 5, 8, 17])
 def test_layouts_vs_oracle(kd, vd, probing, bucket):
     rng = np.random.default_rng(bucket * 10 + probing)
     n = 30000
@@ -223,17 +223,20 @@ def test_erase_with_and_without_tombstone(group):
 
 @pytest.mark.parametrize("group", GROUPS)
 @pytest.mark.parametrize("probing", [O.PROBE_LINEAR, O.PROBE_DOUBLE])
-def test_probe_stats_match_oracle_line_count(group, probing):
+@pytest.mark.parametrize("kd", [np.int64, np.int32])
+def test_probe_stats_match_oracle_line_count(group, probing, kd):
     """The roofline's algorithmic bytes use the lines Find's walk reads; the
-    kernel's count equals the oracle's replay on the same slot array."""
+    kernel's count equals the oracle's replay on the same slot array (16-byte
+    slots read one per vector, 8-byte slots in pairs)."""
     rng = np.random.default_rng(15)
-    m = mx().StaticMap(200000, -1, -1, torch.int64, torch.int64, probing=PROBE[probing], group_size=group)
+    m = mx().StaticMap(200000, -1, -1, TD[kd], TD[kd], probing=PROBE[probing], group_size=group)
     n = int(0.8 * m.capacity())
-    keys = distinct_keys(rng, n, 2**62, np.int64)
+    hi = 2**62 if kd == np.int64 else 2**30
+    keys = distinct_keys(rng, n, hi, kd)
     m.insert(dev_pairs(keys, keys))
-    ref = O.OpenAddressing(np.int64, np.int64, 200000, -1, -1, probing=probing)
+    ref = O.OpenAddressing(kd, kd, 200000, -1, -1, probing=probing)
     ref.load_slots(m.slots().cpu().numpy())
-    q = np.concatenate([keys[:50000], rng.integers(2**62, 2**63 - 1, 50000)])
+    q = np.concatenate([keys[:50000], rng.integers(hi, 2 * hi - 1, 50000).astype(kd)])
     lines, slots = m.probe_stats(dev(q))
     _, _, exp_lines = ref.find(q, lines=True)
     if probing == O.PROBE_LINEAR:
