@@ -39,7 +39,8 @@ typedef enum mxco_status {
     MXCO_OUT_OF_MEMORY = 3,    /* std::bad_alloc (device allocation failed) */
     MXCO_RUNTIME_ERROR = 4,    /* std::runtime_error (HIP runtime failure) */
     MXCO_LOGIC_ERROR = 5,      /* std::logic_error (moved-from / uninitialised object) */
-    MXCO_UNSUPPORTED = 6       /* configuration this build does not provide */
+    MXCO_UNSUPPORTED = 6,      /* configuration this build does not provide */
+    MXCO_OVERFLOW_ERROR = 7    /* std::overflow_error (a count that may not fit 64 bits) */
 } mxco_status;
 
 /* Last error message of the calling thread ("" after success). */
@@ -278,6 +279,55 @@ MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* map, const void* keys, uint64_t n
  * the stored pairs as a multiset (order not part of the contract). */
 MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* map, void* key_out, void* value_out, uint64_t output_capacity,
                                           hipStream_t stream, uint64_t* num_out);
+
+/* ---------------------------------------------------------------------- */
+/* StaticMultiset (include/static_multiset.h:36-575): int32 / int64 keys,  */
+/* exact multiplicities, accepted keys kept in insertion order.           */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_ms mxco_ms;
+
+/* Query kinds of mxco_ms_query. */
+enum {
+    MXCO_MS_CONTAINS = 0,        /* Contains / ContainsIf: 1-byte 0/1 */
+    MXCO_MS_FIND = 1,            /* Find / FindIf: the key, or the empty key */
+    MXCO_MS_COUNT_EACH = 2,      /* CountEach: uint64 multiplicity */
+    MXCO_MS_COUNT_EACH_OUTER = 3 /* CountEachOuter: uint64 multiplicity, zero written as one */
+};
+
+/* StaticMultiset(capacity, emptyKey, stream) (static_multiset.h:55-76):
+ * capacity elements including duplicates (> 0); key_size 4 or 8. */
+MXCO_API mxco_status mxco_ms_create(uint32_t key_size, uint64_t capacity, const void* empty_key, hipStream_t stream,
+                                    mxco_ms** out);
+MXCO_API mxco_status mxco_ms_destroy(mxco_ms* set);
+/* Capacity() (static_multiset.h:127-130; 0 for NULL) / Size(stream) (:136-141) / Clear(stream) (:146-157). */
+MXCO_API mxco_status mxco_ms_capacity(const mxco_ms* set, uint64_t* capacity);
+MXCO_API mxco_status mxco_ms_size(mxco_ms* set, hipStream_t stream, uint64_t* size);
+MXCO_API mxco_status mxco_ms_clear(mxco_ms* set, hipStream_t stream);
+/* Insert / InsertIf<Stencil, Predicate> (static_multiset.h:163-189): keeps
+ * the earliest selected non-sentinel keys that fit; *num_failed = selected
+ * keys not kept + selected sentinel keys (unselected inputs are not
+ * failures).  NULL keys with n > 0: *num_failed = n, nothing changes. */
+MXCO_API mxco_status mxco_ms_insert(mxco_ms* set, const void* keys, const mxco_stencil* stencil, uint64_t n,
+                                    hipStream_t stream, uint64_t* num_failed);
+/* Contains / Find / CountEach / CountEachOuter and the If forms
+ * (static_multiset.h:194-260): one output per query; unselected queries
+ * read as absent. */
+MXCO_API mxco_status mxco_ms_query(mxco_ms* set, int kind, const void* keys, const mxco_stencil* stencil, void* out,
+                                   uint64_t n, hipStream_t stream);
+/* Count(keys, n, stream) (static_multiset.h:290-294): total multiplicity;
+ * MXCO_OVERFLOW_ERROR when n * Size() may exceed 64 bits. */
+MXCO_API mxco_status mxco_ms_count(mxco_ms* set, const void* keys, uint64_t n, hipStream_t stream, uint64_t* count);
+/* Retrieve(keys, n, probeOutput, matchOutput, outputCapacity, stream)
+ * (static_multiset.h:296-321): every match in query order (query key as
+ * probe and match); MXCO_LENGTH_ERROR, nothing written, when the matches
+ * exceed output_capacity. */
+MXCO_API mxco_status mxco_ms_retrieve(mxco_ms* set, const void* keys, uint64_t n, void* probe_out, void* match_out,
+                                      uint64_t output_capacity, hipStream_t stream, uint64_t* num_out);
+/* RetrieveAll(output, outputCapacity, stream) (static_multiset.h:322-345):
+ * the kept keys in accepted insertion order. */
+MXCO_API mxco_status mxco_ms_retrieve_all(mxco_ms* set, void* out, uint64_t output_capacity, hipStream_t stream,
+                                          uint64_t* num_out);
 
 /* ---------------------------------------------------------------------- */
 /* Serving engine (no reference counterpart: a request loop over resident */

@@ -159,6 +159,7 @@ inline void check(mxco_status s)
     case MXCO_LENGTH_ERROR: throw std::length_error(msg);
     case MXCO_OUT_OF_MEMORY: throw std::bad_alloc();
     case MXCO_LOGIC_ERROR: throw std::logic_error(msg);
+    case MXCO_OVERFLOW_ERROR: throw std::overflow_error(msg);
     default: throw std::runtime_error(msg);
     }
 }

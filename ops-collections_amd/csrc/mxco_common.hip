@@ -176,6 +176,7 @@ MXCO_API const char* mxco_status_string(mxco_status s)
     case MXCO_RUNTIME_ERROR: return "runtime error";
     case MXCO_LOGIC_ERROR: return "logic error";
     case MXCO_UNSUPPORTED: return "unsupported";
+    case MXCO_OVERFLOW_ERROR: return "overflow error";
     }
     return "unknown";
 }

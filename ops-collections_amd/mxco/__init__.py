@@ -27,7 +27,7 @@ u64 = ctypes.c_uint64
 i32 = ctypes.c_int
 pu64 = ctypes.POINTER(ctypes.c_uint64)
 
-OK, INVALID_ARGUMENT, LENGTH_ERROR, OUT_OF_MEMORY, RUNTIME_ERROR, LOGIC_ERROR, UNSUPPORTED = range(7)
+OK, INVALID_ARGUMENT, LENGTH_ERROR, OUT_OF_MEMORY, RUNTIME_ERROR, LOGIC_ERROR, UNSUPPORTED, OVERFLOW_ERROR = range(8)
 PROBING = {"linear": 0, "double": 1}
 HASH = {"murmurhash3_32": 0, "xxhash_32": 1, "xxhash_64": 2, "murmurhash3_fmix32": 3, "murmurhash3_fmix64": 4}
 PRED = {"always": 0, "nonzero": 1, "odd": 2, "even": 3}
@@ -51,7 +51,8 @@ class LogicError(RuntimeError):
 
 
 _EXC = {INVALID_ARGUMENT: ValueError, LENGTH_ERROR: LengthError, OUT_OF_MEMORY: MemoryError,
-        RUNTIME_ERROR: RuntimeError, LOGIC_ERROR: LogicError, UNSUPPORTED: NotImplementedError}
+        RUNTIME_ERROR: RuntimeError, LOGIC_ERROR: LogicError, UNSUPPORTED: NotImplementedError,
+        OVERFLOW_ERROR: OverflowError}
 
 
 class OaDesc(ctypes.Structure):
@@ -607,6 +608,109 @@ class StaticMultimap:
         c = u64()
         check(_lib.mxco_mm_retrieve_all(self._h, _ptr(key_out), _ptr(value_out), cap, _stream(stream),
                                         ctypes.byref(c)))
+        return c.value
+
+
+# ---------------------------------------------------------------------------
+# StaticMultiset
+# ---------------------------------------------------------------------------
+
+MS_CONTAINS, MS_FIND, MS_COUNT_EACH, MS_COUNT_EACH_OUTER = 0, 1, 2, 3
+_sig("mxco_ms_create", [u32, u64, vp, vp, ctypes.POINTER(vp)])
+_sig("mxco_ms_destroy", [vp])
+_sig("mxco_ms_capacity", [vp, pu64])
+_sig("mxco_ms_size", [vp, vp, pu64])
+_sig("mxco_ms_clear", [vp, vp])
+_sig("mxco_ms_insert", [vp, vp, ctypes.POINTER(Stencil), u64, vp, pu64])
+_sig("mxco_ms_query", [vp, i32, vp, ctypes.POINTER(Stencil), vp, u64, vp])
+_sig("mxco_ms_count", [vp, vp, u64, vp, pu64])
+_sig("mxco_ms_retrieve", [vp, vp, u64, vp, vp, u64, vp, pu64])
+_sig("mxco_ms_retrieve_all", [vp, vp, u64, vp, pu64])
+
+
+class StaticMultiset:
+    """aclco::StaticMultiset<Key> (include/static_multiset.h:36): int32 or
+    int64 keys with exact multiplicities, kept keys in insertion order."""
+
+    def __init__(self, capacity, empty_key, key_dtype=torch.int64, stream=None):
+        self.key_dtype = key_dtype
+        self.empty_key = empty_key
+        self._ek = _scalar_bytes(empty_key, key_dtype)
+        h = vp()
+        check(_lib.mxco_ms_create(torch.empty(0, dtype=key_dtype).element_size(), capacity, self._ek,
+                                  _stream(stream), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        _release(self, "mxco_ms_destroy")
+
+    def capacity(self):
+        c = u64()
+        check(_lib.mxco_ms_capacity(self._h, ctypes.byref(c)))
+        return c.value
+
+    def size(self, stream=None):
+        c = u64()
+        check(_lib.mxco_ms_size(self._h, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def clear(self, stream=None):
+        check(_lib.mxco_ms_clear(self._h, _stream(stream)))
+
+    def insert(self, keys, n=None, stream=None):
+        n = (0 if keys is None else keys.shape[0]) if n is None else n
+        f = u64()
+        check(_lib.mxco_ms_insert(self._h, _ptr(keys), None, n, _stream(stream), ctypes.byref(f)))
+        return f.value
+
+    def insert_if(self, keys, stencil, pred, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        if n and keys is not None and stencil is None:
+            raise ValueError("StaticMultiset: null pointer for nonempty operation")
+        f = u64()
+        check(_lib.mxco_ms_insert(self._h, _ptr(keys), _stencil(stencil, pred), n, _stream(stream),
+                                  ctypes.byref(f)))
+        return f.value
+
+    def _query(self, kind, keys, out, dtype, n, stream, stencil, pred):
+        n = keys.shape[0] if n is None else n
+        if out is None and n:
+            out = torch.empty(n, dtype=dtype, device="cuda")
+        check(_lib.mxco_ms_query(self._h, kind, _ptr(keys), _stencil(stencil, pred), _ptr(out), n,
+                                 _stream(stream)))
+        return out
+
+    def contains(self, keys, out=None, n=None, stream=None, stencil=None, pred="always"):
+        return self._query(MS_CONTAINS, keys, out, torch.bool, n, stream, stencil, pred)
+
+    def find(self, keys, out=None, n=None, stream=None, stencil=None, pred="always"):
+        return self._query(MS_FIND, keys, out, self.key_dtype, n, stream, stencil, pred)
+
+    def count_each(self, keys, out=None, n=None, stream=None):
+        """uint64 multiplicities (returned as int64)."""
+        return self._query(MS_COUNT_EACH, keys, out, torch.int64, n, stream, None, "always")
+
+    def count_each_outer(self, keys, out=None, n=None, stream=None):
+        return self._query(MS_COUNT_EACH_OUTER, keys, out, torch.int64, n, stream, None, "always")
+
+    def count(self, keys, n=None, stream=None):
+        n = (0 if keys is None else keys.shape[0]) if n is None else n
+        c = u64()
+        check(_lib.mxco_ms_count(self._h, _ptr(keys), n, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def retrieve(self, keys, probe_out, match_out, output_capacity=None, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        cap = (0 if match_out is None else match_out.shape[0]) if output_capacity is None else output_capacity
+        c = u64()
+        check(_lib.mxco_ms_retrieve(self._h, _ptr(keys), n, _ptr(probe_out), _ptr(match_out), cap,
+                                    _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def retrieve_all(self, out, output_capacity=None, stream=None):
+        cap = (0 if out is None else out.shape[0]) if output_capacity is None else output_capacity
+        c = u64()
+        check(_lib.mxco_ms_retrieve_all(self._h, _ptr(out), cap, _stream(stream), ctypes.byref(c)))
         return c.value
 
 

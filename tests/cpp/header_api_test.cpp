@@ -11,6 +11,7 @@
 #include <vector>
 
 #include <algorithm>
+#include <limits>
 
 #include "mxco/aclco.h"
 
@@ -190,6 +191,37 @@ int main()
         CHECK(hipFree(dq) == hipSuccess);
         CHECK(hipFree(dv) == hipSuccess);
         CHECK(hipFree(dprobe) == hipSuccess);
+    }
+    {  // StaticMultiset<int32> (tests/static_multiset/semantics_test.cpp:87-129)
+        aclco::StaticMultiset<int32_t> ms(aclco::Extent<size_t>{5}, std::numeric_limits<int32_t>::lowest(), s);
+        std::vector<int32_t> keys = {7, 2, 7, std::numeric_limits<int32_t>::max(), -4, 2, 9};
+        auto* dk = to_device(keys);
+        CHECK(ms.Insert(dk, keys.size(), s) == 2);
+        CHECK(ms.Size(s) == 5);
+        int32_t* dout = nullptr;
+        CHECK(hipMalloc(&dout, 8 * sizeof(int32_t)) == hipSuccess);
+        CHECK(ms.RetrieveAll(dout, 8, s) == 5);
+        auto all = to_host(dout, 5);
+        CHECK(std::equal(all.begin(), all.end(), keys.begin()));
+        std::vector<int32_t> q = {7, 8, 7, -4};
+        auto* dq = to_device(q);
+        uint64_t* dc = nullptr;
+        CHECK(hipMalloc(&dc, 4 * sizeof(uint64_t)) == hipSuccess);
+        ms.CountEach(dq, dc, q.size(), s);
+        auto c = to_host(dc, 4);
+        CHECK(c[0] == 2 && c[1] == 0 && c[2] == 2 && c[3] == 1);
+        CHECK(ms.Count(dq, q.size(), s) == 5);
+        bool threw = false;
+        try {
+            (void)ms.RetrieveAll(dout, 4, s);
+        } catch (std::length_error const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        CHECK(hipFree(dk) == hipSuccess);
+        CHECK(hipFree(dq) == hipSuccess);
+        CHECK(hipFree(dc) == hipSuccess);
+        CHECK(hipFree(dout) == hipSuccess);
     }
     CHECK(hipStreamDestroy(s) == hipSuccess);
     std::printf("OK\n");

@@ -430,3 +430,49 @@ def test_multimap_oracle_vs_brute_force():
     assert list(zip(probe.tolist(), vals.tolist())) == exp
     with pytest.raises(ValueError):
         mm.insert(np.array([1, -1]), np.array([2, 3]))
+
+
+# --------------------------------------------------------------------------
+# StaticMultiset restatement against the reference's own test vectors
+# --------------------------------------------------------------------------
+
+def test_multiset_oracle_reference_vectors():
+    from multiset import MultisetOracle
+    # tests/static_multiset/semantics_test.cpp:87-129 (int32 and int64 alike)
+    for big in (2**31 - 1, 2**63 - 1):
+        ms = MultisetOracle(5, -big - 1)
+        keys = [7, 2, 7, big, -4, 2, 9]
+        assert ms.insert(keys) == 2
+        assert ms.retrieve_all().tolist() == keys[:5]
+        q = [7, 8, 7, -4]
+        assert ms.count(q) == 5
+        probe, match = ms.retrieve(q)
+        assert probe.tolist() == [7, 7, 7, 7, -4] and match.tolist() == probe.tolist()
+    # tests/static_multiset/insert_test.cpp:60-74: capacity boundary
+    ms = MultisetOracle(10, -2**31)
+    assert ms.insert(list(range(25))) == 15 and ms.size() == 10
+    # sentinel keys fail, unselected inputs do not
+    ms = MultisetOracle(4, -1)
+    assert ms.insert([1, -1, 2, -1, 3], selected=[1, 1, 0, 0, 1]) == 1
+    assert ms.retrieve_all().tolist() == [1, 3]
+
+
+def test_multiset_oracle_brute_force():
+    from multiset import MultisetOracle
+    rng = np.random.default_rng(12)
+    ms = MultisetOracle(300, -1)
+    kept = []
+    for _ in range(5):
+        k = rng.integers(-1, 40, 100)
+        sel = rng.integers(0, 3, 100) > 0
+        failed = ms.insert(k, sel)
+        chosen = [int(a) for a, s in zip(k, sel) if s]
+        valid = [a for a in chosen if a != -1]
+        take = valid[:300 - len(kept)]
+        kept += take
+        assert failed == len(chosen) - len(take)
+    assert ms.retrieve_all().tolist() == kept
+    q = np.arange(-2, 45)
+    assert ms.count_each(q).tolist() == [kept.count(int(x)) if x != -1 else 0 for x in q]
+    assert ms.count(q) == sum(kept.count(int(x)) for x in q if x != -1)
+    assert ms.count_each_outer(q).tolist() == [max(1, kept.count(int(x))) if x != -1 else 1 for x in q]
