@@ -101,7 +101,7 @@ typedef struct mxco_oa_desc {
     uint32_t probing;     /* mxco_probing */
     uint32_t hash;        /* mxco_hash */
     uint32_t group_size;  /* lanes per key: 0 = tuned default, or 1, 2, 4, 8 */
-    uint32_t reserved;
+    uint32_t value_kind;  /* mxco_key_kind of the payload (read by DynamicMap's Find merge) */
 } mxco_oa_desc;
 
 typedef struct mxco_oa mxco_oa;
@@ -279,6 +279,46 @@ MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* map, const void* keys, uint64_t n
  * the stored pairs as a multiset (order not part of the contract). */
 MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* map, void* key_out, void* value_out, uint64_t output_capacity,
                                           hipStream_t stream, uint64_t* num_out);
+
+/* ---------------------------------------------------------------------- */
+/* DynamicMap (include/dynamic_map.h:26-625): StaticMap submaps grown on   */
+/* demand (room floor(0.6 c) - 10000 per submap, doubling capacities),     */
+/* exact deduplication across submaps, erase by tombstone.                 */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_dm mxco_dm;
+
+/* InsertMode (dynamic_map.h:36): exact deduplication, or unique keys appended. */
+enum { MXCO_DM_EXACT_DEDUP = 0, MXCO_DM_APPEND_UNIQUE = 1 };
+
+/* DynamicMap(initialCapacity, emptyKey, emptyValue, erasedKey, ..., stream)
+ * (dynamic_map.h:92-130): maps only; MXCO_INVALID_ARGUMENT when emptyKey ==
+ * erasedKey. */
+MXCO_API mxco_status mxco_dm_create(const mxco_oa_desc* desc, uint64_t initial_capacity, const void* empty_key,
+                                    const void* empty_value, const void* erased_key, hipStream_t stream, mxco_dm** out);
+MXCO_API mxco_status mxco_dm_destroy(mxco_dm* map);
+/* Size() / Capacity() (sum of submap capacities) / NumSubmaps() (dynamic_map.h:133-137). */
+MXCO_API mxco_status mxco_dm_info(const mxco_dm* map, uint64_t* size, uint64_t* capacity, uint64_t* num_submaps);
+/* Reserve(n, stream) (dynamic_map.h:142-165). */
+MXCO_API mxco_status mxco_dm_reserve(mxco_dm* map, uint64_t n, hipStream_t stream);
+/* Insert(values, n, stream[, mode]) (sync != 0, dynamic_map.h:168-219): the
+ * number of pairs newly inserted; InsertAsync (sync == 0, :221-235) and
+ * MXCO_DM_APPEND_UNIQUE count every pair as inserted.  NULL / 0: nothing. */
+MXCO_API mxco_status mxco_dm_insert(mxco_dm* map, const void* pairs, uint64_t n, int mode, int sync,
+                                    hipStream_t stream, uint64_t* inserted);
+/* InsertOrAssign(values, n, stream) (dynamic_map.h:237-259): new keys inserted (assignments not counted). */
+MXCO_API mxco_status mxco_dm_insert_or_assign(mxco_dm* map, const void* pairs, uint64_t n, hipStream_t stream,
+                                              uint64_t* inserted);
+/* Find / Contains (dynamic_map.h:262-315): every non-empty submap, merged. */
+MXCO_API mxco_status mxco_dm_find(mxco_dm* map, const void* keys, void* out, uint64_t n, hipStream_t stream);
+MXCO_API mxco_status mxco_dm_contains(mxco_dm* map, const void* keys, uint8_t* out, uint64_t n, hipStream_t stream);
+/* Erase (sync != 0: the number erased, dynamic_map.h:318-335) / EraseAsync (:337-347, sizes unchanged). */
+MXCO_API mxco_status mxco_dm_erase(mxco_dm* map, const void* keys, uint64_t n, int sync, hipStream_t stream,
+                                   uint64_t* erased);
+/* Clear(stream) (dynamic_map.h:349-357): every submap cleared, submaps kept. */
+MXCO_API mxco_status mxco_dm_clear(mxco_dm* map, hipStream_t stream);
+/* Submap i (owned by the DynamicMap), e.g. for its slot array. */
+MXCO_API mxco_status mxco_dm_submap(const mxco_dm* map, uint32_t index, mxco_oa** submap);
 
 /* ---------------------------------------------------------------------- */
 /* StaticMultiset (include/static_multiset.h:36-575): int32 / int64 keys,  */

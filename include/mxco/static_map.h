@@ -47,6 +47,7 @@ public:
         d.bucket_size = static_cast<uint32_t>(bucketSize);
         d.probing = ProbingScheme::id;
         d.hash = ProbingScheme::hash;
+        d.value_kind = std::is_floating_point_v<T> ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
         mxco_oa* h = nullptr;
         detail::check(mxco_oa_create(&d, static_cast<SizeType>(capacity), &emptyKey, &emptyValue, stream, &h));
         impl_.reset(h);

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void arena_fill_kernel(uint8_t* p, uint64_t by
 {
     const uint64_t n16 = bytes / 16;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(pattern, (u64x2*)p + i);
+        ((u64x2*)p)[i] = pattern;
 }
 
 }  // namespace
@@ -269,7 +269,8 @@ MXCO_API mxco_status mxco_batch_clear(mxco_batch* b, hipStream_t stream)
     }
     u64x2 pat;
     memcpy(&pat, img, 16);
-    arena_fill_kernel<<<grid_for(b->arena_bytes / 16, 8), 256, 0, stream>>>(b->arena, b->arena_bytes, pat);
+    arena_fill_kernel<<<grid_for(b->arena_bytes / 16, kFillBlocksPerCu), 256, 0, stream>>>(b->arena, b->arena_bytes,
+                                                                                           pat);
     MXCO_HIP(hipGetLastError(), "mxco_batch_clear");
     return ok();
 }
@@ -364,7 +365,7 @@ MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, vo
 {
     if (!p || !in) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_tune: null argument");
     static const int groups[] = {1, 2, 4, 8};
-    static const unsigned bpcs[] = {8, 32, 128, 256};
+    static const unsigned bpcs[] = {8, 32, 64, 128, 256, 512};
     hipEvent_t e0, e1;
     MXCO_HIP(hipEventCreate(&e0), "tune");
     MXCO_HIP(hipEventCreate(&e1), "tune");

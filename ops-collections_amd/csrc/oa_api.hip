@@ -149,7 +149,7 @@ mxco_status fill(mxco_oa* t, hipStream_t stream)
     u64x2 pat;
     memcpy(&pat, img, 16);
     const uint64_t bytes = t->cap * t->ss;
-    const unsigned grid = grid_for(bytes / 16, 8);
+    const unsigned grid = grid_for(bytes / 16, kFillBlocksPerCu);
     oa_fill_kernel<<<grid, 256, 0, stream>>>(t->slots, bytes, pat);
     MXCO_HIP(hipGetLastError(), "Clear");
     return MXCO_OK;

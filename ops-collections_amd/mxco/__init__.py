@@ -57,7 +57,7 @@ _EXC = {INVALID_ARGUMENT: ValueError, LENGTH_ERROR: LengthError, OUT_OF_MEMORY: 
 
 class OaDesc(ctypes.Structure):
     _fields_ = [("key_size", u32), ("value_size", u32), ("key_kind", u32), ("bucket_size", u32),
-                ("probing", u32), ("hash", u32), ("group_size", u32), ("reserved", u32)]
+                ("probing", u32), ("hash", u32), ("group_size", u32), ("value_kind", u32)]
 
 
 class Stencil(ctypes.Structure):
@@ -180,7 +180,7 @@ class _OpenAddressing:
         ks = torch.empty(0, dtype=key_dtype).element_size()
         vs = 0 if value_dtype is None else torch.empty(0, dtype=value_dtype).element_size()
         d = OaDesc(ks, vs, 1 if key_dtype.is_floating_point else 0, bucket_size, PROBING[probing], HASH[hash],
-                   group_size, 0)
+                   group_size, 1 if value_dtype is not None and value_dtype.is_floating_point else 0)
         self._ek = _scalar_bytes(empty_key, key_dtype)
         self._ev = _scalar_bytes(empty_value, value_dtype) if value_dtype is not None else None
         h = vp()
