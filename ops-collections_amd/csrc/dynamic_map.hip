@@ -8,8 +8,11 @@
 //   - Find / Contains query every non-empty submap and merge on the device
 //     (a key lives in at most one submap); Erase visits every non-empty
 //     submap; submaps use the erased key as tombstone.
-// The reference also pre-clears the next submap on a second stream; growth
-// here is synchronous, which changes only when the clearing happens.
+// The reference prepares the next submap ahead of need (StartPrepareNext,
+// dynamic_map.h:470-486) and clears it on a second stream; the same rule
+// decides here when that submap is made and what capacity it gets, so
+// Capacity() and NumSubmaps() follow the reference, but it is cleared at
+// once on the caller's stream.
 #include <math.h>
 #include <string.h>
 
@@ -102,6 +105,8 @@ struct mxco_dm {
     std::vector<mxco_oa*> subs;
     std::vector<uint64_t> sub_size;
     uint64_t size = 0, total_cap = 0, next_cap = 0;
+    mxco_oa* next_sub = nullptr;  // prepared next submap (not yet active)
+    uint64_t next_sub_cap = 0;    // its requested capacity
     uint8_t* scratch = nullptr;  // Find / Contains merge input
     size_t scratch_bytes = 0;
     uint8_t* dedup = nullptr;  // keys | mask | tmp
@@ -125,7 +130,7 @@ uint64_t capacity_of(const mxco_oa* t)
     return c;
 }
 
-mxco_status add_submap(mxco_dm* m, uint64_t cap, hipStream_t s)
+mxco_status make_submap(mxco_dm* m, uint64_t cap, hipStream_t s, mxco_oa** out)
 {
     mxco_oa* t = nullptr;
     if (mxco_status e = mxco_oa_create(&m->d, cap, m->empty_key, m->empty_val, s, &t)) return e;
@@ -133,10 +138,30 @@ mxco_status add_submap(mxco_dm* m, uint64_t cap, hipStream_t s)
         (void)mxco_oa_destroy(t);
         return e;
     }
+    *out = t;
+    return MXCO_OK;
+}
+
+void push_submap(mxco_dm* m, mxco_oa* t)
+{
     m->subs.push_back(t);
     m->sub_size.push_back(0);
     m->total_cap += capacity_of(t);
+}
+
+mxco_status add_submap(mxco_dm* m, uint64_t cap, hipStream_t s)
+{
+    mxco_oa* t = nullptr;
+    if (mxco_status e = make_submap(m, cap, s, &t)) return e;
+    push_submap(m, t);
     return MXCO_OK;
+}
+
+void discard_prepared(mxco_dm* m)
+{
+    if (m->next_sub) (void)mxco_oa_destroy(m->next_sub);
+    m->next_sub = nullptr;
+    m->next_sub_cap = 0;
 }
 
 size_t active(const mxco_dm* m)
@@ -157,9 +182,30 @@ uint64_t grow_target(const mxco_dm* m, uint64_t min_usable)
     return by_growth > min_usable ? by_growth : min_usable;
 }
 
-// ActivateNext (dynamic_map.h:488-510), synchronous form.
+// StartPrepareNext (dynamic_map.h:470-486): the next submap, sized for
+// GrowTarget(min_usable), made before it is needed.
+mxco_status start_prepare_next(mxco_dm* m, uint64_t min_usable, hipStream_t s)
+{
+    if (m->next_sub) return MXCO_OK;
+    uint64_t cap = m->next_cap;
+    while (usable(cap) < grow_target(m, min_usable)) cap *= kGrowth;
+    if (mxco_status e = make_submap(m, cap, s, &m->next_sub)) return e;
+    m->next_sub_cap = cap;
+    return MXCO_OK;
+}
+
+// ActivateNext (dynamic_map.h:488-510): the prepared submap when its room
+// suffices, else a new one of the next capacity doubled until it does.
 mxco_status activate_next(mxco_dm* m, uint64_t min_usable, hipStream_t s)
 {
+    if (m->next_sub && usable(m->next_sub_cap) >= min_usable) {
+        push_submap(m, m->next_sub);
+        m->next_cap = m->next_sub_cap * kGrowth;
+        m->next_sub = nullptr;
+        m->next_sub_cap = 0;
+        return MXCO_OK;
+    }
+    discard_prepared(m);
     uint64_t cap = m->next_cap;
     while (usable(cap) < min_usable) cap *= kGrowth;
     if (mxco_status e = add_submap(m, cap, s)) return e;
@@ -335,6 +381,7 @@ MXCO_API mxco_status mxco_dm_destroy(mxco_dm* m)
 {
     if (!m) return ok();
     for (mxco_oa* t : m->subs) (void)mxco_oa_destroy(t);
+    discard_prepared(m);
     if (m->scratch) (void)hipFree(m->scratch);
     if (m->dedup) (void)hipFree(m->dedup);
     if (m->counter) (void)hipFree(m->counter);
@@ -386,6 +433,8 @@ MXCO_API mxco_status mxco_dm_insert(mxco_dm* m, const void* pairs, uint64_t n, i
     const uint8_t* p = (const uint8_t*)pairs;
     if (!sync || mode == MXCO_DM_APPEND_UNIQUE) {
         // optimistic count: InsertAsync, or unique keys appended in sub-batches
+        if (!sync && active_room(m) < n * 2)
+            if (mxco_status e = start_prepare_next(m, n, stream)) return e;
         const uint64_t step = sync ? kSubBatch : n;
         for (uint64_t off = 0; off < n; off += step) {
             const uint64_t cur = std::min(step, n - off);
@@ -412,6 +461,8 @@ MXCO_API mxco_status mxco_dm_insert(mxco_dm* m, const void* pairs, uint64_t n, i
             if (mxco_status e = activate_next(m, grow_target(m, n - off), stream)) return e;
             continue;
         }
+        if (room < (n - off) * 2)
+            if (mxco_status e = start_prepare_next(m, n - off, stream)) return e;
         const uint64_t chunk = std::min(n - off, room);
         uint64_t ins = 0;
         if (mxco_status e = insert_chunk(m, p + off * m->ss, chunk, stream, &ins)) return e;
@@ -438,6 +489,8 @@ MXCO_API mxco_status mxco_dm_insert_or_assign(mxco_dm* m, const void* pairs, uin
             if (mxco_status e = activate_next(m, grow_target(m, n - off), stream)) return e;
             continue;
         }
+        if (room < (n - off) * 2)
+            if (mxco_status e = start_prepare_next(m, n - off, stream)) return e;
         const uint64_t chunk = std::min(n - off, room);
         uint64_t ins = 0;
         if (mxco_status e = insert_or_assign_chunk(m, p + off * m->ss, chunk, stream, &ins)) return e;
@@ -525,6 +578,7 @@ MXCO_API mxco_status mxco_dm_erase(mxco_dm* m, const void* keys, uint64_t n, int
 MXCO_API mxco_status mxco_dm_clear(mxco_dm* m, hipStream_t stream)
 {
     if (mxco_status e = check_dm(m)) return e;
+    discard_prepared(m);
     for (mxco_oa* t : m->subs)
         if (mxco_status e = mxco_oa_clear(t, stream, 1)) return e;
     for (uint64_t& s : m->sub_size) s = 0;

@@ -612,6 +612,109 @@ class StaticMultimap:
 
 
 # ---------------------------------------------------------------------------
+# DynamicMap
+# ---------------------------------------------------------------------------
+
+DM_EXACT_DEDUP, DM_APPEND_UNIQUE = 0, 1
+_sig("mxco_dm_create", [ctypes.POINTER(OaDesc), u64, vp, vp, vp, vp, ctypes.POINTER(vp)])
+_sig("mxco_dm_destroy", [vp])
+_sig("mxco_dm_info", [vp, pu64, pu64, pu64])
+_sig("mxco_dm_reserve", [vp, u64, vp])
+_sig("mxco_dm_insert", [vp, vp, u64, i32, i32, vp, pu64])
+_sig("mxco_dm_insert_or_assign", [vp, vp, u64, vp, pu64])
+_sig("mxco_dm_find", [vp, vp, vp, u64, vp])
+_sig("mxco_dm_contains", [vp, vp, vp, u64, vp])
+_sig("mxco_dm_erase", [vp, vp, u64, i32, vp, pu64])
+_sig("mxco_dm_clear", [vp, vp])
+_sig("mxco_dm_submap", [vp, u32, ctypes.POINTER(vp)])
+
+
+class DynamicMap:
+    """aclco::DynamicMap<Key, T> (include/dynamic_map.h:26): StaticMap
+    submaps (LinearProbing<murmurhash3_32>, bucket 5) grown on demand."""
+
+    def __init__(self, initial_capacity, empty_key, empty_value, erased_key, key_dtype=torch.int64,
+                 value_dtype=torch.int64, bucket_size=5, probing="linear", hash="murmurhash3_32", stream=None):
+        self.key_dtype, self.value_dtype = key_dtype, value_dtype
+        ks = torch.empty(0, dtype=key_dtype).element_size()
+        vs = torch.empty(0, dtype=value_dtype).element_size()
+        d = OaDesc(ks, vs, 1 if key_dtype.is_floating_point else 0, bucket_size, PROBING[probing], HASH[hash], 0,
+                   1 if value_dtype.is_floating_point else 0)
+        self._ek = _scalar_bytes(empty_key, key_dtype)
+        self._ev = _scalar_bytes(empty_value, value_dtype)
+        self._rk = _scalar_bytes(erased_key, key_dtype)
+        h = vp()
+        check(_lib.mxco_dm_create(ctypes.byref(d), initial_capacity, self._ek, self._ev, self._rk, _stream(stream),
+                                  ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        _release(self, "mxco_dm_destroy")
+
+    def _info(self):
+        a, b, c = u64(), u64(), u64()
+        check(_lib.mxco_dm_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def size(self):
+        return self._info()[0]
+
+    def capacity(self):
+        return self._info()[1]
+
+    def num_submaps(self):
+        return self._info()[2]
+
+    def reserve(self, n, stream=None):
+        check(_lib.mxco_dm_reserve(self._h, n, _stream(stream)))
+
+    def insert(self, pairs, n=None, stream=None, mode=DM_EXACT_DEDUP):
+        n = (0 if pairs is None else pairs.shape[0]) if n is None else n
+        c = u64()
+        check(_lib.mxco_dm_insert(self._h, _ptr(pairs), n, mode, 1, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def insert_async(self, pairs, n=None, stream=None):
+        n = pairs.shape[0] if n is None else n
+        c = u64()
+        check(_lib.mxco_dm_insert(self._h, _ptr(pairs), n, DM_EXACT_DEDUP, 0, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def insert_or_assign(self, pairs, n=None, stream=None):
+        n = pairs.shape[0] if n is None else n
+        c = u64()
+        check(_lib.mxco_dm_insert_or_assign(self._h, _ptr(pairs), n, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def find(self, keys, out=None, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        if out is None and n:
+            out = torch.empty(n, dtype=self.value_dtype, device="cuda")
+        check(_lib.mxco_dm_find(self._h, _ptr(keys), _ptr(out), n, _stream(stream)))
+        return out
+
+    def contains(self, keys, out=None, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        if out is None and n:
+            out = torch.empty(n, dtype=torch.bool, device="cuda")
+        check(_lib.mxco_dm_contains(self._h, _ptr(keys), _ptr(out), n, _stream(stream)))
+        return out
+
+    def erase(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        c = u64()
+        check(_lib.mxco_dm_erase(self._h, _ptr(keys), n, 1, _stream(stream), ctypes.byref(c)))
+        return c.value
+
+    def erase_async(self, keys, n=None, stream=None):
+        n = keys.shape[0] if n is None else n
+        check(_lib.mxco_dm_erase(self._h, _ptr(keys), n, 0, _stream(stream), None))
+
+    def clear(self, stream=None):
+        check(_lib.mxco_dm_clear(self._h, _stream(stream)))
+
+
+# ---------------------------------------------------------------------------
 # StaticMultiset
 # ---------------------------------------------------------------------------
 

@@ -476,3 +476,30 @@ def test_multiset_oracle_brute_force():
     assert ms.count_each(q).tolist() == [kept.count(int(x)) if x != -1 else 0 for x in q]
     assert ms.count(q) == sum(kept.count(int(x)) for x in q if x != -1)
     assert ms.count_each_outer(q).tolist() == [max(1, kept.count(int(x))) if x != -1 else 1 for x in q]
+
+
+# --------------------------------------------------------------------------
+# DynamicMap growth rules (include/dynamic_map.h), hand-computed values and
+# the properties the reference's tests check (tests/dynamic_map/*.cpp)
+# --------------------------------------------------------------------------
+
+def test_dynamic_map_oracle_growth_rules():
+    import dynamic_map as D
+    # floor(0.60f * c) - 10000: 0.60f is 0.60000002384..., so 100000 -> 60000
+    assert D.usable(100000) == 50000 and D.usable(16666) == 0 and D.usable(200000) == 110000
+    m = D.DynamicMapOracle(100000, -1, -2)
+    assert m.num_submaps() == 1 and m.total_cap == 100000        # create_test.cpp:41-49
+    assert m.insert(range(50000), range(50000)) == 50000 and m.num_submaps() == 1
+    # one more key: room 0 -> GrowTarget(1) = 2 * size = 100000 -> next capacity 200000 (room 110000)
+    assert m.insert([10**9], [1]) == 1
+    assert m.caps == [100000, 200000] and m.next_cap == 400000
+    # re-inserting old keys after growth inserts nothing (insert_test.cpp:122-160)
+    assert m.insert(range(1000), range(5000, 6000)) == 0
+    assert m.find([0, 999], -1) == [0, 999] and m.size == 50001
+    # zero initial capacity grows (reserve_test.cpp:82-106)
+    z = D.DynamicMapOracle(0, -1, -2)
+    assert z.insert([7], [8]) == 1 and z.size == 1 and z.num_submaps() == 2
+    with pytest.raises(ValueError):
+        D.DynamicMapOracle(10, -1, -1)
+    # erase counts distinct present keys once
+    assert m.erase([0, 0, 1, 10**9, 123456789]) == 3 and m.size == 49998
