@@ -10,6 +10,9 @@
 #include <utility>
 
 #include "common.h"
+#if defined(__HIPCC__)
+#include "for_each.h"
+#endif
 
 namespace mxco {
 
@@ -194,6 +197,23 @@ public:
         return static_cast<ValueType*>(p);
     }
     Key EmptyKeySentinel() const noexcept { return emptyKey_; }
+
+#if defined(__HIPCC__)
+    // ForEach / ForEachAsync (the callback is device code: callers compile with hipcc).
+    template <typename CallbackOp>
+    void ForEachAsync(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)
+    {
+        detail::for_each_launch<Key, ValueType, CallbackOp>(Data(), Capacity(), static_cast<uint32_t>(bucketSize),
+                                                     ProbingScheme::id, ProbingScheme::hash, keys,
+                                                     static_cast<SizeType>(keyNum), emptyKey_, callbackArgs, stream);
+    }
+    template <typename CallbackOp>
+    void ForEach(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)
+    {
+        ForEachAsync<CallbackOp>(keys, keyNum, callbackArgs, stream);
+        if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("ForEach: synchronize failed");
+    }
+#endif
     T EmptyValueSentinel() const noexcept { return emptyValue_; }
 
 private:

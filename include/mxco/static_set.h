@@ -6,6 +6,9 @@
 #include <memory>
 
 #include "common.h"
+#if defined(__HIPCC__)
+#include "for_each.h"
+#endif
 
 namespace mxco {
 
@@ -146,6 +149,23 @@ public:
         return static_cast<Key*>(p);
     }
     Key EmptyKeySentinel() const noexcept { return emptyKey_; }
+
+#if defined(__HIPCC__)
+    // ForEach / ForEachAsync (the callback is device code: callers compile with hipcc).
+    template <typename CallbackOp>
+    void ForEachAsync(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)
+    {
+        detail::for_each_launch<Key, Key, CallbackOp>(Data(), Capacity(), static_cast<uint32_t>(bucketSize),
+                                                     ProbingScheme::id, ProbingScheme::hash, keys,
+                                                     static_cast<SizeType>(keyNum), emptyKey_, callbackArgs, stream);
+    }
+    template <typename CallbackOp>
+    void ForEach(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)
+    {
+        ForEachAsync<CallbackOp>(keys, keyNum, callbackArgs, stream);
+        if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("ForEach: synchronize failed");
+    }
+#endif
 
 private:
     struct Deleter {

@@ -49,15 +49,19 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
-    # C++ header-API check program (run by tests/test_header_api_gpu.py).
-    test_src = os.path.join(HERE, "..", "tests", "cpp", "header_api_test.cpp")
-    if os.path.exists(test_src):
-        exe = os.path.join(LIBDIR, "header_api_test")
-        cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), test_src, "-L", LIBDIR,
-               "-lmxco", "-Wl,-rpath,$ORIGIN", "-o", exe]
+    # C++ header-API check programs (run by tests/test_header_api_gpu.py): the
+    # host-only header use, and ForEach, whose callback is device code.
+    for name, src in (("header_api_test", "header_api_test.cpp"), ("for_each_test", "for_each_test.hip")):
+        test_src = os.path.join(HERE, "..", "tests", "cpp", src)
+        if not os.path.exists(test_src):
+            continue
+        exe = os.path.join(LIBDIR, name)
+        cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include"),
+               test_src, "-L", LIBDIR, "-lmxco", "-Wl,-rpath,$ORIGIN", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError("header API test build failed:\n%s" % r.stderr)
+            raise RuntimeError("%s build failed:\n%s\n%s" % (name, " ".join(cmd), r.stderr))
     if verbose:
         print("built", LIB)
     return LIB

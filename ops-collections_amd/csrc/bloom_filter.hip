@@ -28,7 +28,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "mxco_common.h"
-#include "mxco_hash.h"
+#include "mxco/detail/hash.h"
 
 using namespace mxco;
 

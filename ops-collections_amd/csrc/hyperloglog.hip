@@ -22,7 +22,7 @@
 #include <string>
 
 #include "mxco_common.h"
-#include "mxco_hash.h"
+#include "mxco/detail/hash.h"
 
 using namespace mxco;
 

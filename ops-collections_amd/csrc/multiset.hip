@@ -24,7 +24,7 @@
 #include <string>
 
 #include "mxco_common.h"
-#include "mxco_hash.h"
+#include "mxco/detail/hash.h"
 
 using namespace mxco;
 

@@ -18,7 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "mxco_hash.h"
+#include "mxco/detail/hash.h"
 
 namespace mxco {
 
