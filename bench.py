@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-streams", action="store_true", help="skip the Zipf duplicate-heavy insert stream")
     ap.add_argument("--no-filters", action="store_true", help="skip the BloomFilter and HyperLogLog legs")
     ap.add_argument("--hll-sketches", type=int, default=64)
+    ap.add_argument("--no-roaring", action="store_true", help="skip the RoaringBitmap query leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -280,6 +281,52 @@ def run_hll(torch, mxco, sketches=64, n=1 << 28):
             "merge_is_register_max": merged_ok, "within_tolerance": rel <= tol,
             "roofline_add": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_key": 8}}
+
+
+def run_roaring(torch, mxco, steps=5):
+    """RoaringBitmap Contains of 2^28 uint32 queries on two bitmaps:
+    "spec", the RoaringFormatSpec key set of the reference's tests (200,100
+    keys in 11 array/bitset/run containers, 48 KB) queried uniformly over
+    [0, 2^20); "wide", 2^26 uniform keys over the whole 2^32 range (65,536
+    array containers of ~1,024 values, 128 MiB) queried half from the keys,
+    half uniformly.  Each query reads a 4-byte key and writes a 1-byte
+    result; the container data it touches is extra (cache-resident for
+    "spec")."""
+    import numpy as np
+    from mxco import workloads as W
+    n = 1 << 28
+    g = torch.Generator(device="cuda")
+    g.manual_seed(41)
+    out = torch.empty(n, dtype=torch.bool, device="cuda")
+    res = {}
+    spec = (list(range(0, 100000, 1000)) + [3 * x for x in range(100000, 200000)]
+            + list(range(700000, 800000)))
+    wide = np.random.default_rng(41).integers(0, 1 << 32, 1 << 26, dtype=np.uint64)
+    for name, keys, lo_hi in (("spec", spec, (0, 1 << 20)), ("wide", wide, None)):
+        bm = W.roaring_portable32(keys, "optimize")
+        rb = mxco.RoaringBitmap(bm, torch.int32)
+        if lo_hi:
+            q = torch.randint(lo_hi[0], lo_hi[1], (n,), dtype=torch.int64, device="cuda", generator=g)
+        else:
+            kd = torch.from_numpy(np.unique(keys).astype(np.int64)).cuda()
+            pick = torch.randint(0, kd.numel(), (n // 2,), device="cuda", generator=g)
+            q = torch.cat([kd[pick], torch.randint(0, 1 << 32, (n - n // 2,), dtype=torch.int64, device="cuda",
+                                                   generator=g)])
+            del kd
+        q = torch.where(q >= 1 << 31, q - (1 << 32), q).to(torch.int32)
+        rb.contains(q, out=out)
+        ms = _timed(torch, lambda: rb.contains(q, out=out, sync=False), steps)
+        # spot check of 2^16 results against the key set
+        qs = q[: 1 << 16].cpu().numpy().view(np.uint32).astype(np.uint64)
+        ok = bool(np.array_equal(out[: 1 << 16].cpu().numpy(), np.isin(qs, np.asarray(keys, dtype=np.uint64))))
+        gbs = n * 5 / (ms * 1e-3) / 1e9
+        res[name] = {"bitmap_bytes": len(bm), "keys": rb.size(), "queries": n, "contains_ms": round(ms, 4),
+                     "contains_gkeys_s": round(n / ms / 1e6, 3), "hit_rate": round(out.float().mean().item(), 4),
+                     "spot_check": ok,
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_key": 5}}
+        del rb, q
+    return res
 
 
 def run_batched(args, torch, mxco, steps=3, warmup=1):
@@ -534,6 +581,10 @@ def main():
         log("bench: hyperloglog")
         line["hll"] = run_hll(torch, mxco, sketches=args.hll_sketches)
         log("bench: hll %s" % json.dumps(line["hll"]))
+    if not args.no_roaring:
+        log("bench: roaring bitmap")
+        line["roaring"] = run_roaring(torch, mxco)
+        log("bench: roaring %s" % json.dumps(line["roaring"]))
     if not args.no_batched:
         log("bench: batched 1024 tables")
         line["batched"] = run_batched(args, torch, mxco)

@@ -321,6 +321,41 @@ MXCO_API mxco_status mxco_dm_clear(mxco_dm* map, hipStream_t stream);
 MXCO_API mxco_status mxco_dm_submap(const mxco_dm* map, uint32_t index, mxco_oa** submap);
 
 /* ---------------------------------------------------------------------- */
+/* RoaringBitmap (include/roaring_bitmap.h:24-143): membership of uint32 /  */
+/* uint64 keys in a bitmap given in the Roaring portable serialization.    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mxco_rb mxco_rb;
+
+/* RoaringBitmap(bitmap[, bitmapBytes], allocator, stream) (roaring_bitmap.h:58-77):
+ * key_size 4 (uint32) or 8 (uint64, the 64-bit portable bucket format);
+ * bitmap is host memory, bitmap_bytes its length (SIZE_MAX: not known, the
+ * caller vouches for complete input).  Malformed, truncated or unordered
+ * input: MXCO_INVALID_ARGUMENT. */
+MXCO_API mxco_status mxco_rb_create(uint32_t key_size, const void* bitmap, size_t bitmap_bytes, hipStream_t stream,
+                                    mxco_rb** out);
+MXCO_API mxco_status mxco_rb_destroy(mxco_rb* bitmap);
+/* Size(), SizeBytes(), Data() and the device view behind Ref() (roaring_bitmap.h:111-121):
+ * bytes is the device copy of the serialized bitmap, buckets / containers
+ * the device bucket table and container directory that
+ * mxco/detail/roaring.h (RbBucket, RbContainer, rb_contains) searches. */
+typedef struct mxco_rb_view {
+    const uint8_t* bytes;
+    const void* buckets;
+    const void* containers;
+    uint32_t num_buckets;
+    uint32_t reserved;
+    uint64_t num_keys;   /* Size() */
+    uint64_t size_bytes; /* SizeBytes() */
+} mxco_rb_view;
+MXCO_API mxco_status mxco_rb_info(const mxco_rb* bitmap, mxco_rb_view* view);
+/* Contains / ContainsAsync(keys, outputValues, keyNum, stream) (roaring_bitmap.h:95-108): 1-byte bool per key.
+ * keyNum 0 returns at once; null keys or outputValues, or a moved-from
+ * (null) bitmap: MXCO_INVALID_ARGUMENT. */
+MXCO_API mxco_status mxco_rb_contains(const mxco_rb* bitmap, const void* keys, uint8_t* out, uint64_t n,
+                                      hipStream_t stream, int sync);
+
+/* ---------------------------------------------------------------------- */
 /* StaticMultiset (include/static_multiset.h:36-575): int32 / int64 keys,  */
 /* exact multiplicities, accepted keys kept in insertion order.           */
 /* ---------------------------------------------------------------------- */

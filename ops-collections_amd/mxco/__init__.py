@@ -612,6 +612,58 @@ class StaticMultimap:
 
 
 # ---------------------------------------------------------------------------
+# RoaringBitmap
+# ---------------------------------------------------------------------------
+
+_sig("mxco_rb_create", [u32, vp, ctypes.c_size_t, vp, ctypes.POINTER(vp)])
+_sig("mxco_rb_destroy", [vp])
+class RbView(ctypes.Structure):
+    """mxco_rb_view: device bytes, bucket table and container directory."""
+    _fields_ = [("bytes", vp), ("buckets", vp), ("containers", vp), ("num_buckets", u32), ("reserved", u32),
+                ("num_keys", u64), ("size_bytes", u64)]
+
+
+_sig("mxco_rb_info", [vp, ctypes.POINTER(RbView)])
+_sig("mxco_rb_contains", [vp, vp, vp, u64, vp, i32])
+
+
+class RoaringBitmap:
+    """aclco::RoaringBitmap<Key> (include/roaring_bitmap.h:24): `bitmap` is
+    the portable serialization (bytes); key_dtype int32 (uint32 keys) or int64
+    (uint64 keys)."""
+
+    def __init__(self, bitmap, key_dtype=torch.int32, stream=None):
+        self.key_dtype = key_dtype
+        self._bytes = bytes(bitmap)
+        buf = ctypes.create_string_buffer(self._bytes, len(self._bytes))
+        h = vp()
+        check(_lib.mxco_rb_create(torch.empty(0, dtype=key_dtype).element_size(), buf, len(self._bytes),
+                                  _stream(stream), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        _release(self, "mxco_rb_destroy")
+
+    def view(self):
+        v = RbView()
+        check(_lib.mxco_rb_info(self._h, ctypes.byref(v)))
+        return v
+
+    def size(self):
+        return self.view().num_keys
+
+    def size_bytes(self):
+        return self.view().size_bytes
+
+    def contains(self, keys, out=None, n=None, stream=None, sync=True):
+        n = keys.shape[0] if n is None else n
+        if out is None:
+            out = torch.empty(n, dtype=torch.bool, device="cuda")
+        check(_lib.mxco_rb_contains(self._h, _ptr(keys), _ptr(out), n, _stream(stream), int(sync)))
+        return out
+
+
+# ---------------------------------------------------------------------------
 # DynamicMap
 # ---------------------------------------------------------------------------
 

@@ -151,3 +151,61 @@ class Zipf:
     @staticmethod
     def keys(ranks):
         return ranks * ZIPF_MULT
+
+
+def roaring_portable32(keys, runs=False):
+    """Roaring portable serialization (RoaringFormatSpec) of a set of uint32
+    keys, vectorised for large key sets (bench inputs).  runs=False writes
+    array (<= 4096 values) and bitset containers with cookie 12346; with
+    runs="optimize" a container is written as runs when 2 + 4 * runs is
+    below its array (2 + 2 * cardinality) or bitset (8192) size."""
+    k = np.unique(np.asarray(keys, dtype=np.uint64) & 0xFFFFFFFF).astype(np.uint32)
+    hi, lo = (k >> 16).astype(np.int64), (k & 0xFFFF).astype(np.int64)
+    hks, first, cards = np.unique(hi, return_index=True, return_counts=True)
+    n = len(hks)
+    brk = np.ones(len(k), dtype=bool)  # first value of each run
+    if len(k):
+        brk[1:] = (hi[1:] != hi[:-1]) | (lo[1:] != lo[:-1] + 1)
+    run_counts = np.add.reduceat(brk.astype(np.int64), first) if n else np.zeros(0, np.int64)
+    if runs == "optimize":
+        as_run = np.where(cards <= 4096, 4 * run_counts < 2 * cards, 2 + 4 * run_counts < 8192)
+    else:
+        as_run = np.zeros(n, dtype=bool)
+    has_run = bool(as_run.any())
+    head = bytearray()
+    if has_run:
+        head += np.uint32(12347 | ((n - 1) << 16)).tobytes()
+        head += np.packbits(as_run, bitorder="little").tobytes()
+    else:
+        head += np.array([12346, n], dtype="<u4").tobytes()
+    head += np.stack([hks, cards - 1], 1).astype("<u2").tobytes()
+    offsets = (not has_run) or n >= 4
+    bodies = []
+    for i in range(n):
+        v = lo[first[i]:first[i] + cards[i]]
+        if as_run[i]:
+            idx = np.flatnonzero(brk[first[i]:first[i] + cards[i]])
+            starts, ends = v[idx], v[np.append(idx[1:] - 1, len(v) - 1)]
+            bodies.append(np.uint16(len(starts)).tobytes() + np.stack([starts, ends - starts], 1).astype("<u2").tobytes())
+        elif cards[i] <= 4096:
+            bodies.append(v.astype("<u2").tobytes())
+        else:
+            bits = np.zeros(65536, dtype=bool)
+            bits[v] = True
+            bodies.append(np.packbits(bits, bitorder="little").tobytes())
+    sizes = np.array([len(b) for b in bodies], dtype=np.int64)
+    if offsets:
+        base = len(head) + 4 * n
+        head += (base + np.concatenate([[0], np.cumsum(sizes)[:-1]]) if n else np.zeros(0)).astype("<u4").tobytes()
+    return bytes(head) + b"".join(bodies)
+
+
+def roaring_portable64(keys, runs=False):
+    """64-bit portable format: uint64 bucket count, then per distinct high
+    32-bit half its key and the 32-bit bitmap of the low halves."""
+    k = np.unique(np.asarray(keys, dtype=np.uint64))
+    his = np.unique(k >> np.uint64(32))
+    out = bytearray(np.uint64(len(his)).tobytes())
+    for h in his:
+        out += np.uint32(h).tobytes() + roaring_portable32(k[(k >> np.uint64(32)) == h], runs)
+    return bytes(out)

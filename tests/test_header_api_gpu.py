@@ -1,6 +1,7 @@
-"""Runs the C++ header-API programs (tests/cpp/header_api_test.cpp and the
-ForEach program tests/cpp/for_each_test.hip), built against libmxco.so by
-ops-collections_amd/build.py."""
+"""Runs the C++ header-API programs (tests/cpp/header_api_test.cpp, the
+ForEach program tests/cpp/for_each_test.hip and the RoaringBitmap /
+RoaringBitmapRef program tests/cpp/roaring_ref_test.hip), built against
+libmxco.so by ops-collections_amd/build.py."""
 import os
 import subprocess
 
@@ -8,7 +9,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "ops-collections_amd", "lib")
-PROGRAMS = ["header_api_test", "for_each_test"]
+PROGRAMS = ["header_api_test", "for_each_test", "roaring_ref_test"]
 
 
 @pytest.mark.parametrize("name", PROGRAMS)

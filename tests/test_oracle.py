@@ -503,3 +503,65 @@ def test_dynamic_map_oracle_growth_rules():
         D.DynamicMapOracle(10, -1, -1)
     # erase counts distinct present keys once
     assert m.erase([0, 0, 1, 10**9, 123456789]) == 3 and m.size == 49998
+
+
+# --------------------------------------------------------------------------
+# Roaring portable serialization writer (used to build GPU test bitmaps)
+# --------------------------------------------------------------------------
+
+def test_roaring_writer_layout_by_hand():
+    import roaring as R
+    # {1, 2, 3} without runs: cookie 12346, 1 container, header (key 0,
+    # cardinality-1 = 2), offset 16 (4 + 4 + 4 + 4 bytes before it), values.
+    assert R.serialize32([1, 2, 3]) == bytes.fromhex("3a300000" "01000000" "00000200" "10000000" "010002000300")
+    # the same as one run container: cookie 12347 | (1 - 1) << 16, run bitmap
+    # 0b1, header, no offsets (fewer than 4 containers), 1 run (start 1, length-1 2)
+    assert R.serialize32([1, 2, 3], run_containers=True) == bytes.fromhex("3b300000" "01" "00000200" "0100" "01000200")
+    # empty 32-bit bitmap and empty 64-bit bitmap (reference contains_test.cpp:211-224)
+    assert R.serialize32([]) == bytes.fromhex("3a300000" "00000000")
+    assert R.serialize64([]) == bytes(8)
+    # more than 4096 values in one container: an 8192-byte bitset
+    b = R.serialize32(range(5000))
+    assert len(b) == 4 + 4 + 4 + 4 + 8192 and b[16] == 0xFF
+
+
+def test_roaring_writer_reproduces_reference_spec_files():
+    """The reference's RoaringFormatSpec test files (written by CRoaring
+    v5.1.0 from tests/utility/generate_roaring_testdata.cpp:48-101) are pinned
+    by size and SHA-256 in cmake/roaring_testdata.cmake:35-48; the writer
+    rebuilds all three byte for byte from the same key sets."""
+    import hashlib
+    import roaring as R
+    k = list(range(0, 100000, 1000)) + [3 * x for x in range(100000, 200000)] + list(range(700000, 800000))
+    low = list(range(0, 0x9001)) + list(range(0xA000, 0x10001)) + [0x20000, 0x20005]
+    low += [0x80000 + j for j in range(0, 0x10000, 2)]
+    cases = [
+        (R.serialize32(k), 72616, "d719ae2e0150a362ef7cf51c361527585891f01460b1a92bcfb6a7257282a442"),
+        (R.serialize32(k, run_containers="optimize"), 48056,
+         "1f1909bfdd354fa2f0694fe88b8076833ca5383ad9fc3f68f2709c84a2ab70e3"),
+        (R.serialize64(low + [(1 << 32) | x for x in low], run_containers="optimize"), 16506,
+         "b5a553a759167f5f9ccb3fa21552d943b4c73235635b753376f4faf62067d178"),
+    ]
+    for data, size, sha in cases:
+        assert len(data) == size and hashlib.sha256(data).hexdigest() == sha
+
+
+def test_bench_roaring_writer_matches_oracle_writer():
+    """The vectorised writer that builds the bench's bitmaps produces the same
+    bytes as the oracle writer (itself pinned by the reference's SHA-256s)."""
+    import roaring as R
+    from mxco import workloads as W
+    k = list(range(0, 100000, 1000)) + [3 * x for x in range(100000, 200000)] + list(range(700000, 800000))
+    for mode in (False, "optimize"):
+        assert W.roaring_portable32(k, mode) == R.serialize32(k, run_containers=mode)
+    rng = np.random.default_rng(3)
+    for t in range(12):
+        ks = np.concatenate([rng.integers(0, 1 << int(rng.integers(8, 33)), int(rng.integers(0, 20000))),
+                             np.concatenate([np.arange(s, s + int(rng.integers(1, 3000)))
+                                             for s in rng.integers(0, 1 << 20, 5)])]) & 0xFFFFFFFF
+        for mode in (False, "optimize"):
+            assert W.roaring_portable32(ks, mode) == R.serialize32(ks, run_containers=mode), (t, mode)
+    low = list(range(0, 0x9001)) + list(range(0xA000, 0x10001)) + [0x20000, 0x20005]
+    low += [0x80000 + j for j in range(0, 0x10000, 2)]
+    kk = low + [(1 << 32) | x for x in low]
+    assert W.roaring_portable64(kk, "optimize") == R.serialize64(kk, run_containers="optimize")
