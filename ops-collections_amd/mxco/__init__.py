@@ -307,6 +307,30 @@ class _OpenAddressing:
               else RUNTIME_ERROR)
         return out.view(self._capacity, self.slot_size)
 
+    def dump(self):
+        """Live slots in slot order, read from Data() as the reference's
+        DumpTable does (tests/common/dump_table.h:20-57): Capacity() slots of
+        Pair<Key, Value> (key at byte 0, value at the next multiple of its
+        own size) or of Key for a set, skipping slots whose key is bitwise the
+        empty or, once set, the erased sentinel.  Returns host numpy arrays:
+        keys for a set, (keys, values) for a map."""
+        import numpy as np
+        raw = self.slots().cpu().numpy()
+        ks = torch.empty(0, dtype=self.key_dtype).element_size()
+        kb = raw[:, :ks]
+        live = ~np.all(kb == np.frombuffer(self._ek, np.uint8), axis=1)
+        erk = getattr(self, "_erk", None)
+        if erk is not None:
+            live &= ~np.all(kb == np.frombuffer(erk, np.uint8), axis=1)
+        keys = np.ascontiguousarray(kb[live]).view(torch.empty(0, dtype=self.key_dtype).numpy().dtype).reshape(-1)
+        if self.value_dtype is None:
+            return keys
+        _, off = slot_layout(self.key_dtype, self.value_dtype)
+        vs = torch.empty(0, dtype=self.value_dtype).element_size()
+        vals = np.ascontiguousarray(raw[live, off:off + vs]).view(
+            torch.empty(0, dtype=self.value_dtype).numpy().dtype).reshape(-1)
+        return keys, vals
+
 
 class StaticMap(_OpenAddressing):
     """aclco::StaticMap<Key, T> (include/static_map.h:29-35).
