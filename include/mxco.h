@@ -336,13 +336,17 @@ MXCO_API mxco_status mxco_rb_create(uint32_t key_size, const void* bitmap, size_
                                     mxco_rb** out);
 MXCO_API mxco_status mxco_rb_destroy(mxco_rb* bitmap);
 /* Size(), SizeBytes(), Data() and the device view behind Ref() (roaring_bitmap.h:111-121):
- * bytes is the device copy of the serialized bitmap, buckets / containers
- * the device bucket table and container directory that
- * mxco/detail/roaring.h (RbBucket, RbContainer, rb_contains) searches. */
+ * bytes is the device copy of the serialized bitmap; the other pointers are
+ * the device search image that mxco/detail/roaring.h describes and
+ * rb_contains searches (RbBucket table, dense container tables, container
+ * keys and descriptors, aligned container data). */
 typedef struct mxco_rb_view {
     const uint8_t* bytes;
     const void* buckets;
-    const void* containers;
+    const uint64_t* dense;
+    const uint16_t* container_keys;
+    const uint64_t* container_descs;
+    const uint8_t* image;
     uint32_t num_buckets;
     uint32_t reserved;
     uint64_t num_keys;   /* Size() */

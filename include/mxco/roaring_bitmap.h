@@ -36,9 +36,8 @@ public:
 #if defined(__HIPCC__)
     __device__ bool Contains(Key value) const noexcept
     {
-        return detail::rb_contains(v_.bytes, static_cast<detail::RbBucket const*>(v_.buckets), v_.num_buckets,
-                                   static_cast<detail::RbContainer const*>(v_.containers),
-                                   static_cast<uint64_t>(value));
+        return detail::rb_contains(static_cast<detail::RbBucket const*>(v_.buckets), v_.num_buckets, v_.dense,
+                                   v_.container_keys, v_.container_descs, v_.image, static_cast<uint64_t>(value));
     }
     __host__ __device__ uint64_t Size() const noexcept { return v_.num_keys; }
     __host__ __device__ bool Empty() const noexcept { return v_.num_keys == 0; }

@@ -12,9 +12,11 @@
 //           uint16) or bitset (8192 bytes).
 //   64-bit: uint64 bucket count, then per bucket a uint32 high key and a
 //           32-bit bitmap for the low halves.
-// The host parses and validates the headers once; the serialized bytes and a
-// flat container directory go to the device, and each query is a binary
-// search of its bucket, then of its container, then inside the container.
+// The host parses and validates the headers once, keeps a device copy of the
+// serialized bytes (Data()), and builds the line-aligned search image of
+// mxco/detail/roaring.h: dense per-bucket container tables, fenced array and
+// run containers, bitsets.  Each query is then a bucket lookup, a container
+// lookup and a search inside one line of container data.
 #include <string.h>
 
 #include <limits>
@@ -37,8 +39,18 @@ constexpr uint32_t kBitsetBytes = 8192;
 constexpr uint32_t kNoOffsetThreshold = 4;
 
 constexpr uint32_t kArray = detail::kRbArray, kBitset = detail::kRbBitset, kRun = detail::kRbRun;
+constexpr uint32_t kDenseMinContainers = 16;       // smaller buckets search a key list ...
+constexpr uint32_t kDenseMinSingle = 2;            // ... unless the bitmap has one bucket
+constexpr uint64_t kDenseBudget = 128ull << 20;    // bytes of dense tables per bitmap (256 tables)
+constexpr uint64_t kBitsetImageMax = 1ull << 30;   // all containers as bitsets up to this many bytes
 using detail::RbBucket;
-using detail::RbContainer;
+
+struct HostContainer {  // one container of the parsed serialization
+    uint64_t offset;    // byte offset of its data in the serialized bytes
+    uint32_t key;       // bits 16..31 of the keys it holds
+    uint32_t type;
+    uint32_t cardinality;
+};
 
 // Bounds-checked little-endian reads of the host copy.
 struct Reader {
@@ -58,7 +70,7 @@ struct Reader {
 
 // Parses one 32-bit bitmap starting at `base`; appends its containers to
 // `dir` (offsets relative to the whole buffer) and returns its byte length.
-mxco_status parse32(Reader& r, size_t base, std::vector<RbContainer>& dir, uint64_t* keys, size_t* length)
+mxco_status parse32(Reader& r, size_t base, std::vector<HostContainer>& dir, uint64_t* keys, size_t* length)
 {
     const uint32_t cookie = r.u32(base);
     size_t cur = base + 4;
@@ -116,7 +128,7 @@ mxco_status parse32(Reader& r, size_t base, std::vector<RbContainer>& dir, uint6
             bytes = kBitsetBytes;
         }
         if (!r.has(off, bytes)) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated serialized bitmap");
-        dir.push_back({off, key, type, card, 0});
+        dir.push_back({off, key, type, card});
         *keys += card;
         end = off + bytes;
     }
@@ -125,31 +137,230 @@ mxco_status parse32(Reader& r, size_t base, std::vector<RbContainer>& dir, uint6
     return MXCO_OK;
 }
 
-template <int KS>
-__global__ __launch_bounds__(256) void rb_contains_kernel(const uint8_t* bytes, const RbBucket* buckets,
-                                                          uint32_t nbuckets, const RbContainer* dir,
-                                                          const uint8_t* keys, uint8_t* out, uint64_t n)
+struct Search {  // device pointers of the search image
+    const RbBucket* buckets;
+    const uint64_t* dense;
+    const uint16_t* ckeys;
+    const uint64_t* cdesc;
+    const uint8_t* image;
+    uint32_t nbuckets;
+};
+
+template <int KS, bool BitsetsOnly>
+__global__ __launch_bounds__(256) void rb_contains_kernel(Search s, const uint8_t* keys, uint8_t* out, uint64_t n)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint64_t key;
         if constexpr (KS == 4) key = ((const uint32_t*)keys)[i];
         else key = ((const uint64_t*)keys)[i];
-        out[i] = detail::rb_contains(bytes, buckets, nbuckets, dir, key) ? 1 : 0;
+        out[i] = detail::rb_contains<BitsetsOnly>(s.buckets, s.nbuckets, s.dense, s.ckeys, s.cdesc, s.image, key) ? 1 : 0;
     }
+}
+
+// One-bucket bitmap with a dense container table over a bitset image: a
+// query is a descriptor load and a byte load, and each lane keeps U queries
+// in flight.  An absent container has descriptor 0, whose byte load reads
+// the first bitset (the image holds at least two) and is masked out.
+template <int KS>
+__global__ __launch_bounds__(256) void rb_dense_bits_kernel(const uint64_t* dense, const uint8_t* image,
+                                                            uint32_t bucket_key, const uint8_t* keys, uint8_t* out,
+                                                            uint64_t n)
+{
+    constexpr int U = 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += stride * U) {
+        uint64_t key[U], d[U];
+        uint32_t b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = i0 + u * stride;
+            key[u] = 0;
+            if (i < n) key[u] = KS == 4 ? ((const uint32_t*)keys)[i] : ((const uint64_t*)keys)[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = dense[(key[u] >> 16) & 0xffffu];
+#pragma unroll
+        for (int u = 0; u < U; ++u) b[u] = image[(d[u] & ((1ull << 40) - 1)) * 16 + ((key[u] & 0xffffu) >> 3)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = i0 + u * stride;
+            const bool hit = (d[u] >> 63) && (uint32_t)(key[u] >> 32) == bucket_key && ((b[u] >> (key[u] & 7)) & 1u);
+            if (i < n) out[i] = hit ? 1 : 0;
+        }
+    }
+}
+
+// Element count of a container in the image: values, or runs (the run
+// count leads the serialized run container).
+uint32_t elements(const uint8_t* ser, const HostContainer& c)
+{
+    if (c.type != kRun) return c.cardinality;
+    return (uint32_t)ser[c.offset] | (uint32_t)ser[c.offset + 1] << 8;
+}
+
+uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Image bytes of a container, and its alignment.
+uint64_t image_bytes(const HostContainer& c, uint32_t n, bool as_bitset, uint64_t* align)
+{
+    if (c.type == kBitset || as_bitset) {
+        *align = detail::kRbLine;
+        return kBitsetBytes;
+    }
+    const uint32_t esz = c.type == kRun ? 4 : 2;
+    const uint32_t fb = detail::rb_fence_bytes(n, esz);
+    *align = fb ? detail::kRbLine : 16;
+    return fb + align_up((uint64_t)n * esz, 16);
+}
+
+// Writes container c into the image at dst: fences (the first uint16 of every
+// 128-byte line of elements), then the elements.
+void write_image(uint8_t* dst, const uint8_t* ser, const HostContainer& c, uint32_t n, bool as_bitset)
+{
+    if (c.type == kBitset) {
+        memcpy(dst, ser + c.offset, kBitsetBytes);
+        return;
+    }
+    const uint32_t esz = c.type == kRun ? 4 : 2;
+    const uint8_t* src = ser + c.offset + (c.type == kRun ? 2 : 0);  // runs: skip the count
+    if (as_bitset) {  // dst holds 8192 zero bytes
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t start = (uint32_t)src[esz * i] | (uint32_t)src[esz * i + 1] << 8;
+            const uint32_t len = c.type == kRun ? ((uint32_t)src[4 * i + 2] | (uint32_t)src[4 * i + 3] << 8) : 0;
+            for (uint32_t v = start; v <= start + len && v < 65536; ++v) dst[v >> 3] |= (uint8_t)(1u << (v & 7));
+        }
+        return;
+    }
+    const uint32_t nf = detail::rb_fences(n, esz), per = detail::kRbLine / esz;
+    for (uint32_t f = 0; f < nf; ++f) memcpy(dst + 2 * f, src + (uint64_t)f * per * esz, 2);
+    memcpy(dst + detail::rb_fence_bytes(n, esz), src, (size_t)n * esz);
 }
 
 }  // namespace
 
 struct mxco_rb {
     uint32_t ks;
-    uint8_t* bytes;  // device copy of the serialized bitmap
-    RbBucket* buckets;
-    RbContainer* dir;
-    uint32_t nbuckets;
-    uint64_t num_keys;
-    uint64_t size_bytes;
+    uint8_t* bytes = nullptr;  // device copy of the serialized bitmap
+    RbBucket* buckets = nullptr;
+    uint64_t* dense = nullptr;
+    uint16_t* ckeys = nullptr;
+    uint64_t* cdesc = nullptr;
+    uint8_t* image = nullptr;
+    uint32_t nbuckets = 0;
+    bool bitsets_only = false;  // every container of the image is a bitset
+    bool dense_bits = false;    // ... and the bitmap is one bucket with a dense table
+    uint32_t bucket_key = 0;
+    uint64_t num_keys = 0;
+    uint64_t size_bytes = 0;
 };
+
+namespace {
+
+template <class T>
+hipError_t upload(T** dst, const T* src, size_t count, hipStream_t stream)
+{
+    hipError_t e = hipMalloc((void**)dst, (count ? count : 1) * sizeof(T));
+    if (e == hipSuccess && count) e = hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, stream);
+    return e;
+}
+
+// Parses a whole 32-bit (key_size 4) or 64-bit bitmap: containers, buckets,
+// key count and serialized length.
+mxco_status parse_all(const uint8_t* ser, size_t bitmap_bytes, uint32_t key_size, std::vector<HostContainer>& dir,
+                      std::vector<RbBucket>& buckets, uint64_t* keys, size_t* total)
+{
+    Reader r{ser, bitmap_bytes};
+    if (key_size == 4) {
+        size_t len = 0;
+        if (mxco_status e = parse32(r, 0, dir, keys, &len)) return e;
+        buckets.push_back({0, 0, (uint32_t)dir.size(), detail::kRbNoDense});
+        *total = len;
+        return MXCO_OK;
+    }
+    const uint64_t nb = r.u64(0);
+    if (!r.ok) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated serialized bitmap");
+    if (bitmap_bytes != SIZE_MAX && nb > (bitmap_bytes - 8) / 12)
+        return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated 64-bit bucket table");
+    if (nb > 0xffffffffull) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: bucket count is out of range");
+    size_t cur = 8;
+    for (uint64_t b = 0; b < nb; ++b) {
+        const uint32_t hk = r.u32(cur);
+        if (!r.ok) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated serialized bitmap");
+        if (b && hk <= buckets.back().key)
+            return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: bucket keys must be strictly increasing");
+        cur += 4;
+        const uint32_t first = (uint32_t)dir.size();
+        size_t len = 0;
+        if (mxco_status e = parse32(r, cur, dir, keys, &len)) return e;
+        buckets.push_back({hk, first, (uint32_t)(dir.size() - first), detail::kRbNoDense});
+        cur += len;
+    }
+    *total = cur;
+    return MXCO_OK;
+}
+
+// The search image of mxco/detail/roaring.h for parsed containers `dir`:
+// every container expanded to an 8 KiB bitset while they all fit
+// MXCO_RB_BITSET_MAX_MB (read at each construction; default 1024: one byte
+// read per query), else
+// aligned container copies; per bucket its container keys and descriptors
+// from a 16-byte aligned index (buckets[].first is rewritten to it); a dense
+// table for each bucket of at least kDenseMinContainers containers (of
+// kDenseMinSingle in a one-bucket bitmap) within the budget; 128 bytes of
+// padding after the key list and the image.
+mxco_status build_search(const uint8_t* ser, const std::vector<HostContainer>& dir, std::vector<RbBucket>& buckets,
+                         std::vector<uint8_t>& image, std::vector<uint64_t>& dense, std::vector<uint16_t>& ckeys,
+                         std::vector<uint64_t>& cdesc, bool* bitsets_only)
+{
+    constexpr uint32_t kPadKeys = detail::kRbLine / sizeof(uint16_t);
+    const uint64_t bitset_max = (uint64_t)env_int("MXCO_RB_BITSET_MAX_MB", (int)(kBitsetImageMax >> 20)) << 20;
+    const bool as_bitset = (uint64_t)dir.size() * kBitsetBytes <= bitset_max;
+    *bitsets_only = as_bitset;
+    std::vector<uint64_t> at(dir.size());
+    std::vector<uint64_t> desc(dir.size());
+    uint64_t image_total = 0;
+    try {
+        for (size_t i = 0; i < dir.size(); ++i) {
+            uint64_t align;
+            const uint32_t n = elements(ser, dir[i]);
+            const uint64_t bytes = image_bytes(dir[i], n, as_bitset, &align);
+            at[i] = align_up(image_total, align);
+            image_total = at[i] + bytes;
+            desc[i] = as_bitset ? detail::rb_desc(at[i], kBitset, dir[i].cardinality)
+                                : detail::rb_desc(at[i], dir[i].type, n);
+        }
+        uint64_t slots = 0;
+        for (const RbBucket& b : buckets) slots = align_up(slots, 8) + b.count;
+        ckeys.assign(slots + kPadKeys, 0);
+        cdesc.assign(slots + kPadKeys, 0);
+        uint64_t pos = 0;
+        uint32_t ndense = 0;
+        for (RbBucket& b : buckets) {
+            pos = align_up(pos, 8);
+            for (uint32_t j = 0; j < b.count; ++j) {
+                ckeys[pos + j] = (uint16_t)dir[b.first + j].key;
+                cdesc[pos + j] = desc[b.first + j];
+            }
+            b.first = (uint32_t)pos;
+            pos += b.count;
+            const uint32_t min_count = buckets.size() == 1 ? kDenseMinSingle : kDenseMinContainers;
+            if (b.count < min_count || (uint64_t)(ndense + 1) * 65536 * 8 > kDenseBudget) continue;
+            b.dense = ndense++;
+            dense.resize((size_t)ndense * 65536, 0);
+            for (uint32_t c = b.first; c < b.first + b.count; ++c)
+                dense[(size_t)b.dense * 65536 + ckeys[c]] = cdesc[c];
+        }
+        image.assign(image_total + detail::kRbLine, 0);
+    } catch (const std::bad_alloc&) {
+        return fail(MXCO_OUT_OF_MEMORY, "RoaringBitmap: host allocation");
+    }
+    for (size_t i = 0; i < dir.size(); ++i)
+        write_image(image.data() + at[i], ser, dir[i], elements(ser, dir[i]), as_bitset);
+    return MXCO_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -161,53 +372,35 @@ MXCO_API mxco_status mxco_rb_create(uint32_t key_size, const void* bitmap, size_
     if (key_size != 4 && key_size != 8)
         return fail(MXCO_UNSUPPORTED, "RoaringBitmap key type must be uint32_t or uint64_t");
     if (!bitmap) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: bitmap must not be null");
-    Reader r{(const uint8_t*)bitmap, bitmap_bytes};
-    std::vector<RbContainer> dir;
+    const uint8_t* ser = (const uint8_t*)bitmap;
+    std::vector<HostContainer> dir;
     std::vector<RbBucket> buckets;
     uint64_t keys = 0;
     size_t total = 0;
-    if (key_size == 4) {
-        size_t len = 0;
-        if (mxco_status e = parse32(r, 0, dir, &keys, &len)) return e;
-        buckets.push_back({0, 0, (uint32_t)dir.size(), 0});
-        total = len;
-    } else {
-        const uint64_t nb = r.u64(0);
-        if (!r.ok) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated serialized bitmap");
-        if (bitmap_bytes != SIZE_MAX && nb > (bitmap_bytes - 8) / 12)
-            return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated 64-bit bucket table");
-        if (nb > 0xffffffffull) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: bucket count is out of range");
-        size_t cur = 8;
-        for (uint64_t b = 0; b < nb; ++b) {
-            const uint32_t hk = r.u32(cur);
-            if (!r.ok) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: truncated serialized bitmap");
-            if (b && hk <= buckets.back().key)
-                return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: bucket keys must be strictly increasing");
-            cur += 4;
-            const uint32_t first = (uint32_t)dir.size();
-            size_t len = 0;
-            if (mxco_status e = parse32(r, cur, dir, &keys, &len)) return e;
-            buckets.push_back({hk, first, (uint32_t)(dir.size() - first), 0});
-            cur += len;
-        }
-        total = cur;
-    }
+    if (mxco_status e = parse_all(ser, bitmap_bytes, key_size, dir, buckets, &keys, &total)) return e;
+    std::vector<uint8_t> image;
+    std::vector<uint64_t> dense;
+    std::vector<uint16_t> ckeys;
+    std::vector<uint64_t> cdesc;
+    bool bitsets_only = false;
+    if (mxco_status e = build_search(ser, dir, buckets, image, dense, ckeys, cdesc, &bitsets_only)) return e;
+
     mxco_rb* rb = new (std::nothrow) mxco_rb();
     if (!rb) return fail(MXCO_OUT_OF_MEMORY, "RoaringBitmap: host allocation");
     rb->ks = key_size;
     rb->nbuckets = (uint32_t)buckets.size();
+    rb->bitsets_only = bitsets_only;
+    rb->dense_bits = bitsets_only && buckets.size() == 1 && buckets[0].dense != detail::kRbNoDense;
+    rb->bucket_key = buckets.empty() ? 0 : buckets[0].key;
     rb->num_keys = keys;
     rb->size_bytes = total;
-    hipError_t e = hipMalloc((void**)&rb->bytes, total ? total : 1);
-    if (e == hipSuccess) e = hipMalloc((void**)&rb->buckets, (buckets.size() + 1) * sizeof(RbBucket));
-    if (e == hipSuccess) e = hipMalloc((void**)&rb->dir, (dir.size() + 1) * sizeof(RbContainer));
-    if (e == hipSuccess && total) e = hipMemcpyAsync(rb->bytes, bitmap, total, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess && !buckets.empty())
-        e = hipMemcpyAsync(rb->buckets, buckets.data(), buckets.size() * sizeof(RbBucket), hipMemcpyHostToDevice,
-                           stream);
-    if (e == hipSuccess && !dir.empty())
-        e = hipMemcpyAsync(rb->dir, dir.data(), dir.size() * sizeof(RbContainer), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    hipError_t e = upload(&rb->bytes, ser, total, stream);
+    if (e == hipSuccess) e = upload(&rb->buckets, buckets.data(), buckets.size(), stream);
+    if (e == hipSuccess) e = upload(&rb->dense, dense.data(), dense.size(), stream);
+    if (e == hipSuccess) e = upload(&rb->ckeys, ckeys.data(), ckeys.size(), stream);
+    if (e == hipSuccess) e = upload(&rb->cdesc, cdesc.data(), cdesc.size(), stream);
+    if (e == hipSuccess) e = upload(&rb->image, image.data(), image.size(), stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);  // the host vectors end here
     if (e != hipSuccess) {
         (void)mxco_rb_destroy(rb);
         return hip_fail(e, "RoaringBitmap: device copy");
@@ -219,9 +412,9 @@ MXCO_API mxco_status mxco_rb_create(uint32_t key_size, const void* bitmap, size_
 MXCO_API mxco_status mxco_rb_destroy(mxco_rb* rb)
 {
     if (!rb) return ok();
-    if (rb->bytes) (void)hipFree(rb->bytes);
-    if (rb->buckets) (void)hipFree(rb->buckets);
-    if (rb->dir) (void)hipFree(rb->dir);
+    for (void* p : {(void*)rb->bytes, (void*)rb->buckets, (void*)rb->dense, (void*)rb->ckeys, (void*)rb->cdesc,
+                    (void*)rb->image})
+        if (p) (void)hipFree(p);
     delete rb;
     return ok();
 }
@@ -232,7 +425,10 @@ MXCO_API mxco_status mxco_rb_info(const mxco_rb* rb, mxco_rb_view* view)
     if (!view) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap: null output");
     view->bytes = rb->bytes;
     view->buckets = rb->buckets;
-    view->containers = rb->dir;
+    view->dense = rb->dense;
+    view->container_keys = rb->ckeys;
+    view->container_descs = rb->cdesc;
+    view->image = rb->image;
     view->num_buckets = rb->nbuckets;
     view->reserved = 0;
     view->num_keys = rb->num_keys;
@@ -248,12 +444,19 @@ MXCO_API mxco_status mxco_rb_contains(const mxco_rb* rb, const void* keys, uint8
         return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap::ContainsAsync keys and outputValues must not be null");
     if (!rb) return fail(MXCO_INVALID_ARGUMENT, "RoaringBitmap::ContainsAsync bitmap storage is not initialized");
     const unsigned grid = grid_for(n, 8);
-    if (rb->ks == 4)
-        rb_contains_kernel<4><<<grid, 256, 0, stream>>>(rb->bytes, rb->buckets, rb->nbuckets, rb->dir,
-                                                        (const uint8_t*)keys, out, n);
-    else
-        rb_contains_kernel<8><<<grid, 256, 0, stream>>>(rb->bytes, rb->buckets, rb->nbuckets, rb->dir,
-                                                        (const uint8_t*)keys, out, n);
+    const Search s{rb->buckets, rb->dense, rb->ckeys, rb->cdesc, rb->image, rb->nbuckets};
+    const uint8_t* k = (const uint8_t*)keys;
+    if (rb->dense_bits) {
+        const unsigned g = grid_for((n + 3) / 4, 8);
+        if (rb->ks == 4) rb_dense_bits_kernel<4><<<g, 256, 0, stream>>>(rb->dense, rb->image, 0, k, out, n);
+        else rb_dense_bits_kernel<8><<<g, 256, 0, stream>>>(rb->dense, rb->image, rb->bucket_key, k, out, n);
+    } else if (rb->ks == 4) {
+        if (rb->bitsets_only) rb_contains_kernel<4, true><<<grid, 256, 0, stream>>>(s, k, out, n);
+        else rb_contains_kernel<4, false><<<grid, 256, 0, stream>>>(s, k, out, n);
+    } else {
+        if (rb->bitsets_only) rb_contains_kernel<8, true><<<grid, 256, 0, stream>>>(s, k, out, n);
+        else rb_contains_kernel<8, false><<<grid, 256, 0, stream>>>(s, k, out, n);
+    }
     MXCO_HIP(hipGetLastError(), "RoaringBitmap: contains");
     if (sync) MXCO_HIP(hipStreamSynchronize(stream), "RoaringBitmap: contains");
     return ok();

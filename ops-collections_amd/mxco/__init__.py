@@ -642,9 +642,9 @@ class StaticMultimap:
 _sig("mxco_rb_create", [u32, vp, ctypes.c_size_t, vp, ctypes.POINTER(vp)])
 _sig("mxco_rb_destroy", [vp])
 class RbView(ctypes.Structure):
-    """mxco_rb_view: device bytes, bucket table and container directory."""
-    _fields_ = [("bytes", vp), ("buckets", vp), ("containers", vp), ("num_buckets", u32), ("reserved", u32),
-                ("num_keys", u64), ("size_bytes", u64)]
+    """mxco_rb_view: device copy of the serialized bytes and the search image."""
+    _fields_ = [("bytes", vp), ("buckets", vp), ("dense", vp), ("container_keys", vp), ("container_descs", vp),
+                ("image", vp), ("num_buckets", u32), ("reserved", u32), ("num_keys", u64), ("size_bytes", u64)]
 
 
 _sig("mxco_rb_info", [vp, ctypes.POINTER(RbView)])

@@ -37,7 +37,7 @@ def spec_keys64():
 
 
 @pytest.mark.parametrize("runs", [False, "optimize"])
-def test_spec_file_key_sets(runs):
+def test_spec_file_key_sets(runs, image):
     keys = spec_keys32()
     rb = mx().RoaringBitmap(R.serialize32(keys, run_containers=runs), torch.int32)
     assert rb.size() == len(np.unique(keys))
@@ -52,8 +52,17 @@ def test_spec_file_key_sets(runs):
     assert not query(rb64, absent64, torch.int64).any()
 
 
+@pytest.fixture(params=["bitset_image", "container_image"])
+def image(request, monkeypatch):
+    """The library expands containers to bitsets within a byte budget
+    (MXCO_RB_BITSET_MAX_MB); 0 keeps array and run containers as they are."""
+    if request.param == "container_image":
+        monkeypatch.setenv("MXCO_RB_BITSET_MAX_MB", "0")
+    return request.param
+
+
 @pytest.mark.parametrize("layout", ["array", "bitset", "run", "mixed", "few_runs", "empty"])
-def test_random_bitmaps_vs_brute_force(layout):
+def test_random_bitmaps_vs_brute_force(layout, image):
     rng = np.random.default_rng(len(layout))
     if layout == "empty":
         keys = np.array([], np.uint64)
@@ -95,3 +104,31 @@ def test_malformed_inputs_raise():
         mxco.RoaringBitmap(bytes(two), torch.int64)
     rb = mxco.RoaringBitmap(good, torch.int32)
     assert rb.contains(torch.empty(0, dtype=torch.int32, device="cuda")).numel() == 0
+
+
+def test_search_paths_dense_budget_and_long_run_lists(image):
+    """Buckets past the dense-table budget (binary search of the key list)
+    and a run container of 32,768 runs (fence list longer than one line)."""
+    rng = np.random.default_rng(12)
+    hk = np.arange(300, dtype=np.uint64) << np.uint64(32)
+    lo = (np.arange(100, dtype=np.uint64) << np.uint64(16)) + np.uint64(5)
+    keys = np.unique((hk[:, None] + lo[None, :]).ravel())
+    rb = mx().RoaringBitmap(R.serialize64(keys), torch.int64)
+    q = np.concatenate([keys, keys + 1, rng.integers(0, 1 << 41, 100000, dtype=np.uint64)])
+    assert np.array_equal(query(rb, q, torch.int64), np.isin(q, keys))
+    alt = np.arange(0, 65536, 2, dtype=np.uint64)
+    keys = np.concatenate([alt, np.uint64(1 << 16) + alt[:3000]])
+    rb = mx().RoaringBitmap(R.serialize32(keys, run_containers=True), torch.int32)
+    q = np.concatenate([keys, keys + 1, rng.integers(0, 1 << 18, 100000, dtype=np.uint64)]) & 0xFFFFFFFF
+    assert np.array_equal(query(rb, q, torch.int32), np.isin(q, keys))
+
+
+def test_one_bucket_64bit_bitmap(image):
+    """A 64-bit bitmap of one bucket (high half 7): keys of other high halves
+    are absent even when their low half is present."""
+    rng = np.random.default_rng(13)
+    low = np.unique(rng.integers(0, 1 << 20, 30000, dtype=np.uint64))
+    keys = low | np.uint64(7 << 32)
+    rb = mx().RoaringBitmap(R.serialize64(keys, run_containers="optimize"), torch.int64)
+    q = np.concatenate([keys, low, low | np.uint64(8 << 32), keys + 1])
+    assert np.array_equal(query(rb, q, torch.int64), np.isin(q, keys))
