@@ -465,24 +465,42 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
 
 
 def cpu_baseline(args):
+    """The oracle's restatement of the probe/CAS algorithm on the host cores,
+    on bounded samples of the measured workload: Insert of round(0.8 * cap)
+    distinct int64 keys then Find of cap queries (50 % hit) on a 2^27-slot
+    table with T threads, and on a 2^23-slot table with one thread (about
+    10 s of CPU time together)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    cap_req = 1 << 24
-    rng = np.random.default_rng(5)
-    t = O.OpenAddressing(np.int64, np.int64, cap_req, -1, -1)
-    n_ins = int(round(0.8 * t.capacity))
-    keys = rng.choice(1 << 62, n_ins, replace=False).astype(np.int64)
-    vals = keys * 7 + 1
-    q = np.concatenate([rng.choice(keys, cap_req // 2), rng.integers(1 << 62, (1 << 63) - 1, cap_req // 2)])
-    rng.shuffle(q)
-    t0 = time.perf_counter()
-    failed = t.mt_insert(keys, vals, threads)
-    t1 = time.perf_counter()
-    out = t.mt_find(q, threads)
-    t2 = time.perf_counter()
-    ok = failed == 0 and np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1))
+
+    def leg(cap_log2, nthreads):
+        cap_req = 1 << cap_log2
+        rng = np.random.default_rng(5)
+        t = O.OpenAddressing(np.int64, np.int64, cap_req, -1, -1)
+        n_ins = int(round(0.8 * t.capacity))
+        # distinct keys below 2^62: an odd multiplier is a bijection mod 2^62
+        keys = ((np.arange(n_ins, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) & np.uint64((1 << 62) - 1))
+        keys = keys.astype(np.int64)
+        vals = keys * 7 + 1
+        q = np.concatenate([keys[rng.integers(0, n_ins, cap_req // 2)],
+                            rng.integers(1 << 62, (1 << 63) - 1, cap_req // 2)])
+        rng.shuffle(q)
+        t0 = time.perf_counter()
+        failed = t.mt_insert(keys, vals, nthreads)
+        t1 = time.perf_counter()
+        out = t.mt_find(q, nthreads)
+        t2 = time.perf_counter()
+        ok = failed == 0 and np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1))
+        del t
+        return {"value": (n_ins + len(q)) / (t2 - t0) / 1e9, "insert_s": round(t1 - t0, 3),
+                "find_s": round(t2 - t1, 3), "verified": bool(ok),
+                "sample": "Insert of %d keys + Find of %d queries (50%% hit) on a 2^%d-slot table at LF 0.8, "
+                          "%d thread%s" % (n_ins, len(q), cap_log2, nthreads, "s" if nthreads > 1 else "")}
+
+    mt = leg(27, threads)
+    st = leg(23, 1)
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -491,11 +509,12 @@ def cpu_baseline(args):
                 break
     except OSError:
         pass
-    keys_total = n_ins + len(q)
-    return {"value": keys_total / (t2 - t0) / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
-            "sample": "oracle restatement (std-thread-style pthreads, __atomic CAS) of StaticMap int64 Insert of "
-                      "%d keys + Find of %d queries (50%% hit) on a 2^24-slot table at LF 0.8" % (n_ins, len(q)),
-            "insert_s": t1 - t0, "find_s": t2 - t1, "verified": bool(ok), "cpu_model": model}
+    return {"value": mt["value"], "unit": "Gkeys/s", "cores": threads, "kind": "port",
+            "sample": "oracle restatement (pthreads, __atomic 64-bit CAS) of StaticMap int64 " + mt["sample"],
+            "insert_s": mt["insert_s"], "find_s": mt["find_s"], "verified": mt["verified"],
+            "single_thread": {"value": st["value"], "unit": "Gkeys/s", "cores": 1, "sample": st["sample"],
+                              "verified": st["verified"]},
+            "cpu_model": model}
 
 
 def main():
