@@ -94,6 +94,7 @@ __global__ __launch_bounds__(256) void oa_batch_kernel(OaArgs base, const BatchT
 // w, w + waves, ... (skipping the empty tails of short chunks).
 template <int KS, int OUTB>
 struct BatchChunks {
+    static constexpr bool kOneTable = false;
     const OaArgs* base;
     const BatchTable* tables;
     const mxco_batch_item* items;

@@ -367,12 +367,17 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
         uint64_t& prev_line = PREV[u];
         Raw (&s)[K] = S[u];
         bool key_done = false;
+        // Counter increments of this step, added once at its end: separate
+        // conditional increments of the two counters get merged by the
+        // compiler into one increment through a dynamically indexed (scratch)
+        // pair.
+        uint32_t add0 = 0, add1 = 0;
         if (w.done) {
             // Walk exhausted: full table / miss after a full cycle.
             if (leader) {
                 if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + i * OUTS, miss_out);
                 if constexpr (OP == OP_CONTAINS) a.out[i] = 0;
-                if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ERASE) cnt0++;
+                if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ERASE) add0 = 1;
                 if constexpr (OP == OP_INSERT_AND_FIND) {
                     store_bits<OUTS>(a.out + i * OUTS, miss_out);
                     a.out2[i] = 0;
@@ -425,8 +430,8 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                 const uint64_t first = ((uint64_t)w.pos * SS) / 128;
                 const uint64_t last = ((uint64_t)(w.pos + used - 1) * SS + SS - 1) / 128;
                 if (leader) {
-                    cnt0 += last - first + 1 - (first == prev_line ? 1 : 0);
-                    cnt1 += used;
+                    add0 = (uint32_t)(last - first + 1 - (first == prev_line ? 1 : 0));
+                    add1 = used;
                 }
                 prev_line = last;
             }
@@ -446,7 +451,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                         } else if constexpr (OP == OP_CONTAINS) {
                             a.out[i] = weq ? 1 : 0;
                         } else if constexpr (OP == OP_COUNT) {
-                            cnt0 += weq ? 1 : 0;
+                            add0 = weq ? 1 : 0;
                         }
                     }
                     key_done = true;
@@ -544,8 +549,8 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                         consume(a, w, woff + 1);
                     } else {
                         if (leader) {
-                            if constexpr (OP == OP_INSERT) cnt0 += (r == 2);
-                            if constexpr (OP == OP_INSERT_OR_ASSIGN) cnt1 += (r == 2);
+                            if constexpr (OP == OP_INSERT) add0 = (r == 2);
+                            if constexpr (OP == OP_INSERT_OR_ASSIGN) add1 = (r == 2);
                         }
                         if constexpr (OP == OP_INSERT_AND_FIND) {
                             if (winner) {
@@ -570,6 +575,8 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                 }
             }
         }
+        cnt0 += add0;
+        cnt1 += add1;
         if (key_done) advance(u, i + (uint64_t)U * ngroups);
     };
 
@@ -695,6 +702,7 @@ struct WsChunk {
 // Chunks of one table: wavefront w takes chunks w, w + waves, ...
 template <int KS, int OUTB>
 struct PlainChunks {
+    static constexpr bool kOneTable = true;
     const OaArgs* a;
     uint64_t next, step;
     __device__ __forceinline__ WsChunk open()
@@ -787,9 +795,11 @@ __device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint
                 }
                 if (fresh) {
                     active = true;
-                    ga.table = B ? c1.table : c0.table;
-                    ga.cap = B ? c1.cap : c0.cap;
-                    ga.nbuckets = B ? c1.nbuckets : c0.nbuckets;
+                    if constexpr (!Src::kOneTable) {  // one table: the fields stay wave-uniform
+                        ga.table = B ? c1.table : c0.table;
+                        ga.cap = B ? c1.cap : c0.cap;
+                        ga.nbuckets = B ? c1.nbuckets : c0.nbuckets;
+                    }
                     ga.stencil = B ? c1.stencil : c0.stencil;
                     if (!stencil_ok(ga, I)) {
                         if (leader) put(miss_out);
@@ -889,6 +899,19 @@ __device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint
 
 template <int KS, int VS, int G, int OP>
 __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
+{
+    constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
+    __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
+    const uint32_t wib = threadIdx.x >> 6;
+    PlainChunks<KS, OUTB> src{&a, (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib,
+                              (uint64_t)gridDim.x * (blockDim.x >> 6)};
+    wave_staged_find<KS, VS, G, OP>(a, src, stage[wib][0], stage[wib][1]);
+}
+
+// The same kernel held to 64 VGPRs (8 waves per SIMD); used where that
+// costs no spills.
+template <int KS, int VS, int G, int OP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void oa_wave_staged_kernel_w8(OaArgs a)
 {
     constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
     __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
