@@ -114,15 +114,34 @@ __global__ __launch_bounds__(kThreads) void hll_private_add_kernel(const uint8_t
     for (uint32_t j = threadIdx.x; j < nwords; j += blockDim.x) out[j] = sk[j];
 }
 
+// Byte-max of the nws private sketches into the registers.  A block reduces
+// 64 consecutive words: its 4 wavefronts take every 4th sketch (coalesced
+// 256-byte rows, 8 independent loads in flight per lane) and combine
+// through LDS.  Grid: ceil(words / 64) blocks of 256 lanes.
 __global__ __launch_bounds__(256) void hll_reduce_kernel(uint32_t* words, const uint8_t* workspace, uint32_t nws,
                                                          uint32_t p)
 {
+    __shared__ uint32_t part[4][64];
     const uint32_t nwords = (1u << p) / 4;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwords; j += gridDim.x * blockDim.x) {
-        uint32_t r = words[j];
-        for (uint32_t b = 0; b < nws; ++b) r = bytemax(r, ((const uint32_t*)(workspace + (uint64_t)b * (1u << p)))[j]);
-        words[j] = r;
+    const uint32_t lane = threadIdx.x & 63u, slice = threadIdx.x >> 6;
+    const uint32_t j = blockIdx.x * 64u + lane;
+    uint32_t r = 0;
+    if (j < nwords) {
+        const uint32_t* base = (const uint32_t*)workspace + j;
+        uint32_t b = slice;
+        for (; b + 28u < nws; b += 32u) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) v[u] = base[(uint64_t)(b + 4u * u) * nwords];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) r = bytemax(r, v[u]);
+        }
+        for (; b < nws; b += 4u) r = bytemax(r, base[(uint64_t)b * nwords]);
     }
+    part[slice][lane] = r;
+    __syncthreads();
+    if (slice == 0 && j < nwords)
+        words[j] = bytemax(bytemax(words[j], part[0][lane]), bytemax(bytemax(part[1][lane], part[2][lane]), part[3][lane]));
 }
 
 template <int KS>
@@ -373,7 +392,7 @@ MXCO_API mxco_status mxco_hll_add(mxco_hll* h, const void* keys, uint64_t n, hip
             hll_private_add_kernel<8><<<blocks, kThreads, lds, stream>>>(k, n, h->p, h->workspace);
         MXCO_HIP(hipGetLastError(), "HyperLogLog: add");
         const uint32_t nwords = (1u << h->p) / 4;
-        hll_reduce_kernel<<<grid_for(nwords, 4), 256, 0, stream>>>(h->words, h->workspace, blocks, h->p);
+        hll_reduce_kernel<<<(nwords + 63) / 64, 256, 0, stream>>>(h->words, h->workspace, blocks, h->p);
         MXCO_HIP(hipGetLastError(), "HyperLogLog: reduce");
     } else {
         const unsigned grid = grid_for(n, 8);
