@@ -468,8 +468,9 @@ def cpu_baseline(args):
     """The oracle's restatement of the probe/CAS algorithm on the host cores,
     on bounded samples of the measured workload: Insert of round(0.8 * cap)
     distinct int64 keys then Find of cap queries (50 % hit) on a 2^27-slot
-    table with T threads, and on a 2^23-slot table with one thread (about
-    10 s of CPU time together)."""
+    table with T threads, and on a 2^23-slot table with one thread; and the
+    reference tests' std::unordered_map golden path on the 2^23 sample
+    (about 15 s of CPU time together)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -499,8 +500,35 @@ def cpu_baseline(args):
                 "sample": "Insert of %d keys + Find of %d queries (50%% hit) on a 2^%d-slot table at LF 0.8, "
                           "%d thread%s" % (n_ins, len(q), cap_log2, nthreads, "s" if nthreads > 1 else "")}
 
+    def golden_leg(cap_log2):
+        """The reference tests' host path: a std::unordered_map filled and
+        queried on one thread (oracle/libgolden_map.so)."""
+        import ctypes
+        lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "libgolden_map.so"))
+        cap_req = 1 << cap_log2
+        n_ins = int(round(0.8 * cap_req))
+        rng = np.random.default_rng(5)
+        keys = ((np.arange(n_ins, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+                & np.uint64((1 << 62) - 1)).astype(np.int64)
+        vals = keys * 7 + 1
+        q = np.concatenate([keys[rng.integers(0, n_ins, cap_req // 2)],
+                            rng.integers(1 << 62, (1 << 63) - 1, cap_req // 2)])
+        rng.shuffle(q)
+        out = np.empty_like(q)
+        ins_s, find_s = ctypes.c_double(), ctypes.c_double()
+        P = ctypes.c_void_p
+        lib.golden_map_run(P(keys.ctypes.data), P(vals.ctypes.data), ctypes.c_uint64(n_ins), P(q.ctypes.data),
+                           ctypes.c_uint64(len(q)), ctypes.c_int64(-1), P(out.ctypes.data), ctypes.byref(ins_s),
+                           ctypes.byref(find_s))
+        ok = np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1))
+        return {"value": (n_ins + len(q)) / (ins_s.value + find_s.value) / 1e9, "unit": "Gkeys/s", "cores": 1,
+                "insert_s": round(ins_s.value, 3), "find_s": round(find_s.value, 3), "verified": bool(ok),
+                "sample": "std::unordered_map Insert of %d keys + Find of %d queries (50%% hit), one thread"
+                          % (n_ins, len(q))}
+
     mt = leg(27, threads)
     st = leg(23, 1)
+    gm = golden_leg(23)
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -514,6 +542,7 @@ def cpu_baseline(args):
             "insert_s": mt["insert_s"], "find_s": mt["find_s"], "verified": mt["verified"],
             "single_thread": {"value": st["value"], "unit": "Gkeys/s", "cores": 1, "sample": st["sample"],
                               "verified": st["verified"]},
+            "golden_unordered_map": gm,
             "cpu_model": model}
 
 

@@ -8,6 +8,7 @@ tiny inputs, and closed forms (dict goldens for the map semantics, linear
 counting for HyperLogLog).
 """
 import hashlib
+import os
 import math
 import struct
 
@@ -565,3 +566,19 @@ def test_bench_roaring_writer_matches_oracle_writer():
     low += [0x80000 + j for j in range(0, 0x10000, 2)]
     kk = low + [(1 << 32) | x for x in low]
     assert W.roaring_portable64(kk, "optimize") == R.serialize64(kk, run_containers="optimize")
+
+
+def test_golden_map_baseline_library():
+    """The std::unordered_map golden path timed by the CPU baseline: first
+    insert of a key wins, misses return the given value."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(O.__file__), "libgolden_map.so"))
+    keys = np.array([5, 7, 5, 9], np.int64)
+    vals = np.array([50, 70, 51, 90], np.int64)
+    q = np.array([5, 6, 7, 9, 10], np.int64)
+    out = np.empty_like(q)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    P = ctypes.c_void_p
+    lib.golden_map_run(P(keys.ctypes.data), P(vals.ctypes.data), ctypes.c_uint64(4), P(q.ctypes.data),
+                       ctypes.c_uint64(5), ctypes.c_int64(-1), P(out.ctypes.data), ctypes.byref(a), ctypes.byref(b))
+    assert out.tolist() == [50, -1, 70, 90, -1]
