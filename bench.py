@@ -610,15 +610,30 @@ def main():
         "group": r["group"],
     }
     if args.sweep:
+        # Load factors of the measured configuration at the default group
+        # width, with their algorithmic bytes; then group widths at args.lf.
         sw = []
         for lf in (0.5, 0.7, 0.8, 0.9):
-            for g in (1, 2, 4, 8):
-                x = run_gpu(args, torch, mxco, lf, g, 3, 1, verify=False)
-                sw.append({"lf": lf, "group": g, "insert_ms": round(x["insert_ms"], 3),
-                           "find_ms": round(x["find_ms"], 3),
-                           "insert_gkeys_s": round(x["n_insert"] / x["insert_ms"] / 1e6, 3),
-                           "find_gkeys_s": round(x["n_find"] / x["find_ms"] / 1e6, 3)})
-                log("sweep %s" % json.dumps(sw[-1]))
+            x = run_gpu(args, torch, mxco, lf, args.group, 3, 1, verify=True)
+            lq = x["find_lines"] / x["n_find"]
+            lk = x["insert_lines"] / x["n_insert"]
+            f_gbs = x["n_find"] * (128 * lq + 16) / (x["find_ms"] * 1e-3) / 1e9
+            i_gbs = x["n_insert"] * (128 * lk + 128 + 16) / (x["insert_ms"] * 1e-3) / 1e9
+            sw.append({"lf": lf, "group": args.group, "insert_ms": round(x["insert_ms"], 3),
+                       "find_ms": round(x["find_ms"], 3),
+                       "insert_gkeys_s": round(x["n_insert"] / x["insert_ms"] / 1e6, 3),
+                       "find_gkeys_s": round(x["n_find"] / x["find_ms"] / 1e6, 3),
+                       "lines_per_query": round(lq, 4), "lines_per_key": round(lk, 4),
+                       "find_frac": round(f_gbs / PEAK_HBM_GBS, 4), "insert_frac": round(i_gbs / PEAK_HBM_GBS, 4),
+                       "verified": x["verified"]})
+            log("sweep %s" % json.dumps(sw[-1]))
+        for g in (1, 2, 4, 8):
+            x = run_gpu(args, torch, mxco, args.lf, g, 3, 1, verify=False)
+            sw.append({"lf": args.lf, "group": g, "insert_ms": round(x["insert_ms"], 3),
+                       "find_ms": round(x["find_ms"], 3),
+                       "insert_gkeys_s": round(x["n_insert"] / x["insert_ms"] / 1e6, 3),
+                       "find_gkeys_s": round(x["n_find"] / x["find_ms"] / 1e6, 3)})
+            log("sweep %s" % json.dumps(sw[-1]))
         line["sweep"] = sw
     if not args.no_streams:
         log("bench: zipf insert stream")
