@@ -582,12 +582,15 @@ def main():
     if os.path.exists(tf):
         traffic = json.load(open(tf))
     dominant_insert = r["insert_ms"] >= r["find_ms"]
-    roof_find = {"kernel": "oa_wave_staged_kernel<8,8,G,OP_FIND>", "bound": "hbm", "achieved": round(find_gbs, 1),
+    g = args.group or 4
+    find_kernel = ("oa_wave_staged_kernel_w8<8,8,4,OP_FIND>" if g == 4 and os.environ.get("MXCO_OA_FIND_W8", "1") != "0"
+                   else "oa_wave_staged_kernel<8,8,%d,OP_FIND>" % g)
+    roof_find = {"kernel": find_kernel, "bound": "hbm", "achieved": round(find_gbs, 1),
                  "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(find_gbs / PEAK_HBM_GBS, 4),
                  "traffic": traffic.get("find") if traffic else None,
                  "lines_per_query": r["find_lines"] / r["n_find"], "launch_ms": r["find_ms"],
                  "algorithmic_bytes": find_bytes}
-    roof_ins = {"kernel": "oa_kernel<8,8,G,OP_INSERT>", "bound": "hbm", "achieved": round(ins_gbs, 1),
+    roof_ins = {"kernel": "oa_kernel<8,8,%d,OP_INSERT>" % g, "bound": "hbm", "achieved": round(ins_gbs, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ins_gbs / PEAK_HBM_GBS, 4),
                 "traffic": traffic.get("insert") if traffic else None,
                 "lines_per_key": r["insert_lines"] / r["n_insert"], "launch_ms": r["insert_ms"],
