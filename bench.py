@@ -583,8 +583,7 @@ def main():
         traffic = json.load(open(tf))
     dominant_insert = r["insert_ms"] >= r["find_ms"]
     g = args.group or 4
-    find_kernel = ("oa_wave_staged_kernel_w8<8,8,4,OP_FIND>" if g == 4 and os.environ.get("MXCO_OA_FIND_W8", "1") != "0"
-                   else "oa_wave_staged_kernel<8,8,%d,OP_FIND>" % g)
+    find_kernel = "oa_wave_staged_kernel<8,8,%d,OP_FIND>" % g
     roof_find = {"kernel": find_kernel, "bound": "hbm", "achieved": round(find_gbs, 1),
                  "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(find_gbs / PEAK_HBM_GBS, 4),
                  "traffic": traffic.get("find") if traffic else None,

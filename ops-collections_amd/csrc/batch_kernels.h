@@ -137,10 +137,11 @@ __global__ __launch_bounds__(256) void oa_batch_wave_staged_kernel(OaArgs base, 
     static_assert(kChunk % kWaveChunk == 0, "chunks split into whole sub-chunks");
     constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
     __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
+    __shared__ __align__(16) uint8_t keys[4][2][kWaveChunk * KS];
     const uint32_t wib = threadIdx.x >> 6;
     BatchChunks<KS, OUTB> src{&base, tables, items, chunks, nchunks,
                               (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib, (uint64_t)gridDim.x * (blockDim.x >> 6)};
-    wave_staged_find<KS, VS, G, OP>(base, src, stage[wib][0], stage[wib][1]);
+    wave_staged_find<KS, VS, G, OP>(base, src, stage[wib][0], stage[wib][1], keys[wib][0], keys[wib][1]);
 }
 
 // Launches the persistent multi-table kernel (defined in batch.hip).

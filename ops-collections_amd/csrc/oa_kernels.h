@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "mxco/detail/hash.h"
 
 namespace mxco {
@@ -297,12 +299,28 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
     Walk W[U];
     uint32_t WLEN[U];
     Raw S[U][K];
+    // Key (and payload) of slot u's next index, loaded one key ahead so a
+    // group that finishes a key starts the next walk without a load round
+    // trip.
+    uint64_t NKEY[U], NPVAL[U];
+    auto prefetch = [&](int u, uint64_t idx) {
+        if (idx >= end) return;
+        if constexpr (kInsertFamily) {
+            NKEY[u] = load_bits<KS>(a.in + idx * SS);
+            if constexpr (VS) NPVAL[u] = load_bits<VS>(a.in + idx * SS + VOFF);
+        } else {
+            NKEY[u] = load_bits<KS>(a.in + idx * KS);
+        }
+    };
 
     // Sets up key `idx`.  Returns false when its outcome needs no walk
     // (false predicate, reserved key) -- the outcome is written here.
     auto begin = [&](int u, uint64_t idx) -> bool {
         uint64_t& key = KEY[u];
         uint64_t& pval = PVAL[u];
+        key = NKEY[u];
+        if constexpr (kInsertFamily && VS) pval = NPVAL[u];
+        prefetch(u, idx + (uint64_t)U * ngroups);
         if (!stencil_ok(a, idx)) {
             if (leader) {
                 if constexpr (OP == OP_FIND) store_bits<OUTS>(a.out + idx * OUTS, miss_out);
@@ -311,8 +329,6 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
             return false;
         }
         if constexpr (kInsertFamily) {
-            key = load_bits<KS>(a.in + idx * SS);
-            if constexpr (VS) pval = load_bits<VS>(a.in + idx * SS + VOFF);
             // The reserved empty key is never inserted (open_addressing_ref_impl.h:494-496).
             if (key == ekey) {
                 if (leader) {
@@ -324,8 +340,6 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                 }
                 return false;
             }
-        } else {
-            key = load_bits<KS>(a.in + idx * KS);
         }
         W[u] = make_walk<KS>(a, key);
         PREV[u] = ~0ull;
@@ -339,6 +353,9 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         W[u].done = true;
+        NKEY[u] = 0;
+        NPVAL[u] = 0;
+        prefetch(u, gid + (uint64_t)u * ngroups);
         advance(u, gid + (uint64_t)u * ngroups);
     }
 
@@ -722,7 +739,8 @@ struct PlainChunks {
 };
 
 template <int KS, int VS, int G, int OP, typename Src>
-__device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint8_t* st0, uint8_t* st1)
+__device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint8_t* st0, uint8_t* st1,
+                                                 uint8_t* kb0, uint8_t* kb1)
 {
     static_assert(OP == OP_FIND || OP == OP_CONTAINS, "Find / Contains only");
     using L = Lay<KS, VS>;
@@ -742,8 +760,23 @@ __device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint
 
     // Wave-uniform state of the two open chunks: keys handed out and keys
     // resolved.  `old` is the chunk opened first.
+    // The keys of an open chunk are copied to its LDS key buffer when it
+    // opens (one coalesced load round trip per chunk), so a group that takes
+    // a new key reads it from LDS and issues its first window load in the
+    // same iteration.
+    auto stage_keys = [&](uint8_t* kb, const WsChunk& c) {
+        static_assert(kWaveChunk == 128, "two keys per lane");
+        using KT = typename std::conditional<KS == 8, uint64_t, uint32_t>::type;
+        KT v0 = 0, v1 = 0;
+        if (lane < c.len) v0 = ((const KT*)c.in)[lane];
+        if (lane + 64 < c.len) v1 = ((const KT*)c.in)[lane + 64];
+        ((KT*)kb)[lane] = v0;
+        ((KT*)kb)[lane + 64] = v1;
+    };
     WsChunk c0 = src.open();
     WsChunk c1 = src.open();
+    stage_keys(kb0, c0);
+    stage_keys(kb1, c1);
     uint32_t ch0 = 0, ch1 = 0, cd0 = 0, cd1 = 0;
     int old = 0;
 
@@ -805,7 +838,7 @@ __device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint
                         if (leader) put(miss_out);
                         key_done = true;
                     } else {
-                        key = load_bits<KS>((B ? c1.in : c0.in) + (uint64_t)I * KS);
+                        key = load_bits<KS>((B ? kb1 : kb0) + (uint64_t)I * KS);
                         w = make_walk<KS>(ga, key);
                     }
                 }
@@ -883,6 +916,7 @@ __device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint
         if (c0.len != 0 && cd0 == c0.len) {
             flush(st0, c0);
             c0 = src.open();
+            stage_keys(kb0, c0);
             ch0 = 0;
             cd0 = 0;
             if (old == 0) old = 1;
@@ -890,6 +924,7 @@ __device__ __forceinline__ void wave_staged_find(const OaArgs& a, Src& src, uint
         if (c1.len != 0 && cd1 == c1.len) {
             flush(st1, c1);
             c1 = src.open();
+            stage_keys(kb1, c1);
             ch1 = 0;
             cd1 = 0;
             if (old == 1) old = 0;
@@ -902,24 +937,13 @@ __global__ __launch_bounds__(256) void oa_wave_staged_kernel(OaArgs a)
 {
     constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
     __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
+    __shared__ __align__(16) uint8_t keys[4][2][kWaveChunk * KS];
     const uint32_t wib = threadIdx.x >> 6;
     PlainChunks<KS, OUTB> src{&a, (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib,
                               (uint64_t)gridDim.x * (blockDim.x >> 6)};
-    wave_staged_find<KS, VS, G, OP>(a, src, stage[wib][0], stage[wib][1]);
+    wave_staged_find<KS, VS, G, OP>(a, src, stage[wib][0], stage[wib][1], keys[wib][0], keys[wib][1]);
 }
 
-// The same kernel held to 64 VGPRs (8 waves per SIMD); used where that
-// costs no spills.
-template <int KS, int VS, int G, int OP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void oa_wave_staged_kernel_w8(OaArgs a)
-{
-    constexpr int OUTB = OP == OP_CONTAINS ? 1 : (VS ? VS : KS);
-    __shared__ __align__(16) uint8_t stage[4][2][kWaveChunk * OUTB];
-    const uint32_t wib = threadIdx.x >> 6;
-    PlainChunks<KS, OUTB> src{&a, (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib,
-                              (uint64_t)gridDim.x * (blockDim.x >> 6)};
-    wave_staged_find<KS, VS, G, OP>(a, src, stage[wib][0], stage[wib][1]);
-}
 
 // Sentinel fill (Clear / ClearAsync, open_addressing_impl.h:341-452):
 // streams the 16-byte replicated slot image over the whole slot array.

@@ -4,7 +4,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "mxco_common.h"
 #include "oa_kernels.h"
 
 namespace mxco {
@@ -16,18 +15,7 @@ hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s, int in
     if (inflight == 4) {
         // wave-staged outputs: the bulk Find / Contains paths
         switch (op) {
-        case OP_FIND:
-            if constexpr (KS == 8 && VS == 8 && G == 4) {
-                // 64 VGPRs fit this configuration without spills: 8 waves
-                // per SIMD instead of 7 (3 % faster at 2^28 slots)
-                static const int w8 = env_int("MXCO_OA_FIND_W8", 1);
-                if (w8) {
-                    oa_wave_staged_kernel_w8<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a);
-                    return hipGetLastError();
-                }
-            }
-            oa_wave_staged_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a);
-            return hipGetLastError();
+        case OP_FIND: oa_wave_staged_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a); return hipGetLastError();
         case OP_CONTAINS:
             oa_wave_staged_kernel<KS, VS, G, OP_CONTAINS><<<grid, block, 0, s>>>(a);
             return hipGetLastError();
