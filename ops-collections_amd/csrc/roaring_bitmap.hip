@@ -191,6 +191,55 @@ __global__ __launch_bounds__(256) void rb_dense_bits_kernel(const uint64_t* dens
     }
 }
 
+// Small uint32 bitmaps (one bucket, at most kLdsContainers bitset
+// containers): one persistent workgroup per CU copies the bitsets into LDS
+// once, keeps the container keys in registers, and streams the queries four
+// keys per 16-byte load, four loads in flight per lane, writing four result
+// bytes per store.  A query is then register compares and one LDS byte read.
+constexpr uint32_t kLdsContainers = 12;  // 96 KiB of LDS
+constexpr uint32_t kLdsThreads = 1024;
+
+__global__ __launch_bounds__(kLdsThreads) void rb_lds_bits_kernel(const uint8_t* image, const uint16_t* ckeys,
+                                                                  uint32_t nc, const uint32_t* keys, uint8_t* out,
+                                                                  uint64_t n)
+{
+    __shared__ uint4 lds[kLdsContainers * kBitsetBytes / 16];
+    const uint4* src = (const uint4*)image;
+    for (uint32_t j = threadIdx.x; j < nc * (kBitsetBytes / 16); j += blockDim.x) lds[j] = src[j];
+    uint32_t kk[kLdsContainers];
+#pragma unroll
+    for (uint32_t j = 0; j < kLdsContainers; ++j) kk[j] = j < nc ? ckeys[j] : 0x10000u;  // 0x10000: never a key
+    __syncthreads();
+    const uint8_t* bits = (const uint8_t*)lds;
+    auto test = [&](uint32_t key) -> uint32_t {
+        const uint32_t k16 = key >> 16, v = key & 0xffffu;
+        int c = -1;
+#pragma unroll
+        for (uint32_t j = 0; j < kLdsContainers; ++j) c = k16 == kk[j] ? (int)j : c;
+        const uint32_t b = bits[(uint32_t)max(c, 0) * kBitsetBytes + (v >> 3)];
+        return c >= 0 ? (b >> (v & 7)) & 1u : 0u;
+    };
+    constexpr int U = 4;
+    const uint64_t n4 = n / 4, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < n4; q0 += stride * U) {
+        uint4 kv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t q = q0 + u * stride;
+            kv[u] = q < n4 ? ((const uint4*)keys)[q] : uint4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t q = q0 + u * stride;
+            if (q < n4)
+                ((uint32_t*)out)[q] =
+                    test(kv[u].x) | test(kv[u].y) << 8 | test(kv[u].z) << 16 | test(kv[u].w) << 24;
+        }
+    }
+    for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (uint8_t)test(keys[i]);
+}
+
 // Element count of a container in the image: values, or runs (the run
 // count leads the serialized run container).
 uint32_t elements(const uint8_t* ser, const HostContainer& c)
@@ -251,6 +300,7 @@ struct mxco_rb {
     bool bitsets_only = false;  // every container of the image is a bitset
     bool dense_bits = false;    // ... and the bitmap is one bucket with a dense table
     uint32_t bucket_key = 0;
+    uint32_t bucket_containers = 0;  // containers of the first bucket
     uint64_t num_keys = 0;
     uint64_t size_bytes = 0;
 };
@@ -392,6 +442,7 @@ MXCO_API mxco_status mxco_rb_create(uint32_t key_size, const void* bitmap, size_
     rb->bitsets_only = bitsets_only;
     rb->dense_bits = bitsets_only && buckets.size() == 1 && buckets[0].dense != detail::kRbNoDense;
     rb->bucket_key = buckets.empty() ? 0 : buckets[0].key;
+    rb->bucket_containers = buckets.empty() ? 0 : buckets[0].count;
     rb->num_keys = keys;
     rb->size_bytes = total;
     hipError_t e = upload(&rb->bytes, ser, total, stream);
@@ -446,7 +497,12 @@ MXCO_API mxco_status mxco_rb_contains(const mxco_rb* rb, const void* keys, uint8
     const unsigned grid = grid_for(n, 8);
     const Search s{rb->buckets, rb->dense, rb->ckeys, rb->cdesc, rb->image, rb->nbuckets};
     const uint8_t* k = (const uint8_t*)keys;
-    if (rb->dense_bits) {
+    if (rb->dense_bits && rb->ks == 4 && rb->bucket_containers <= kLdsContainers && n >= (1u << 16) &&
+        ((uintptr_t)keys & 15u) == 0 && ((uintptr_t)out & 3u) == 0) {
+        // one bucket of key 0: its containers are the image's first bitsets
+        rb_lds_bits_kernel<<<cu_count(), kLdsThreads, 0, stream>>>(rb->image, rb->ckeys, rb->bucket_containers,
+                                                                  (const uint32_t*)keys, out, n);
+    } else if (rb->dense_bits) {
         const unsigned g = grid_for((n + 3) / 4, 8);
         if (rb->ks == 4) rb_dense_bits_kernel<4><<<g, 256, 0, stream>>>(rb->dense, rb->image, 0, k, out, n);
         else rb_dense_bits_kernel<8><<<g, 256, 0, stream>>>(rb->dense, rb->image, rb->bucket_key, k, out, n);

@@ -132,3 +132,20 @@ def test_one_bucket_64bit_bitmap(image):
     rb = mx().RoaringBitmap(R.serialize64(keys, run_containers="optimize"), torch.int64)
     q = np.concatenate([keys, low, low | np.uint64(8 << 32), keys + 1])
     assert np.array_equal(query(rb, q, torch.int64), np.isin(q, keys))
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+def test_small_bitmap_lds_path_and_ragged_tail(offset):
+    """A uint32 bitmap of a few containers and a large query batch (the
+    LDS-resident path when the key pointer is 16-byte aligned, the general
+    path at an offset of one key), with a ragged tail."""
+    rng = np.random.default_rng(14)
+    keys = np.unique(np.concatenate([rng.integers(k << 16, (k << 16) + 65536, 9000) for k in (1, 4, 7, 8, 300)]))
+    keys = keys.astype(np.uint64)
+    rb = mx().RoaringBitmap(R.serialize32(keys, run_containers="optimize"), torch.int32)
+    n = (1 << 17) + 3
+    q = np.concatenate([keys[rng.integers(0, len(keys), n // 2)], rng.integers(0, 1 << 25, n - n // 2)])
+    q = q.astype(np.uint64)
+    t = torch.from_numpy(np.concatenate([np.zeros(offset, np.uint32), q.astype(np.uint32)]).view(np.int32)).cuda()
+    got = rb.contains(t[offset:]).cpu().numpy()
+    assert np.array_equal(got, np.isin(q, keys))
