@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-filters", action="store_true", help="skip the BloomFilter and HyperLogLog legs")
     ap.add_argument("--hll-sketches", type=int, default=64)
     ap.add_argument("--no-roaring", action="store_true", help="skip the RoaringBitmap query leg")
+    ap.add_argument("--no-containers", action="store_true",
+                    help="skip the StaticMultimap / StaticMultiset / DynamicMap / Erase+Count leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -329,6 +331,140 @@ def run_roaring(torch, mxco, steps=5):
                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                   "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_key": qbytes}}
         del rb, q
+    return res
+
+
+def run_containers(torch, mxco, steps=3):
+    """The other containers at one shape each, int64 keys, verified:
+    - StaticMultimap (int64 -> int64): capacity 2^26, Insert 2^25 pairs over
+      2^23 distinct keys (Poisson(4) multiplicities), Count and Find of 2^24
+      queries (half present), Retrieve of 2^20 present queries;
+    - StaticMultiset: the same keys, Insert, CountEach and Contains of 2^24;
+    - DynamicMap: initial capacity 2^22, 2^26 distinct pairs inserted in 16
+      batches of 2^22 (the map grows), then Find of 2^26 (half present);
+    - StaticMap Erase and Count on a 2^26-slot table at LF 0.8.
+    Rates are keys (or queries) per second of the op's launches; the lower-
+    bound bytes count one 128-byte line per key or query beside its input
+    and output."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    res = {}
+
+    def rate(n, ms):
+        return round(n / ms / 1e6, 3)
+
+    def lb(n, per, ms):  # lower-bound GB/s and fraction of the HBM peak
+        gbs = n * per / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved_lower_bound": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac_lower_bound": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_key": per}
+
+    # keys with multiplicities: index r in [0, D) -> key r * ZMUL + 1
+    D, N, Q = 1 << 23, 1 << 25, 1 << 24
+    ZMUL = 0x9E3779B97F4A7C1
+    r = torch.randint(0, D, (N,), device="cuda", generator=g)
+    mult = torch.bincount(r, minlength=D)
+    keys = r * ZMUL + 1
+    vals = keys * 3 + r
+    qi = torch.cat([torch.randint(0, D, (Q // 2,), device="cuda", generator=g),
+                    torch.randint(D, 2 * D, (Q - Q // 2,), device="cuda", generator=g)])
+    q = qi * ZMUL + 1
+    exp_mult = torch.where(qi < D, mult[qi.clamp(max=D - 1)], torch.zeros_like(qi))
+
+    mm = mxco.StaticMultimap(1 << 26, -1, -1, torch.int64, torch.int64)
+    pairs = mxco.make_pairs(keys, vals)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    failed = mm.insert(pairs)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_ins = ev0.elapsed_time(ev1)
+    t_cnt = _timed(torch, lambda: mm.count(q), steps)
+    total = mm.count(q)
+    out = torch.empty(Q, dtype=torch.int64, device="cuda")
+    t_find = _timed(torch, lambda: mm.find(q, out=out), steps)
+    present = qi < D
+    rq = q[present][: 1 << 20].contiguous()
+    cap_out = int(mm.count(rq))
+    po = torch.empty(cap_out, dtype=torch.int64, device="cuda")
+    vo = torch.empty(cap_out, dtype=torch.int64, device="cuda")
+    t_ret = _timed(torch, lambda: mm.retrieve(rq, po, vo, cap_out), 1)
+    got = mm.retrieve(rq, po, vo, cap_out)
+    exp_find = torch.where(exp_mult > 0, q * 3 + qi, torch.full_like(q, -1))  # one value per key
+    ok = (failed == 0 and total == int(exp_mult.sum().item()) and got == cap_out
+          and bool(torch.equal(out, exp_find)))
+    res["multimap"] = {"capacity": mm.capacity(), "inserted": N, "distinct": int((mult > 0).sum().item()),
+                       "queries": Q, "insert_ms": round(t_ins, 3), "insert_gkeys_s": rate(N, t_ins),
+                       "count_ms": round(t_cnt, 3), "count_gkeys_s": rate(Q, t_cnt),
+                       "find_ms": round(t_find, 3), "find_gkeys_s": rate(Q, t_find),
+                       "retrieve_queries": rq.numel(), "retrieve_matches": got, "retrieve_ms": round(t_ret, 3),
+                       "verified": bool(ok), "roofline_find": lb(Q, 128 + 16, t_find),
+                       "roofline_insert": lb(N, 128 + 128 + 16, t_ins)}
+    del mm, po, vo
+
+    ms = mxco.StaticMultiset(1 << 26, -1, torch.int64)
+    ev0.record()
+    failed = ms.insert(keys)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_ins = ev0.elapsed_time(ev1)
+    ce = torch.empty(Q, dtype=torch.int64, device="cuda")
+    t_ce = _timed(torch, lambda: ms.count_each(q, out=ce), steps)
+    con = torch.empty(Q, dtype=torch.bool, device="cuda")
+    t_con = _timed(torch, lambda: ms.contains(q, out=con), steps)
+    ok = failed == 0 and bool(torch.equal(ce.to(torch.int64), exp_mult)) and bool(torch.equal(con, exp_mult > 0))
+    res["multiset"] = {"capacity": ms.capacity(), "inserted": N, "queries": Q, "insert_ms": round(t_ins, 3),
+                       "insert_gkeys_s": rate(N, t_ins), "count_each_ms": round(t_ce, 3),
+                       "count_each_gkeys_s": rate(Q, t_ce), "contains_ms": round(t_con, 3),
+                       "contains_gkeys_s": rate(Q, t_con), "verified": bool(ok),
+                       "roofline_count_each": lb(Q, 128 + 16, t_ce)}
+    del ms, ce, con, keys, vals, pairs, r, mult
+
+    # DynamicMap growth
+    NB, B = 16, 1 << 22
+    dk = torch.unique(torch.randint(0, 1 << 62, (NB * B + (1 << 16),), device="cuda", generator=g))[: NB * B]
+    dk = dk[torch.randperm(dk.numel(), device="cuda", generator=g)]
+    dpairs = mxco.make_pairs(dk, dk * 7 + 1)
+    dm = mxco.DynamicMap(1 << 22, -1, -1, -2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in range(NB):
+        dm.insert(dpairs[b * B:(b + 1) * B])
+    torch.cuda.synchronize()
+    t_dins = (time.perf_counter() - t0) * 1e3
+    dq = torch.cat([dk[torch.randint(0, dk.numel(), (dk.numel() // 2,), device="cuda", generator=g)],
+                    torch.randint(1 << 62, (1 << 63) - 1, (dk.numel() // 2,), device="cuda", generator=g)])
+    dout = torch.empty(dq.numel(), dtype=torch.int64, device="cuda")
+    t_dfind = _timed(torch, lambda: dm.find(dq, out=dout), steps)
+    ok = bool(torch.equal(dout, torch.where(dq < (1 << 62), dq * 7 + 1, torch.full_like(dq, -1)))) and \
+        dm.size() == dk.numel()
+    res["dynamic_map"] = {"initial_capacity": 1 << 22, "batches": NB, "batch": B, "final_capacity": dm.capacity(),
+                          "submaps": dm.num_submaps(), "insert_ms": round(t_dins, 3),
+                          "insert_gkeys_s": rate(NB * B, t_dins), "find_ms": round(t_dfind, 3),
+                          "find_gkeys_s": rate(dq.numel(), t_dfind), "verified": ok,
+                          "note": "insert time is wall clock over the 16 synchronous batches, growth included"}
+    del dm, dk, dpairs, dq, dout
+
+    # StaticMap Erase + Count
+    m = mxco.StaticMap(1 << 26, -1, -1, torch.int64, torch.int64)
+    m.set_erased_key(-2)
+    ek, ev, eq = make_workload(torch, m.capacity(), 0.8, seed=99)
+    m.insert(mxco.make_pairs(ek, ev))
+    t_c = _timed(torch, lambda: m.count(eq), steps)
+    present = int(m.count(eq))
+    half = ek[: ek.numel() // 2].contiguous()
+    ev0.record()
+    failed = m.erase(half)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_e = ev0.elapsed_time(ev1)
+    left = int(m.count(ek))
+    ok = failed == 0 and present == eq.numel() // 2 and left == ek.numel() - half.numel()
+    res["erase_count"] = {"capacity": m.capacity(), "inserted": ek.numel(), "count_queries": eq.numel(),
+                          "count_ms": round(t_c, 3), "count_gkeys_s": rate(eq.numel(), t_c),
+                          "erase_keys": half.numel(), "erase_ms": round(t_e, 3),
+                          "erase_gkeys_s": rate(half.numel(), t_e), "verified": bool(ok)}
+    del m
+    torch.cuda.empty_cache()
     return res
 
 
@@ -653,6 +789,10 @@ def main():
         log("bench: roaring bitmap")
         line["roaring"] = run_roaring(torch, mxco)
         log("bench: roaring %s" % json.dumps(line["roaring"]))
+    if not args.no_containers:
+        log("bench: multimap, multiset, dynamic map, erase/count")
+        line["containers"] = run_containers(torch, mxco)
+        log("bench: containers %s" % json.dumps(line["containers"]))
     if not args.no_batched:
         log("bench: batched 1024 tables")
         line["batched"] = run_batched(args, torch, mxco)
