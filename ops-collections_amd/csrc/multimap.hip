@@ -140,6 +140,11 @@ __global__ __launch_bounds__(256) void mm_insert_kernel(uint8_t* table, uint64_t
 
 enum { Q_CONTAINS = 0, Q_FIND = 1, Q_COUNT = 2, Q_GATHER = 3 };
 
+// Retrieve's gather sorts a query's values in place (insertion into its
+// output segment) when it has at most this many; larger segments go to a
+// scratch copy and a segmented sort.
+constexpr uint64_t kInlineSort = 64;
+
 // One lane per query; the walk covers the run of occupied slots from the
 // key's start slot, one line-remainder window per round trip.
 template <int KS, int VS, int MODE>
@@ -162,6 +167,15 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
         int64_t best = 0;
         bool any = false;
         uint64_t wpos = (MODE == Q_GATHER) ? offsets[i] : 0;
+        const uint64_t seg_off = wpos;
+        const uint64_t seg_len = (MODE == Q_GATHER) ? offsets[i + 1] - offsets[i] : 0;
+        if (MODE == Q_GATHER && seg_len > kInlineSort) {
+            // listed for the segmented sort: begins in counts[0..n],
+            // ends in counts[n + 1..2n + 1]
+            const unsigned long long j = atomicAdd(total, 1ull);
+            counts[j] = seg_off;
+            counts[n + 1 + j] = seg_off + seg_len;
+        }
         if (sel && key != ek) {
             uint64_t pos = mm_start<KS, VS>(key, mask);
             uint64_t left = mask + 1;
@@ -184,8 +198,19 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
                         const int64_t v = sval<VS>(L::val(s[k]));
                         if (MODE == Q_CONTAINS) stop = true;
                         if (MODE == Q_GATHER) {
+                            // out: scratch for segments sorted afterwards; value_out: final output
                             store_bits<KS>(probe_out + wpos * KS, key);
-                            store_bits<VS>(value_out + wpos * VS, (uint64_t)v);
+                            if (seg_len > kInlineSort) {
+                                store_bits<VS>(out + wpos * VS, (uint64_t)v);
+                            } else {
+                                // insertion into the sorted prefix of the segment (ascending, signed)
+                                uint64_t p = wpos;
+                                while (p > seg_off && sval<VS>(load_bits<VS>(value_out + (p - 1) * VS)) > v) {
+                                    store_bits<VS>(value_out + p * VS, load_bits<VS>(value_out + (p - 1) * VS));
+                                    --p;
+                                }
+                                store_bits<VS>(value_out + p * VS, (uint64_t)v);
+                            }
                             ++wpos;
                         }
                         best = (!any || v < best) ? v : best;
@@ -360,9 +385,9 @@ mxco_status query(mxco_mm* m, int mode, const void* keys, const mxco_stencil* st
         break;                                                                                                    \
     default:                                                                                                      \
         mm_query_kernel<K, V, Q_GATHER><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz,      \
-                                                             pred, n, m->empty_key, m->empty_val, nullptr,        \
-                                                             nullptr, offsets, (uint8_t*)probe_out,              \
-                                                             (uint8_t*)value_out, nullptr);                       \
+                                                             pred, n, m->empty_key, m->empty_val, (uint8_t*)out,  \
+                                                             counts, offsets, (uint8_t*)probe_out,               \
+                                                             (uint8_t*)value_out, total);                         \
     }
     MM_DISPATCH(m, QL);
 #undef QL
@@ -549,8 +574,9 @@ MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* m, const void* keys, uint64_t n, 
     *num_out = 0;
     if (n == 0) return ok();
     if (!keys) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
-    // counts -> exclusive offsets (n + 1 entries)
-    if (mxco_status e = reserve(m, 0, (n + 1) * sizeof(uint64_t))) return e;
+    // counts -> exclusive offsets (n + 1 entries); afterwards counts[] holds
+    // the list of segments too long to sort in place (2n + 2 entries)
+    if (mxco_status e = reserve(m, 0, (2 * n + 2) * sizeof(uint64_t))) return e;
     if (mxco_status e = reserve(m, 4, (n + 1) * sizeof(uint64_t))) return e;
     if (mxco_status e = reserve(m, 2, sizeof(unsigned long long))) return e;
     uint64_t* counts = sbuf<uint64_t>(m, 0);
@@ -564,31 +590,40 @@ MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* m, const void* keys, uint64_t n, 
     if (total > output_capacity) return fail(MXCO_LENGTH_ERROR, "StaticMultimap: output capacity is insufficient");
     if (total == 0) return ok();
     if (!probe_out || !value_out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
-    // gather unsorted values into scratch, then sort every query's segment into value_out
+    // Gather: segments of up to kInlineSort values are sorted in place in
+    // value_out; longer ones go unsorted to scratch and are listed.
     if (mxco_status e = reserve(m, 1, total * m->vs)) return e;
-    if (mxco_status e = query(m, Q_GATHER, keys, nullptr, nullptr, n, nullptr, offsets, probe_out, m->buf[1],
-                              nullptr, s))
+    MXCO_HIP(hipMemsetAsync(total_dev, 0, 8, s), "StaticMultimap: retrieve");
+    if (mxco_status e = query(m, Q_GATHER, keys, nullptr, m->buf[1], n, counts, offsets, probe_out, value_out,
+                              total_dev, s))
         return e;
-    size_t temp = 0;
-    hipError_t he;
-    if (m->vs == 4) {
-        he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int32_t*)m->buf[1], (int32_t*)value_out,
-                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
-    } else {
-        he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int64_t*)m->buf[1], (int64_t*)value_out,
-                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
-    }
-    MXCO_HIP(he, "StaticMultimap: sort");
-    if (mxco_status e = reserve(m, 3, temp)) return e;
-    if (m->vs == 4) {
-        he = hipcub::DeviceSegmentedSort::SortKeys(m->buf[3], temp, (const int32_t*)m->buf[1], (int32_t*)value_out,
-                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
-    } else {
-        he = hipcub::DeviceSegmentedSort::SortKeys(m->buf[3], temp, (const int64_t*)m->buf[1], (int64_t*)value_out,
-                                                  (int64_t)total, (int64_t)n, offsets, offsets + 1, s);
-    }
-    MXCO_HIP(he, "StaticMultimap: sort");
+    MXCO_HIP(hipMemcpyAsync(m->host, total_dev, 8, hipMemcpyDeviceToHost, s), "StaticMultimap: retrieve");
     MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: retrieve");
+    const int64_t nlong = (int64_t)m->host[0];
+    if (nlong > 0) {
+        const uint64_t* begins = counts;
+        const uint64_t* ends = counts + n + 1;
+        size_t temp = 0;
+        hipError_t he;
+        if (m->vs == 4) {
+            he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int32_t*)m->buf[1], (int32_t*)value_out,
+                                                      (int64_t)total, nlong, begins, ends, s);
+        } else {
+            he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int64_t*)m->buf[1], (int64_t*)value_out,
+                                                      (int64_t)total, nlong, begins, ends, s);
+        }
+        MXCO_HIP(he, "StaticMultimap: sort");
+        if (mxco_status e = reserve(m, 3, temp)) return e;
+        if (m->vs == 4) {
+            he = hipcub::DeviceSegmentedSort::SortKeys(m->buf[3], temp, (const int32_t*)m->buf[1], (int32_t*)value_out,
+                                                      (int64_t)total, nlong, begins, ends, s);
+        } else {
+            he = hipcub::DeviceSegmentedSort::SortKeys(m->buf[3], temp, (const int64_t*)m->buf[1], (int64_t*)value_out,
+                                                      (int64_t)total, nlong, begins, ends, s);
+        }
+        MXCO_HIP(he, "StaticMultimap: sort");
+        MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: retrieve");
+    }
     *num_out = total;
     return ok();
 }

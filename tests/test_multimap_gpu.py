@@ -94,3 +94,27 @@ def test_multimap_errors():
     with pytest.raises(ValueError):
         mm.insert(0, n=3)
     assert mm.find(dev(np.array([-1], np.int64))).item() == -1  # the empty key always misses
+
+
+@pytest.mark.parametrize("kd,vd", [(np.int32, np.int32), (np.int64, np.int64)])
+def test_retrieve_long_and_short_segments(kd, vd):
+    """Retrieve sorts short segments in place and long ones (more than 64
+    values) with a segmented sort: multiplicities 1, 63, 64, 65, 500 and 3000,
+    negative values included, queries repeated and absent."""
+    rng = np.random.default_rng(2)
+    mult = {11: 1, 12: 63, 13: 64, 14: 65, 15: 500, 16: 3000}
+    k = np.concatenate([np.full(c, key) for key, c in mult.items()]).astype(kd)
+    v = rng.integers(-10**6, 10**6, len(k)).astype(vd)
+    perm = rng.permutation(len(k))
+    k, v = k[perm], v[perm]
+    mm = mx().StaticMultimap(20000, -1, -1, TD[kd], TD[vd])
+    ref = MultimapOracle(20000, -1, -1, kd, vd)
+    assert mm.insert(pairs(k, v)) == ref.insert(k, v)
+    q = np.array([16, 11, 99, 13, 15, 14, 12, 16, 12], kd)
+    total = ref.count(q)
+    po = torch.empty(total, dtype=TD[kd], device="cuda")
+    vo = torch.empty(total, dtype=TD[vd], device="cuda")
+    assert mm.retrieve(dev(q), po, vo) == total
+    ep, ev = ref.retrieve(q)
+    np.testing.assert_array_equal(po.cpu().numpy(), ep)
+    np.testing.assert_array_equal(vo.cpu().numpy(), ev)
