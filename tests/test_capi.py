@@ -57,3 +57,17 @@ def test_status_strings():
     lib.mxco_status_string.restype = ctypes.c_char_p
     assert lib.mxco_status_string(0) == b"ok"
     assert lib.mxco_status_string(2) == b"length error"
+
+
+def test_create_rejects_capacity_beyond_32_bit_slot_index_before_allocating():
+    """The device probe sequence indexes slots with 32 bits: a capacity that
+    rounds above 2^32 - 1 fails in mxco_oa_create before any allocation, with
+    MXCO_INVALID_ARGUMENT and a message naming the limit."""
+    import mxco
+    d = mxco.OaDesc(8, 8, 0, 5, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    ek, ev = ctypes.c_int64(-1), ctypes.c_int64(-1)
+    st = mxco._lib.mxco_oa_create(ctypes.byref(d), 1 << 33, ctypes.byref(ek), ctypes.byref(ev), None,
+                                  ctypes.byref(h))
+    assert st == mxco.INVALID_ARGUMENT and h.value is None
+    assert b"32-bit slot index" in mxco._lib.mxco_last_error()
