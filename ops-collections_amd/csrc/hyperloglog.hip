@@ -81,32 +81,59 @@ __device__ __forceinline__ void raise_byte(W* w, uint32_t shift, uint32_t rank)
     }
 }
 
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
 template <int KS>
 __global__ __launch_bounds__(kThreads) void hll_private_add_kernel(const uint8_t* keys, uint64_t n, uint32_t p,
-                                                                   uint8_t* workspace)
+                                                                   uint8_t* workspace, int pairs16)
 {
     extern __shared__ uint32_t sk[];
     const uint32_t nwords = (1u << p) / 4;
     for (uint32_t j = threadIdx.x; j < nwords; j += blockDim.x) sk[j] = 0;
     __syncthreads();
-    // Contiguous chunk per workgroup so each input line is read by one CU.
-    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
-    const uint64_t begin = (uint64_t)blockIdx.x * per;
+    auto add = [&](uint64_t key) {
+        uint32_t idx, rank;
+        hll_index_rank<KS>(key, p, idx, rank);
+        raise_byte(&sk[idx >> 2], (idx & 3u) * 8u, rank);
+    };
+    // Contiguous chunk per workgroup so each input line is read by one CU;
+    // an even chunk length keeps every chunk 16-byte aligned for int64 keys.
+    const uint64_t per = ((n + gridDim.x - 1) / gridDim.x + 1) & ~1ull;
+    const uint64_t begin = min(n, (uint64_t)blockIdx.x * per);
     const uint64_t end = min(n, begin + per);
-    constexpr int U = 4;  // key loads in flight per lane
-    for (uint64_t i0 = begin + threadIdx.x; i0 < end; i0 += (uint64_t)U * blockDim.x) {
-        uint64_t k[U];
+    constexpr int U = 4;  // loads in flight per lane
+    if (KS == 8 && pairs16) {
+        // two keys per 16-byte load (keys 16-byte aligned)
+        const uint64_t np = (end - begin) / 2;
+        const u64x2* kp = (const u64x2*)(keys + begin * 8);
+        for (uint64_t j0 = threadIdx.x; j0 < np; j0 += (uint64_t)U * blockDim.x) {
+            u64x2 k[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t i = i0 + (uint64_t)u * blockDim.x;
-            k[u] = i < end ? key_at<KS>(keys, i) : 0;
+            for (int u = 0; u < U; ++u) {
+                const uint64_t j = j0 + (uint64_t)u * blockDim.x;
+                k[u] = j < np ? kp[j] : u64x2{0, 0};
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (j0 + (uint64_t)u * blockDim.x >= np) break;
+                add(k[u].x);
+                add(k[u].y);
+            }
         }
+        if (((end - begin) & 1) && threadIdx.x == 0) add(key_at<KS>(keys, end - 1));
+    } else {
+        for (uint64_t i0 = begin + threadIdx.x; i0 < end; i0 += (uint64_t)U * blockDim.x) {
+            uint64_t k[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (i0 + (uint64_t)u * blockDim.x >= end) break;
-            uint32_t idx, rank;
-            hll_index_rank<KS>(k[u], p, idx, rank);
-            raise_byte(&sk[idx >> 2], (idx & 3u) * 8u, rank);
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+                k[u] = i < end ? key_at<KS>(keys, i) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (i0 + (uint64_t)u * blockDim.x >= end) break;
+                add(k[u]);
+            }
         }
     }
     __syncthreads();
@@ -387,9 +414,10 @@ MXCO_API mxco_status mxco_hll_add(mxco_hll* h, const void* keys, uint64_t n, hip
         const uint32_t blocks = h->ws_blocks;
         const size_t lds = (size_t)1 << h->p;
         if (h->key_size == 4)
-            hll_private_add_kernel<4><<<blocks, kThreads, lds, stream>>>(k, n, h->p, h->workspace);
+            hll_private_add_kernel<4><<<blocks, kThreads, lds, stream>>>(k, n, h->p, h->workspace, 0);
         else
-            hll_private_add_kernel<8><<<blocks, kThreads, lds, stream>>>(k, n, h->p, h->workspace);
+            hll_private_add_kernel<8><<<blocks, kThreads, lds, stream>>>(k, n, h->p, h->workspace,
+                                                                          ((uintptr_t)k & 15u) == 0);
         MXCO_HIP(hipGetLastError(), "HyperLogLog: add");
         const uint32_t nwords = (1u << h->p) / 4;
         hll_reduce_kernel<<<(nwords + 63) / 64, 256, 0, stream>>>(h->words, h->workspace, blocks, h->p);
