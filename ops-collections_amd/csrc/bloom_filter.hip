@@ -176,6 +176,103 @@ __global__ __launch_bounds__(256) void bloom_scatter_kernel(uint32_t nb, const u
     });
 }
 
+// The same scatter with the segment's keys staged in LDS tiles of kTileKeys:
+// a tile is sorted by range in LDS (counting sort), then each range's run of
+// the tile is written to its bin as one contiguous piece, so the bins fill
+// in multi-entry runs instead of one 8-byte write per key.  Dynamic LDS:
+// cursors (4 B), tile counts (2 B, two per 32-bit word) and tile offsets
+// (2 B) per range, and 10 B per tile key.
+constexpr uint32_t kTileKeys = 8192;
+constexpr uint32_t kScatterThreads = 1024;
+constexpr uint32_t kTilePerThread = kTileKeys / kScatterThreads;
+
+size_t staged_scatter_lds(uint32_t nranges)
+{
+    return (size_t)nranges * 4 + ((size_t)nranges + 1) / 2 * 4 + (size_t)nranges * 2 + (size_t)kTileKeys * 10;
+}
+
+template <int KS>
+__global__ __launch_bounds__(kScatterThreads) void bloom_staged_scatter_kernel(
+    uint32_t nb, const uint8_t* keys, const uint8_t* stencil, uint64_t n, uint64_t seed, uint32_t nranges,
+    const uint32_t* offsets, uint64_t* entries)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint64_t* ent = (uint64_t*)smem;                             // kTileKeys
+    uint32_t* cur = (uint32_t*)(ent + kTileKeys);                // nranges
+    uint32_t* tcnt = cur + nranges;                              // (nranges + 1) / 2 words of two u16
+    uint16_t* toff = (uint16_t*)(tcnt + (nranges + 1) / 2);      // nranges
+    uint16_t* rid = toff + nranges;                              // kTileKeys
+    typedef hipcub::BlockScan<uint32_t, kScatterThreads> Scan;
+    __shared__ typename Scan::TempStorage scan_tmp;
+
+    for (uint32_t j = threadIdx.x; j < nranges; j += blockDim.x) cur[j] = offsets[(uint64_t)j * gridDim.x + blockIdx.x];
+    // the count kernel's segment of this workgroup (same grid, same split)
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t begin = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(n, begin + per);
+    const uint32_t span = (nranges + kScatterThreads - 1) / kScatterThreads;  // ranges per thread in the scan
+    for (uint64_t t0 = begin; t0 < end; t0 += kTileKeys) {
+        const uint32_t tlen = (uint32_t)min<uint64_t>(kTileKeys, end - t0);
+        for (uint32_t j = threadIdx.x; j < (nranges + 1) / 2; j += blockDim.x) tcnt[j] = 0;
+        __syncthreads();
+        uint32_t rr[kTilePerThread], rank[kTilePerThread];
+        uint64_t ev[kTilePerThread];
+#pragma unroll
+        for (uint32_t u = 0; u < kTilePerThread; ++u) {
+            const uint32_t j = threadIdx.x + u * kScatterThreads;
+            rr[u] = 0xffffffffu;
+            if (j < tlen && (!stencil || stencil[t0 + j] != 0)) {
+                const uint64_t h = xxhash64<KS>(key_at<KS>(keys, t0 + j), seed);
+                const uint32_t blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * nb) >> 32);
+                rr[u] = blk >> kRangeShift;
+                ev[u] = ((uint64_t)(blk & (kRangeBlocks - 1)) << 32) | (uint32_t)h;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kTilePerThread; ++u) {
+            if (rr[u] == 0xffffffffu) continue;
+            const uint32_t sh = (rr[u] & 1u) * 16u;
+            rank[u] = (atomicAdd(&tcnt[rr[u] >> 1], 1u << sh) >> sh) & 0xffffu;
+        }
+        __syncthreads();
+        // exclusive scan of the tile counts: `span` consecutive ranges per thread
+        uint32_t local = 0;
+        for (uint32_t q = 0; q < span; ++q) {
+            const uint32_t r = threadIdx.x * span + q;
+            if (r < nranges) local += (tcnt[r >> 1] >> ((r & 1u) * 16u)) & 0xffffu;
+        }
+        uint32_t excl;
+        Scan(scan_tmp).ExclusiveSum(local, excl);
+        for (uint32_t q = 0; q < span; ++q) {
+            const uint32_t r = threadIdx.x * span + q;
+            if (r < nranges) {
+                toff[r] = (uint16_t)excl;
+                excl += (tcnt[r >> 1] >> ((r & 1u) * 16u)) & 0xffffu;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t u = 0; u < kTilePerThread; ++u) {
+            if (rr[u] == 0xffffffffu) continue;
+            const uint32_t pos = toff[rr[u]] + rank[u];
+            ent[pos] = ev[u];
+            rid[pos] = (uint16_t)rr[u];
+        }
+        __syncthreads();
+        // the tile's selected keys occupy positions [0, selected)
+        const uint32_t selected = (uint32_t)toff[nranges - 1] +
+                                  ((tcnt[(nranges - 1) >> 1] >> (((nranges - 1) & 1u) * 16u)) & 0xffffu);
+        for (uint32_t pos = threadIdx.x; pos < selected; pos += blockDim.x) {
+            const uint32_t r = rid[pos];
+            entries[cur[r] + (pos - toff[r])] = ent[pos];
+        }
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < nranges; r += blockDim.x)
+            cur[r] += (tcnt[r >> 1] >> ((r & 1u) * 16u)) & 0xffffu;
+        __syncthreads();
+    }
+}
+
 // One range at a time: load its words into LDS, OR its bin, write it back.
 // offsets[r * segments] is the first entry of range r; offsets[nranges *
 // segments] is the total.
@@ -287,10 +384,33 @@ mxco_status add_partitioned(mxco_bloom* f, const uint8_t* keys, const uint8_t* s
     MXCO_HIP(hipGetLastError(), "BloomFilter::Add: count");
     MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(f->ws + off_temp, temp, counts, offsets, (int64_t)ncnt, stream),
              "BloomFilter::Add: scan");
-    if (f->key_size == 4)
-        bloom_scatter_kernel<4><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, offsets, entries);
-    else
-        bloom_scatter_kernel<8><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, offsets, entries);
+    // LDS-staged scatter; the per-key scatter remains for MXCO_BLOOM_STAGED=0
+    static const int staged = env_int("MXCO_BLOOM_STAGED", 1);
+    static const hipError_t attr = [] {
+        const int bytes = (int)staged_scatter_lds(kMaxRanges);
+        hipError_t e = hipFuncSetAttribute((const void*)bloom_staged_scatter_kernel<4>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)bloom_staged_scatter_kernel<8>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        return e;
+    }();
+    if (staged && attr == hipSuccess) {
+        const size_t slds = staged_scatter_lds(nranges);
+        if (f->key_size == 4)
+            bloom_staged_scatter_kernel<4><<<segs, kScatterThreads, slds, stream>>>(nb, keys, stencil, n, f->seed,
+                                                                                    nranges, offsets, entries);
+        else
+            bloom_staged_scatter_kernel<8><<<segs, kScatterThreads, slds, stream>>>(nb, keys, stencil, n, f->seed,
+                                                                                    nranges, offsets, entries);
+    } else {
+        if (f->key_size == 4)
+            bloom_scatter_kernel<4><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, offsets,
+                                                                entries);
+        else
+            bloom_scatter_kernel<8><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, offsets,
+                                                                entries);
+    }
     MXCO_HIP(hipGetLastError(), "BloomFilter::Add: scatter");
     const unsigned grid = std::min<unsigned>(nranges, (unsigned)cu_count() * 2);
     bloom_apply_kernel<<<grid, 1024, 0, stream>>>(f->words, nb, nranges, segs, offsets, entries);
