@@ -280,7 +280,7 @@ __global__ __launch_bounds__(1024) void bloom_apply_kernel(uint32_t* words, uint
                                                            uint32_t segments, const uint32_t* offsets,
                                                            const uint64_t* entries)
 {
-    __shared__ uint32_t lw[kRangeBlocks * 8];
+    __shared__ __align__(16) uint32_t lw[kRangeBlocks * 8];
     for (uint32_t r = blockIdx.x; r < nranges; r += gridDim.x) {
         const uint32_t s = offsets[(uint64_t)r * segments];
         const uint32_t e = offsets[(uint64_t)(r + 1) * segments];
@@ -291,12 +291,28 @@ __global__ __launch_bounds__(1024) void bloom_apply_kernel(uint32_t* words, uint
         u64x2* l = (u64x2*)lw;
         for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) l[j] = g[j];
         __syncthreads();
-        for (uint32_t k = s + threadIdx.x; k < e; k += blockDim.x) {
-            const uint64_t v = entries[k];
-            const uint32_t lo = (uint32_t)v;
-            uint32_t* w = lw + (uint32_t)(v >> 32) * 8;
+        // 8 entry loads in flight per lane
+        for (uint32_t k0 = s + threadIdx.x; k0 < e; k0 += 8 * blockDim.x) {
+            uint64_t v[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) atomicOr(w + q, 1u << ((kSalts[q] * lo) >> 27));
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t k = k0 + u * blockDim.x;
+                v[u] = k < e ? entries[k] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (k0 + u * blockDim.x >= e) break;
+                const uint32_t lo = (uint32_t)v[u];
+                // the block's 8 words as 4 64-bit words: one LDS atomic per
+                // pair of words, none when the bits are already set
+                unsigned long long* w = (unsigned long long*)(lw + (uint32_t)(v[u] >> 32) * 8);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const unsigned long long m = (unsigned long long)(1u << ((kSalts[2 * q] * lo) >> 27)) |
+                                                 (unsigned long long)(1u << ((kSalts[2 * q + 1] * lo) >> 27)) << 32;
+                    if ((w[q] & m) != m) atomicOr(w + q, m);
+                }
+            }
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) g[j] = l[j];
