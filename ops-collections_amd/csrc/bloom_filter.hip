@@ -211,8 +211,19 @@ __global__ __launch_bounds__(kScatterThreads) void bloom_staged_scatter_kernel(
     const uint64_t begin = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(n, begin + per);
     const uint32_t span = (nranges + kScatterThreads - 1) / kScatterThreads;  // ranges per thread in the scan
+    // the next tile's keys are loaded while the current tile is sorted
+    uint64_t kc[kTilePerThread], kn[kTilePerThread];
+    auto load_tile = [&](uint64_t t, uint64_t (&k)[kTilePerThread]) {
+#pragma unroll
+        for (uint32_t u = 0; u < kTilePerThread; ++u) {
+            const uint64_t i = t + threadIdx.x + u * kScatterThreads;
+            k[u] = i < end ? key_at<KS>(keys, i) : 0;
+        }
+    };
+    load_tile(begin, kc);
     for (uint64_t t0 = begin; t0 < end; t0 += kTileKeys) {
         const uint32_t tlen = (uint32_t)min<uint64_t>(kTileKeys, end - t0);
+        if (t0 + kTileKeys < end) load_tile(t0 + kTileKeys, kn);
         for (uint32_t j = threadIdx.x; j < (nranges + 1) / 2; j += blockDim.x) tcnt[j] = 0;
         __syncthreads();
         uint32_t rr[kTilePerThread], rank[kTilePerThread];
@@ -222,7 +233,7 @@ __global__ __launch_bounds__(kScatterThreads) void bloom_staged_scatter_kernel(
             const uint32_t j = threadIdx.x + u * kScatterThreads;
             rr[u] = 0xffffffffu;
             if (j < tlen && (!stencil || stencil[t0 + j] != 0)) {
-                const uint64_t h = xxhash64<KS>(key_at<KS>(keys, t0 + j), seed);
+                const uint64_t h = xxhash64<KS>(kc[u], seed);
                 const uint32_t blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * nb) >> 32);
                 rr[u] = blk >> kRangeShift;
                 ev[u] = ((uint64_t)(blk & (kRangeBlocks - 1)) << 32) | (uint32_t)h;
@@ -269,6 +280,8 @@ __global__ __launch_bounds__(kScatterThreads) void bloom_staged_scatter_kernel(
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < nranges; r += blockDim.x)
             cur[r] += (tcnt[r >> 1] >> ((r & 1u) * 16u)) & 0xffffu;
+#pragma unroll
+        for (uint32_t u = 0; u < kTilePerThread; ++u) kc[u] = kn[u];
         __syncthreads();
     }
 }
