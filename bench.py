@@ -401,6 +401,7 @@ def run_containers(torch, mxco, steps=3):
                        "roofline_insert": lb(N, 128 + 128 + 16, t_ins)}
     del mm, po, vo
 
+    log("bench: containers: multiset")
     ms = mxco.StaticMultiset(1 << 26, -1, torch.int64)
     ev0.record()
     failed = ms.insert(keys)
@@ -420,6 +421,7 @@ def run_containers(torch, mxco, steps=3):
     del ms, ce, con, keys, vals, pairs, r, mult
 
     # DynamicMap growth
+    log("bench: containers: dynamic map")
     NB, B = 16, 1 << 22
     dk = torch.unique(torch.randint(0, 1 << 62, (NB * B + (1 << 16),), device="cuda", generator=g))[: NB * B]
     dk = dk[torch.randperm(dk.numel(), device="cuda", generator=g)]
@@ -445,12 +447,35 @@ def run_containers(torch, mxco, steps=3):
     del dm, dk, dpairs, dq, dout
 
     # StaticMap Erase + Count
+    log("bench: containers: erase, count, insert_and_find, contains")
     m = mxco.StaticMap(1 << 26, -1, -1, torch.int64, torch.int64)
     m.set_erased_key(-2)
     ek, ev, eq = make_workload(torch, m.capacity(), 0.8, seed=99)
     m.insert(mxco.make_pairs(ek, ev))
     t_c = _timed(torch, lambda: m.count(eq), steps)
     present = int(m.count(eq))
+    cq = torch.empty(eq.numel(), dtype=torch.bool, device="cuda")
+    t_con = _timed(torch, lambda: m.contains(eq, out=cq, sync=False), steps)
+    ok_con = int(cq.sum().item()) == eq.numel() // 2
+    # Contains, then InsertAndFind with half the pairs already present, on
+    # the table at load factor 0.8 (before the Erase).  Only 2^22 new keys are
+    # added (0.86 after): a batch larger than the EMPTY slots left would walk
+    # full cycles.
+    iaf_keys = torch.cat([ek[ek.numel() // 2:ek.numel() // 2 + (1 << 22)],
+                          torch.randint(1 << 62, (1 << 63) - 1, (1 << 22,), device="cuda", generator=g)])
+    iaf_pairs = mxco.make_pairs(iaf_keys, iaf_keys * 7 + 1)
+    of = torch.empty(iaf_keys.numel(), dtype=torch.int64, device="cuda")
+    oi = torch.empty(iaf_keys.numel(), dtype=torch.bool, device="cuda")
+    ev0.record()
+    m.insert_and_find(iaf_pairs, out_find=of, out_inserted=oi)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_iaf = ev0.elapsed_time(ev1)
+    ok_iaf = bool(torch.equal(of, iaf_keys * 7 + 1)) and int(oi.sum().item()) == (1 << 22)
+    res["insert_and_find"] = {"keys": iaf_keys.numel(), "already_present": 1 << 22, "ms": round(t_iaf, 3),
+                              "gkeys_s": rate(iaf_keys.numel(), t_iaf), "verified": ok_iaf}
+    res["contains"] = {"queries": eq.numel(), "ms": round(t_con, 3), "gkeys_s": rate(eq.numel(), t_con),
+                       "verified": ok_con}
     half = ek[: ek.numel() // 2].contiguous()
     ev0.record()
     failed = m.erase(half)
