@@ -514,9 +514,11 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                     // (InsertVerdict, open_addressing_ref_impl.h:490-532).
                     int r = 0;          // 0 lost to another key, 1 inserted, 2 duplicate
                     uint64_t seen = 0;  // stored key (set) / payload (8-byte map slot) on a duplicate
+                    bool seen_window = false;  // `seen` came from the window load (16-byte slots)
                     if (weq) {
                         r = 2;
                         seen = VS ? L::val(ws) : L::key(ws);
+                        seen_window = true;
                         if constexpr (OP == OP_INSERT_OR_ASSIGN) {
                             // AssignPayload (ref_impl.h:210-220)
                             if (winner) store_bits<VS>(wslot + VOFF, pval);
@@ -577,7 +579,10 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                                 } else if constexpr (VS && SS == 16) {
                                     // WaitForPayload (ref_impl.h:474-486): a racing inserter may
                                     // have stored the key but not yet the payload.  Bounded here.
-                                    v = atomic_load_bits<VS>(wslot + 8);
+                                    // The window's 16-byte load already holds the final payload
+                                    // unless it still read the empty value.
+                                    v = seen_window ? seen : a.empty_val;
+                                    if (v == a.empty_val) v = atomic_load_bits<VS>(wslot + 8);
                                     for (int spin = 0; spin < (1 << 22) && v == a.empty_val; ++spin)
                                         v = atomic_load_bits<VS>(wslot + 8);
                                 } else {
