@@ -496,35 +496,55 @@ def run_containers(torch, mxco, steps=3):
 def run_batched(args, torch, mxco, steps=3, warmup=1):
     """1024 StaticMap int32 tables, capacities log-uniform 2^10..2^22, LF
     U[0.5,0.85], finds log-uniform 2^6..2^20 at 50 % hit: one graph replay
-    (Clear + one persistent Insert launch + one persistent Find launch)
-    against 1024 plain per-table Clear/Insert/Find calls."""
+    (a BUILD plan -- Clear + Insert built region by region -- and one
+    persistent Find launch) against 1024 plain per-table Clear/Insert/Find
+    calls.  The graph of Clear + the persistent CAS Insert + Find is timed
+    beside it (insert_plan_ms)."""
     from mxco import workloads
     w = workloads.batched_tables(1024, (10, 22), (0.5, 0.85), (6, 20), seed=4,
                                  valid_extent=lambda c: mxco.valid_extent(c, 5, "linear"))
     T = len(w["caps"])
     pairs, q = w["pairs"], w["queries"]
     b = mxco.BatchedStaticMap(w["caps"], -1, -1)
-    ins = b.plan(mxco.BATCH_INSERT, [(t, w["ins_off"][t], w["n_ins"][t]) for t in range(T)])
+    items = [(t, w["ins_off"][t], w["n_ins"][t]) for t in range(T)]
+    ins = b.plan(mxco.BATCH_INSERT, items)
+    bld = b.plan(mxco.BATCH_BUILD, items)
     fnd = b.plan(mxco.BATCH_FIND, [(t, w["find_off"][t], w["n_find"][t]) for t in range(T)])
     out_b = torch.empty_like(q)
     # the insert tuner leaves the tables filled, so the find plan is tuned on them
-    tuned = {"insert": ins.tune(pairs), "find": fnd.tune(q, out_b)}
+    tuned = {"insert": ins.tune(pairs), "find": fnd.tune(q, out_b), "build": bld.tune(pairs)}
 
-    def record(s):
+    def timed_graph(record):
+        out_b.fill_(0)
+        g = mxco.Graph(record)
+        for _ in range(warmup):
+            g.launch()
+        g.stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.launch()
+        g.stream.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        ok = bool(torch.equal(out_b, w["expected"]))
+        return t, ok, g
+
+    # BUILD (Clear + Insert in one region-built pass) + Find
+    def record_build(s):
+        bld.run(pairs, stream=s)
+        fnd.run(q, out_b, stream=s)
+
+    t_batched, ok_b, g = timed_graph(record_build)
+    ok_b = ok_b and bld.failures(g.stream) == [0] * T
+    del g
+
+    # Clear + Insert (CAS walk) + Find
+    def record_insert(s):
         b.clear(s)
         ins.run(pairs, stream=s)
         fnd.run(q, out_b, stream=s)
 
-    g = mxco.Graph(record)
-    for _ in range(warmup):
-        g.launch()
-    g.stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.launch()
-    g.stream.synchronize()
-    t_batched = (time.perf_counter() - t0) / steps
-    ok_b = bool(torch.equal(out_b, w["expected"])) and ins.failures(g.stream) == [0] * T
+    t_cas, ok_c, g = timed_graph(record_insert)
+    ok_b = ok_b and ok_c and ins.failures(g.stream) == [0] * T
     del g
     # plain: one StaticMap per table, the reference's synchronous calls
     maps = [mxco.StaticMap(w["caps"][t], -1, -1) for t in range(T)]
@@ -552,13 +572,15 @@ def run_batched(args, torch, mxco, steps=3, warmup=1):
     t_plain = (time.perf_counter() - t0) / steps
     ok_p = failed == 0 and bool(torch.equal(out_p, w["expected"]))
     keys = sum(w["n_ins"]) + sum(w["n_find"])
-    del maps, b
+    cfg = {"insert": ins.config(), "find": fnd.config()}
+    del maps, b, ins, bld, fnd
     torch.cuda.empty_cache()
     return {"tables": T, "slots": sum(w["caps"]), "inserts": sum(w["n_ins"]), "finds": sum(w["n_find"]),
             "batched_ms": round(t_batched * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
             "batched_gkeys_s": round(keys / t_batched / 1e9, 4), "plain_gkeys_s": round(keys / t_plain / 1e9, 4),
             "speedup": round(t_plain / t_batched, 3), "identical_results": ok_b and ok_p,
-            "tuned_config": {"insert": ins.config(), "find": fnd.config()},
+            "insert_plan_ms": round(t_cas * 1e3, 3), "insert_plan_speedup": round(t_plain / t_cas, 3),
+            "tuned_config": cfg,
             "tune_ms": {k: round(v, 3) for k, v in tuned.items()}}
 
 

@@ -204,7 +204,12 @@ typedef struct mxco_batch_item {
     uint64_t count;
 } mxco_batch_item;
 
-enum { MXCO_BATCH_INSERT = 0, MXCO_BATCH_FIND = 1, MXCO_BATCH_CONTAINS = 2 };
+/* MXCO_BATCH_BUILD: Clear of every table of the batch followed by Insert of
+ * the plan's items, as one operation (each table in at most one item).  With
+ * LinearProbing the tables are built region by region in LDS; its results
+ * (Find / Contains outputs, per-item failures) equal mxco_batch_clear + an
+ * insert plan's.  No other operation may touch the batch while it runs. */
+enum { MXCO_BATCH_INSERT = 0, MXCO_BATCH_FIND = 1, MXCO_BATCH_CONTAINS = 2, MXCO_BATCH_BUILD = 3 };
 
 /* num_tables tables of one descriptor, table t with requested capacity
  * capacities[t] (rounded as mxco_valid_extent), all in one slot arena. */

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "mxco_common.h"
+#include "batch_build.h"
 #include "batch_kernels.h"
 #include "oa_kernels.h"
 
@@ -141,6 +142,19 @@ struct mxco_batch_plan {
     unsigned long long* host_counters;
     int group;
     unsigned blocks_per_cu;
+    // MXCO_BATCH_BUILD
+    bool build;   // Clear of every table, then this insert plan
+    bool region;  // ... by the region build (otherwise Clear + the insert kernel)
+    uint32_t nreg, ntiles, maxbins;
+    uint64_t total;
+    BatchChunk* tiles_dev;
+    uint32_t* reg0_dev;
+    uint32_t* reg_table_dev;
+    int32_t* table_item_dev;
+    uint32_t* counts;   // nreg + 1
+    uint32_t* offsets;  // nreg + 1
+    uint32_t* ovc;      // nreg
+    uint8_t* part;      // total pairs
 };
 
 namespace {
@@ -164,6 +178,138 @@ hipError_t dispatch(const mxco_batch* b, int g, int op, const OaArgs& a, const m
 {
     return launch_batch_any(b->d.key_size, b->d.value_size, g, op, false, a, b->tables_dev, p->items_dev,
                             p->chunks_dev, p->nchunks, op == OP_INSERT ? p->counters : nullptr, grid, s);
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+template <int KS, int VS>
+hipError_t launch_build(const BuildArgs& a, uint32_t ntiles, hipStream_t s)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    static const hipError_t attr = hipFuncSetAttribute((const void*)build_scatter_kernel<KS, VS>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)build_scatter_lds<SS>(kBuildMaxBins));
+    if (attr != hipSuccess) return attr;
+    if (ntiles) build_count_kernel<KS, VS><<<ntiles, kBuildThreads, 0, s>>>(a);
+    build_scan_kernel<<<1, 1024, 0, s>>>(a.counts, a.offsets, a.nreg);
+    if (ntiles) build_scatter_kernel<KS, VS><<<ntiles, kBuildThreads, build_scatter_lds<SS>(a.maxbins), s>>>(a);
+    build_apply_kernel<KS, VS><<<a.nreg, kBuildThreads, 0, s>>>(a);
+    const unsigned waves_grid = std::max(1u, std::min<unsigned>((a.nreg + 3) / 4, (unsigned)cu_count() * 8));
+    build_overflow_kernel<KS, VS><<<waves_grid, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_any(int ks, int vs, const BuildArgs& a, uint32_t ntiles, hipStream_t s)
+{
+    if (ks == 4 && vs == 4) return launch_build<4, 4>(a, ntiles, s);
+    if (ks == 8 && vs == 8) return launch_build<8, 8>(a, ntiles, s);
+    if (ks == 4 && vs == 8) return launch_build<4, 8>(a, ntiles, s);
+    if (ks == 8 && vs == 4) return launch_build<8, 4>(a, ntiles, s);
+    if (ks == 4 && vs == 0) return launch_build<4, 0>(a, ntiles, s);
+    if (ks == 8 && vs == 0) return launch_build<8, 0>(a, ntiles, s);
+    return hipErrorInvalidValue;
+}
+
+mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, int g,
+                     unsigned bpc);
+
+// MXCO_BATCH_BUILD: Clear + Insert (capture-safe: kernels only).
+mxco_status run_build(mxco_batch_plan* p, const void* in, hipStream_t s)
+{
+    mxco_batch* b = p->batch;
+    if (!p->region) {
+        if (mxco_status st = mxco_batch_clear(b, s)) return st;
+        return run_plan(p, in, nullptr, nullptr, s, p->group, p->blocks_per_cu);
+    }
+    zero_u64_kernel<<<grid_for(2ull * p->nitems, 1), 256, 0, s>>>(p->counters, 2ull * p->nitems);
+    zero_u32_kernel<<<grid_for(p->nreg + 1ull, 1), 256, 0, s>>>(p->counts, p->nreg + 1ull);
+    MXCO_HIP(hipGetLastError(), "batch build: zero");
+    BuildArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = (const uint8_t*)in;
+    a.part = p->part;
+    a.tables = b->tables_dev;
+    a.items = p->items_dev;
+    a.tiles = p->tiles_dev;
+    a.reg0 = p->reg0_dev;
+    a.reg_table = p->reg_table_dev;
+    a.table_item = p->table_item_dev;
+    a.counts = p->counts;
+    a.offsets = p->offsets;
+    a.ovc = p->ovc;
+    a.counters = p->counters;
+    a.empty_key = b->empty_key;
+    a.empty_val = b->empty_val;
+    a.nreg = p->nreg;
+    a.maxbins = p->maxbins;
+    a.bucket = b->d.bucket_size;
+    a.hash = b->d.hash;
+    a.key_float = b->d.key_kind == MXCO_KEY_FLOAT;
+    MXCO_HIP(launch_build_any(b->d.key_size, b->d.value_size, a, p->ntiles, s), "batch build: launch");
+    return MXCO_OK;
+}
+
+// Region build data of a build plan; leaves p->region false when the plan
+// does not qualify (double hashing, a table in two items, a table of more
+// than kBuildMaxBins regions, 2^32 pairs or more).
+hipError_t prepare_build(mxco_batch_plan* p, const mxco_batch_item* items)
+{
+    mxco_batch* b = p->batch;
+    static const int enabled = env_int("MXCO_BATCH_REGION_BUILD", 1);
+    if (!enabled || b->d.probing != 0) return hipSuccess;
+    const uint32_t R = kBuildBytes / b->ss;
+    const uint32_t tile = kBuildTileBytes / b->ss;
+    std::vector<int32_t> table_item(b->ntables, -1);
+    std::vector<uint32_t> reg0(b->ntables + 1);
+    uint64_t nreg = 0, total = 0;
+    uint32_t maxbins = 1;
+    for (uint32_t t = 0; t < b->ntables; ++t) {
+        reg0[t] = (uint32_t)nreg;
+        nreg += (b->tables[t].cap + R - 1) / R;
+        if (nreg >= 0x7fffffffull) return hipSuccess;
+    }
+    reg0[b->ntables] = (uint32_t)nreg;
+    std::vector<BatchChunk> tiles;
+    for (uint32_t i = 0; i < p->nitems; ++i) {
+        const uint32_t t = items[i].table;
+        if (table_item[t] >= 0) return hipSuccess;
+        table_item[t] = (int32_t)i;
+        const uint32_t bins = reg0[t + 1] - reg0[t];
+        if (bins > kBuildMaxBins) return hipSuccess;
+        maxbins = std::max(maxbins, bins);
+        total += items[i].count;
+        for (uint64_t s = 0; s < items[i].count; s += tile)
+            tiles.push_back({i, (uint32_t)std::min<uint64_t>(tile, items[i].count - s), s});
+    }
+    if (total >= 0xffffffffull) return hipSuccess;
+    std::vector<uint32_t> reg_table(nreg);
+    for (uint32_t t = 0; t < b->ntables; ++t)
+        for (uint32_t r = reg0[t]; r < reg0[t + 1]; ++r) reg_table[r] = t;
+    p->nreg = (uint32_t)nreg;
+    p->ntiles = (uint32_t)tiles.size();
+    p->maxbins = maxbins;
+    p->total = total;
+    hipError_t e = hipMalloc((void**)&p->tiles_dev, std::max<size_t>(1, tiles.size()) * sizeof(BatchChunk));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->reg0_dev, reg0.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->reg_table_dev, nreg * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->table_item_dev, b->ntables * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->counts, (nreg + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->offsets, (nreg + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->ovc, nreg * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->part, std::max<uint64_t>(16, total * b->ss));
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpy(p->tiles_dev, tiles.data(), tiles.size() * sizeof(BatchChunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->reg0_dev, reg0.data(), reg0.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->reg_table_dev, reg_table.data(), nreg * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->table_item_dev, table_item.data(), b->ntables * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) p->region = true;
+    return e;
 }
 
 mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, int g,
@@ -279,7 +425,7 @@ MXCO_API mxco_status mxco_batch_plan_create(mxco_batch* b, int op, const mxco_ba
                                             uint32_t group_size, mxco_batch_plan** out)
 {
     if (!b || !items || !out || num_items == 0) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: bad args");
-    if (op != MXCO_BATCH_INSERT && op != MXCO_BATCH_FIND && op != MXCO_BATCH_CONTAINS)
+    if (op != MXCO_BATCH_INSERT && op != MXCO_BATCH_FIND && op != MXCO_BATCH_CONTAINS && op != MXCO_BATCH_BUILD)
         return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: unknown operation");
     if (group_size != 0 && group_size != 1 && group_size != 2 && group_size != 4 && group_size != 8)
         return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_create: group size must be 0, 1, 2, 4 or 8");
@@ -296,7 +442,9 @@ MXCO_API mxco_status mxco_batch_plan_create(mxco_batch* b, int op, const mxco_ba
     mxco_batch_plan* p = new (std::nothrow) mxco_batch_plan();
     if (!p) return fail(MXCO_OUT_OF_MEMORY, "mxco_batch_plan_create: host allocation");
     p->batch = b;
-    p->op = op == MXCO_BATCH_INSERT ? OP_INSERT : (op == MXCO_BATCH_FIND ? OP_FIND : OP_CONTAINS);
+    p->op = (op == MXCO_BATCH_INSERT || op == MXCO_BATCH_BUILD) ? OP_INSERT
+                                                                 : (op == MXCO_BATCH_FIND ? OP_FIND : OP_CONTAINS);
+    p->build = op == MXCO_BATCH_BUILD;
     p->nitems = num_items;
     p->nchunks = (uint32_t)chunks.size();
     p->group = group_size ? (int)group_size : (b->ss == 16 ? 4 : 4);
@@ -309,6 +457,7 @@ MXCO_API mxco_status mxco_batch_plan_create(mxco_batch* b, int op, const mxco_ba
         e = hipMemcpy(p->items_dev, items, num_items * sizeof(mxco_batch_item), hipMemcpyHostToDevice);
     if (e == hipSuccess && !chunks.empty())
         e = hipMemcpy(p->chunks_dev, chunks.data(), chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess && p->build) e = prepare_build(p, items);
     if (e != hipSuccess) {
         (void)mxco_batch_plan_destroy(p);
         return hip_fail(e, "mxco_batch_plan_create: allocation");
@@ -324,6 +473,9 @@ MXCO_API mxco_status mxco_batch_plan_destroy(mxco_batch_plan* p)
     if (p->chunks_dev) (void)hipFree(p->chunks_dev);
     if (p->counters) (void)hipFree(p->counters);
     if (p->host_counters) (void)hipHostFree(p->host_counters);
+    for (void* d : {(void*)p->tiles_dev, (void*)p->reg0_dev, (void*)p->reg_table_dev, (void*)p->table_item_dev,
+                    (void*)p->counts, (void*)p->offsets, (void*)p->ovc, (void*)p->part})
+        if (d) (void)hipFree(d);
     delete p;
     return ok();
 }
@@ -332,6 +484,10 @@ MXCO_API mxco_status mxco_batch_run(mxco_batch_plan* p, const void* in, void* ou
 {
     if (!p || !in) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_run: null argument");
     if (p->op != OP_INSERT && !out) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_run: null output");
+    if (p->build) {
+        if (mxco_status s = run_build(p, in, stream)) return s;
+        return ok();
+    }
     if (mxco_status s = run_plan(p, in, out, out2, stream, p->group, p->blocks_per_cu)) return s;
     return ok();
 }
@@ -364,6 +520,28 @@ MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, vo
                                           hipStream_t stream, float* best_ms)
 {
     if (!p || !in) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_tune: null argument");
+    if (p->build && p->region) {
+        // the region build has no launch shape to choose: time it (3 runs, the first discarded)
+        hipEvent_t e0, e1;
+        MXCO_HIP(hipEventCreate(&e0), "tune");
+        MXCO_HIP(hipEventCreate(&e1), "tune");
+        mxco_status st = MXCO_OK;
+        float total = 0;
+        for (int rep = 0; rep < 3 && st == MXCO_OK; ++rep) {
+            if (hipEventRecord(e0, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune: event");
+            if (st == MXCO_OK) st = run_build(p, in, stream);
+            if (st == MXCO_OK && hipEventRecord(e1, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
+            if (st == MXCO_OK && hipEventSynchronize(e1) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
+            float ms = 0;
+            if (st == MXCO_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) total += ms;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (st != MXCO_OK) return st;
+        if (best_ms) *best_ms = total / 2;
+        return ok();
+    }
     static const int groups[] = {1, 2, 4, 8};
     static const unsigned bpcs[] = {8, 32, 64, 128, 256, 512};
     hipEvent_t e0, e1;

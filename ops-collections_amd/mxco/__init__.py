@@ -901,7 +901,7 @@ class BatchItem(ctypes.Structure):
     _fields_ = [("table", u32), ("reserved", u32), ("offset", u64), ("count", u64)]
 
 
-BATCH_INSERT, BATCH_FIND, BATCH_CONTAINS = 0, 1, 2
+BATCH_INSERT, BATCH_FIND, BATCH_CONTAINS, BATCH_BUILD = 0, 1, 2, 3  # BUILD: Clear of every table + Insert
 _sig("mxco_batch_create", [ctypes.POINTER(OaDesc), u32, ctypes.POINTER(u64), vp, vp, vp, ctypes.POINTER(vp)])
 _sig("mxco_batch_destroy", [vp])
 _sig("mxco_batch_table", [vp, u32, pu64, ctypes.POINTER(vp)])
@@ -964,6 +964,7 @@ class BatchedStaticMap:
         d = OaDesc(ks, vs, 1 if key_dtype.is_floating_point else 0, bucket_size, PROBING[probing], HASH[hash], 0, 0)
         self._ek = _scalar_bytes(empty_key, key_dtype)
         self._ev = _scalar_bytes(empty_value, value_dtype) if value_dtype is not None else None
+        self.slot_size = ks if not vs else (8 if ks == 4 and vs == 4 else 16)
         caps = (u64 * len(capacities))(*capacities)
         h = vp()
         check(_lib.mxco_batch_create(ctypes.byref(d), len(capacities), caps, self._ek, self._ev, _stream(stream),
@@ -978,6 +979,12 @@ class BatchedStaticMap:
         c = u64()
         check(_lib.mxco_batch_table(self._h, t, ctypes.byref(c), None))
         return c.value
+
+    def slots(self, t):
+        """Device copy of table t's raw slot array as uint8 (capacity, slot_size)."""
+        c, p = u64(), vp()
+        check(_lib.mxco_batch_table(self._h, t, ctypes.byref(c), ctypes.byref(p)))
+        return _dev_view(p.value, c.value * self.slot_size).view(c.value, self.slot_size)
 
     def clear(self, stream=None):
         check(_lib.mxco_batch_clear(self._h, _stream(stream)))
