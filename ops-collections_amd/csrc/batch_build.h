@@ -125,10 +125,16 @@ __global__ __launch_bounds__(1024) void build_scan_kernel(const uint32_t* counts
 {
     typedef hipcub::BlockScan<uint32_t, 1024> Scan;
     __shared__ typename Scan::TempStorage tmp;
-    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t per = (n + 4095) / 4096 * 4;  // a multiple of 4: 16-byte loads
     const uint32_t b = min(n, threadIdx.x * per), e = min(n, b + per);
     uint32_t local = 0;
-    for (uint32_t i = b; i < e; ++i) local += counts[i];
+    uint32_t i = b;
+#pragma unroll 4
+    for (; i + 4 <= e; i += 4) {
+        const uint4 v = *(const uint4*)(counts + i);
+        local += v.x + v.y + v.z + v.w;
+    }
+    for (; i < e; ++i) local += counts[i];
     uint32_t excl, total;
     Scan(tmp).ExclusiveSum(local, excl, total);
     for (uint32_t i = b; i < e; ++i) {
