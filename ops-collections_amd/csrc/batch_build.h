@@ -23,6 +23,14 @@
 //               CAS walk produces for some order of the same inserts.
 // The walk order and the EMPTY / EQUAL rules are those of the CAS insert, so
 // Find, Contains and the per-item failure counts equal Clear + Insert's.
+//
+// The same kernels serve Insert into one large table (oa_api.hip).  Its
+// regions outnumber what one LDS histogram holds, so the partition has two
+// levels: count / scan / scatter group the pairs by runs of `group` regions,
+// and build_regroup_kernel regroups each run by region.  The apply then loads
+// each region from the table (the table may hold keys and tombstones
+// already), and a region with more than `heavy_limit` pairs (a skewed stream
+// of duplicates) is left to the CAS walk of build_heavy_kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -46,24 +54,57 @@ struct BuildShape {
 
 struct BuildArgs {
     const uint8_t* in;  // the plan's input pairs (slot layout)
-    uint8_t* part;      // the pairs grouped by region (total * SS bytes)
+    uint8_t* part;      // the pairs grouped by bin (total * SS bytes)
     const BatchTable* tables;
     const mxco_batch_item* items;
-    const BatchChunk* tiles;    // tiles of one item each, at most kTile pairs
-    const uint32_t* reg0;       // first region of each table (num_tables + 1)
-    const uint32_t* reg_table;  // table of each region
-    const int32_t* table_item;  // item of each table, or -1
-    uint32_t* counts;           // per region; zero again after the scatter
-    uint32_t* offsets;          // exclusive sum of the counts (nreg + 1)
+    const BatchChunk* tiles;    // tiles of one item each, at most kTile pairs; null: one table,
+                                // pairs [0, single_n) of `in` cut into whole tiles
+    const uint32_t* reg0;       // first bin of each table (num_tables + 1)
+    const uint32_t* reg_table;  // table of each region (null: table 0)
+    const int32_t* table_item;  // item of each table, or -1 (null: item 0)
+    uint32_t* counts;           // per bin; zero again after the scatter
+    uint32_t* offsets;          // exclusive sum of the counts (nbins + 1)
     uint32_t* ovc;              // overflow pairs of each region, kept at its offset
-    unsigned long long* counters;  // 2 per item: [0] failed inserts
+    uint32_t* heavy;            // [0] count, then the regions left to the CAS walk
+    unsigned long long* counters;  // 2 per item: [0] failed inserts (null: not counted)
     uint64_t empty_key, empty_val;
+    uint64_t single_n;
     uint32_t nreg;
-    uint32_t maxbins;  // most regions of a table with an item (stride of the scatter's LDS arrays)
+    uint32_t maxbins;     // most bins of a table with an item (stride of the scatter's LDS arrays)
+    uint32_t bin_slots;   // slots per bin of count / scan / scatter (a multiple of the region)
+    uint32_t load_region; // apply: start from the table's slots instead of empty ones
+    uint32_t heavy_limit; // apply: regions with more pairs go to build_heavy_kernel
     uint32_t bucket;
     uint32_t hash;
     uint32_t key_float;
 };
+
+// The tile of workgroup b: its table, its first pair and its length.
+struct TileRef {
+    uint32_t table;
+    const uint8_t* src;
+    uint32_t len;
+};
+
+template <int SS>
+__device__ __forceinline__ TileRef tile_ref(const BuildArgs& a, uint32_t b)
+{
+    constexpr uint32_t kTile = BuildShape<SS>::kTile;
+    if (a.tiles) {
+        const BatchChunk tl = a.tiles[b];
+        const mxco_batch_item it = a.items[tl.item];
+        return TileRef{it.table, a.in + (it.offset + tl.start) * SS, tl.len};
+    }
+    const uint64_t st = (uint64_t)b * kTile;
+    return TileRef{0u, a.in + st * SS, (uint32_t)min<uint64_t>(kTile, a.single_n - st)};
+}
+
+__device__ __forceinline__ void count_failures(const BuildArgs& a, uint32_t table, uint32_t n)
+{
+    if (!n || !a.counters) return;
+    const int32_t item = a.table_item ? a.table_item[table] : 0;
+    if (item >= 0) atomicAdd(&a.counters[2ull * item], (unsigned long long)n);
+}
 
 template <int KS>
 __device__ __forceinline__ uint32_t first_slot(const BuildArgs& a, uint32_t nbuckets, uint64_t key)
@@ -96,13 +137,12 @@ __global__ __launch_bounds__(kBuildThreads) void build_count_kernel(BuildArgs a)
     using Raw = typename L::Raw;
     using S = BuildShape<L::SS>;
     __shared__ uint32_t hist[kBuildMaxBins];
-    const BatchChunk tl = a.tiles[blockIdx.x];
-    const mxco_batch_item it = a.items[tl.item];
-    const BatchTable t = a.tables[it.table];
-    const uint32_t nbins = (t.cap + S::kRegion - 1) / S::kRegion;
+    const TileRef tl = tile_ref<L::SS>(a, blockIdx.x);
+    const BatchTable t = a.tables[tl.table];
+    const uint32_t nbins = (t.cap + a.bin_slots - 1) / a.bin_slots;
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x) hist[j] = 0;
     __syncthreads();
-    const Raw* src = (const Raw*)(a.in + (it.offset + tl.start) * L::SS);
+    const Raw* src = (const Raw*)tl.src;
     Raw p[S::kPer];
 #pragma unroll
     for (uint32_t u = 0; u < S::kPer; ++u) {
@@ -112,16 +152,16 @@ __global__ __launch_bounds__(kBuildThreads) void build_count_kernel(BuildArgs a)
 #pragma unroll
     for (uint32_t u = 0; u < S::kPer; ++u) {
         const uint32_t j = threadIdx.x + u * kBuildThreads;
-        if (j < tl.len) atomicAdd(&hist[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / S::kRegion], 1u);
+        if (j < tl.len) atomicAdd(&hist[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / a.bin_slots], 1u);
     }
     __syncthreads();
-    const uint32_t g0 = a.reg0[it.table];
+    const uint32_t g0 = a.reg0[tl.table];
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x)
         if (hist[j]) atomicAdd(&a.counts[g0 + j], hist[j]);
 }
 
 // 2. offsets[0..n] = exclusive sum of counts[0..n) (one workgroup of 1024).
-__global__ __launch_bounds__(1024) void build_scan_kernel(const uint32_t* counts, uint32_t* offsets, uint32_t n)
+static __global__ __launch_bounds__(1024) void build_scan_kernel(const uint32_t* counts, uint32_t* offsets, uint32_t n)
 {
     typedef hipcub::BlockScan<uint32_t, 1024> Scan;
     __shared__ typename Scan::TempStorage tmp;
@@ -168,14 +208,13 @@ __global__ __launch_bounds__(kBuildThreads) void build_scatter_kernel(BuildArgs 
     typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
     __shared__ typename Scan::TempStorage scan_tmp;
 
-    const BatchChunk tl = a.tiles[blockIdx.x];
-    const mxco_batch_item it = a.items[tl.item];
-    const BatchTable t = a.tables[it.table];
-    const uint32_t nbins = (t.cap + S::kRegion - 1) / S::kRegion;
-    const uint32_t g0 = a.reg0[it.table];
+    const TileRef tl = tile_ref<L::SS>(a, blockIdx.x);
+    const BatchTable t = a.tables[tl.table];
+    const uint32_t nbins = (t.cap + a.bin_slots - 1) / a.bin_slots;
+    const uint32_t g0 = a.reg0[tl.table];
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x) tcnt[j] = 0;
     __syncthreads();
-    const Raw* src = (const Raw*)(a.in + (it.offset + tl.start) * L::SS);
+    const Raw* src = (const Raw*)tl.src;
     Raw p[S::kPer];
     uint32_t bin[S::kPer], rank[S::kPer];
 #pragma unroll
@@ -187,7 +226,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_scatter_kernel(BuildArgs 
     for (uint32_t u = 0; u < S::kPer; ++u) {
         const uint32_t j = threadIdx.x + u * kBuildThreads;
         if (j < tl.len) {
-            bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / S::kRegion;
+            bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / a.bin_slots;
             rank[u] = atomicAdd(&tcnt[bin[u]], 1u);
         }
     }
@@ -265,20 +304,35 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     __shared__ __align__(16) Raw ls[R];
     __shared__ uint32_t s_nov, s_fail;
     const uint32_t r = blockIdx.x;
-    const uint32_t ti = a.reg_table[r];
+    const uint32_t ti = a.reg_table ? a.reg_table[r] : 0u;
     const BatchTable t = a.tables[ti];
     const uint32_t r0 = (r - a.reg0[ti]) * R;
     const uint32_t rlen = min(R, t.cap - r0);
     const bool whole = rlen == t.cap;  // a one-region table: the walk wraps inside the region
+    const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
+    if (e - s > a.heavy_limit) {
+        // left untouched here; build_heavy_kernel inserts its pairs by the CAS walk
+        if (threadIdx.x == 0) {
+            a.ovc[r] = 0;
+            a.heavy[1 + atomicAdd(&a.heavy[0], 1u)] = r;
+        }
+        return;
+    }
     const uint64_t ekey = a.empty_key;
-    const Raw empty = empty_slot<KS, VS>(ekey, a.empty_val);
-    for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) ls[j] = empty;
+    uint8_t* dst = t.slots + (uint64_t)r0 * SS;
+    const uint32_t bytes = rlen * SS, n16 = bytes / 16;
+    if (a.load_region) {
+        for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)ls)[j] = ((const u64x2*)dst)[j];
+        for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ls[j] = ((const Raw*)dst)[j];
+    } else {
+        const Raw empty = empty_slot<KS, VS>(ekey, a.empty_val);
+        for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) ls[j] = empty;
+    }
     if (threadIdx.x == 0) {
         s_nov = 0;
         s_fail = 0;
     }
     __syncthreads();
-    const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
     Raw* part = (Raw*)a.part;
     uint32_t fails = 0;
     for (uint32_t b = s; b < e; b += blockDim.x) {
@@ -322,15 +376,40 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     }
     if (fails) atomicAdd(&s_fail, fails);
     __syncthreads();
-    uint8_t* dst = t.slots + (uint64_t)r0 * SS;
-    const uint32_t bytes = rlen * SS, n16 = bytes / 16;
     for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)dst)[j] = ((const u64x2*)ls)[j];
     for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ((Raw*)dst)[j] = ls[j];
     if (threadIdx.x == 0) {
         a.ovc[r] = s_nov;
-        const int32_t item = a.table_item[ti];
-        if (s_fail && item >= 0) atomicAdd(&a.counters[2ull * item], (unsigned long long)s_fail);
+        count_failures(a, ti, s_fail);
     }
+}
+
+// The CAS walk of pair p (key `key`) on table t from slot `pos` over `left`
+// slots (open_addressing_ref_impl.h:490-532): the first slot that is empty
+// or holds the key ends it.  Returns true when the pair was not inserted
+// (a duplicate, or no empty slot on the walk).
+template <int KS, int VS>
+__device__ __forceinline__ bool cas_walk(const BuildArgs& a, const BatchTable& t, typename Lay<KS, VS>::Raw p,
+                                         uint64_t key, uint32_t pos, uint32_t left)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr uint32_t SS = L::SS;
+    const uint64_t ekey = a.empty_key;
+    while (left > 0) {
+        Raw* slot = (Raw*)(t.slots + (uint64_t)pos * SS);
+        const Raw v = *slot;
+        uint64_t kk = L::key(v);
+        if (kk == ekey) {
+            kk = claim_slot<KS, VS>(slot, v, p, ekey);
+            if (kk == ekey) return false;
+            // lost: the slot now holds kk
+        }
+        if (key_eq<KS>(kk, key, a.key_float)) return true;
+        --left;
+        if (++pos == t.cap) pos = 0;
+    }
+    return true;
 }
 
 // 5. Keys whose walk left their region: the CAS walk on the table, resumed
@@ -344,11 +423,10 @@ __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
     constexpr uint32_t R = BuildShape<SS>::kRegion;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (blockDim.x / 64);
-    const uint64_t ekey = a.empty_key;
     for (uint32_t r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < a.nreg; r += waves) {
         const uint32_t n = a.ovc[r];
         if (n == 0) continue;
-        const uint32_t ti = a.reg_table[r];
+        const uint32_t ti = a.reg_table ? a.reg_table[r] : 0u;
         const BatchTable t = a.tables[ti];
         const uint32_t r0 = (r - a.reg0[ti]) * R;
         const uint32_t rend = r0 + min(R, t.cap - r0);
@@ -358,35 +436,133 @@ __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
             const Raw p = part[k];
             const uint64_t key = L::key(p);
             const uint32_t first = first_slot<KS>(a, t.nbuckets, key);
-            uint32_t left = t.cap - (rend - first);  // slots of the walk not yet examined
-            uint32_t pos = rend == t.cap ? 0 : rend;
-            bool done = false;
-            while (left > 0) {
-                Raw* slot = (Raw*)(t.slots + (uint64_t)pos * SS);
-                const Raw v = *slot;
-                uint64_t kk = L::key(v);
-                if (kk == ekey) {
-                    kk = claim_slot<KS, VS>(slot, v, p, ekey);
-                    if (kk == ekey) {
-                        done = true;
-                        break;
-                    }
-                    // lost: the slot now holds kk
-                }
-                if (key_eq<KS>(kk, key, a.key_float)) {
-                    ++fails;
-                    done = true;
-                    break;
-                }
-                --left;
-                if (++pos == t.cap) pos = 0;
-            }
-            if (!done) ++fails;  // no empty slot on the walk
+            // the region's slots from `first` on are full and hold other keys
+            fails += cas_walk<KS, VS>(a, t, p, key, rend == t.cap ? 0 : rend, t.cap - (rend - first));
         }
         // one add per wavefront
         for (int o = 32; o > 0; o >>= 1) fails += __shfl_xor(fails, o);
-        const int32_t item = a.table_item[ti];
-        if (lane == 0 && fails && item >= 0) atomicAdd(&a.counters[2ull * item], (unsigned long long)fails);
+        if (lane == 0) count_failures(a, ti, fails);
+    }
+}
+
+// 6. Pairs of the heavy regions: the CAS walk from their first slot, every
+// heavy region spread over the whole grid.
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void build_heavy_kernel(BuildArgs a)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    const uint32_t nh = a.heavy[0];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t r = a.heavy[1 + h];
+        const uint32_t ti = a.reg_table ? a.reg_table[r] : 0u;
+        const BatchTable t = a.tables[ti];
+        const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
+        const Raw* part = (const Raw*)a.part;
+        uint32_t fails = 0;
+        for (uint64_t j = s + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < e; j += stride) {
+            const Raw p = part[j];
+            const uint64_t key = L::key(p);
+            if (key == a.empty_key) {
+                ++fails;
+                continue;
+            }
+            fails += cas_walk<KS, VS>(a, t, p, key, first_slot<KS>(a, t.nbuckets, key), t.cap);
+        }
+        for (int o = 32; o > 0; o >>= 1) fails += __shfl_xor(fails, o);
+        if (lane == 0) count_failures(a, ti, fails);
+    }
+}
+
+// Two-level partition of one table: workgroup b regroups the pairs of bin b
+// (`group` consecutive regions, part1[offsets1[b] .. offsets1[b + 1]))
+// by region into part2 at the same positions, and writes the regions'
+// offsets (a.offsets, nreg + 1 entries).  Dynamic LDS: build_regroup_lds.
+template <int SS>
+constexpr size_t build_regroup_lds(uint32_t group)
+{
+    return (size_t)BuildShape<SS>::kTile * SS + (size_t)BuildShape<SS>::kTile * 2 + (size_t)group * 16;
+}
+
+template <int KS, int VS>
+__global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs a, const uint8_t* part1,
+                                                                      const uint32_t* offsets1, uint32_t group)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    using S = BuildShape<L::SS>;
+    extern __shared__ __align__(16) uint8_t smem[];
+    Raw* staged = (Raw*)smem;                          // kTile
+    uint32_t* rcnt = (uint32_t*)(staged + S::kTile);   // group: pairs of each region in the bin
+    uint32_t* rcur = rcnt + group;                     // group: next position of each region
+    uint32_t* tcnt = rcur + group;                     // group: pairs of each region in the tile
+    uint32_t* toff = tcnt + group;                     // group: first of each region in the tile
+    uint16_t* rid = (uint16_t*)(toff + group);         // kTile
+    const BatchTable t = a.tables[0];
+    const uint32_t b = blockIdx.x;
+    const uint32_t rfirst = b * group;
+    const uint32_t nr = min(group, a.nreg - rfirst);
+    const uint32_t s = offsets1[b], e = offsets1[b + 1];
+    const Raw* src = (const Raw*)part1;
+    Raw* dst = (Raw*)a.part;
+    for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) rcnt[j] = 0;
+    __syncthreads();
+    for (uint32_t j = s + threadIdx.x; j < e; j += blockDim.x)
+        atomicAdd(&rcnt[first_slot<KS>(a, t.nbuckets, L::key(src[j])) / S::kRegion - rfirst], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t o = s;
+        for (uint32_t j = 0; j < nr; ++j) {
+            rcur[j] = o;
+            a.offsets[rfirst + j] = o;
+            o += rcnt[j];
+        }
+        if (rfirst + nr == a.nreg) a.offsets[a.nreg] = e;
+    }
+    __syncthreads();
+    for (uint32_t t0 = s; t0 < e; t0 += S::kTile) {
+        const uint32_t tlen = min(S::kTile, e - t0);
+        for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) tcnt[j] = 0;
+        __syncthreads();
+        Raw p[S::kPer];
+        uint32_t bin[S::kPer], rank[S::kPer];
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < tlen) {
+                p[u] = src[t0 + j];
+                bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / S::kRegion - rfirst;
+                rank[u] = atomicAdd(&tcnt[bin[u]], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t o = 0;
+            for (uint32_t j = 0; j < nr; ++j) {
+                toff[j] = o;
+                o += tcnt[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < tlen) {
+                const uint32_t pos = toff[bin[u]] + rank[u];
+                staged[pos] = p[u];
+                rid[pos] = (uint16_t)bin[u];
+            }
+        }
+        __syncthreads();
+        for (uint32_t pos = threadIdx.x; pos < tlen; pos += blockDim.x) {
+            const uint32_t r = rid[pos];
+            dst[rcur[r] + (pos - toff[r])] = staged[pos];
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) rcur[j] += tcnt[j];
+        __syncthreads();
     }
 }
 

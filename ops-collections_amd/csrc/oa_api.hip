@@ -12,10 +12,27 @@
 #include <new>
 #include <string>
 
+#include "batch_build.h"
 #include "mxco_common.h"
 #include "oa_launch.h"
 
 using namespace mxco;
+
+// Workspace of the region-built Insert, allocated on first use and grown
+// with the batch (batch_build.h).
+struct RegionWs {
+    uint64_t pairs;  // pairs part1 / part2 hold
+    uint8_t* part1;
+    uint8_t* part2;
+    uint32_t* counts1;   // nbins1 + 1
+    uint32_t* offsets1;  // nbins1 + 1
+    uint32_t* offsets2;  // nreg + 1
+    uint32_t* ovc;       // nreg
+    uint32_t* heavy;     // nreg + 1
+    BatchTable* table;   // the table's descriptor
+    uint32_t* reg0;      // {0, nbins1}
+    uint32_t nreg, nbins1, group;
+};
 
 struct mxco_oa {
     mxco_oa_desc d;
@@ -35,6 +52,7 @@ struct mxco_oa {
     uint32_t store_mode;
     int inflight;
     uint32_t probe_flags;
+    RegionWs* rws;
 };
 
 namespace {
@@ -155,6 +173,155 @@ mxco_status fill(mxco_oa* t, hipStream_t stream)
     return MXCO_OK;
 }
 
+__global__ void zero_u32_pair_kernel(uint32_t* a, uint32_t na, uint32_t* b)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) a[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) b[0] = 0;
+}
+
+void free_region_ws(mxco_oa* t)
+{
+    RegionWs* w = t->rws;
+    if (!w) return;
+    for (void* p : {(void*)w->part1, (void*)w->part2, (void*)w->counts1, (void*)w->offsets1, (void*)w->offsets2,
+                    (void*)w->ovc, (void*)w->heavy, (void*)w->table, (void*)w->reg0})
+        if (p) (void)hipFree(p);
+    delete w;
+    t->rws = nullptr;
+}
+
+// With MXCO_OA_REGION_INSERT=1 (read at each call; off by default), Insert of
+// n pairs takes the region build when the batch is large next to the table
+// (n >= 2^20 and n >= capacity / 4) under LinearProbing without a stencil.
+// Under stream capture the workspace must already be large enough.
+bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream_t stream)
+{
+    const char* env = getenv("MXCO_OA_REGION_INSERT");
+    if (!env || atoi(env) == 0 || t->d.probing != 0 || a.stencil) return false;
+    if (n < (1ull << 20) || n * 4 < t->cap || n >= 0xffffffffull) return false;
+    // the null stream is never captured
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (stream && hipStreamIsCapturing(stream, &cs) != hipSuccess) return false;
+    if (cs != hipStreamCaptureStatusNone && (!t->rws || t->rws->pairs < n)) return false;
+    return true;
+}
+
+hipError_t region_ws(mxco_oa* t, uint64_t n)
+{
+    if (!t->rws) {
+        RegionWs* w = new (std::nothrow) RegionWs();
+        if (!w) return hipErrorOutOfMemory;
+        t->rws = w;
+        const uint32_t R = kBuildBytes / t->ss;
+        w->nreg = (uint32_t)((t->cap + R - 1) / R);
+        w->group = (w->nreg + kBuildMaxBins - 1) / kBuildMaxBins;
+        w->nbins1 = (w->nreg + w->group - 1) / w->group;
+        hipError_t e = hipMalloc((void**)&w->counts1, (w->nbins1 + 1ull) * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->offsets1, (w->nbins1 + 1ull) * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->offsets2, (w->nreg + 1ull) * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->ovc, w->nreg * 4ull);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->heavy, (w->nreg + 1ull) * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->table, sizeof(BatchTable));
+        if (e == hipSuccess) e = hipMalloc((void**)&w->reg0, 2 * sizeof(uint32_t));
+        if (e == hipSuccess) {
+            BatchTable bt{t->slots, (uint32_t)t->cap, (uint32_t)(t->cap / t->d.bucket_size)};
+            const uint32_t r0[2] = {0u, w->nbins1};
+            e = hipMemcpy(w->table, &bt, sizeof(bt), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(w->reg0, r0, sizeof(r0), hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) {
+            free_region_ws(t);
+            return e;
+        }
+    }
+    RegionWs* w = t->rws;
+    if (w->pairs < n) {
+        if (w->part1) (void)hipFree(w->part1);
+        if (w->part2) (void)hipFree(w->part2);
+        w->part1 = w->part2 = nullptr;
+        w->pairs = 0;
+        hipError_t e = hipMalloc((void**)&w->part1, n * t->ss);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->part2, n * t->ss);
+        if (e != hipSuccess) return e;
+        w->pairs = n;
+    }
+    return hipSuccess;
+}
+
+template <int KS, int VS>
+hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t s)
+{
+    constexpr int SS = Lay<KS, VS>::SS;
+    constexpr uint32_t R = BuildShape<SS>::kRegion;
+    static const hipError_t attr = [] {
+        hipError_t e = hipFuncSetAttribute((const void*)build_scatter_kernel<KS, VS>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)build_scatter_lds<SS>(kBuildMaxBins));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)build_regroup_kernel<KS, VS>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)build_regroup_lds<SS>(256));
+        return e;
+    }();
+    if (attr != hipSuccess) return attr;
+    const RegionWs& w = *t->rws;
+    if (w.group > 256) return hipErrorInvalidValue;
+    const uint64_t n = oa.n;
+    zero_u32_pair_kernel<<<grid_for(w.nbins1 + 1ull, 1), 256, 0, s>>>(w.counts1, w.nbins1 + 1, w.heavy);
+    BuildArgs a1;
+    memset(&a1, 0, sizeof(a1));
+    a1.in = oa.in;
+    a1.part = w.part1;
+    a1.tables = w.table;
+    a1.reg0 = w.reg0;
+    a1.counts = w.counts1;
+    a1.offsets = w.offsets1;
+    a1.counters = oa.counters;
+    a1.empty_key = oa.empty_key;
+    a1.empty_val = oa.empty_val;
+    a1.single_n = n;
+    a1.nreg = w.nbins1;
+    a1.maxbins = w.nbins1;
+    a1.bin_slots = w.group * R;
+    a1.heavy_limit = 0xffffffffu;
+    a1.bucket = oa.bucket;
+    a1.hash = oa.hash;
+    a1.key_float = oa.key_float;
+    const unsigned ntiles = (unsigned)((n + BuildShape<SS>::kTile - 1) / BuildShape<SS>::kTile);
+    build_count_kernel<KS, VS><<<ntiles, kBuildThreads, 0, s>>>(a1);
+    build_scan_kernel<<<1, 1024, 0, s>>>(w.counts1, w.offsets1, w.nbins1);
+    build_scatter_kernel<KS, VS><<<ntiles, kBuildThreads, build_scatter_lds<SS>(w.nbins1), s>>>(a1);
+    BuildArgs a2 = a1;
+    a2.part = w.part2;
+    a2.offsets = w.offsets2;
+    a2.ovc = w.ovc;
+    a2.heavy = w.heavy;
+    a2.nreg = w.nreg;
+    a2.bin_slots = R;
+    a2.load_region = 1;
+    a2.heavy_limit = 2 * R;
+    build_regroup_kernel<KS, VS><<<w.nbins1, kBuildThreads, build_regroup_lds<SS>(w.group), s>>>(a2, w.part1,
+                                                                                               w.offsets1, w.group);
+    build_apply_kernel<KS, VS><<<w.nreg, kBuildThreads, 0, s>>>(a2);
+    const unsigned ogrid = std::max(1u, std::min<unsigned>((w.nreg + 3) / 4, (unsigned)cu_count() * 8));
+    build_overflow_kernel<KS, VS><<<ogrid, 256, 0, s>>>(a2);
+    build_heavy_kernel<KS, VS><<<(unsigned)cu_count() * 4, 256, 0, s>>>(a2);
+    return hipGetLastError();
+}
+
+hipError_t region_insert(mxco_oa* t, const OaArgs& a, hipStream_t s)
+{
+    hipError_t e = region_ws(t, a.n);
+    if (e != hipSuccess) return e;
+    const uint32_t ks = t->d.key_size, vs = t->d.value_size;
+    if (ks == 4 && vs == 4) return launch_region_insert<4, 4>(t, a, s);
+    if (ks == 8 && vs == 8) return launch_region_insert<8, 8>(t, a, s);
+    if (ks == 4 && vs == 8) return launch_region_insert<4, 8>(t, a, s);
+    if (ks == 8 && vs == 4) return launch_region_insert<8, 4>(t, a, s);
+    if (ks == 4 && vs == 0) return launch_region_insert<4, 0>(t, a, s);
+    if (ks == 8 && vs == 0) return launch_region_insert<8, 0>(t, a, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
 
 extern "C" {
@@ -243,6 +410,7 @@ MXCO_API mxco_status mxco_oa_destroy(mxco_oa* t)
     if (t->slots) (void)hipFree(t->slots);
     if (t->counters) (void)hipFree(t->counters);
     if (t->host_counters) (void)hipHostFree(t->host_counters);
+    free_region_ws(t);
     delete t;
     return ok();
 }
@@ -308,12 +476,25 @@ MXCO_API mxco_status mxco_oa_insert(mxco_oa* t, const void* pairs, const mxco_st
         if (mxco_status s = zero_counters(t, stream)) return s;
         a.counters = t->counters;
     }
-    if (mxco_status s = launch(t, OP_INSERT, a, stream, "Insert")) return s;
+    if (region_insert_fits(t, n, a, stream)) {
+        MXCO_HIP(region_insert(t, a, stream), "Insert");
+    } else if (mxco_status s = launch(t, OP_INSERT, a, stream, "Insert")) {
+        return s;
+    }
     if (sync) {
         uint64_t f = 0;
         if (mxco_status s = read_counters(t, stream, &f, nullptr)) return s;
         if (num_failed) *num_failed = f;
     }
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_insert_path(const mxco_oa* t, uint64_t n, hipStream_t stream, int* region_build)
+{
+    if (mxco_status s = check_table(t, "insert_path")) return s;
+    if (!region_build) return fail(MXCO_INVALID_ARGUMENT, "insert_path: null output");
+    OaArgs a = base_args(t);
+    *region_build = region_insert_fits(t, n, a, stream) ? 1 : 0;
     return ok();
 }
 

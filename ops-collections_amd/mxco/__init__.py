@@ -298,6 +298,12 @@ class _OpenAddressing:
         check(_lib.mxco_oa_probe_stats(self._h, _ptr(keys), n, _stream(stream), ctypes.byref(ln), ctypes.byref(sl)))
         return ln.value, sl.value
 
+    def insert_uses_region_build(self, n, stream=None):
+        """Whether an Insert of n pairs takes the region build (a diagnostic)."""
+        r = i32()
+        check(_lib.mxco_oa_insert_path(self._h, n, _stream(stream), ctypes.byref(r)))
+        return bool(r.value)
+
     def slots(self):
         """Device copy of the raw slot array as uint8 (capacity, slot_size)."""
         n = self._capacity * self.slot_size
@@ -902,6 +908,7 @@ class BatchItem(ctypes.Structure):
 
 
 BATCH_INSERT, BATCH_FIND, BATCH_CONTAINS, BATCH_BUILD = 0, 1, 2, 3  # BUILD: Clear of every table + Insert
+_sig("mxco_oa_insert_path", [vp, u64, vp, ctypes.POINTER(i32)])
 _sig("mxco_batch_create", [ctypes.POINTER(OaDesc), u32, ctypes.POINTER(u64), vp, vp, vp, ctypes.POINTER(vp)])
 _sig("mxco_batch_destroy", [vp])
 _sig("mxco_batch_table", [vp, u32, pu64, ctypes.POINTER(vp)])
