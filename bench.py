@@ -119,7 +119,25 @@ def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
     t_find = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
 
     res = {"capacity": cap, "n_insert": keys.numel(), "n_find": q.numel(), "wall_ms_per_step": wall * 1e3 / steps,
-           "clear_ms": t_clear, "insert_ms": t_ins, "find_ms": t_find, "group": group, "lf": lf}
+           "clear_ms": t_clear, "insert_ms": t_ins, "find_ms": t_find, "group": group, "lf": lf,
+           "insert_path": "region build" if m.insert_uses_region_build(keys.numel()) else "CAS walk"}
+    if verify and res["insert_path"] == "region build":
+        # the CAS walk on the same batch, for comparison (MXCO_OA_REGION_INSERT=0 is read at each call)
+        os.environ["MXCO_OA_REGION_INSERT"] = "0"
+        try:
+            def clear_insert(e):
+                m.clear_async()
+                e[0].record(s)
+                m.insert_async(pairs)
+                e[1].record(s)
+            evc = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+            clear_insert(evc[0])
+            for e in evc:
+                clear_insert(e)
+            torch.cuda.synchronize()
+            res["insert_cas_walk_ms"] = sum(e[0].elapsed_time(e[1]) for e in evc) / steps
+        finally:
+            del os.environ["MXCO_OA_REGION_INSERT"]
     if verify:
         # Inserted keys hit with f(key); absent keys miss; no insert failed.
         exp = torch.where(q < (1 << 62), q * 7 + 1, torch.full_like(q, -1))
@@ -772,11 +790,24 @@ def main():
                  "traffic": traffic.get("find") if traffic else None,
                  "lines_per_query": r["find_lines"] / r["n_find"], "launch_ms": r["find_ms"],
                  "algorithmic_bytes": find_bytes}
-    roof_ins = {"kernel": "oa_kernel<8,8,%d,OP_INSERT>" % g, "bound": "hbm", "achieved": round(ins_gbs, 1),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ins_gbs / PEAK_HBM_GBS, 4),
-                "traffic": traffic.get("insert") if traffic else None,
-                "lines_per_key": r["insert_lines"] / r["n_insert"], "launch_ms": r["insert_ms"],
-                "algorithmic_bytes": ins_bytes}
+    if r["insert_path"] == "region build":
+        # the bytes the build's passes move: count 16, scatter 32, regroup 48
+        # and apply 16 per pair, plus each region read and written once
+        build_bytes = 112.0 * r["n_insert"] + 32.0 * r["capacity"]
+        build_gbs = build_bytes / (r["insert_ms"] * 1e-3) / 1e9
+        roof_ins = {"kernel": "build_{count,scatter,regroup,apply,overflow}_kernel<8,8> (region build)",
+                    "bound": "hbm", "achieved": round(build_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(build_gbs / PEAK_HBM_GBS, 4),
+                    "traffic": traffic.get("insert") if traffic else None, "launch_ms": r["insert_ms"],
+                    "algorithmic_bytes": build_bytes,
+                    "cas_walk_bytes": ins_bytes, "cas_walk_equivalent_frac": round(ins_gbs / PEAK_HBM_GBS, 4),
+                    "cas_walk_ms": r.get("insert_cas_walk_ms")}
+    else:
+        roof_ins = {"kernel": "oa_kernel<8,8,%d,OP_INSERT>" % g, "bound": "hbm", "achieved": round(ins_gbs, 1),
+                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ins_gbs / PEAK_HBM_GBS, 4),
+                    "traffic": traffic.get("insert") if traffic else None,
+                    "lines_per_key": r["insert_lines"] / r["n_insert"], "launch_ms": r["insert_ms"],
+                    "algorithmic_bytes": ins_bytes}
     line = {
         "metric": "Gkeys/s StaticMap Insert/Find int64 @LF0.8; batched vs plain speedup; p99 ms",
         "value": round(value, 4), "unit": "Gkeys/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -789,6 +820,7 @@ def main():
         "insert_gkeys_s": round(r["n_insert"] / (r["insert_ms"] * 1e-3) / 1e9, 4),
         "find_gkeys_s": round(r["n_find"] / (r["find_ms"] * 1e-3) / 1e9, 4),
         "clear_ms": round(r["clear_ms"], 4),
+        "insert_path": r["insert_path"],
         "verified": r["verified"] and r["insert_failed"] == 0,
         "roofline": roof_ins if dominant_insert else roof_find,
         "roofline_other": roof_find if dominant_insert else roof_ins,

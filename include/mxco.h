@@ -180,12 +180,14 @@ MXCO_API mxco_status mxco_oa_erase(mxco_oa* table, const void* keys, uint64_t n,
                                    uint64_t* num_failed);
 
 /* Whether an Insert of n pairs (no stencil) on `stream` takes the region
- * build (no reference counterpart; a diagnostic).  With the environment
- * variable MXCO_OA_REGION_INSERT=1, batches of at least 2^20 pairs and a
- * quarter of the capacity under LinearProbing are partitioned by 64 KiB slot
- * region and inserted region by region in LDS, with the CAS walk's results;
- * concurrent bulk calls on one table, on any streams, then need external
- * serialization. */
+ * build (no reference counterpart; a diagnostic).  Batches of at least 2^20
+ * pairs and half the capacity, on tables of more than 256 MiB of slots
+ * (MXCO_OA_REGION_INSERT_MIN_BYTES), under LinearProbing are partitioned by
+ * 64 KiB slot region and inserted region by region in LDS, with the CAS
+ * walk's results (MXCO_OA_REGION_INSERT=0 turns this off).  The table keeps a
+ * workspace of 2 x n slots for it, and concurrent bulk calls on one table,
+ * on any streams, need external serialization (as the reference's
+ * StaticMultimap states for its calls). */
 MXCO_API mxco_status mxco_oa_insert_path(const mxco_oa* table, uint64_t n, hipStream_t stream, int* region_build);
 
 /* Probe statistics of Find's walk over `keys` (no reference counterpart;

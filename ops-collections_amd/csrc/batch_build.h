@@ -293,6 +293,169 @@ __device__ __forceinline__ uint64_t claim_slot(typename Lay<KS, VS>::Raw* s, typ
     }
 }
 
+// Placement without walks (an empty region of at most kPlaceMax slots, at
+// most kPlaceMax pairs, integer keys).  Inserting keys one after another in
+// the order of their first slots, each walk ends at the first slot after the
+// previous key's that is not below its own first slot: the i-th kept key
+// lands at slot i + max_{j <= i}(first_j - j), a prefix maximum.  The pairs
+// are counting-sorted by first slot in LDS, and the second and later copies
+// of a key within its first-slot group are duplicates (equal integer keys
+// share a first slot).  A key placed past the region's end goes to the
+// overflow kernel.  Returns false, with ls refilled with the empty slot, when
+// a first-slot group exceeds kPlaceGroup pairs; the caller then walks.
+constexpr uint32_t kPlaceMax = 4096;
+constexpr uint32_t kPlaceGroup = 64;
+
+template <int KS, int VS>
+__device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTable& t, uint32_t r0, uint32_t rlen,
+                                             uint32_t s, uint32_t e, typename Lay<KS, VS>::Raw* ls,
+                                             typename Lay<KS, VS>::Raw* part, uint32_t& s_nov, uint32_t& fails)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr uint32_t SS = L::SS;
+    constexpr uint32_t kPer = kPlaceMax / kBuildThreads;  // pairs (and sorted positions) per thread
+    static_assert(kPlaceMax * (8 + 2 + 2) <= BuildShape<SS>::kRegion * SS, "sort scratch fits the region's LDS");
+    typedef hipcub::BlockScan<int32_t, kBuildThreads> Scan;
+    __shared__ typename Scan::TempStorage scan_tmp;
+    __shared__ uint32_t hcnt[kPlaceMax / 2];  // two 16-bit counts per word: pairs per first slot, then group starts
+    __shared__ uint32_t s_big;
+    uint64_t* skey = (uint64_t*)ls;               // sorted keys
+    uint16_t* shome = (uint16_t*)(skey + kPlaceMax);  // sorted first slots | 0x8000 for a kept pair
+    uint16_t* sq = shome + kPlaceMax;             // slot of each kept sorted pair, 0xffff past the region
+    const uint64_t ekey = a.empty_key;
+    const uint32_t tid = threadIdx.x;
+
+    Raw p[kPer];
+    uint32_t home[kPer], rank[kPer], si[kPer];
+    bool valid[kPer];
+    for (uint32_t j = tid; j < kPlaceMax / 2; j += kBuildThreads) hcnt[j] = 0;
+    if (tid == 0) s_big = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u) {
+        const uint32_t i = s + tid + u * kBuildThreads;
+        valid[u] = false;
+        if (i < e) {
+            p[u] = part[i];
+            const uint64_t key = L::key(p[u]);
+            if (key == ekey) {
+                ++fails;  // the reserved empty key is never inserted
+            } else {
+                valid[u] = true;
+                home[u] = first_slot<KS>(a, t.nbuckets, key) - r0;
+                const uint32_t sh = (home[u] & 1u) * 16u;
+                rank[u] = (atomicAdd(&hcnt[home[u] >> 1], 1u << sh) >> sh) & 0xffffu;
+            }
+        }
+    }
+    __syncthreads();
+    // group starts: exclusive sum of the counts, kPer first slots per thread;
+    // `total` = pairs sorted (all but the reserved key)
+    uint32_t total;
+    {
+        uint32_t c[kPer], sum = 0, big = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t h = tid * kPer + k;
+            c[k] = (hcnt[h >> 1] >> ((h & 1u) * 16u)) & 0xffffu;
+            sum += c[k];
+            big |= c[k] > kPlaceGroup;
+        }
+        int32_t excl, agg;
+        Scan(scan_tmp).ExclusiveSum((int32_t)sum, excl, agg);
+        total = (uint32_t)agg;
+        if (big) s_big = 1;
+        __syncthreads();  // every count read before the starts overwrite them
+        uint32_t o = (uint32_t)excl;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k += 2) {
+            hcnt[(tid * kPer + k) >> 1] = o | (o + c[k]) << 16;
+            o += c[k] + c[k + 1];
+        }
+    }
+    __syncthreads();
+    if (s_big) {
+        const Raw empty = empty_slot<KS, VS>(ekey, a.empty_val);
+        for (uint32_t j = tid; j < rlen; j += kBuildThreads) ls[j] = empty;
+        __syncthreads();
+        fails = 0;
+        return false;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u) {
+        if (!valid[u]) continue;
+        const uint32_t h = home[u];
+        si[u] = ((hcnt[h >> 1] >> ((h & 1u) * 16u)) & 0xffffu) + rank[u];
+        skey[si[u]] = L::key(p[u]);
+    }
+    __syncthreads();
+    // duplicates: an equal key earlier in the same first-slot group
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u) {
+        if (!valid[u]) continue;
+        const uint32_t h = home[u];
+        const uint32_t g0 = (hcnt[h >> 1] >> ((h & 1u) * 16u)) & 0xffffu;
+        const uint64_t key = skey[si[u]];
+        bool dup = false;
+        for (uint32_t j = g0; j < si[u]; ++j) dup |= skey[j] == key;
+        shome[si[u]] = (uint16_t)(h | (dup ? 0u : 0x8000u));
+        if (dup) {
+            ++fails;
+            valid[u] = false;
+        }
+    }
+    __syncthreads();
+    // kept pairs in sorted order: rank K (exclusive count) and the running
+    // maximum of first - K, kPer sorted positions per thread
+    {
+        uint32_t kept[kPer], hh[kPer], nk = 0;
+        int32_t lmax = INT32_MIN;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t j = tid * kPer + k;
+            const uint16_t v = j < total ? shome[j] : 0;
+            kept[k] = v >> 15;
+            hh[k] = v & 0x7fffu;
+            if (kept[k]) lmax = max(lmax, (int32_t)hh[k] - (int32_t)nk);  // relative to the thread's first K
+            nk += kept[k];
+        }
+        int32_t kx;
+        Scan(scan_tmp).ExclusiveSum((int32_t)nk, kx);
+        __syncthreads();
+        // the thread's own maximum, shifted by its first K
+        const int32_t tmax = lmax == INT32_MIN ? INT32_MIN : lmax - kx;
+        int32_t mx;
+        Scan(scan_tmp).ExclusiveScan(tmax, mx, INT32_MIN, hipcub::Max());
+        int32_t K = kx;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t j = tid * kPer + k;
+            if (j < total && kept[k]) {
+                mx = max(mx, (int32_t)hh[k] - K);
+                const int32_t q = K + mx;
+                sq[j] = q < (int32_t)rlen ? (uint16_t)q : (uint16_t)0xffffu;
+                ++K;
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t q[kPer];
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u) q[u] = valid[u] ? sq[si[u]] : 0xffffu;
+    __syncthreads();
+    const Raw empty = empty_slot<KS, VS>(ekey, a.empty_val);
+    for (uint32_t j = tid; j < rlen; j += kBuildThreads) ls[j] = empty;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u) {
+        if (!valid[u]) continue;
+        if (q[u] != 0xffffu) ls[q[u]] = p[u];
+        else part[s + atomicAdd(&s_nov, 1u)] = p[u];  // every pair was read before the first barrier
+    }
+    return true;
+}
+
 // 4. One region per workgroup: build it in LDS and write it once.
 template <int KS, int VS>
 __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
@@ -332,9 +495,27 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
         s_nov = 0;
         s_fail = 0;
     }
-    __syncthreads();
     Raw* part = (Raw*)a.part;
     uint32_t fails = 0;
+    if constexpr (R <= kPlaceMax) {
+        // an empty region, integer keys: placed without walks
+        bool empty_region = true;
+        if (a.load_region)
+            for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) empty_region &= L::key(ls[j]) == ekey;
+        if (__syncthreads_and(empty_region && !a.key_float && e - s <= kPlaceMax) &&
+            place_sorted<KS, VS>(a, t, r0, rlen, s, e, ls, part, s_nov, fails)) {
+            if (fails) atomicAdd(&s_fail, fails);
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)dst)[j] = ((const u64x2*)ls)[j];
+            for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ((Raw*)dst)[j] = ls[j];
+            if (threadIdx.x == 0) {
+                a.ovc[r] = s_nov;
+                count_failures(a, ti, s_fail);
+            }
+            return;
+        }
+    }
+    __syncthreads();
     for (uint32_t b = s; b < e; b += blockDim.x) {
         const uint32_t j = b + threadIdx.x;
         Raw p;

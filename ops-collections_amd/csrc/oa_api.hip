@@ -190,15 +190,22 @@ void free_region_ws(mxco_oa* t)
     t->rws = nullptr;
 }
 
-// With MXCO_OA_REGION_INSERT=1 (read at each call; off by default), Insert of
-// n pairs takes the region build when the batch is large next to the table
-// (n >= 2^20 and n >= capacity / 4) under LinearProbing without a stencil.
-// Under stream capture the workspace must already be large enough.
+// Insert of n pairs takes the region build when the batch is large next to
+// the table (n >= 2^20 and n >= capacity / 2) and the table is larger than
+// the 256 MiB last-level cache (MXCO_OA_REGION_INSERT_MIN_BYTES), under
+// LinearProbing without a stencil; MXCO_OA_REGION_INSERT=0 keeps the CAS
+// walk (both read at each call).  The build moves about 112 bytes per pair
+// plus 32 per slot; the CAS walk about 300 per distinct key at load factor
+// 0.8 on a table that misses the cache, less on one that fits it or for a
+// stream of repeated keys, so those keep the walk.  Under stream capture the
+// workspace must already be large enough.
 bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream_t stream)
 {
     const char* env = getenv("MXCO_OA_REGION_INSERT");
-    if (!env || atoi(env) == 0 || t->d.probing != 0 || a.stencil) return false;
-    if (n < (1ull << 20) || n * 4 < t->cap || n >= 0xffffffffull) return false;
+    if ((env && atoi(env) == 0) || t->d.probing != 0 || a.stencil) return false;
+    if (n < (1ull << 20) || n * 2 < t->cap || n >= 0xffffffffull) return false;
+    const char* mb = getenv("MXCO_OA_REGION_INSERT_MIN_BYTES");
+    if (t->cap * t->ss < (mb ? strtoull(mb, nullptr, 10) : (256ull << 20))) return false;
     // the null stream is never captured
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (stream && hipStreamIsCapturing(stream, &cs) != hipSuccess) return false;
@@ -214,7 +221,9 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         t->rws = w;
         const uint32_t R = kBuildBytes / t->ss;
         w->nreg = (uint32_t)((t->cap + R - 1) / R);
-        w->group = (w->nreg + kBuildMaxBins - 1) / kBuildMaxBins;
+        // about 512 bins: runs of a line or more per bin and 4096-pair tile
+        // in the scatter; at most 256 regions per bin (the regroup's LDS)
+        w->group = std::min<uint32_t>(256u, (w->nreg + 511) / 512);
         w->nbins1 = (w->nreg + w->group - 1) / w->group;
         hipError_t e = hipMalloc((void**)&w->counts1, (w->nbins1 + 1ull) * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&w->offsets1, (w->nbins1 + 1ull) * 4);
@@ -264,7 +273,7 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     }();
     if (attr != hipSuccess) return attr;
     const RegionWs& w = *t->rws;
-    if (w.group > 256) return hipErrorInvalidValue;
+    if (w.group > 256 || w.nbins1 > kBuildMaxBins) return hipErrorInvalidValue;
     const uint64_t n = oa.n;
     zero_u32_pair_kernel<<<grid_for(w.nbins1 + 1ull, 1), 256, 0, s>>>(w.counts1, w.nbins1 + 1, w.heavy);
     BuildArgs a1;
@@ -310,8 +319,6 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
 
 hipError_t region_insert(mxco_oa* t, const OaArgs& a, hipStream_t s)
 {
-    hipError_t e = region_ws(t, a.n);
-    if (e != hipSuccess) return e;
     const uint32_t ks = t->d.key_size, vs = t->d.value_size;
     if (ks == 4 && vs == 4) return launch_region_insert<4, 4>(t, a, s);
     if (ks == 8 && vs == 8) return launch_region_insert<8, 8>(t, a, s);
@@ -476,7 +483,12 @@ MXCO_API mxco_status mxco_oa_insert(mxco_oa* t, const void* pairs, const mxco_st
         if (mxco_status s = zero_counters(t, stream)) return s;
         a.counters = t->counters;
     }
-    if (region_insert_fits(t, n, a, stream)) {
+    bool region = region_insert_fits(t, n, a, stream);
+    if (region && region_ws(t, n) != hipSuccess) {
+        (void)hipGetLastError();  // no room for the workspace: the CAS walk needs none
+        region = false;
+    }
+    if (region) {
         MXCO_HIP(region_insert(t, a, stream), "Insert");
     } else if (mxco_status s = launch(t, OP_INSERT, a, stream, "Insert")) {
         return s;

@@ -1,8 +1,8 @@
-"""With MXCO_OA_REGION_INSERT=1, Insert of a large batch (n >= 2^20 and
-n >= capacity / 4, LinearProbing) is built region by region in LDS.  Its
-failures and Find outputs equal the oracle's sequential CAS walk on the same
-inputs, and the table holds every
-inserted key exactly once: on empty tables, tables of 2^24 slots (a two-level
+"""Insert of a large batch (n >= 2^20 and n >= capacity / 2, LinearProbing)
+on a table of more than 256 MiB of slots is built region by region in LDS
+(MXCO_OA_REGION_INSERT=0 turns it off; these tests lower the size threshold).
+Its failures and Find outputs equal the oracle's sequential CAS walk on the
+same inputs, and the table holds every inserted key exactly once: on empty tables, tables of 2^24 slots (a two-level
 partition), tables already holding keys and tombstones, streams with one
 very hot key (regions left to the CAS walk), full tables, InsertAsync and
 graph replays."""
@@ -17,7 +17,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(autouse=True)
 def region_insert_on(monkeypatch):
+    # these tables are smaller than the default 256 MiB threshold
     monkeypatch.setenv("MXCO_OA_REGION_INSERT", "1")
+    monkeypatch.setenv("MXCO_OA_REGION_INSERT_MIN_BYTES", "0")
 
 TD = {np.int32: torch.int32, np.int64: torch.int64}
 
@@ -167,15 +169,17 @@ def test_region_insert_full_table(kd, vd):
 def test_region_insert_thresholds(monkeypatch):
     m = mx().StaticMap(1 << 22, -1, -1, torch.int64, torch.int64)
     cap = m.capacity()
-    assert m.insert_uses_region_build(cap // 4 + 1)
+    assert m.insert_uses_region_build(cap // 2 + 1)
     monkeypatch.setenv("MXCO_OA_REGION_INSERT", "0")
-    assert not m.insert_uses_region_build(cap // 4 + 1)
+    assert not m.insert_uses_region_build(cap // 2 + 1)
     monkeypatch.setenv("MXCO_OA_REGION_INSERT", "1")
-    assert not m.insert_uses_region_build(cap // 4 - 5)
+    assert not m.insert_uses_region_build(cap // 2 - 5)
     small = mx().StaticMap(1 << 20, -1, -1, torch.int64, torch.int64)
     assert not small.insert_uses_region_build((1 << 20) - 1)
     dh = mx().StaticMap(1 << 22, -1, -1, torch.int64, torch.int64, probing="double")
     assert not dh.insert_uses_region_build(1 << 22)
+    monkeypatch.delenv("MXCO_OA_REGION_INSERT_MIN_BYTES")
+    assert not m.insert_uses_region_build(cap // 2 + 1)  # 64 MiB of slots: below the default threshold
 
 
 def test_region_insert_async_and_graph_replays():
