@@ -53,6 +53,11 @@ struct mxco_oa {
     int inflight;
     uint32_t probe_flags;
     RegionWs* rws;
+    // No key can be in the slots: set by creation and Clear, cleared by every
+    // call that can add keys (Insert, InsertOrAssign, InsertAndFind, and Data,
+    // whose pointer device code may write through).  The region build then
+    // starts its regions empty instead of reading them.
+    mutable bool empty;
 };
 
 namespace {
@@ -258,7 +263,7 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
 }
 
 template <int KS, int VS>
-hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t s)
+hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t s, bool table_empty)
 {
     constexpr int SS = Lay<KS, VS>::SS;
     constexpr uint32_t R = BuildShape<SS>::kRegion;
@@ -306,7 +311,7 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     a2.heavy = w.heavy;
     a2.nreg = w.nreg;
     a2.bin_slots = R;
-    a2.load_region = 1;
+    a2.load_region = table_empty ? 0 : 1;  // an empty table's regions start from the empty slot
     a2.heavy_limit = 2 * R;
     build_regroup_kernel<KS, VS><<<w.nbins1, kBuildThreads, build_regroup_lds<SS>(w.group), s>>>(a2, w.part1,
                                                                                                w.offsets1, w.group);
@@ -317,15 +322,18 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t region_insert(mxco_oa* t, const OaArgs& a, hipStream_t s)
+hipError_t region_insert(mxco_oa* t, const OaArgs& a, hipStream_t s, bool table_empty)
 {
+    // a captured Insert may be replayed on a table that holds keys by then
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s && (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)) table_empty = false;
     const uint32_t ks = t->d.key_size, vs = t->d.value_size;
-    if (ks == 4 && vs == 4) return launch_region_insert<4, 4>(t, a, s);
-    if (ks == 8 && vs == 8) return launch_region_insert<8, 8>(t, a, s);
-    if (ks == 4 && vs == 8) return launch_region_insert<4, 8>(t, a, s);
-    if (ks == 8 && vs == 4) return launch_region_insert<8, 4>(t, a, s);
-    if (ks == 4 && vs == 0) return launch_region_insert<4, 0>(t, a, s);
-    if (ks == 8 && vs == 0) return launch_region_insert<8, 0>(t, a, s);
+    if (ks == 4 && vs == 4) return launch_region_insert<4, 4>(t, a, s, table_empty);
+    if (ks == 8 && vs == 8) return launch_region_insert<8, 8>(t, a, s, table_empty);
+    if (ks == 4 && vs == 8) return launch_region_insert<4, 8>(t, a, s, table_empty);
+    if (ks == 8 && vs == 4) return launch_region_insert<8, 4>(t, a, s, table_empty);
+    if (ks == 4 && vs == 0) return launch_region_insert<4, 0>(t, a, s, table_empty);
+    if (ks == 8 && vs == 0) return launch_region_insert<8, 0>(t, a, s, table_empty);
     return hipErrorInvalidValue;
 }
 
@@ -396,6 +404,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
         return hip_fail(e, "mxco_oa_create: counters");
     }
     mxco_status st = fill(t, stream);
+    t->empty = true;
     if (st == MXCO_OK) {
         e = hipStreamSynchronize(stream);
         if (e != hipSuccess) st = hip_fail(e, "mxco_oa_create: clear");
@@ -435,6 +444,7 @@ MXCO_API mxco_status mxco_oa_data(const mxco_oa* t, void** slots)
     if (mxco_status s = check_table(t, "Data")) return s;
     if (!slots) return fail(MXCO_INVALID_ARGUMENT, "Data: null");
     *slots = t->slots;
+    t->empty = false;
     return ok();
 }
 
@@ -459,6 +469,7 @@ MXCO_API mxco_status mxco_oa_clear(mxco_oa* t, hipStream_t stream, int sync)
 {
     if (mxco_status s = check_table(t, "Clear")) return s;
     if (mxco_status s = fill(t, stream)) return s;
+    t->empty = true;
     if (sync) MXCO_HIP(hipStreamSynchronize(stream), "Clear: hipStreamSynchronize");
     return ok();
 }
@@ -488,8 +499,10 @@ MXCO_API mxco_status mxco_oa_insert(mxco_oa* t, const void* pairs, const mxco_st
         (void)hipGetLastError();  // no room for the workspace: the CAS walk needs none
         region = false;
     }
+    const bool was_empty = t->empty;
+    t->empty = false;
     if (region) {
-        MXCO_HIP(region_insert(t, a, stream), "Insert");
+        MXCO_HIP(region_insert(t, a, stream, was_empty), "Insert");
     } else if (mxco_status s = launch(t, OP_INSERT, a, stream, "Insert")) {
         return s;
     }
@@ -523,6 +536,7 @@ MXCO_API mxco_status mxco_oa_insert_or_assign(mxco_oa* t, const void* pairs, uin
         return ok();
     }
     if (mxco_status s = check_batch(n, "InsertOrAssign")) return s;
+    t->empty = false;
     OaArgs a = base_args(t);
     a.in = (const uint8_t*)pairs;
     a.n = n;
@@ -564,6 +578,7 @@ MXCO_API mxco_status mxco_oa_insert_and_find(mxco_oa* t, const void* pairs, void
     if (n == 0 || !pairs) return ok();
     if (!out_found || !out_inserted) return fail(MXCO_INVALID_ARGUMENT, "InsertAndFind: null output");
     if (mxco_status s = check_batch(n, "InsertAndFind")) return s;
+    t->empty = false;
     OaArgs a = base_args(t);
     a.in = (const uint8_t*)pairs;
     a.out = (uint8_t*)out_found;

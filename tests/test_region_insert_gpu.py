@@ -166,6 +166,35 @@ def test_region_insert_full_table(kd, vd):
     assert np.array_equal(np.sort(k[found]), np.sort(got))
 
 
+def test_region_insert_after_clear_and_captured_insert_on_filled_table():
+    """An Insert right after Clear builds its regions from the empty slot; an
+    Insert captured on an empty table and replayed after other keys went in
+    still reads the regions, so those keys stay."""
+    import mxco
+    kd = vd = np.int64
+    rng = np.random.default_rng(7)
+    m = mxco.StaticMap(1 << 21, -1, -1, TD[kd], TD[vd])
+    cap = m.capacity()
+    a = distinct(rng, int(0.6 * cap), kd)
+    b = distinct(rng, 200000, kd, exclude=a)
+    pa, pb = dev_pairs(a, value_of(a, vd)), dev_pairs(b, value_of(b, vd))
+    assert m.insert(pb) == 0
+    m.clear()
+    assert m.insert_uses_region_build(len(a))
+    assert m.insert(pa) == 0  # after Clear: regions start empty
+    assert not m.contains(dev(b)).cpu().numpy().any()
+    assert np.array_equal(m.find(dev(a)).cpu().numpy(), value_of(a, vd))
+    m.clear()
+    g = mxco.Graph(lambda s: m.insert_async(pa, stream=s))  # captured while the table is empty
+    for rep in range(2):
+        m.clear()
+        assert m.insert(pb) == 0  # small batch: the CAS walk
+        g.launch()
+        g.stream.synchronize()
+        assert m.contains(dev(b)).cpu().numpy().all(), rep
+        assert np.array_equal(m.find(dev(a)).cpu().numpy(), value_of(a, vd)), rep
+
+
 def test_region_insert_thresholds(monkeypatch):
     m = mx().StaticMap(1 << 22, -1, -1, torch.int64, torch.int64)
     cap = m.capacity()
