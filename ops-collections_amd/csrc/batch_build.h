@@ -86,11 +86,11 @@ struct TileRef {
     uint32_t len;
 };
 
-template <int SS>
+template <int SS, bool kOne>
 __device__ __forceinline__ TileRef tile_ref(const BuildArgs& a, uint32_t b)
 {
     constexpr uint32_t kTile = BuildShape<SS>::kTile;
-    if (a.tiles) {
+    if constexpr (!kOne) {
         const BatchChunk tl = a.tiles[b];
         const mxco_batch_item it = a.items[tl.item];
         return TileRef{it.table, a.in + (it.offset + tl.start) * SS, tl.len};
@@ -129,17 +129,21 @@ __device__ __forceinline__ typename Lay<KS, VS>::Raw empty_slot(uint64_t ek, uin
     }
 }
 
+// kOne: the one-table Insert (implicit tiles, bins of a.bin_slots slots, a
+// two-level partition); otherwise a batched plan (bins are regions).
+
 // 1. Region histogram of one tile per workgroup.
-template <int KS, int VS>
+template <int KS, int VS, bool kOne>
 __global__ __launch_bounds__(kBuildThreads) void build_count_kernel(BuildArgs a)
 {
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
     using S = BuildShape<L::SS>;
     __shared__ uint32_t hist[kBuildMaxBins];
-    const TileRef tl = tile_ref<L::SS>(a, blockIdx.x);
+    const TileRef tl = tile_ref<L::SS, kOne>(a, blockIdx.x);
     const BatchTable t = a.tables[tl.table];
-    const uint32_t nbins = (t.cap + a.bin_slots - 1) / a.bin_slots;
+    const uint32_t bs = kOne ? a.bin_slots : S::kRegion;
+    const uint32_t nbins = (t.cap + bs - 1) / bs;
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x) hist[j] = 0;
     __syncthreads();
     const Raw* src = (const Raw*)tl.src;
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_count_kernel(BuildArgs a)
 #pragma unroll
     for (uint32_t u = 0; u < S::kPer; ++u) {
         const uint32_t j = threadIdx.x + u * kBuildThreads;
-        if (j < tl.len) atomicAdd(&hist[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / a.bin_slots], 1u);
+        if (j < tl.len) atomicAdd(&hist[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / bs], 1u);
     }
     __syncthreads();
     const uint32_t g0 = a.reg0[tl.table];
@@ -193,7 +197,7 @@ constexpr size_t build_scatter_lds(uint32_t maxbins)
 
 // 3. One tile per workgroup: sort its pairs by region in LDS, claim a run in
 // every region it touches (counts count down to zero) and write the runs.
-template <int KS, int VS>
+template <int KS, int VS, bool kOne>
 __global__ __launch_bounds__(kBuildThreads) void build_scatter_kernel(BuildArgs a)
 {
     using L = Lay<KS, VS>;
@@ -208,9 +212,10 @@ __global__ __launch_bounds__(kBuildThreads) void build_scatter_kernel(BuildArgs 
     typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
     __shared__ typename Scan::TempStorage scan_tmp;
 
-    const TileRef tl = tile_ref<L::SS>(a, blockIdx.x);
+    const TileRef tl = tile_ref<L::SS, kOne>(a, blockIdx.x);
     const BatchTable t = a.tables[tl.table];
-    const uint32_t nbins = (t.cap + a.bin_slots - 1) / a.bin_slots;
+    const uint32_t bs = kOne ? a.bin_slots : S::kRegion;
+    const uint32_t nbins = (t.cap + bs - 1) / bs;
     const uint32_t g0 = a.reg0[tl.table];
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x) tcnt[j] = 0;
     __syncthreads();
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_scatter_kernel(BuildArgs 
     for (uint32_t u = 0; u < S::kPer; ++u) {
         const uint32_t j = threadIdx.x + u * kBuildThreads;
         if (j < tl.len) {
-            bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / a.bin_slots;
+            bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / bs;
             rank[u] = atomicAdd(&tcnt[bin[u]], 1u);
         }
     }
@@ -457,7 +462,7 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
 }
 
 // 4. One region per workgroup: build it in LDS and write it once.
-template <int KS, int VS>
+template <int KS, int VS, bool kOne>
 __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
 {
     using L = Lay<KS, VS>;
@@ -467,13 +472,13 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     __shared__ __align__(16) Raw ls[R];
     __shared__ uint32_t s_nov, s_fail;
     const uint32_t r = blockIdx.x;
-    const uint32_t ti = a.reg_table ? a.reg_table[r] : 0u;
+    const uint32_t ti = kOne ? 0u : a.reg_table[r];
     const BatchTable t = a.tables[ti];
     const uint32_t r0 = (r - a.reg0[ti]) * R;
     const uint32_t rlen = min(R, t.cap - r0);
     const bool whole = rlen == t.cap;  // a one-region table: the walk wraps inside the region
     const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
-    if (e - s > a.heavy_limit) {
+    if (kOne && e - s > a.heavy_limit) {
         // left untouched here; build_heavy_kernel inserts its pairs by the CAS walk
         if (threadIdx.x == 0) {
             a.ovc[r] = 0;
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     const uint64_t ekey = a.empty_key;
     uint8_t* dst = t.slots + (uint64_t)r0 * SS;
     const uint32_t bytes = rlen * SS, n16 = bytes / 16;
-    if (a.load_region) {
+    if (kOne && a.load_region) {
         for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)ls)[j] = ((const u64x2*)dst)[j];
         for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ls[j] = ((const Raw*)dst)[j];
     } else {
@@ -497,7 +502,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     }
     Raw* part = (Raw*)a.part;
     uint32_t fails = 0;
-    if constexpr (R <= kPlaceMax) {
+    if constexpr (kOne && R <= kPlaceMax) {
         // an empty region, integer keys: placed without walks
         bool empty_region = true;
         if (a.load_region)
@@ -595,7 +600,7 @@ __device__ __forceinline__ bool cas_walk(const BuildArgs& a, const BatchTable& t
 
 // 5. Keys whose walk left their region: the CAS walk on the table, resumed
 // at the slot after the region.  One wavefront per region.
-template <int KS, int VS>
+template <int KS, int VS, bool kOne>
 __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
 {
     using L = Lay<KS, VS>;
@@ -607,7 +612,7 @@ __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
     for (uint32_t r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < a.nreg; r += waves) {
         const uint32_t n = a.ovc[r];
         if (n == 0) continue;
-        const uint32_t ti = a.reg_table ? a.reg_table[r] : 0u;
+        const uint32_t ti = kOne ? 0u : a.reg_table[r];
         const BatchTable t = a.tables[ti];
         const uint32_t r0 = (r - a.reg0[ti]) * R;
         const uint32_t rend = r0 + min(R, t.cap - r0);
@@ -638,7 +643,7 @@ __global__ __launch_bounds__(256) void build_heavy_kernel(BuildArgs a)
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint32_t h = 0; h < nh; ++h) {
         const uint32_t r = a.heavy[1 + h];
-        const uint32_t ti = a.reg_table ? a.reg_table[r] : 0u;
+        const uint32_t ti = 0;  // the one-table Insert only
         const BatchTable t = a.tables[ti];
         const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
         const Raw* part = (const Raw*)a.part;

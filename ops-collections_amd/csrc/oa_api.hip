@@ -268,7 +268,7 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     constexpr int SS = Lay<KS, VS>::SS;
     constexpr uint32_t R = BuildShape<SS>::kRegion;
     static const hipError_t attr = [] {
-        hipError_t e = hipFuncSetAttribute((const void*)build_scatter_kernel<KS, VS>,
+        hipError_t e = hipFuncSetAttribute((const void*)build_scatter_kernel<KS, VS, true>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)build_scatter_lds<SS>(kBuildMaxBins));
         if (e == hipSuccess)
@@ -301,9 +301,9 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     a1.hash = oa.hash;
     a1.key_float = oa.key_float;
     const unsigned ntiles = (unsigned)((n + BuildShape<SS>::kTile - 1) / BuildShape<SS>::kTile);
-    build_count_kernel<KS, VS><<<ntiles, kBuildThreads, 0, s>>>(a1);
+    build_count_kernel<KS, VS, true><<<ntiles, kBuildThreads, 0, s>>>(a1);
     build_scan_kernel<<<1, 1024, 0, s>>>(w.counts1, w.offsets1, w.nbins1);
-    build_scatter_kernel<KS, VS><<<ntiles, kBuildThreads, build_scatter_lds<SS>(w.nbins1), s>>>(a1);
+    build_scatter_kernel<KS, VS, true><<<ntiles, kBuildThreads, build_scatter_lds<SS>(w.nbins1), s>>>(a1);
     BuildArgs a2 = a1;
     a2.part = w.part2;
     a2.offsets = w.offsets2;
@@ -315,9 +315,9 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     a2.heavy_limit = 2 * R;
     build_regroup_kernel<KS, VS><<<w.nbins1, kBuildThreads, build_regroup_lds<SS>(w.group), s>>>(a2, w.part1,
                                                                                                w.offsets1, w.group);
-    build_apply_kernel<KS, VS><<<w.nreg, kBuildThreads, 0, s>>>(a2);
+    build_apply_kernel<KS, VS, true><<<w.nreg, kBuildThreads, 0, s>>>(a2);
     const unsigned ogrid = std::max(1u, std::min<unsigned>((w.nreg + 3) / 4, (unsigned)cu_count() * 8));
-    build_overflow_kernel<KS, VS><<<ogrid, 256, 0, s>>>(a2);
+    build_overflow_kernel<KS, VS, true><<<ogrid, 256, 0, s>>>(a2);
     build_heavy_kernel<KS, VS><<<(unsigned)cu_count() * 4, 256, 0, s>>>(a2);
     return hipGetLastError();
 }
