@@ -197,7 +197,7 @@ hipError_t launch_build(const BuildArgs& a, uint32_t ntiles, hipStream_t s)
     if (ntiles) build_count_kernel<KS, VS, false><<<ntiles, kBuildThreads, 0, s>>>(a);
     build_scan_kernel<<<1, 1024, 0, s>>>(a.counts, a.offsets, a.nreg);
     if (ntiles) build_scatter_kernel<KS, VS, false><<<ntiles, kBuildThreads, build_scatter_lds<SS>(a.maxbins), s>>>(a);
-    build_apply_kernel<KS, VS, false><<<a.nreg, kBuildThreads, 0, s>>>(a);
+    build_apply_kernel<KS, VS, false, false><<<a.nreg, kBuildThreads, 0, s>>>(a);
     const unsigned waves_grid = std::max(1u, std::min<unsigned>((a.nreg + 3) / 4, (unsigned)cu_count() * 8));
     build_overflow_kernel<KS, VS, false><<<waves_grid, 256, 0, s>>>(a);
     return hipGetLastError();

@@ -29,8 +29,9 @@
 // levels: count / scan / scatter group the pairs by runs of `group` regions,
 // and build_regroup_kernel regroups each run by region.  The apply then loads
 // each region from the table (the table may hold keys and tombstones
-// already), and a region with more than `heavy_limit` pairs (a skewed stream
-// of duplicates) is left to the CAS walk of build_heavy_kernel.
+// already, unless it is known to be empty), and a region with more than
+// `heavy_limit` pairs (a skewed stream of duplicates) is left to the CAS walk
+// of build_heavy_kernel: the regroup gives the apply an empty range for it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -66,14 +67,14 @@ struct BuildArgs {
     uint32_t* offsets;          // exclusive sum of the counts (nbins + 1)
     uint32_t* ovc;              // overflow pairs of each region, kept at its offset
     uint32_t* heavy;            // [0] count, then the regions left to the CAS walk
+    uint32_t* apply_end;        // one-table: end of each region's pairs for the apply (its start for a hot region)
     unsigned long long* counters;  // 2 per item: [0] failed inserts (null: not counted)
     uint64_t empty_key, empty_val;
     uint64_t single_n;
     uint32_t nreg;
     uint32_t maxbins;     // most bins of a table with an item (stride of the scatter's LDS arrays)
     uint32_t bin_slots;   // slots per bin of count / scan / scatter (a multiple of the region)
-    uint32_t load_region; // apply: start from the table's slots instead of empty ones
-    uint32_t heavy_limit; // apply: regions with more pairs go to build_heavy_kernel
+    uint32_t heavy_limit; // regroup: regions with more pairs go to build_heavy_kernel
     uint32_t bucket;
     uint32_t hash;
     uint32_t key_float;
@@ -462,7 +463,9 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
 }
 
 // 4. One region per workgroup: build it in LDS and write it once.
-template <int KS, int VS, bool kOne>
+// kLoad: start from the table's slots (otherwise from the empty slot: a
+// batched plan, or a table known to be empty).
+template <int KS, int VS, bool kOne, bool kLoad>
 __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
 {
     using L = Lay<KS, VS>;
@@ -477,19 +480,11 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     const uint32_t r0 = (r - a.reg0[ti]) * R;
     const uint32_t rlen = min(R, t.cap - r0);
     const bool whole = rlen == t.cap;  // a one-region table: the walk wraps inside the region
-    const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
-    if (kOne && e - s > a.heavy_limit) {
-        // left untouched here; build_heavy_kernel inserts its pairs by the CAS walk
-        if (threadIdx.x == 0) {
-            a.ovc[r] = 0;
-            a.heavy[1 + atomicAdd(&a.heavy[0], 1u)] = r;
-        }
-        return;
-    }
+    const uint32_t s = a.offsets[r], e = kOne ? a.apply_end[r] : a.offsets[r + 1];
     const uint64_t ekey = a.empty_key;
     uint8_t* dst = t.slots + (uint64_t)r0 * SS;
     const uint32_t bytes = rlen * SS, n16 = bytes / 16;
-    if (kOne && a.load_region) {
+    if constexpr (kLoad) {
         for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)ls)[j] = ((const u64x2*)dst)[j];
         for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ls[j] = ((const Raw*)dst)[j];
     } else {
@@ -505,7 +500,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     if constexpr (kOne && R <= kPlaceMax) {
         // an empty region, integer keys: placed without walks
         bool empty_region = true;
-        if (a.load_region)
+        if constexpr (kLoad)
             for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) empty_region &= L::key(ls[j]) == ekey;
         if (__syncthreads_and(empty_region && !a.key_float && e - s <= kPlaceMax) &&
             place_sorted<KS, VS>(a, t, r0, rlen, s, e, ls, part, s_nov, fails)) {
@@ -703,6 +698,9 @@ __global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs 
         for (uint32_t j = 0; j < nr; ++j) {
             rcur[j] = o;
             a.offsets[rfirst + j] = o;
+            const bool hot = rcnt[j] > a.heavy_limit;
+            a.apply_end[rfirst + j] = hot ? o : o + rcnt[j];
+            if (hot) a.heavy[1 + atomicAdd(&a.heavy[0], 1u)] = rfirst + j;
             o += rcnt[j];
         }
         if (rfirst + nr == a.nreg) a.offsets[a.nreg] = e;

@@ -29,6 +29,7 @@ struct RegionWs {
     uint32_t* offsets2;  // nreg + 1
     uint32_t* ovc;       // nreg
     uint32_t* heavy;     // nreg + 1
+    uint32_t* apply_end; // nreg
     BatchTable* table;   // the table's descriptor
     uint32_t* reg0;      // {0, nbins1}
     uint32_t nreg, nbins1, group;
@@ -189,7 +190,7 @@ void free_region_ws(mxco_oa* t)
     RegionWs* w = t->rws;
     if (!w) return;
     for (void* p : {(void*)w->part1, (void*)w->part2, (void*)w->counts1, (void*)w->offsets1, (void*)w->offsets2,
-                    (void*)w->ovc, (void*)w->heavy, (void*)w->table, (void*)w->reg0})
+                    (void*)w->ovc, (void*)w->heavy, (void*)w->apply_end, (void*)w->table, (void*)w->reg0})
         if (p) (void)hipFree(p);
     delete w;
     t->rws = nullptr;
@@ -235,6 +236,7 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         if (e == hipSuccess) e = hipMalloc((void**)&w->offsets2, (w->nreg + 1ull) * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&w->ovc, w->nreg * 4ull);
         if (e == hipSuccess) e = hipMalloc((void**)&w->heavy, (w->nreg + 1ull) * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->apply_end, w->nreg * 4ull);
         if (e == hipSuccess) e = hipMalloc((void**)&w->table, sizeof(BatchTable));
         if (e == hipSuccess) e = hipMalloc((void**)&w->reg0, 2 * sizeof(uint32_t));
         if (e == hipSuccess) {
@@ -309,13 +311,17 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     a2.offsets = w.offsets2;
     a2.ovc = w.ovc;
     a2.heavy = w.heavy;
+    a2.apply_end = w.apply_end;
     a2.nreg = w.nreg;
     a2.bin_slots = R;
-    a2.load_region = table_empty ? 0 : 1;  // an empty table's regions start from the empty slot
     a2.heavy_limit = 2 * R;
     build_regroup_kernel<KS, VS><<<w.nbins1, kBuildThreads, build_regroup_lds<SS>(w.group), s>>>(a2, w.part1,
                                                                                                w.offsets1, w.group);
-    build_apply_kernel<KS, VS, true><<<w.nreg, kBuildThreads, 0, s>>>(a2);
+    // an empty table's regions start from the empty slot
+    if (table_empty)
+        build_apply_kernel<KS, VS, true, false><<<w.nreg, kBuildThreads, 0, s>>>(a2);
+    else
+        build_apply_kernel<KS, VS, true, true><<<w.nreg, kBuildThreads, 0, s>>>(a2);
     const unsigned ogrid = std::max(1u, std::min<unsigned>((w.nreg + 3) / 4, (unsigned)cu_count() * 8));
     build_overflow_kernel<KS, VS, true><<<ogrid, 256, 0, s>>>(a2);
     build_heavy_kernel<KS, VS><<<(unsigned)cu_count() * 4, 256, 0, s>>>(a2);
