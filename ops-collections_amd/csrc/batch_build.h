@@ -141,26 +141,35 @@ __global__ __launch_bounds__(kBuildThreads) void build_count_kernel(BuildArgs a)
     using Raw = typename L::Raw;
     using S = BuildShape<L::SS>;
     __shared__ uint32_t hist[kBuildMaxBins];
-    const TileRef tl = tile_ref<L::SS, kOne>(a, blockIdx.x);
-    const BatchTable t = a.tables[tl.table];
+    // a batched plan: one tile per workgroup (tiles of different tables);
+    // one table: tiles blockIdx.x, blockIdx.x + gridDim.x, ... into one
+    // histogram, added once (every tile shares the bins, so per-tile adds
+    // would contend on the same counters)
+    const uint32_t ntiles = kOne ? (uint32_t)((a.single_n + S::kTile - 1) / S::kTile) : blockIdx.x + 1;
+    const BatchTable t = a.tables[kOne ? 0u : tile_ref<L::SS, kOne>(a, blockIdx.x).table];
     const uint32_t bs = kOne ? a.bin_slots : S::kRegion;
     const uint32_t nbins = (t.cap + bs - 1) / bs;
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x) hist[j] = 0;
     __syncthreads();
-    const Raw* src = (const Raw*)tl.src;
-    Raw p[S::kPer];
+    uint32_t table = 0;
+    for (uint32_t b = blockIdx.x; b < ntiles; b += kOne ? gridDim.x : 1u) {
+        const TileRef tl = tile_ref<L::SS, kOne>(a, b);
+        table = tl.table;
+        const Raw* src = (const Raw*)tl.src;
+        Raw p[S::kPer];
 #pragma unroll
-    for (uint32_t u = 0; u < S::kPer; ++u) {
-        const uint32_t j = threadIdx.x + u * kBuildThreads;
-        if (j < tl.len) p[u] = src[j];
-    }
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < tl.len) p[u] = src[j];
+        }
 #pragma unroll
-    for (uint32_t u = 0; u < S::kPer; ++u) {
-        const uint32_t j = threadIdx.x + u * kBuildThreads;
-        if (j < tl.len) atomicAdd(&hist[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / bs], 1u);
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < tl.len) atomicAdd(&hist[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / bs], 1u);
+        }
     }
     __syncthreads();
-    const uint32_t g0 = a.reg0[tl.table];
+    const uint32_t g0 = a.reg0[table];
     for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x)
         if (hist[j]) atomicAdd(&a.counts[g0 + j], hist[j]);
 }

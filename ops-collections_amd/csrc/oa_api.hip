@@ -303,7 +303,7 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     a1.hash = oa.hash;
     a1.key_float = oa.key_float;
     const unsigned ntiles = (unsigned)((n + BuildShape<SS>::kTile - 1) / BuildShape<SS>::kTile);
-    build_count_kernel<KS, VS, true><<<ntiles, kBuildThreads, 0, s>>>(a1);
+    build_count_kernel<KS, VS, true><<<std::min<unsigned>(ntiles, (unsigned)cu_count() * 4), kBuildThreads, 0, s>>>(a1);
     build_scan_kernel<<<1, 1024, 0, s>>>(w.counts1, w.offsets1, w.nbins1);
     build_scatter_kernel<KS, VS, true><<<ntiles, kBuildThreads, build_scatter_lds<SS>(w.nbins1), s>>>(a1);
     BuildArgs a2 = a1;
