@@ -29,9 +29,11 @@
 // levels: count / scan / scatter group the pairs by runs of `group` regions,
 // and build_regroup_kernel regroups each run by region.  The apply then loads
 // each region from the table (the table may hold keys and tombstones
-// already, unless it is known to be empty), and a region with more than
-// `heavy_limit` pairs (a skewed stream of duplicates) is left to the CAS walk
-// of build_heavy_kernel: the regroup gives the apply an empty range for it.
+// already, unless it is known to be empty).  An empty region of at most
+// kPlaceMax pairs with integer keys is placed without walks (place_sorted);
+// others walk as above.  A region with more than `heavy_limit` pairs (a
+// skewed stream of duplicates) is left to the CAS walk of build_heavy_kernel:
+// the regroup gives the apply an empty range for it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -43,7 +45,7 @@ namespace mxco {
 
 constexpr uint32_t kBuildBytes = 65536;    // slot bytes of one region (held in LDS)
 constexpr uint32_t kBuildThreads = 1024;   // count / scatter / apply workgroups
-constexpr uint32_t kBuildMaxBins = 4096;   // regions of one table
+constexpr uint32_t kBuildMaxBins = 4096;   // bins of one table in count / scatter
 constexpr uint32_t kBuildTileBytes = 65536;  // pair bytes of one scatter tile
 
 template <int SS>
