@@ -246,7 +246,7 @@ mxco_status run_build(mxco_batch_plan* p, const void* in, hipStream_t s)
     a.empty_val = b->empty_val;
     a.nreg = p->nreg;
     a.maxbins = p->maxbins;
-    a.bin_slots = kBuildBytes / b->ss;  // a bin is one region
+    a.bin_slots = build_region_slots(b->ss);  // a bin is one region
     a.heavy_limit = 0xffffffffu;
     a.bucket = b->d.bucket_size;
     a.hash = b->d.hash;
@@ -263,7 +263,7 @@ hipError_t prepare_build(mxco_batch_plan* p, const mxco_batch_item* items)
     mxco_batch* b = p->batch;
     static const int enabled = env_int("MXCO_BATCH_REGION_BUILD", 1);
     if (!enabled || b->d.probing != 0) return hipSuccess;
-    const uint32_t R = kBuildBytes / b->ss;
+    const uint32_t R = build_region_slots(b->ss);
     const uint32_t tile = kBuildTileBytes / b->ss;
     std::vector<int32_t> table_item(b->ntables, -1);
     std::vector<uint32_t> reg0(b->ntables + 1);

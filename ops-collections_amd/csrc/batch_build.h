@@ -38,19 +38,28 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "batch_kernels.h"
 #include "oa_kernels.h"
 
 namespace mxco {
 
-constexpr uint32_t kBuildBytes = 65536;    // slot bytes of one region (held in LDS)
+constexpr uint32_t kBuildBytes = 65536;    // most slot bytes of one region (held in LDS)
+constexpr uint32_t kRegionSlots = 4096;    // most slots of one region
 constexpr uint32_t kBuildThreads = 1024;   // count / scatter / apply workgroups
 constexpr uint32_t kBuildMaxBins = 4096;   // bins of one table in count / scatter
 constexpr uint32_t kBuildTileBytes = 65536;  // pair bytes of one scatter tile
 
+// Slots of one region for slot size ss (BuildShape<ss>::kRegion on the host).
+constexpr uint32_t build_region_slots(uint32_t ss)
+{
+    return kBuildBytes / ss < kRegionSlots ? kBuildBytes / ss : kRegionSlots;
+}
+
 template <int SS>
 struct BuildShape {
-    static constexpr uint32_t kRegion = kBuildBytes / SS;     // slots per region
+    static constexpr uint32_t kRegion = kBuildBytes / SS < kRegionSlots ? kBuildBytes / SS : kRegionSlots;  // slots per region
     static constexpr uint32_t kTile = kBuildTileBytes / SS;   // pairs per tile
     static constexpr uint32_t kPer = kTile / kBuildThreads;   // pairs per thread of a tile
 };
@@ -311,7 +320,8 @@ __device__ __forceinline__ uint64_t claim_slot(typename Lay<KS, VS>::Raw* s, typ
 }
 
 // Placement without walks (an empty region of at most kPlaceMax slots, at
-// most kPlaceMax pairs, integer keys).  Inserting keys one after another in
+// most kPlaceMax pairs, integer keys; the sort scratch, keys of KS bytes and
+// two 16-bit words per pair, must fit the region's LDS).  Inserting keys one after another in
 // the order of their first slots, each walk ends at the first slot after the
 // previous key's that is not below its own first slot: the i-th kept key
 // lands at slot i + max_{j <= i}(first_j - j), a prefix maximum.  The pairs
@@ -321,6 +331,12 @@ __device__ __forceinline__ uint64_t claim_slot(typename Lay<KS, VS>::Raw* s, typ
 // overflow kernel.  Returns false, with ls refilled with the empty slot, when
 // a first-slot group exceeds kPlaceGroup pairs; the caller then walks.
 constexpr uint32_t kPlaceMax = 4096;
+
+template <int KS, int SS>
+constexpr bool place_fits()
+{
+    return BuildShape<SS>::kRegion <= kPlaceMax && kPlaceMax * (KS + 2 + 2) <= BuildShape<SS>::kRegion * SS;
+}
 constexpr uint32_t kPlaceGroup = 64;
 
 template <int KS, int VS>
@@ -332,12 +348,13 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
     using Raw = typename L::Raw;
     constexpr uint32_t SS = L::SS;
     constexpr uint32_t kPer = kPlaceMax / kBuildThreads;  // pairs (and sorted positions) per thread
-    static_assert(kPlaceMax * (8 + 2 + 2) <= BuildShape<SS>::kRegion * SS, "sort scratch fits the region's LDS");
+    using SKey = typename std::conditional<KS == 4, uint32_t, uint64_t>::type;
+    static_assert(place_fits<KS, SS>(), "sort scratch fits the region's LDS");
     typedef hipcub::BlockScan<int32_t, kBuildThreads> Scan;
     __shared__ typename Scan::TempStorage scan_tmp;
     __shared__ uint32_t hcnt[kPlaceMax / 2];  // two 16-bit counts per word: pairs per first slot, then group starts
     __shared__ uint32_t s_big;
-    uint64_t* skey = (uint64_t*)ls;               // sorted keys
+    SKey* skey = (SKey*)ls;                       // sorted keys
     uint16_t* shome = (uint16_t*)(skey + kPlaceMax);  // sorted first slots | 0x8000 for a kept pair
     uint16_t* sq = shome + kPlaceMax;             // slot of each kept sorted pair, 0xffff past the region
     const uint64_t ekey = a.empty_key;
@@ -404,7 +421,7 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
         if (!valid[u]) continue;
         const uint32_t h = home[u];
         si[u] = ((hcnt[h >> 1] >> ((h & 1u) * 16u)) & 0xffffu) + rank[u];
-        skey[si[u]] = L::key(p[u]);
+        skey[si[u]] = (SKey)L::key(p[u]);
     }
     __syncthreads();
     // duplicates: an equal key earlier in the same first-slot group
@@ -413,7 +430,7 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
         if (!valid[u]) continue;
         const uint32_t h = home[u];
         const uint32_t g0 = (hcnt[h >> 1] >> ((h & 1u) * 16u)) & 0xffffu;
-        const uint64_t key = skey[si[u]];
+        const SKey key = skey[si[u]];
         bool dup = false;
         for (uint32_t j = g0; j < si[u]; ++j) dup |= skey[j] == key;
         shome[si[u]] = (uint16_t)(h | (dup ? 0u : 0x8000u));
@@ -508,7 +525,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
     }
     Raw* part = (Raw*)a.part;
     uint32_t fails = 0;
-    if constexpr (kOne && R <= kPlaceMax) {
+    if constexpr (place_fits<KS, SS>()) {
         // an empty region, integer keys: placed without walks
         bool empty_region = true;
         if constexpr (kLoad)

@@ -225,7 +225,7 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         RegionWs* w = new (std::nothrow) RegionWs();
         if (!w) return hipErrorOutOfMemory;
         t->rws = w;
-        const uint32_t R = kBuildBytes / t->ss;
+        const uint32_t R = build_region_slots(t->ss);
         w->nreg = (uint32_t)((t->cap + R - 1) / R);
         // about 512 bins: runs of a line or more per bin and 4096-pair tile
         // in the scatter; at most 256 regions per bin (the regroup's LDS)

@@ -25,7 +25,8 @@ def slot_size(kd, vd):
 
 
 def region_slots(kd, vd):
-    return 65536 // slot_size(kd, vd)
+    """Slots of one region: 64 KiB of slots, at most 4096."""
+    return min(65536 // slot_size(kd, vd), 4096)
 
 
 def value_of(keys, vd):
