@@ -216,6 +216,16 @@ constexpr size_t build_scatter_lds(uint32_t maxbins)
     return (size_t)BuildShape<SS>::kTile * SS + (size_t)BuildShape<SS>::kTile * 2 + (size_t)maxbins * 12;
 }
 
+// Most bins of count / scatter with slot size ss: the count kernel's LDS
+// histogram, and the scatter's LDS of 160 KiB less 8 KiB for its static scan
+// storage (4096 for every slot size).
+constexpr uint32_t build_max_bins(uint32_t ss)
+{
+    const uint32_t tile = kBuildTileBytes / ss;
+    const uint32_t fit = (163840u - 8192u - tile * ss - tile * 2) / 12;
+    return fit < kBuildMaxBins ? fit : kBuildMaxBins;
+}
+
 // 3. One tile per workgroup: sort its pairs by region in LDS, claim a run in
 // every region it touches (counts count down to zero) and write the runs.
 template <int KS, int VS, bool kOne>

@@ -205,6 +205,22 @@ void free_region_ws(mxco_oa* t)
 // 0.8 on a table that misses the cache, less on one that fits it or for a
 // stream of repeated keys, so those keep the walk.  Under stream capture the
 // workspace must already be large enough.
+// Regions, regions per first-level bin and first-level bins of the region
+// build: about 512 bins, at most 256 regions per bin.
+struct RegionShape {
+    uint32_t nreg, group, nbins1;
+};
+
+RegionShape region_shape(const mxco_oa* t)
+{
+    const uint32_t R = build_region_slots(t->ss);
+    RegionShape r;
+    r.nreg = (uint32_t)((t->cap + R - 1) / R);
+    r.group = std::min<uint32_t>(256u, (r.nreg + 511) / 512);
+    r.nbins1 = (r.nreg + r.group - 1) / r.group;
+    return r;
+}
+
 bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream_t stream)
 {
     const char* env = getenv("MXCO_OA_REGION_INSERT");
@@ -212,6 +228,7 @@ bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream
     if (n < (1ull << 20) || n * 2 < t->cap || n >= 0xffffffffull) return false;
     const char* mb = getenv("MXCO_OA_REGION_INSERT_MIN_BYTES");
     if (t->cap * t->ss < (mb ? strtoull(mb, nullptr, 10) : (256ull << 20))) return false;
+    if (region_shape(t).nbins1 > build_max_bins(t->ss)) return false;  // the scatter's LDS
     // the null stream is never captured
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (stream && hipStreamIsCapturing(stream, &cs) != hipSuccess) return false;
@@ -225,12 +242,12 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         RegionWs* w = new (std::nothrow) RegionWs();
         if (!w) return hipErrorOutOfMemory;
         t->rws = w;
-        const uint32_t R = build_region_slots(t->ss);
-        w->nreg = (uint32_t)((t->cap + R - 1) / R);
         // about 512 bins: runs of a line or more per bin and 4096-pair tile
         // in the scatter; at most 256 regions per bin (the regroup's LDS)
-        w->group = std::min<uint32_t>(256u, (w->nreg + 511) / 512);
-        w->nbins1 = (w->nreg + w->group - 1) / w->group;
+        const RegionShape rs = region_shape(t);
+        w->nreg = rs.nreg;
+        w->group = rs.group;
+        w->nbins1 = rs.nbins1;
         hipError_t e = hipMalloc((void**)&w->counts1, (w->nbins1 + 1ull) * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&w->offsets1, (w->nbins1 + 1ull) * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&w->offsets2, (w->nreg + 1ull) * 4);
