@@ -69,10 +69,10 @@ struct BuildArgs {
     uint8_t* part;      // the pairs grouped by bin (total * SS bytes)
     const BatchTable* tables;
     const mxco_batch_item* items;
-    const BatchChunk* tiles;    // tiles of one item each, at most kTile pairs; null: one table,
-                                // pairs [0, single_n) of `in` cut into whole tiles
+    const BatchChunk* tiles;    // batched plans: tiles of one item each, at most kTile pairs (the
+                                // one-table Insert cuts pairs [0, single_n) of `in` into tiles)
     const uint32_t* reg0;       // first bin of each table (num_tables + 1)
-    const uint32_t* reg_table;  // table of each region (null: table 0)
+    const uint32_t* reg_table;  // batched plans: table of each region (the one-table Insert: table 0)
     const int32_t* table_item;  // item of each table, or -1 (null: item 0)
     uint32_t* counts;           // per bin; zero again after the scatter
     uint32_t* offsets;          // exclusive sum of the counts (nbins + 1)
@@ -84,7 +84,8 @@ struct BuildArgs {
     uint64_t single_n;
     uint32_t nreg;
     uint32_t maxbins;     // most bins of a table with an item (stride of the scatter's LDS arrays)
-    uint32_t bin_slots;   // slots per bin of count / scan / scatter (a multiple of the region)
+    uint32_t bin_slots;   // one-table Insert: slots per bin of count / scatter (a multiple of the region;
+                          // batched plans bin by region)
     uint32_t heavy_limit; // regroup: regions with more pairs go to build_heavy_kernel
     uint32_t bucket;
     uint32_t hash;
