@@ -10,8 +10,10 @@
 //    probe kernel), queued on the map stream without waiting for earlier
 //    ticks -- requests that arrive while a tick runs are admitted into the
 //    next one as tick buffers free up;
-//  - StaticMultimap requests run in arrival order on a second stream by a
-//    worker thread, overlapping the map ticks on the device;
+//  - StaticMultimap requests run on kMmWorkers worker threads, each with
+//    its own stream, overlapping the map ticks and each other on the device;
+//    a table's requests all go to worker `table % kMmWorkers`, in arrival
+//    order;
 //  - a completion thread retires ticks in order (event wait), stamps each
 //    request's completion time and publishes its result.
 // Per table, requests take effect in arrival order: a tick takes a table's
@@ -63,6 +65,7 @@ struct Table {
 };
 
 constexpr int kTickSlots = 4;
+constexpr int kMmWorkers = 4;  // StaticMultimap worker threads (and streams)
 
 struct Tick {
     // One upload per tick: [items | chunks | zeroed insert counters].
@@ -89,13 +92,13 @@ struct mxco_server {
     std::mutex mu;
     std::condition_variable cv_pending, cv_done, cv_ticks, cv_mm;
     std::deque<Req*> pending;
-    std::deque<Req*> mm_pending;
+    std::deque<Req*> mm_pending[kMmWorkers];
     std::unordered_map<uint64_t, Req*> reqs;
     uint64_t next_ticket = 1;
     Tick ticks[kTickSlots];
     std::deque<Tick*> inflight;
-    hipStream_t map_stream = nullptr, mm_stream = nullptr;
-    std::thread dispatcher, completer, mm_worker;
+    hipStream_t map_stream = nullptr, mm_streams[kMmWorkers] = {};
+    std::thread dispatcher, completer, mm_workers[kMmWorkers];
     bool stop = false;
     std::string worker_error;
     // statistics
@@ -249,7 +252,7 @@ void dispatcher_loop(mxco_server* s)
         for (Req* r : s->pending) {
             const uint32_t tb = r->r.table;
             if (!is_map_op(r->r.op)) {
-                s->mm_pending.push_back(r);
+                s->mm_pending[tb % kMmWorkers].push_back(r);
                 continue;
             }
             int& ph = phase[tb];
@@ -265,7 +268,7 @@ void dispatcher_loop(mxco_server* s)
             t->reqs.push_back(r);
         }
         s->pending.swap(keep);
-        s->cv_mm.notify_one();
+        s->cv_mm.notify_all();
         if (t->reqs.empty()) continue;
         t->busy = true;
         ++s->n_ticks;
@@ -317,29 +320,31 @@ void completer_loop(mxco_server* s)
     }
 }
 
-// Multimap requests, in arrival order, on the second stream.
-void mm_loop(mxco_server* s)
+// Multimap requests of worker w's tables, in arrival order, on its stream.
+void mm_loop(mxco_server* s, int w)
 {
     (void)hipSetDevice(s->device);
+    hipStream_t stream = s->mm_streams[w];
+    std::deque<Req*>& queue = s->mm_pending[w];
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        s->cv_mm.wait(lk, [&] { return s->stop || !s->mm_pending.empty(); });
-        if (s->mm_pending.empty()) {
+        s->cv_mm.wait(lk, [&] { return s->stop || !queue.empty(); });
+        if (queue.empty()) {
             if (s->stop) break;
             continue;
         }
-        Req* r = s->mm_pending.front();
-        s->mm_pending.pop_front();
+        Req* r = queue.front();
+        queue.pop_front();
         mxco_mm* mm = s->tables[r->r.table].mm;
         lk.unlock();
         uint64_t result = 0;
         mxco_status st = MXCO_OK;
         switch (r->r.op) {
-        case MXCO_REQ_MM_INSERT: st = mxco_mm_insert(mm, r->r.in, nullptr, r->r.n, s->mm_stream, &result); break;
-        case MXCO_REQ_MM_FIND: st = mxco_mm_find(mm, r->r.in, nullptr, r->r.out, r->r.n, s->mm_stream); break;
-        case MXCO_REQ_MM_COUNT: st = mxco_mm_count(mm, r->r.in, r->r.n, s->mm_stream, &result); break;
+        case MXCO_REQ_MM_INSERT: st = mxco_mm_insert(mm, r->r.in, nullptr, r->r.n, stream, &result); break;
+        case MXCO_REQ_MM_FIND: st = mxco_mm_find(mm, r->r.in, nullptr, r->r.out, r->r.n, stream); break;
+        case MXCO_REQ_MM_COUNT: st = mxco_mm_count(mm, r->r.in, r->r.n, stream, &result); break;
         case MXCO_REQ_MM_RETRIEVE:
-            st = mxco_mm_retrieve(mm, r->r.in, r->r.n, r->r.out, r->r.out2, r->r.out_capacity, s->mm_stream, &result);
+            st = mxco_mm_retrieve(mm, r->r.in, r->r.n, r->r.out, r->r.out2, r->r.out_capacity, stream, &result);
             break;
         default: st = fail(MXCO_INVALID_ARGUMENT, "server: unknown request");
         }
@@ -398,7 +403,8 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
     s->tables_cap = 4096;
     hipError_t e = hipMalloc((void**)&s->tables_dev, s->tables_cap * sizeof(BatchTable));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->map_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->mm_stream, hipStreamNonBlocking);
+    for (int w = 0; w < kMmWorkers && e == hipSuccess; ++w)
+        e = hipStreamCreateWithFlags(&s->mm_streams[w], hipStreamNonBlocking);
     for (int i = 0; i < kTickSlots && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&s->ticks[i].done, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)mxco_server_destroy(s);
@@ -406,7 +412,7 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
     }
     s->dispatcher = std::thread(dispatcher_loop, s);
     s->completer = std::thread(completer_loop, s);
-    s->mm_worker = std::thread(mm_loop, s);
+    for (int w = 0; w < kMmWorkers; ++w) s->mm_workers[w] = std::thread(mm_loop, s, w);
     *out = s;
     return ok();
 }
@@ -423,7 +429,8 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
     s->cv_ticks.notify_all();
     s->cv_mm.notify_all();
     if (s->dispatcher.joinable()) s->dispatcher.join();
-    if (s->mm_worker.joinable()) s->mm_worker.join();
+    for (std::thread& w : s->mm_workers)
+        if (w.joinable()) w.join();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
@@ -431,7 +438,8 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
     s->cv_done.notify_all();
     if (s->completer.joinable()) s->completer.join();
     if (s->map_stream) (void)hipStreamSynchronize(s->map_stream);
-    if (s->mm_stream) (void)hipStreamSynchronize(s->mm_stream);
+    for (hipStream_t st : s->mm_streams)
+        if (st) (void)hipStreamSynchronize(st);
     for (Table& t : s->tables) {
         if (t.map) (void)mxco_oa_destroy(t.map);
         if (t.mm) (void)mxco_mm_destroy(t.mm);
@@ -445,7 +453,8 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
     for (auto& kv : s->reqs) delete kv.second;
     if (s->tables_dev) (void)hipFree(s->tables_dev);
     if (s->map_stream) (void)hipStreamDestroy(s->map_stream);
-    if (s->mm_stream) (void)hipStreamDestroy(s->mm_stream);
+    for (hipStream_t st : s->mm_streams)
+        if (st) (void)hipStreamDestroy(st);
     delete s;
     return ok();
 }
@@ -492,7 +501,8 @@ MXCO_API mxco_status mxco_server_add_multimap(mxco_server* s, uint64_t capacity,
     while (p < 2 * capacity) p <<= 1;
     if (mxco_status e = admit(s, p * 16)) return e;
     mxco_mm* mm = nullptr;
-    if (mxco_status e = mxco_mm_create(8, 8, capacity, empty_key, empty_value, s->mm_stream, &mm)) return e;
+    hipStream_t stream = s->mm_streams[s->tables.size() % kMmWorkers];  // the stream of the table's worker
+    if (mxco_status e = mxco_mm_create(8, 8, capacity, empty_key, empty_value, stream, &mm)) return e;
     Table t;
     t.kind = 1;
     t.mm = mm;
@@ -566,7 +576,9 @@ MXCO_API mxco_status mxco_server_drain(mxco_server* s)
     if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
-        if (!s->pending.empty() || !s->mm_pending.empty() || !s->inflight.empty()) return false;
+        if (!s->pending.empty() || !s->inflight.empty()) return false;
+        for (const std::deque<Req*>& q : s->mm_pending)
+            if (!q.empty()) return false;
         for (auto& kv : s->reqs)
             if (!kv.second->done) return false;
         for (Tick& t : s->ticks)
