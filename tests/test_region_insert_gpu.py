@@ -90,6 +90,29 @@ def test_region_insert_matches_oracle(kd, vd):
     check_against(m, ref, kd, vd, np.concatenate([k[:200000], misses(rng, 200000, kd)]))
 
 
+@pytest.mark.parametrize("kd", [np.int32, np.int64])
+def test_region_insert_sets_match_oracle(kd):
+    """StaticSet under LinearProbing: 4- and 8-byte slots, whose regions walk
+    (their sort scratch does not fit the region's LDS)."""
+    rng = np.random.default_rng(8)
+    cap_req = 1 << 21
+    s = mx().StaticSet(cap_req, -1, TD[kd], probing="linear")
+    cap = s.capacity()
+    k = distinct(rng, int(0.8 * cap), kd)
+    allk = np.concatenate([k, k[rng.integers(0, len(k), len(k) // 10)], np.full(3, -1, kd)])
+    rng.shuffle(allk)
+    assert s.insert_uses_region_build(len(allk))
+    failed = s.insert(dev(allk))
+    ref = O.OpenAddressing(kd, None, cap_req, -1, None, probing=O.PROBE_LINEAR, hash_family=O.HASH_XXHASH_32)
+    assert failed == ref.insert(allk) == len(allk) - len(k)
+    got = table_keys(s, kd)
+    assert len(got) == len(k) and np.array_equal(np.sort(got), np.sort(k))
+    q = np.concatenate([k[:200000], misses(rng, 200000, kd)])
+    rv, rf = ref.find(q)
+    assert s.find(dev(q)).cpu().numpy().tobytes() == rv.tobytes()
+    assert s.contains(dev(q)).cpu().numpy().astype(bool).tolist() == rf.tolist()
+
+
 def test_region_insert_two_level_partition():
     """2^24 + 2^20 slots of 16 bytes: 4353 regions, grouped two per bin."""
     kd = vd = np.int64
