@@ -41,7 +41,7 @@ struct mxco_oa {
     uint32_t ss;
     uint32_t voff;
     uint8_t* slots;
-    unsigned long long* counters;       // device [failed, assigned]
+    unsigned long long* counters;       // device: kCounterSlots pairs [failed, assigned]
     unsigned long long* host_counters;  // pinned mirror
     uint64_t empty_key;
     uint64_t empty_val;
@@ -146,19 +146,25 @@ mxco_status launch(const mxco_oa* t, int op, OaArgs& a, hipStream_t stream, cons
 
 mxco_status zero_counters(mxco_oa* t, hipStream_t stream)
 {
-    zero_counters_kernel<<<1, 64, 0, stream>>>(t->counters, 2);
+    zero_counters_kernel<<<1, 128, 0, stream>>>(t->counters, 2 * kCounterSlots);
     MXCO_HIP(hipGetLastError(), "zero counters");
     return MXCO_OK;
 }
 
 mxco_status read_counters(mxco_oa* t, hipStream_t stream, uint64_t* c0, uint64_t* c1)
 {
-    MXCO_HIP(hipMemcpyAsync(t->host_counters, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    MXCO_HIP(hipMemcpyAsync(t->host_counters, t->counters, 2 * kCounterSlots * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost,
                             stream),
              "read counters");
     MXCO_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    if (c0) *c0 = t->host_counters[0];
-    if (c1) *c1 = t->host_counters[1];
+    uint64_t s0 = 0, s1 = 0;
+    for (uint32_t i = 0; i < kCounterSlots; ++i) {
+        s0 += t->host_counters[2 * i];
+        s1 += t->host_counters[2 * i + 1];
+    }
+    if (c0) *c0 = s0;
+    if (c1) *c1 = s1;
     return MXCO_OK;
 }
 
@@ -418,8 +424,8 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
         delete t;
         return hip_fail(e, "mxco_oa_create: slot array");
     }
-    e = hipMalloc((void**)&t->counters, 2 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&t->host_counters, 2 * sizeof(unsigned long long), 0);
+    e = hipMalloc((void**)&t->counters, 2 * kCounterSlots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&t->host_counters, 2 * kCounterSlots * sizeof(unsigned long long), 0);
     if (e != hipSuccess) {
         (void)hipFree(t->slots);
         if (t->counters) (void)hipFree(t->counters);

@@ -615,6 +615,17 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
 
 }
 
+// A table's counters: kCounterSlots pairs [failed / count, assigned], each
+// workgroup adding into pair blockIdx.x % kCounterSlots and the host summing
+// them, so that a batch with many failures (duplicates) does not serialise
+// on one address.
+constexpr uint32_t kCounterSlots = 64;
+
+__device__ __forceinline__ unsigned long long* counter_slot(unsigned long long* counters)
+{
+    return counters + 2 * (blockIdx.x % kCounterSlots);
+}
+
 // Wave reduction of the two counters, one atomic per wavefront.
 template <int OP>
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long cnt0,
@@ -644,7 +655,7 @@ __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
     oa_process<KS, VS, G, OP, U>(a, gid, a.n, ngroups, cnt0, cnt1);
-    if (a.counters) flush_counters<OP>(a.counters, cnt0, cnt1);
+    if (a.counters) flush_counters<OP>(counter_slot(a.counters), cnt0, cnt1);
 }
 
 // Chunk-staged variant for the streaming bulk operations.  A workgroup takes
@@ -696,7 +707,7 @@ __global__ __launch_bounds__(256) void oa_staged_kernel(OaArgs a)
         }
         if constexpr (OUT != 0) __syncthreads();
     }
-    if (a.counters) flush_counters<OP>(a.counters, cnt0, cnt1);
+    if (a.counters) flush_counters<OP>(counter_slot(a.counters), cnt0, cnt1);
 }
 
 // Find / Contains with wave-staged outputs.  Each wavefront owns chunks of
@@ -971,7 +982,7 @@ static __global__ __launch_bounds__(256) void oa_fill_kernel(uint8_t* table, uin
 
 static __global__ void zero_counters_kernel(unsigned long long* c, int n)
 {
-    if (threadIdx.x < (unsigned)n) c[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
 }
 
 }  // namespace mxco
