@@ -112,10 +112,16 @@ __device__ __forceinline__ TileRef tile_ref(const BuildArgs& a, uint32_t b)
     return TileRef{0u, a.in + st * SS, (uint32_t)min<uint64_t>(kTile, a.single_n - st)};
 }
 
+// Failures of a table: into its item's counter (batched plans), or into one
+// of the table's counter slots (the one-table Insert: table_item is null).
 __device__ __forceinline__ void count_failures(const BuildArgs& a, uint32_t table, uint32_t n)
 {
     if (!n || !a.counters) return;
-    const int32_t item = a.table_item ? a.table_item[table] : 0;
+    if (!a.table_item) {
+        atomicAdd(counter_slot(a.counters), (unsigned long long)n);
+        return;
+    }
+    const int32_t item = a.table_item[table];
     if (item >= 0) atomicAdd(&a.counters[2ull * item], (unsigned long long)n);
 }
 
