@@ -264,7 +264,7 @@ hipError_t prepare_build(mxco_batch_plan* p, const mxco_batch_item* items)
     static const int enabled = env_int("MXCO_BATCH_REGION_BUILD", 1);
     if (!enabled || b->d.probing != 0) return hipSuccess;
     const uint32_t R = build_region_slots(b->ss);
-    const uint32_t tile = kBuildTileBytes / b->ss;
+    const uint32_t tile = build_tile_pairs(b->ss);
     std::vector<int32_t> table_item(b->ntables, -1);
     std::vector<uint32_t> reg0(b->ntables + 1);
     uint64_t nreg = 0, total = 0;

@@ -49,7 +49,14 @@ constexpr uint32_t kBuildBytes = 65536;    // most slot bytes of one region (hel
 constexpr uint32_t kRegionSlots = 4096;    // most slots of one region
 constexpr uint32_t kBuildThreads = 1024;   // count / scatter / apply workgroups
 constexpr uint32_t kBuildMaxBins = 4096;   // bins of one table in count / scatter
-constexpr uint32_t kBuildTileBytes = 65536;  // pair bytes of one scatter tile
+
+// Pairs of one scatter / regroup tile: 64 KiB of pairs.  (Halving it for the
+// narrower slots lets two of their scatter workgroups share a CU, but doubles
+// the count pass's tiles and per-bin adds: a net loss.)
+constexpr uint32_t build_tile_pairs(uint32_t ss)
+{
+    return 65536u / ss;
+}
 
 // Slots of one region for slot size ss (BuildShape<ss>::kRegion on the host).
 constexpr uint32_t build_region_slots(uint32_t ss)
@@ -60,7 +67,7 @@ constexpr uint32_t build_region_slots(uint32_t ss)
 template <int SS>
 struct BuildShape {
     static constexpr uint32_t kRegion = kBuildBytes / SS < kRegionSlots ? kBuildBytes / SS : kRegionSlots;  // slots per region
-    static constexpr uint32_t kTile = kBuildTileBytes / SS;   // pairs per tile
+    static constexpr uint32_t kTile = build_tile_pairs(SS);   // pairs per tile
     static constexpr uint32_t kPer = kTile / kBuildThreads;   // pairs per thread of a tile
 };
 
@@ -228,7 +235,7 @@ constexpr size_t build_scatter_lds(uint32_t maxbins)
 // storage (4096 for every slot size).
 constexpr uint32_t build_max_bins(uint32_t ss)
 {
-    const uint32_t tile = kBuildTileBytes / ss;
+    const uint32_t tile = build_tile_pairs(ss);
     const uint32_t fit = (163840u - 8192u - tile * ss - tile * 2) / 12;
     return fit < kBuildMaxBins ? fit : kBuildMaxBins;
 }
@@ -248,7 +255,11 @@ __global__ __launch_bounds__(kBuildThreads) void build_scatter_kernel(BuildArgs 
     uint32_t* gpos = toff + a.maxbins;                    // nbins (run start in `part`)
     uint16_t* rid = (uint16_t*)(gpos + a.maxbins);        // kTile
     typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
-    __shared__ typename Scan::TempStorage scan_tmp;
+    // The scan's storage shares `staged`, which is first written after the
+    // scan and the barrier behind it: with no static LDS a workgroup of
+    // 16-byte pairs over 512 bins needs 78 KiB, and two fit on a CU.
+    static_assert(sizeof(typename Scan::TempStorage) <= S::kTile * sizeof(Raw), "scan storage exceeds the tile");
+    typename Scan::TempStorage& scan_tmp = *reinterpret_cast<typename Scan::TempStorage*>(smem);
 
     const TileRef tl = tile_ref<L::SS, kOne>(a, blockIdx.x);
     const BatchTable t = a.tables[tl.table];
@@ -712,6 +723,34 @@ constexpr size_t build_regroup_lds(uint32_t group)
     return (size_t)BuildShape<SS>::kTile * SS + (size_t)BuildShape<SS>::kTile * 2 + (size_t)group * 16;
 }
 
+// Exclusive prefix sums of cnt[0 .. n) into off[0 .. n), n <= 256, by the 64
+// lanes of wavefront 0 (four entries a lane, then a shuffle scan of the lane
+// sums); the other wavefronts do not call it.
+__device__ __forceinline__ void wave0_offsets(const uint32_t* cnt, uint32_t* off, uint32_t n)
+{
+    const uint32_t lane = threadIdx.x;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t j = lane * 4 + q;
+        v[q] = j < n ? cnt[j] : 0u;
+        sum += v[q];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t j = lane * 4 + q;
+        if (j < n) off[j] = run;
+        run += v[q];
+    }
+}
+
 template <int KS, int VS>
 __global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs a, const uint8_t* part1,
                                                                       const uint32_t* offsets1, uint32_t group)
@@ -735,8 +774,20 @@ __global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs 
     Raw* dst = (Raw*)a.part;
     for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) rcnt[j] = 0;
     __syncthreads();
-    for (uint32_t j = s + threadIdx.x; j < e; j += blockDim.x)
-        atomicAdd(&rcnt[first_slot<KS>(a, t.nbuckets, L::key(src[j])) / S::kRegion - rfirst], 1u);
+    // kPer loads in flight per thread before their histogram updates
+    for (uint32_t j0 = s; j0 < e; j0 += S::kTile) {
+        Raw p[S::kPer];
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = j0 + threadIdx.x + u * kBuildThreads;
+            if (j < e) p[u] = src[j];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = j0 + threadIdx.x + u * kBuildThreads;
+            if (j < e) atomicAdd(&rcnt[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / S::kRegion - rfirst], 1u);
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t o = s;
@@ -767,13 +818,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs 
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t o = 0;
-            for (uint32_t j = 0; j < nr; ++j) {
-                toff[j] = o;
-                o += tcnt[j];
-            }
-        }
+        if (threadIdx.x < 64) wave0_offsets(tcnt, toff, nr);
         __syncthreads();
 #pragma unroll
         for (uint32_t u = 0; u < S::kPer; ++u) {
