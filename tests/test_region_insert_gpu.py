@@ -3,7 +3,7 @@ on a table of more than 256 MiB of slots is built region by region in LDS
 (MXCO_OA_REGION_INSERT=0 turns it off; these tests lower the size threshold).
 Its failures and Find outputs equal the oracle's sequential CAS walk on the
 same inputs, and the table holds every inserted key exactly once: on empty tables, tables of 2^24 slots (a two-level
-partition), tables already holding keys and tombstones, streams with one
+partition), an int32 set of 2^29 + 2^26 slots (256 regions per first-level bin, the most), tables already holding keys and tombstones, streams with one
 very hot key (regions left to the CAS walk), full tables, InsertAsync and
 graph replays."""
 import numpy as np
@@ -127,6 +127,28 @@ def test_region_insert_two_level_partition():
     assert ref.insert(k, value_of(k, vd)) == 0
     assert len(table_keys(m, kd)) == len(k)
     check_against(m, ref, kd, vd, np.concatenate([k[::7], misses(rng, 500000, kd)]))
+
+
+def test_region_insert_most_regions_per_bin():
+    """2^29 + 2^26 slots of an int32 set: 147,457 regions of 4096 slots, so
+    the first level groups the most regions a bin may hold (256; 577 bins).
+    At this size the checks are properties: no failures, every key found,
+    no miss found, and the table holds exactly the inserted keys."""
+    s = mx().StaticSet((1 << 29) + (1 << 26), -1, torch.int32, probing="linear")
+    cap = s.capacity()
+    n = int(0.6 * cap)
+    # i * odd mod 2^30 is a bijection of [0, 2^30): n distinct keys, none -1
+    k = ((torch.arange(n, dtype=torch.int64, device="cuda") * 2654435761) % (1 << 30)).to(torch.int32)
+    assert s.insert_uses_region_build(n)
+    assert s.insert(k) == 0
+    assert bool(s.contains(k).bool().all())
+    miss = torch.randint(1 << 30, (1 << 31) - 1, (1 << 22,), dtype=torch.int64, device="cuda").to(torch.int32)
+    assert not bool(s.contains(miss).bool().any())
+    t = s.slots().view(torch.int32).reshape(-1)
+    t = t[t != -1]
+    assert t.numel() == n
+    assert torch.equal(torch.sort(t).values, torch.sort(k).values)
+    del t
 
 
 def test_region_insert_into_table_with_keys_and_tombstones():
