@@ -25,7 +25,10 @@
  *                  spec and tests/common/bloom_filter_golden.h:67-146;
  *   hll         -- per-key register closed form
  *                  (tests/hyperloglog/add_test.cpp:321-335), linear-counting
- *                  and HLL closed forms for the estimate.
+ *                  and HLL closed forms for the estimate, and, for the
+ *                  bias-corrected regime, a brute-force nearest-six mean over
+ *                  the whole raw-estimate table (the method of
+ *                  tests/hyperloglog/hll_finalizer_neighbor_count.cpp:36-55).
  */
 #define _GNU_SOURCE
 #include <stdint.h>
@@ -838,7 +841,7 @@ static double hll_alpha_mm(double m)
 
 static double hll_bias(double e, const double* raw, const double* bias, int n)
 {
-    const int k = 6;
+    const int k = PLANT(13) ? 5 : 6;
     int left = 0, right = n - 1, anchor = -1, found = 0;
     while (left <= right) {
         int mid = left + (right - left) / 2;
@@ -859,6 +862,16 @@ static double hll_bias(double e, const double* raw, const double* bias, int n)
     return s / (high - low);
 }
 
+/* HLL++ bias correction below 5m (finalizer.h:95-98). */
+static double hll_corrected(double e, double m, const double* raw, const double* bias, int n)
+{
+    if (e >= 5.0 * m) return e;
+    return PLANT(14) ? e + hll_bias(e, raw, bias, n) : e - hll_bias(e, raw, bias, n);
+}
+
+/* finalizer.h:46-74: linear counting up to 2.5m; above it the linear-counting
+ * value while it stays at or below the precision's threshold, else the
+ * bias-corrected raw estimate. */
 ORC_API uint64_t orc_hll_finalize(uint32_t p, double z, int v, const double* raw, const double* bias, int n,
                                   double threshold)
 {
@@ -868,9 +881,9 @@ ORC_API uint64_t orc_hll_finalize(uint32_t p, double z, int v, const double* raw
     if (v > 0) {
         double h = m * log(m / (PLANT(11) ? v + 1 : v));
         if (e <= 2.5 * m) return (uint64_t)round(h);
-        e = (h <= threshold) ? h : ((e < 5.0 * m) ? e - hll_bias(e, raw, bias, n) : e);
+        e = (h <= threshold && !PLANT(15)) ? h : hll_corrected(e, m, raw, bias, n);
     } else {
-        e = (e < 5.0 * m) ? e - hll_bias(e, raw, bias, n) : e;
+        e = hll_corrected(e, m, raw, bias, n);
     }
     if (e >= 18446744073709551616.0) return UINT64_MAX;
     return (uint64_t)round(e);

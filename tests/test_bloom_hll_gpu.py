@@ -187,6 +187,57 @@ def test_hll_registers_and_estimate(p, dtype):
         assert s.estimate() == est
 
 
+def _alpha_mm(m):
+    return (0.7213 / (1.0 + 1.079 / m)) * m * m
+
+
+@pytest.mark.parametrize("p", [13, 14, 15, 16, 17, 18])
+def test_hll_bias_regime_estimate(p):
+    """About 3m and 4.5m distinct keys put the raw estimate E inside
+    (2.5m, 5m), where HLL++ subtracts the interpolated bias: registers are
+    bit-exact and the estimate equals the oracle's finalizer (itself checked
+    against a brute-force nearest-six mean on the CPU)."""
+    m = 1 << p
+    for mult in (3.0, 4.5):
+        n = int(mult * m)
+        keys = (np.arange(n, dtype=np.int64) * np.int64(0x9E3779B97F4A7C15 & (2**62 - 1)) + p).astype(np.int64)
+        s = mx().HyperLogLog(p)
+        s.add(dev(keys))
+        ref = np.zeros(m, np.uint8)
+        O.hll_add(ref, p, keys)
+        np.testing.assert_array_equal(s.registers().cpu().numpy(), ref)
+        est, hist = oracle_estimate(ref, p)
+        z = O.lib().orc_hll_z(O._ptr(hist))
+        e = _alpha_mm(m) / z
+        assert 2.5 * m < e < 5.0 * m, (p, n, e / m)
+        assert est != int(math.floor(e + 0.5))  # the bias term moved it
+        assert s.estimate() == est
+        assert abs(est - n) / n < 3 * 1.04 / math.sqrt(m)
+
+
+def test_hll_threshold_branch_estimate():
+    """p = 18 registers with about a quarter at zero and the rest at rank 40:
+    E is above 2.5m while h = m ln(m/V) is at or below the threshold, so the
+    estimate is round(h).  Registers are written straight into the sketch."""
+    import ctypes
+    p, m = 18, 1 << 18
+    s = mx().HyperLogLog(p)
+    for zeros in (int(0.27 * m), int(0.28 * m), int(0.25 * m)):
+        regs = np.full(m, 40, np.uint8)
+        regs[:zeros] = 0
+        np.random.default_rng(zeros).shuffle(regs)
+        d = torch.from_numpy(regs).cuda()
+        torch.cuda.synchronize()
+        assert mx()._hip.hipMemcpy(ctypes.c_void_p(s.data()), ctypes.c_void_p(d.data_ptr()),
+                                   ctypes.c_size_t(m), 3) == 0
+        est, hist = oracle_estimate(regs, p)
+        lin = m * math.log(m / zeros)
+        if zeros != int(0.25 * m):
+            assert est == int(math.floor(lin + 0.5))
+        assert s.histogram() == hist.tolist()
+        assert s.estimate() == est
+
+
 def test_hll_merge_and_accuracy():
     p = 15
     rng = np.random.default_rng(9)

@@ -379,11 +379,38 @@ def test_hll_registers_closed_form(dtype):
     np.testing.assert_array_equal(regs, ref)
 
 
-def test_hll_estimate_linear_counting_and_bias_paths():
+def _hll_tables(p):
+    t = np.load(O.os.path.join(O._HERE, "data", "hllpp_p%d.npz" % p))
+    return t["raw"], t["bias"], float(t["threshold"])
+
+
+def _finalize(p, z, v):
+    raw, bias, thr = _hll_tables(p)
+    return int(L.orc_hll_finalize(p, z, v, O._ptr(raw), O._ptr(bias), len(raw), thr))
+
+
+def _alpha_mm(m):
+    return (0.7213 / (1.0 + 1.079 / m)) * m * m
+
+
+def _nearest_six(p, z):
+    """Brute-force HLL++ bias correction: the six raw estimates nearest to E
+    over the WHOLE table (sort by |raw - E|, stable), their biases averaged
+    (tests/hyperloglog/hll_finalizer_neighbor_count.cpp:36-55).  It searches
+    no window, so it shares no step with the oracle's anchor-and-slide."""
+    raw, bias, _ = _hll_tables(p)
+    e = _alpha_mm(1 << p) / z
+    order = sorted(range(len(raw)), key=lambda i: (abs(float(raw[i]) - e), i))
+    corr = 0.0
+    for i in order[:6]:
+        corr += float(bias[i])
+    return int(math.floor(e - corr / 6 + 0.5))
+
+
+def test_hll_estimate_linear_counting_and_raw_regime():
     p = 14
     m = 1 << p
-    tables = np.load(O.os.path.join(O._HERE, "data", "hllpp_p14.npz"))
-    raw, bias, thr = tables["raw"], tables["bias"], float(tables["threshold"])
+    raw, bias, thr = _hll_tables(p)
     # Linear counting: few keys -> estimate = round(m ln(m/V))
     regs = np.zeros(m, np.uint8)
     O.hll_add(regs, p, np.arange(1000, dtype=np.int64))
@@ -399,6 +426,68 @@ def test_hll_estimate_linear_counting_and_bias_paths():
     alpha = 0.7213 / (1 + 1.079 / m)
     assert est == round(alpha * m * m / z)
     assert abs(z - m * 2.0 ** -20) < 1e-12
+    # no registers set: z = 0 -> the u64 maximum; all zero: linear counting of 0
+    assert _finalize(p, 0.0, 0) == 2**64 - 1
+    assert _finalize(p, float(m), m) == 0
+
+
+@pytest.mark.parametrize("p", [13, 14, 15, 16, 17, 18])
+def test_hll_bias_regime_nearest_six(p):
+    """2.5m < E < 5m: the finalizer subtracts the mean bias of the six
+    nearest raw estimates.  Checked against the brute-force nearest six on
+    (i) a sweep of E over the whole regime, V = 0 and V = 1 (h = m ln m, above
+    every precision's threshold), (ii) the reference's own tail cases (z built
+    from registers of rank 2 and 3 with E beyond the last raw estimate) and
+    its control sums (hll_finalizer_neighbor_count.cpp:74-113)."""
+    m = 1 << p
+    raw, _, thr = _hll_tables(p)
+    assert m * math.log(m) > thr
+    sums = [_alpha_mm(m) / e for e in np.linspace(2.5 * m * 1.0001, 5.0 * m * 0.9999, 301)]
+    tail = []
+    for r2 in range(0, m + 1, max(1, m >> 9)):
+        z = r2 / 4.0 + (m - r2) / 8.0
+        e = _alpha_mm(m) / z
+        if raw[-1] < e < 5.0 * m:
+            tail.append(z)
+    assert tail, "no tail case at p=%d" % p
+    controls = [r2 / 4.0 + (m - r2) / 8.0 for r2 in (m // 2, 3 * m // 4, m)]
+    moved = 0
+    for z in sums + tail + controls:
+        e = _alpha_mm(m) / z
+        exp = _nearest_six(p, z)
+        if 2.5 * m < e < 5.0 * m:
+            moved += exp != int(math.floor(e + 0.5))
+        else:
+            assert e >= 5.0 * m and exp is not None
+            exp = int(math.floor(e + 0.5))
+        assert _finalize(p, z, 0) == exp, (p, z, e)
+        if e > 2.5 * m:
+            assert _finalize(p, z, 1) == exp, (p, z, e)
+    assert moved > 250  # the correction is not a no-op in this regime
+
+
+def test_hll_threshold_branch():
+    """V > 0, E > 2.5m and h = m ln(m/V) <= threshold: the estimate is the
+    rounded linear-counting value h (finalizer.h:60-62).  Registers can reach
+    this only at p = 18 (threshold 350000 > m ln(2.5/0.7213) needs
+    threshold/m > 1.243), with about a quarter of the registers at zero and
+    the rest at high ranks.  A control with fewer zeros crosses the threshold
+    and is bias-corrected (nearest-six)."""
+    p = 18
+    m = 1 << p
+    _, _, thr = _hll_tables(p)
+    for zeros, in_branch in ((int(0.27 * m), True), (int(0.28 * m), True), (int(0.265 * m), True), (int(0.25 * m), False)):
+        regs = np.full(m, 40, np.uint8)
+        regs[:zeros] = 0
+        h = O.hll_histogram(regs, p)
+        z = L.orc_hll_z(O._ptr(h))
+        e = _alpha_mm(m) / z
+        lin = m * math.log(m / zeros)
+        assert 2.5 * m < e < 5.0 * m
+        assert (lin <= thr) == in_branch
+        exp = int(math.floor(lin + 0.5)) if in_branch else _nearest_six(p, z)
+        assert _finalize(p, z, zeros) == exp
+        assert exp != _nearest_six(p, z) or not in_branch
 
 
 # --------------------------------------------------------------------------

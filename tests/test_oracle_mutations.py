@@ -26,6 +26,9 @@ PLANTS = {
     10: ("Pair: second member not naturally aligned", "pair_layout"),
     11: ("HLL++ linear counting: m ln(m/(V+1))", "hll_estimate_linear_counting"),
     12: ("xxh32: stripe rotation 12 instead of 13", "xxhash_against_library"),
+    13: ("HLL++ bias: mean over 5 neighbours instead of 6", "hll_bias_regime_nearest_six"),
+    14: ("HLL++ bias added instead of subtracted", "hll_bias_regime_nearest_six"),
+    15: ("HLL++: the h <= threshold branch dropped", "hll_threshold_branch"),
 }
 
 
