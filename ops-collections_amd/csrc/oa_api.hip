@@ -33,6 +33,16 @@ struct RegionWs {
     BatchTable* table;   // the table's descriptor
     uint32_t* reg0;      // {0, nbins1}
     uint32_t nreg, nbins1, group;
+    // A captured region build keeps part1 / part2 in its kernel arguments,
+    // so once one is captured the buffers are never reallocated: a larger
+    // batch takes the CAS walk instead.
+    bool captured;
+    // Eager region builds on different streams share the workspace: each
+    // one records `done` on its stream and the next one, on another
+    // stream, waits for it.
+    hipEvent_t done;
+    hipStream_t last_stream;
+    bool has_last;
 };
 
 struct mxco_oa {
@@ -54,11 +64,16 @@ struct mxco_oa {
     int inflight;
     uint32_t probe_flags;
     RegionWs* rws;
-    // No key can be in the slots: set by creation and Clear, cleared by every
-    // call that can add keys (Insert, InsertOrAssign, InsertAndFind, and Data,
-    // whose pointer device code may write through).  The region build then
-    // starts its regions empty instead of reading them.
+    // No key can be in the slots: set by creation and by an eager Clear,
+    // cleared by every call that can add keys (Insert, InsertOrAssign,
+    // InsertAndFind).  The region build then starts its regions empty
+    // instead of reading them.  A captured Clear leaves it unset: the fill
+    // has not run when the capture returns.
     mutable bool empty;
+    // Data() has handed out the slot pointer, through which device code may
+    // write at any later time (also after a Clear): from then on the table
+    // is never taken to be empty.
+    mutable bool exposed;
 };
 
 namespace {
@@ -191,10 +206,20 @@ __global__ void zero_u32_pair_kernel(uint32_t* a, uint32_t na, uint32_t* b)
     if (blockIdx.x == 0 && threadIdx.x == 0) b[0] = 0;
 }
 
+bool capturing(hipStream_t s)
+{
+    // the null stream is never captured
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!s) return false;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) return true;  // unknown: assume captured
+    return cs != hipStreamCaptureStatusNone;
+}
+
 void free_region_ws(mxco_oa* t)
 {
     RegionWs* w = t->rws;
     if (!w) return;
+    if (w->done) (void)hipEventDestroy(w->done);
     for (void* p : {(void*)w->part1, (void*)w->part2, (void*)w->counts1, (void*)w->offsets1, (void*)w->offsets2,
                     (void*)w->ovc, (void*)w->heavy, (void*)w->apply_end, (void*)w->table, (void*)w->reg0})
         if (p) (void)hipFree(p);
@@ -210,7 +235,10 @@ void free_region_ws(mxco_oa* t)
 // plus 32 per slot; the CAS walk about 300 per distinct key at load factor
 // 0.8 on a table that misses the cache, less on one that fits it or for a
 // stream of repeated keys, so those keep the walk.  Under stream capture the
-// workspace must already be large enough.
+// workspace must already be large enough, and once a build has been captured
+// it never grows.  A captured build shares the table's workspace with every
+// other Insert on the table: replays must be ordered with them by the caller,
+// as for any two calls on one container.
 // Regions, regions per first-level bin and first-level bins of the region
 // build: about 512 bins, at most 256 regions per bin.
 struct RegionShape {
@@ -235,10 +263,9 @@ bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream
     const char* mb = getenv("MXCO_OA_REGION_INSERT_MIN_BYTES");
     if (t->cap * t->ss < (mb ? strtoull(mb, nullptr, 10) : (256ull << 20))) return false;
     if (region_shape(t).nbins1 > build_max_bins(t->ss)) return false;  // the scatter's LDS
-    // the null stream is never captured
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (stream && hipStreamIsCapturing(stream, &cs) != hipSuccess) return false;
-    if (cs != hipStreamCaptureStatusNone && (!t->rws || t->rws->pairs < n)) return false;
+    // under capture the workspace must exist and be large enough; after a
+    // capture it never grows
+    if (capturing(stream) || (t->rws && t->rws->captured)) return t->rws && t->rws->pairs >= n;
     return true;
 }
 
@@ -262,6 +289,7 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         if (e == hipSuccess) e = hipMalloc((void**)&w->apply_end, w->nreg * 4ull);
         if (e == hipSuccess) e = hipMalloc((void**)&w->table, sizeof(BatchTable));
         if (e == hipSuccess) e = hipMalloc((void**)&w->reg0, 2 * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
         if (e == hipSuccess) {
             BatchTable bt{t->slots, (uint32_t)t->cap, (uint32_t)(t->cap / t->d.bucket_size)};
             const uint32_t r0[2] = {0u, w->nbins1};
@@ -275,6 +303,8 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
     }
     RegionWs* w = t->rws;
     if (w->pairs < n) {
+        if (w->captured) return hipErrorInvalidValue;  // a captured build holds these buffers
+        // hipFree waits for the device, so no earlier build still reads them
         if (w->part1) (void)hipFree(w->part1);
         if (w->part2) (void)hipFree(w->part2);
         w->part1 = w->part2 = nullptr;
@@ -353,17 +383,31 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
 
 hipError_t region_insert(mxco_oa* t, const OaArgs& a, hipStream_t s, bool table_empty)
 {
+    RegionWs& w = *t->rws;
+    const bool cap = capturing(s);
     // a captured Insert may be replayed on a table that holds keys by then
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (s && (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)) table_empty = false;
+    if (cap) table_empty = false;
+    if (!cap && w.has_last && w.last_stream != s) {
+        hipError_t e = hipStreamWaitEvent(s, w.done, 0);
+        if (e != hipSuccess) return e;
+    }
     const uint32_t ks = t->d.key_size, vs = t->d.value_size;
-    if (ks == 4 && vs == 4) return launch_region_insert<4, 4>(t, a, s, table_empty);
-    if (ks == 8 && vs == 8) return launch_region_insert<8, 8>(t, a, s, table_empty);
-    if (ks == 4 && vs == 8) return launch_region_insert<4, 8>(t, a, s, table_empty);
-    if (ks == 8 && vs == 4) return launch_region_insert<8, 4>(t, a, s, table_empty);
-    if (ks == 4 && vs == 0) return launch_region_insert<4, 0>(t, a, s, table_empty);
-    if (ks == 8 && vs == 0) return launch_region_insert<8, 0>(t, a, s, table_empty);
-    return hipErrorInvalidValue;
+    hipError_t e = hipErrorInvalidValue;
+    if (ks == 4 && vs == 4) e = launch_region_insert<4, 4>(t, a, s, table_empty);
+    else if (ks == 8 && vs == 8) e = launch_region_insert<8, 8>(t, a, s, table_empty);
+    else if (ks == 4 && vs == 8) e = launch_region_insert<4, 8>(t, a, s, table_empty);
+    else if (ks == 8 && vs == 4) e = launch_region_insert<8, 4>(t, a, s, table_empty);
+    else if (ks == 4 && vs == 0) e = launch_region_insert<4, 0>(t, a, s, table_empty);
+    else if (ks == 8 && vs == 0) e = launch_region_insert<8, 0>(t, a, s, table_empty);
+    if (e != hipSuccess) return e;
+    if (cap) {
+        w.captured = true;
+    } else {
+        e = hipEventRecord(w.done, s);
+        w.last_stream = s;
+        w.has_last = true;
+    }
+    return e;
 }
 
 }  // namespace
@@ -433,7 +477,7 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
         return hip_fail(e, "mxco_oa_create: counters");
     }
     mxco_status st = fill(t, stream);
-    t->empty = true;
+    t->empty = !capturing(stream);
     if (st == MXCO_OK) {
         e = hipStreamSynchronize(stream);
         if (e != hipSuccess) st = hip_fail(e, "mxco_oa_create: clear");
@@ -474,6 +518,7 @@ MXCO_API mxco_status mxco_oa_data(const mxco_oa* t, void** slots)
     if (!slots) return fail(MXCO_INVALID_ARGUMENT, "Data: null");
     *slots = t->slots;
     t->empty = false;
+    t->exposed = true;
     return ok();
 }
 
@@ -498,7 +543,7 @@ MXCO_API mxco_status mxco_oa_clear(mxco_oa* t, hipStream_t stream, int sync)
 {
     if (mxco_status s = check_table(t, "Clear")) return s;
     if (mxco_status s = fill(t, stream)) return s;
-    t->empty = true;
+    t->empty = !t->exposed && !capturing(stream);
     if (sync) MXCO_HIP(hipStreamSynchronize(stream), "Clear: hipStreamSynchronize");
     return ok();
 }

@@ -536,6 +536,41 @@ ORC_API uint64_t orc_oa_insert_if(orc_oa* t, const void* pairs, const uint8_t* f
     return failed;
 }
 
+/* AssignIf (kernels.h:183-210 with Assign, open_addressing_ref_impl.h:384-410):
+ * for each selected pair, walk the key's probe sequence; an EMPTY slot ends
+ * the walk with nothing written, an EQUAL slot gets the pair's payload, a
+ * full wrap ends it.  Nothing is inserted.  Returns the pairs assigned. */
+ORC_API uint64_t orc_oa_assign_if(orc_oa* t, const void* pairs, const uint8_t* flags, uint64_t n)
+{
+    const uint8_t* p = (const uint8_t*)pairs;
+    uint64_t assigned = 0;
+    if (!t->vs) return 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (flags && !flags[i]) continue;
+        const uint8_t* pair = p + i * t->ss;
+        probe_it it = make_iter(t, pair);
+        uint32_t init = it.idx;
+        int done = 0;
+        while (!done) {
+            for (uint32_t b = 0; b < t->bucket && !done; ++b) {
+                uint8_t* slot = t->slots + ((uint64_t)it.idx + b) * t->ss;
+                if (is_empty_key(t, slot) && !PLANT(17)) {
+                    if (PLANT(16)) memcpy(slot, pair, t->ss);
+                    done = 1;
+                } else if (key_equal(t, pair, slot)) {
+                    memcpy(slot + t->voff, pair + t->voff, t->vs);
+                    assigned++;
+                    done = 1;
+                }
+            }
+            if (done) break;
+            iter_next(&it);
+            if (it.idx == init) break;
+        }
+    }
+    return assigned;
+}
+
 /* InsertAndFind (open_addressing_ref_impl.h:302-330, :544-565). */
 ORC_API void orc_oa_insert_and_find(orc_oa* t, const void* pairs, uint64_t n, void* out_found, uint8_t* out_inserted)
 {

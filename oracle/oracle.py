@@ -70,6 +70,8 @@ def lib():
         L.orc_oa_insert_if.restype = ctypes.c_uint64
         L.orc_oa_insert_if.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.orc_oa_insert_and_find.argtypes = [vp, vp, ctypes.c_uint64, vp, vp]
+        L.orc_oa_assign_if.restype = ctypes.c_uint64
+        L.orc_oa_assign_if.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.orc_oa_find.restype = ctypes.c_uint64
         L.orc_oa_find.argtypes = [vp, vp, ctypes.c_uint64, vp, vp]
         L.orc_oa_erase.restype = ctypes.c_uint64
@@ -163,6 +165,13 @@ class OpenAddressing:
         assert b.size == self.capacity * self.slot_size
         ctypes.memmove(lib().orc_oa_slots(self.t), b.ctypes.data, b.size)
 
+    def probe_start(self, key):
+        """(first slot, step) of the key's probe sequence."""
+        k = np.array([key], self.kd)
+        st, sp = ctypes.c_uint32(), ctypes.c_uint32()
+        lib().orc_oa_probe_start(self.t, _ptr(k), ctypes.byref(st), ctypes.byref(sp))
+        return st.value, sp.value
+
     def _pairs(self, keys, values):
         return np.ascontiguousarray(pack_pairs(keys.astype(self.kd), None if self.vd is None else
                                                values.astype(self.vd)))
@@ -183,6 +192,13 @@ class OpenAddressing:
         p = self._pairs(keys, values)
         fl = np.ascontiguousarray(flags.astype(np.uint8))
         return int(lib().orc_oa_insert_if(self.t, _ptr(p), _ptr(fl), len(keys)))
+
+    def assign_if(self, keys, values, flags=None):
+        """AssignIf: payloads of present keys overwritten, nothing inserted.
+        Returns the number of selected pairs whose key was found."""
+        p = self._pairs(keys, values)
+        fl = None if flags is None else np.ascontiguousarray(np.asarray(flags).astype(np.uint8))
+        return int(lib().orc_oa_assign_if(self.t, _ptr(p), _ptr(fl), len(keys)))
 
     def insert_and_find(self, keys, values=None):
         p = self._pairs(keys, values)

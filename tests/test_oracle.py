@@ -242,6 +242,51 @@ def test_map_semantics_vs_dict(kd, vd, probing):
     np.testing.assert_array_equal(hit, np.array([k in g for k in q.tolist()]))
 
 
+@pytest.mark.parametrize("kd,vd", [(np.int32, np.int32), (np.int64, np.int64), (np.int64, np.int32)])
+@pytest.mark.parametrize("probing", [O.PROBE_LINEAR, O.PROBE_DOUBLE])
+def test_assign_if_vs_dict(kd, vd, probing):
+    """AssignIf overwrites the payload of selected keys already present and
+    inserts nothing (open_addressing_ref_impl.h:384-410, kernels.h:183-210)."""
+    rng = np.random.default_rng(3)
+    keys = rng.choice(6000, 3000, replace=False).astype(kd)
+    t = O.OpenAddressing(kd, vd, 4000, -1, -1, bucket=5, probing=probing)
+    assert t.insert(keys, (keys * 3).astype(vd)) == 0
+    g = {k: 3 * k for k in keys.tolist()}
+    q = rng.choice(9000, 4000, replace=False).astype(kd)
+    nv = rng.integers(-10**6, 10**6, len(q)).astype(vd)
+    sel = rng.integers(0, 2, len(q)).astype(bool)
+    exp_assigned = 0
+    for k, v, f in zip(q.tolist(), nv.tolist(), sel.tolist()):
+        if f and k in g:
+            g[k] = v
+            exp_assigned += 1
+    assert t.assign_if(q, nv, sel) == exp_assigned > 500
+    probe = np.arange(-1, 9000).astype(kd)
+    out, hit = t.find(probe)
+    np.testing.assert_array_equal(hit, np.array([k in g for k in probe.tolist()]))
+    np.testing.assert_array_equal(out, np.array([g.get(k, -1) for k in probe.tolist()], dtype=vd))
+    # no stencil: every pair selected; the empty key assigns nothing
+    q2 = np.concatenate([keys[:100], np.array([-1], kd)])
+    assert t.assign_if(q2, np.full(len(q2), 11, vd)) == 100
+    np.testing.assert_array_equal(t.find(keys[:100])[0], np.full(100, 11, vd))
+
+
+def test_assign_if_stops_at_the_first_empty_slot():
+    """A key stored past an empty slot of its own walk is not reached: the
+    walk ends at EMPTY (Find alike)."""
+    t = O.OpenAddressing(np.int64, np.int64, 64, -1, -1, bucket=1, probing=O.PROBE_LINEAR)
+    key = np.array([12345], np.int64)
+    start, _ = t.probe_start(12345)
+    slots = t.slots()
+    nxt = (start + 1) % t.capacity
+    slots[nxt, :8] = key.view(np.uint8)
+    slots[nxt, 8:16] = np.array([77], np.int64).view(np.uint8)
+    t.load_slots(slots)
+    assert t.assign_if(key, np.array([5], np.int64)) == 0
+    assert t.slots()[nxt, 8:16].view(np.int64)[0] == 77
+    assert not t.find(key)[1][0]
+
+
 def test_insert_or_assign_and_insert_and_find_vs_dict():
     rng = np.random.default_rng(2)
     keys = rng.integers(0, 500, 2000).astype(np.int64)

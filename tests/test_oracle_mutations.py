@@ -29,6 +29,8 @@ PLANTS = {
     13: ("HLL++ bias: mean over 5 neighbours instead of 6", "hll_bias_regime_nearest_six"),
     14: ("HLL++ bias added instead of subtracted", "hll_bias_regime_nearest_six"),
     15: ("HLL++: the h <= threshold branch dropped", "hll_threshold_branch"),
+    16: ("AssignIf inserts absent keys (InsertOrAssign semantics)", "assign_if_vs_dict"),
+    17: ("AssignIf walks past an empty slot", "assign_if_stops_at_the_first_empty_slot"),
 }
 
 

@@ -287,3 +287,95 @@ def test_region_insert_async_and_graph_replays():
         assert np.array_equal(out.cpu().numpy(), expect), rep
         m.clear()
         torch.cuda.synchronize()
+
+
+def test_captured_clear_does_not_mark_table_empty():
+    """A Clear captured into a graph has not run when the capture returns: an
+    eager region-built Insert that follows must still read the regions, so
+    the keys already in the table stay."""
+    import mxco
+    kd = vd = np.int64
+    rng = np.random.default_rng(11)
+    m = mxco.StaticMap(1 << 21, -1, -1, TD[kd], TD[vd])
+    cap = m.capacity()
+    a = distinct(rng, int(0.3 * cap), kd)
+    b = distinct(rng, int(0.55 * cap), kd, exclude=a)
+    assert m.insert(dev_pairs(a, value_of(a, vd))) == 0
+    g = mxco.Graph(lambda s: m.clear_async(stream=s))  # captured, not launched
+    assert m.insert_uses_region_build(len(b))
+    assert m.insert(dev_pairs(b, value_of(b, vd))) == 0
+    assert m.contains(dev(a)).cpu().numpy().all()
+    assert np.array_equal(m.find(dev(a)).cpu().numpy(), value_of(a, vd))
+    assert np.array_equal(m.find(dev(b)).cpu().numpy(), value_of(b, vd))
+    g.launch()
+    g.stream.synchronize()
+    assert not m.contains(dev(a)).cpu().numpy().any()
+    assert not m.contains(dev(b)).cpu().numpy().any()
+
+
+def test_captured_region_insert_keeps_its_workspace():
+    """Once a region-built Insert is captured, its workspace is never
+    reallocated: a later, larger eager Insert takes the CAS walk (same
+    results), and replays of the graph still read valid buffers."""
+    import mxco
+    kd = vd = np.int64
+    rng = np.random.default_rng(12)
+    cap_req = 1 << 21
+    m = mxco.StaticMap(cap_req, -1, -1, TD[kd], TD[vd])
+    cap = m.capacity()
+    k1 = distinct(rng, int(0.55 * cap), kd)
+    k2 = distinct(rng, int(0.75 * cap), kd, exclude=k1)
+    p1, p2 = dev_pairs(k1, value_of(k1, vd)), dev_pairs(k2, value_of(k2, vd))
+    assert m.insert_uses_region_build(len(k1)) and m.insert_uses_region_build(len(k2))
+    assert m.insert(p1) == 0  # workspace sized for len(k1)
+
+    def record(s):
+        m.clear_async(stream=s)
+        assert m.insert_uses_region_build(len(k1), stream=s)
+        m.insert_async(p1, stream=s)
+
+    g = mxco.Graph(record)
+    assert m.insert_uses_region_build(len(k1))
+    assert not m.insert_uses_region_build(len(k2))  # would grow the captured workspace
+    m.clear()
+    ref = O.OpenAddressing(kd, vd, cap_req, -1, -1)
+    assert m.insert(p2) == ref.insert(k2, value_of(k2, vd)) == 0
+    q = np.concatenate([k1[:100000], k2[:100000], misses(rng, 50000, kd)])
+    check_against(m, ref, kd, vd, q)
+    for rep in range(2):
+        g.launch()
+        g.stream.synchronize()
+        ref = O.OpenAddressing(kd, vd, cap_req, -1, -1)
+        ref.insert(k1, value_of(k1, vd))
+        check_against(m, ref, kd, vd, q)
+
+
+def test_region_inserts_on_two_streams_are_ordered():
+    """Two region-built InsertAsync calls on different streams share the
+    table's workspace; the second waits for the first."""
+    import mxco
+    kd = vd = np.int64
+    rng = np.random.default_rng(13)
+    cap_req = 1 << 21
+    m = mxco.StaticMap(cap_req, -1, -1, TD[kd], TD[vd])
+    cap = m.capacity()
+    u = distinct(rng, int(0.8 * cap), kd)
+    a = u[: int(0.55 * cap)]
+    b = np.concatenate([u[int(0.55 * cap):], a[: int(0.3 * cap)]])
+    rng.shuffle(b)
+    assert len(a) * 2 >= cap and len(b) * 2 >= cap
+    pa, pb = dev_pairs(a, value_of(a, vd)), dev_pairs(b, value_of(b, vd))
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for rep in range(2):
+        m.clear()
+        m.insert_async(pa, stream=s1)
+        m.insert_async(pb, stream=s2)
+        s1.synchronize()
+        s2.synchronize()
+        ref = O.OpenAddressing(kd, vd, cap_req, -1, -1)
+        ref.insert(a, value_of(a, vd))
+        ref.insert(b, value_of(b, vd))
+        got = table_keys(m, kd)
+        assert len(got) == len(u) and np.array_equal(np.sort(got), np.sort(u)), rep
+        check_against(m, ref, kd, vd, np.concatenate([u[:200000], misses(rng, 50000, kd)]))

@@ -264,6 +264,53 @@ def test_stencil_predicates(group):
     assert not c.any()
 
 
+ASSIGN_PRED = {"always": lambda st: np.ones(len(st), bool), "nonzero": lambda st: st != 0,
+               "odd": lambda st: st % 2 == 1, "even": lambda st: st % 2 == 0}
+
+
+@pytest.mark.parametrize("kd,vd", LAYOUTS[:4], ids=lambda x: getattr(x, "__name__", str(x)))
+@pytest.mark.parametrize("probing", [O.PROBE_LINEAR, O.PROBE_DOUBLE])
+@pytest.mark.parametrize("group", GROUPS)
+def test_assign_if_vs_oracle(kd, vd, probing, group):
+    """AssignIf (static_map.h:189): present selected keys get the new payload,
+    absent keys stay absent.  From the same slots, the table after each
+    AssignIf equals the oracle's byte for byte."""
+    rng = np.random.default_rng(17 + group)
+    cap_req = 30000
+    keys = distinct_keys(rng, 20000, 10**8, kd)
+    vals = rng.integers(0, 2**30, len(keys)).astype(vd)
+    m = mx().StaticMap(cap_req, -1, -1, TD[kd], TD[vd], probing=PROBE[probing], group_size=group)
+    ref = O.OpenAddressing(kd, vd, cap_req, -1, -1, probing=probing)
+    assert m.insert(dev_pairs(keys, vals)) == ref.insert(keys, vals) == 0
+    # concurrent Insert places keys in its own order: start both from the GPU's slots
+    ref.load_slots(m.slots().cpu().numpy())
+    q = np.concatenate([keys[rng.permutation(len(keys))[:12000]], absent_keys(rng, 6000, 10**8, keys, kd),
+                        np.full(3, -1, kd)])
+    rng.shuffle(q)
+    nv = rng.integers(0, 2**30, len(q)).astype(vd)
+    st = rng.integers(-3, 50, len(q)).astype(np.int32)
+    for pred, sel in ASSIGN_PRED.items():
+        nv = nv + np.asarray(1, vd)
+        m.assign_if_async(dev_pairs(q, nv), dev(st), pred)
+        torch.cuda.synchronize()
+        assert ref.assign_if(q, nv, sel(st)) > 0
+        assert m.slots().cpu().numpy().tobytes() == ref.slots().tobytes(), pred
+    probe = np.concatenate([keys, absent_keys(rng, 5000, 10**8, keys, kd)])
+    rv, rf = ref.find(probe)
+    assert m.find(dev(probe)).cpu().numpy().tobytes() == rv.tobytes()
+    assert m.contains(dev(probe)).cpu().numpy().astype(bool).tolist() == rf.tolist()
+
+
+def test_assign_if_null_and_empty():
+    m = mx().StaticMap(100, -1, -1)
+    m.assign_if_async(dev_pairs(np.array([1], np.int32), np.array([2], np.int32)), None, "always", n=0)
+    k = np.array([5, 6], np.int32)
+    assert m.insert(dev_pairs(k, k)) == 0
+    m.assign_if_async(dev_pairs(k, k * 10), None, "always")  # no stencil: every pair
+    torch.cuda.synchronize()
+    assert m.find(dev(k)).cpu().numpy().tolist() == [50, 60]
+
+
 def test_float_keys_use_operator_equal():
     # KeyEqual is == on float (so -0.0 equals +0.0 once the walk reaches it);
     # the sentinel test stays bitwise and the probe hash is over the bytes.
