@@ -65,29 +65,21 @@ def copy_calibration():
 
 
 def make_workload(torch, cap, lf, seed=1234):
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
+    """round(lf * cap) distinct keys uniform over the signed 64-bit range
+    (sentinel -1 excluded), value key*7+1; 2^28 queries (cap if smaller) at a
+    50 % hit rate, misses uniform 64-bit keys guaranteed absent."""
+    from mxco.workloads import uniform_map
     n_ins = int(round(lf * cap))
     n_find = 1 << 28 if cap >= (1 << 28) else cap
-    # inserted keys in [0, 2^62); absent queries in [2^62, 2^63): the sentinel -1 never occurs
-    keys = torch.randint(0, 1 << 62, (n_ins,), device="cuda", dtype=torch.int64, generator=g)
-    keys = torch.unique(keys)  # distinct (a 2^28-of-2^62 draw almost never collides)
-    n_ins = keys.numel()
-    keys = keys[torch.randperm(n_ins, device="cuda", generator=g)]
-    vals = keys * 7 + 1
-    hits = keys[torch.randint(0, n_ins, (n_find // 2,), device="cuda", generator=g)]
-    miss = torch.randint(1 << 62, (1 << 63) - 1, (n_find - n_find // 2,), device="cuda", dtype=torch.int64,
-                         generator=g)
-    q = torch.cat([hits, miss])
-    q = q[torch.randperm(n_find, device="cuda", generator=g)]
-    return keys, vals, q
+    keys, q, is_hit = uniform_map(n_ins, n_find, 0.5, seed)
+    return keys, keys * 7 + 1, q, is_hit
 
 
 def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
     cap_req = 1 << args.slots_log2
     m = mxco.StaticMap(cap_req, -1, -1, torch.int64, torch.int64, group_size=group)
     cap = m.capacity()
-    keys, vals, q = make_workload(torch, cap, lf)
+    keys, vals, q, is_hit = make_workload(torch, cap, lf)
     pairs = mxco.make_pairs(keys, vals)
     out = torch.empty(q.numel(), dtype=torch.int64, device="cuda")
     s = torch.cuda.current_stream()
@@ -140,7 +132,7 @@ def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
             del os.environ["MXCO_OA_REGION_INSERT"]
     if verify:
         # Inserted keys hit with f(key); absent keys miss; no insert failed.
-        exp = torch.where(q < (1 << 62), q * 7 + 1, torch.full_like(q, -1))
+        exp = torch.where(is_hit, q * 7 + 1, torch.full_like(q, -1))
         res["verified"] = bool(torch.equal(out, exp))
         m.clear()
         res["insert_failed"] = m.insert(pairs)
@@ -468,7 +460,7 @@ def run_containers(torch, mxco, steps=3):
     log("bench: containers: erase, count, insert_and_find, contains")
     m = mxco.StaticMap(1 << 26, -1, -1, torch.int64, torch.int64)
     m.set_erased_key(-2)
-    ek, ev, eq = make_workload(torch, m.capacity(), 0.8, seed=99)
+    ek, ev, eq, e_hit = make_workload(torch, m.capacity(), 0.8, seed=99)
     m.insert(mxco.make_pairs(ek, ev))
     t_c = _timed(torch, lambda: m.count(eq), steps)
     present = int(m.count(eq))
@@ -479,8 +471,7 @@ def run_containers(torch, mxco, steps=3):
     # the table at load factor 0.8 (before the Erase).  Only 2^22 new keys are
     # added (0.86 after): a batch larger than the EMPTY slots left would walk
     # full cycles.
-    iaf_keys = torch.cat([ek[ek.numel() // 2:ek.numel() // 2 + (1 << 22)],
-                          torch.randint(1 << 62, (1 << 63) - 1, (1 << 22,), device="cuda", generator=g)])
+    iaf_keys = torch.cat([ek[ek.numel() // 2:ek.numel() // 2 + (1 << 22)], eq[~e_hit][:1 << 22]])
     iaf_pairs = mxco.make_pairs(iaf_keys, iaf_keys * 7 + 1)
     of = torch.empty(iaf_keys.numel(), dtype=torch.int64, device="cuda")
     oi = torch.empty(iaf_keys.numel(), dtype=torch.bool, device="cuda")

@@ -16,6 +16,44 @@ def _mix(j, salt):
     return ((j * MULT) & 0x7FFFFFFF) ^ salt
 
 
+def uniform_map(n_ins, n_find, hit=0.5, seed=1234, device="cuda"):
+    """Distinct keys drawn uniformly from the whole signed 64-bit range with
+    the sentinel -1 excluded, in random order, and n_find queries of which
+    round(hit * n_find) are drawn from those keys and the rest are uniform
+    64-bit keys guaranteed absent (rejected against the sorted key set),
+    shuffled.
+
+    Returns (keys, queries, is_hit) as device tensors."""
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    lo, hi = -(1 << 63), (1 << 63) - 1
+
+    def draw(count):
+        c = torch.randint(lo, hi, (count,), device=device, dtype=torch.int64, generator=g)
+        return c[c != -1]
+
+    cand = torch.unique(draw(n_ins + n_ins // 1000 + 64))  # sorted; a collision is rare
+    if cand.numel() < n_ins:
+        raise RuntimeError("too few distinct keys drawn")
+    keys = cand[torch.randperm(cand.numel(), device=device, generator=g)[:n_ins]]
+    del cand
+    srt = torch.sort(keys).values
+    n_hit = int(round(hit * n_find))
+    hits = keys[torch.randint(0, n_ins, (n_hit,), device=device, generator=g)]
+    n_miss = n_find - n_hit
+    miss = draw(n_miss + n_miss // 100 + 64)
+    pos = torch.searchsorted(srt, miss).clamp_(max=n_ins - 1)
+    miss = miss[srt[pos] != miss][:n_miss]
+    del srt, pos
+    if miss.numel() < n_miss:
+        raise RuntimeError("too few absent keys drawn")
+    q = torch.cat([hits, miss])
+    is_hit = torch.cat([torch.ones(n_hit, dtype=torch.bool, device=device),
+                        torch.zeros(n_miss, dtype=torch.bool, device=device)])
+    perm = torch.randperm(n_find, device=device, generator=g)
+    return keys, q[perm], is_hit[perm]
+
+
 def batched_tables(num_tables=1024, cap_log2=(10, 22), lf=(0.5, 0.85), find_log2=(6, 20), hit=0.5, seed=4,
                    valid_extent=None):
     """Capacities log-uniform in 2^cap_log2, per-table load factors uniform in
@@ -65,7 +103,10 @@ def serving(num_maps=48, num_multimaps=16, slots_log2=(20, 26), fill=0.5, num_re
 
     Keys of table t are the bijection of (t, j) used by batched_tables; map
     tables start `fill` full, multimap keys repeat 1..8 times (value = key
-    * 16 + copy).  Returns host plans and device buffers for every request."""
+    * 16 + copy).  Queries draw present keys from the initial fill and
+    absent keys from indices no insert reaches, so a query's hit flag
+    (request["hit"]) holds whatever the order of the inserts.  Returns host
+    plans and device buffers for every request."""
     rng = np.random.default_rng(seed)
     dev = "cuda"
     T = num_maps + num_multimaps
@@ -115,7 +156,7 @@ def serving(num_maps=48, num_multimaps=16, slots_log2=(20, 26), fill=0.5, num_re
             jh = torch.from_numpy(rng.integers(0, max(1, n_present[t]), n)).to(dev)
             jm = torch.from_numpy(rng.integers(2**30, 2**31 - 1, n)).to(dev)
             inp = keys_of(t, torch.where(hitm, jh, jm)).contiguous()
-        reqs.append({"table": t, "op": op, "input": inp, "n": n})
+        reqs.append({"table": t, "op": op, "input": inp, "n": n, "hit": None if op == "insert" else hitm})
     return {"caps": caps, "num_maps": num_maps, "init": init, "requests": reqs, "arrivals_us": arrivals.tolist()}
 
 
