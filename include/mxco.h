@@ -515,6 +515,17 @@ MXCO_API mxco_status mxco_bloom_clear(mxco_bloom* filter, hipStream_t stream, in
  * byte per key (non-zero = add) or NULL for all keys. */
 MXCO_API mxco_status mxco_bloom_add(mxco_bloom* filter, const void* keys, const uint8_t* stencil, uint64_t n,
                                     hipStream_t stream, int sync);
+/* Add that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and the same filter words as mxco_bloom_add; the
+ * partitioned pass runs in a fixed scratch array the library holds on each
+ * GPU (52 MiB) instead of a grown workspace, so a call takes at most
+ * 4,194,304 keys (MXCO_LENGTH_ERROR beyond).  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller. */
+MXCO_API mxco_status mxco_bloom_add_noalloc(mxco_bloom* filter, const void* keys, const uint8_t* stencil, uint64_t n,
+                                            hipStream_t stream, int sync);
 /* Contains / ContainsAsync / ContainsIf / ContainsIfAsync
  * (bloom_filter_query.inl): out[i] = 1 byte bool, 0 where the stencil is 0. */
 MXCO_API mxco_status mxco_bloom_contains(const mxco_bloom* filter, const void* keys, const uint8_t* stencil,

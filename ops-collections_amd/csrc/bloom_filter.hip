@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -384,25 +385,43 @@ mxco_status ensure_ws(mxco_bloom* f, size_t bytes)
     return MXCO_OK;
 }
 
-// Partitioned Add of n <= kPassKeys keys (see the file comment).
+// Workspace of one partitioned pass of n keys: bin entries (8 B per key),
+// the range-by-segment counts (+ the total), their offsets and the scan's
+// temporary storage, each 256-byte aligned.
+struct PassLayout {
+    uint32_t nranges, segs;
+    uint64_t ncnt;
+    size_t off_counts, off_offsets, off_temp, temp, total;
+};
+
+mxco_status pass_layout(const mxco_bloom* f, uint64_t n, hipStream_t stream, PassLayout& L)
+{
+    L.nranges = (uint32_t)((f->num_blocks + kRangeBlocks - 1) >> kRangeShift);
+    L.segs = segments_for(n);
+    L.ncnt = (uint64_t)L.nranges * L.segs + 1;  // + the total
+    L.temp = 0;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, L.temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (int64_t)L.ncnt, stream),
+             "BloomFilter::Add: scan size");
+    L.off_counts = ((n * 8) + 255) & ~size_t(255);
+    L.off_offsets = L.off_counts + ((L.ncnt * 4 + 255) & ~size_t(255));
+    L.off_temp = L.off_offsets + ((L.ncnt * 4 + 255) & ~size_t(255));
+    L.total = L.off_temp + L.temp;
+    return MXCO_OK;
+}
+
+// Partitioned Add of n <= kPassKeys keys (see the file comment) in the
+// workspace `ws` laid out by L.
 mxco_status add_partitioned(mxco_bloom* f, const uint8_t* keys, const uint8_t* stencil, uint64_t n,
-                            hipStream_t stream)
+                            hipStream_t stream, uint8_t* ws, const PassLayout& L)
 {
     const uint32_t nb = (uint32_t)f->num_blocks;
-    const uint32_t nranges = (uint32_t)((f->num_blocks + kRangeBlocks - 1) >> kRangeShift);
-    const uint32_t segs = segments_for(n);
-    const uint64_t ncnt = (uint64_t)nranges * segs + 1;  // + the total
-    size_t temp = 0;
-    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (int64_t)ncnt, stream),
-             "BloomFilter::Add: scan size");
-    const size_t off_counts = ((n * 8) + 255) & ~size_t(255);
-    const size_t off_offsets = off_counts + ((ncnt * 4 + 255) & ~size_t(255));
-    const size_t off_temp = off_offsets + ((ncnt * 4 + 255) & ~size_t(255));
-    if (mxco_status e = ensure_ws(f, off_temp + temp)) return e;
-    uint64_t* entries = (uint64_t*)f->ws;
-    uint32_t* counts = (uint32_t*)(f->ws + off_counts);
-    uint32_t* offsets = (uint32_t*)(f->ws + off_offsets);
+    const uint32_t nranges = L.nranges, segs = L.segs;
+    const uint64_t ncnt = L.ncnt;
+    size_t temp = L.temp;
+    uint64_t* entries = (uint64_t*)ws;
+    uint32_t* counts = (uint32_t*)(ws + L.off_counts);
+    uint32_t* offsets = (uint32_t*)(ws + L.off_offsets);
     const size_t lds = (size_t)nranges * 4;
     // the slot after the last count must read 0 so the scan's last output is the total
     words_zero_tail<<<1, 1, 0, stream>>>(counts + ncnt - 1);
@@ -411,7 +430,7 @@ mxco_status add_partitioned(mxco_bloom* f, const uint8_t* keys, const uint8_t* s
     else
         bloom_count_kernel<8><<<segs, 256, lds, stream>>>(nb, keys, stencil, n, f->seed, nranges, counts);
     MXCO_HIP(hipGetLastError(), "BloomFilter::Add: count");
-    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(f->ws + off_temp, temp, counts, offsets, (int64_t)ncnt, stream),
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(ws + L.off_temp, temp, counts, offsets, (int64_t)ncnt, stream),
              "BloomFilter::Add: scan");
     // LDS-staged scatter; the per-key scatter remains for MXCO_BLOOM_STAGED=0
     static const int staged = env_int("MXCO_BLOOM_STAGED", 1);
@@ -444,6 +463,52 @@ mxco_status add_partitioned(mxco_bloom* f, const uint8_t* keys, const uint8_t* s
     const unsigned grid = std::min<unsigned>(nranges, (unsigned)cu_count() * 2);
     bloom_apply_kernel<<<grid, 1024, 0, stream>>>(f->words, nb, nranges, segs, offsets, entries);
     MXCO_HIP(hipGetLastError(), "BloomFilter::Add: apply");
+    return MXCO_OK;
+}
+
+mxco_status add_direct(mxco_bloom* f, const void* keys, const uint8_t* stencil, uint64_t n, hipStream_t stream)
+{
+    const unsigned grid = grid_for(n, 8);
+    if (f->key_size == 4)
+        bloom_add_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                     stencil, n, f->seed);
+    else
+        bloom_add_kernel<8><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
+                                                     stencil, n, f->seed);
+    MXCO_HIP(hipGetLastError(), "BloomFilter::Add");
+    return MXCO_OK;
+}
+
+// Fixed scratch of mxco_bloom_add_noalloc: one partitioned pass of up to
+// kScratchKeys keys on a filter of up to kMaxRanges ranges (512 MiB): 32 MiB
+// of bin entries, two count matrices of 8 MiB (kMaxRanges x 256 segments)
+// and 4 MiB for the scan.  Every GPU the library runs on holds its own copy
+// until the process ends; calls on one GPU share it.
+constexpr uint64_t kScratchKeys = 1ull << 22;
+constexpr size_t kScratchBytes = (kScratchKeys * 8) + 2 * (((size_t)kMaxRanges * 256 * 4 + 4 + 255) & ~size_t(255)) +
+                                 ((size_t)4 << 20);
+__device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices];
+
+// Device address of the current device's scratch, looked up once per device
+// with that device current (the host-side name of a __device__ array is no
+// device address).
+mxco_status scratch_on_device(uint8_t** out)
+{
+    int dev = 0;
+    MXCO_HIP(hipGetDevice(&dev), "BloomFilter::Add (no-alloc): get device");
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(MXCO_UNSUPPORTED, "BloomFilter::Add (no-alloc): device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MXCO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "BloomFilter::Add (no-alloc): scratch address");
+        g_scratch_ptr[dev] = (uint8_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
     return MXCO_OK;
 }
 
@@ -540,19 +605,40 @@ MXCO_API mxco_status mxco_bloom_add(mxco_bloom* f, const void* keys, const uint8
     if (add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
         for (uint64_t done = 0; done < n; done += kPassKeys) {
             const uint64_t m = std::min(kPassKeys, n - done);
+            PassLayout L;
+            if (mxco_status e = pass_layout(f, m, stream, L)) return e;
+            if (mxco_status e = ensure_ws(f, L.total)) return e;
             if (mxco_status e = add_partitioned(f, (const uint8_t*)keys + done * f->key_size,
-                                                stencil ? stencil + done : nullptr, m, stream))
+                                                stencil ? stencil + done : nullptr, m, stream, f->ws, L))
                 return e;
         }
-    } else {
-        const unsigned grid = grid_for(n, 8);
-        if (f->key_size == 4)
-            bloom_add_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
-                                                         stencil, n, f->seed);
-        else
-            bloom_add_kernel<8><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
-                                                         stencil, n, f->seed);
-        MXCO_HIP(hipGetLastError(), "BloomFilter::Add");
+    } else if (mxco_status e = add_direct(f, keys, stencil, n, stream)) {
+        return e;
+    }
+    if (sync) MXCO_HIP(hipStreamSynchronize(stream), "BloomFilter::Add");
+    return ok();
+}
+
+MXCO_API mxco_status mxco_bloom_add_noalloc(mxco_bloom* f, const void* keys, const uint8_t* stencil, uint64_t n,
+                                            hipStream_t stream, int sync)
+{
+    if (n == 0) return ok();
+    if (mxco_status s = check_filter(f, "Add")) return s;
+    if (!keys) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter::Add: keys must not be null when keyNum is non-zero");
+    if (n > kScratchKeys)
+        return fail(MXCO_LENGTH_ERROR,
+                    "BloomFilter::Add (no-alloc): at most 4,194,304 keys per call (the library's fixed scratch)");
+    const uint64_t nranges = (f->num_blocks + kRangeBlocks - 1) >> kRangeShift;
+    if (add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
+        PassLayout L;
+        if (mxco_status e = pass_layout(f, n, stream, L)) return e;
+        if (L.total > kScratchBytes)
+            return fail(MXCO_LENGTH_ERROR, "BloomFilter::Add (no-alloc): the pass needs more than the fixed scratch");
+        uint8_t* ws = nullptr;
+        if (mxco_status e = scratch_on_device(&ws)) return e;
+        if (mxco_status e = add_partitioned(f, (const uint8_t*)keys, stencil, n, stream, ws, L)) return e;
+    } else if (mxco_status e = add_direct(f, keys, stencil, n, stream)) {
+        return e;
     }
     if (sync) MXCO_HIP(hipStreamSynchronize(stream), "BloomFilter::Add");
     return ok();

@@ -386,6 +386,7 @@ _sig("mxco_bloom_destroy", [vp])
 _sig("mxco_bloom_data", [vp, ctypes.POINTER(vp), pu64])
 _sig("mxco_bloom_clear", [vp, vp, i32])
 _sig("mxco_bloom_add", [vp, vp, vp, u64, vp, i32])
+_sig("mxco_bloom_add_noalloc", [vp, vp, vp, u64, vp, i32])
 _sig("mxco_bloom_contains", [vp, vp, vp, vp, u64, vp, i32])
 _sig("mxco_bloom_merge", [vp, vp, vp, i32])
 _sig("mxco_bloom_intersect", [vp, vp, vp, i32])
@@ -452,6 +453,12 @@ class BloomFilter:
 
     def add_async(self, keys, n=None, stream=None):
         self.add(keys, n, stream, sync=False)
+
+    def add_noalloc(self, keys, n=None, stream=None, sync=True, stencil=None):
+        """Add through the entry point that allocates nothing (at most
+        4,194,304 keys per call; calls on one GPU share its scratch)."""
+        n = keys.shape[0] if n is None else n
+        check(_lib.mxco_bloom_add_noalloc(self._h, _ptr(keys), _ptr(stencil), n, _stream(stream), int(sync)))
 
     def add_if(self, keys, stencil, n=None, stream=None, sync=True):
         n = keys.shape[0] if n is None else n
