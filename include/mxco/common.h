@@ -105,6 +105,15 @@ struct EqualTo {
     constexpr bool operator==(EqualTo const&) const noexcept { return true; }
 };
 
+// Key and payload types compared as floating point: the standard floating
+// types, and IEEE binary16 where the compiler has _Float16.
+template <typename T>
+struct IsFloatKey : std::is_floating_point<T> {};
+#if defined(__FLT16_MAX__)
+template <>
+struct IsFloatKey<_Float16> : std::true_type {};
+#endif
+
 // Pair<F, S> (include/pair.h:41): the slot layout of a map.
 namespace detail {
 constexpr std::size_t bit_ceil(std::size_t n)

@@ -1,13 +1,14 @@
-// Hash functions on 4- and 8-byte keys, callable from host and device.
+// Hash functions on 1-, 2-, 4- and 8-byte keys, callable from host and device.
 //
 // Families and bit-exact behaviour follow the reference's hashers:
 //   murmurhash3_32   include/detail/hash_functions/murmurhash3.h:56-108
 //   xxhash_32        include/detail/hash_functions/xxhash.h:53-175
 //   xxhash_64        include/detail/hash_functions/xxhash.h:185-326
 //   murmurhash3_fmix include/detail/hash_functions/utils.h:42-64
-// Only the key widths the containers accept (4 and 8 bytes) are written
-// out; each is the published algorithm specialised to one or two 32-bit
-// lanes of input, little-endian.
+// Only the key widths the containers accept (1, 2, 4 and 8 bytes) are
+// written out; each is the published algorithm specialised to that many
+// little-endian input bytes: one or two 32-bit blocks, or a tail of one or
+// two bytes.
 #pragma once
 #include <stdint.h>
 
@@ -58,8 +59,18 @@ MXCO_HD uint32_t murmur3_block(uint32_t h, uint32_t k)
 template <int KS>
 MXCO_HD uint32_t murmur3_32(uint64_t key, uint32_t seed)
 {
-    uint32_t h = murmur3_block(seed, (uint32_t)key);
-    if (KS == 8) h = murmur3_block(h, (uint32_t)(key >> 32));
+    uint32_t h = seed;
+    if (KS >= 4) {
+        h = murmur3_block(h, (uint32_t)key);
+        if (KS == 8) h = murmur3_block(h, (uint32_t)(key >> 32));
+    } else {
+        // tail of 1 or 2 bytes: mixed into h without the block rotation
+        uint32_t k = (uint32_t)key & (KS == 1 ? 0xffu : 0xffffu);
+        k *= 0xcc9e2d51u;
+        k = rotl32(k, 15);
+        k *= 0x1b873593u;
+        h ^= k;
+    }
     h ^= (uint32_t)KS;
     return fmix32(h);
 }
@@ -70,11 +81,19 @@ MXCO_HD uint32_t xxhash32(uint64_t key, uint32_t seed)
 {
     const uint32_t p1 = 0x9e3779b1u, p2 = 0x85ebca77u, p3 = 0xc2b2ae3du, p4 = 0x27d4eb2fu, p5 = 0x165667b1u;
     uint32_t h = seed + p5 + (uint32_t)KS;
-    h += (uint32_t)key * p3;
-    h = rotl32(h, 17) * p4;
-    if (KS == 8) {
-        h += (uint32_t)(key >> 32) * p3;
+    if (KS >= 4) {
+        h += (uint32_t)key * p3;
         h = rotl32(h, 17) * p4;
+        if (KS == 8) {
+            h += (uint32_t)(key >> 32) * p3;
+            h = rotl32(h, 17) * p4;
+        }
+    } else {
+        // byte tail
+        for (int b = 0; b < KS; ++b) {
+            h += (uint32_t)((key >> (8 * b)) & 0xffu) * p5;
+            h = rotl32(h, 11) * p1;
+        }
     }
     h ^= h >> 15;
     h *= p2;
@@ -84,7 +103,7 @@ MXCO_HD uint32_t xxhash32(uint64_t key, uint32_t seed)
     return h;
 }
 
-// XXH64 for 4- or 8-byte inputs: the 4-byte or the 8-byte tail step.
+// XXH64 for 1-, 2-, 4- or 8-byte inputs: the byte, 4-byte or 8-byte tail steps.
 template <int KS>
 MXCO_HD uint64_t xxhash64(uint64_t key, uint64_t seed)
 {
@@ -95,9 +114,14 @@ MXCO_HD uint64_t xxhash64(uint64_t key, uint64_t seed)
         uint64_t lane = rotl64(key * p2, 31) * p1;
         h ^= lane;
         h = rotl64(h, 27) * p1 + p4;
-    } else {
+    } else if (KS == 4) {
         h ^= (uint64_t)(uint32_t)key * p1;
         h = rotl64(h, 23) * p2 + p3;
+    } else {
+        for (int b = 0; b < KS; ++b) {
+            h ^= ((key >> (8 * b)) & 0xffull) * p5;
+            h = rotl64(h, 11) * p1;
+        }
     }
     h ^= h >> 33;
     h *= p2;

@@ -22,8 +22,10 @@ template <class Key, class T, class Extent = mxco::Extent<std::size_t>, class Ke
           class ProbingScheme = mxco::LinearProbing<mxco::murmurhash3_32<Key>>,
           class Storage = mxco::Storage<defaultMapBucketSize>>
 class StaticMap {
-    static_assert(sizeof(Key) == 4 || sizeof(Key) == 8, "keys must be 4 or 8 bytes");
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "payloads must be 4 or 8 bytes");
+    static_assert(sizeof(Key) == 1 || sizeof(Key) == 2 || sizeof(Key) == 4 || sizeof(Key) == 8,
+                  "keys must be 1, 2, 4 or 8 bytes");
+    static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8,
+                  "payloads must be 1, 2, 4 or 8 bytes");
     static_assert(std::is_same_v<KeyEqual, mxco::EqualTo<Key>>, "KeyEqual must be mxco::EqualTo<Key>");
 
 public:
@@ -46,11 +48,11 @@ public:
         mxco_oa_desc d{};
         d.key_size = sizeof(Key);
         d.value_size = sizeof(T);
-        d.key_kind = std::is_floating_point_v<Key> ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
+        d.key_kind = IsFloatKey<Key>::value ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
         d.bucket_size = static_cast<uint32_t>(bucketSize);
         d.probing = ProbingScheme::id;
         d.hash = ProbingScheme::hash;
-        d.value_kind = std::is_floating_point_v<T> ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
+        d.value_kind = IsFloatKey<T>::value ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
         mxco_oa* h = nullptr;
         detail::check(mxco_oa_create(&d, static_cast<SizeType>(capacity), &emptyKey, &emptyValue, stream, &h));
         impl_.reset(h);

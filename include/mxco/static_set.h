@@ -18,7 +18,8 @@ template <class Key, class Extent = mxco::Extent<std::size_t>, class KeyEqual = 
           class ProbingScheme = mxco::DoubleHashing<mxco::xxhash_32<Key>>,
           class Storage = mxco::Storage<defaultSetBucketSize>>
 class StaticSet {
-    static_assert(sizeof(Key) == 4 || sizeof(Key) == 8, "keys must be 4 or 8 bytes");
+    static_assert(sizeof(Key) == 1 || sizeof(Key) == 2 || sizeof(Key) == 4 || sizeof(Key) == 8,
+                  "keys must be 1, 2, 4 or 8 bytes");
     static_assert(std::is_same_v<KeyEqual, mxco::EqualTo<Key>>, "KeyEqual must be mxco::EqualTo<Key>");
 
 public:
@@ -39,7 +40,7 @@ public:
         mxco_oa_desc d{};
         d.key_size = sizeof(Key);
         d.value_size = 0;
-        d.key_kind = std::is_floating_point_v<Key> ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
+        d.key_kind = IsFloatKey<Key>::value ? MXCO_KEY_FLOAT : MXCO_KEY_INTEGRAL;
         d.bucket_size = static_cast<uint32_t>(bucketSize);
         d.probing = ProbingScheme::id;
         d.hash = ProbingScheme::hash;

@@ -353,6 +353,8 @@ MXCO_API mxco_status mxco_dm_create(const mxco_oa_desc* desc, uint64_t initial_c
         return fail(MXCO_INVALID_ARGUMENT, "DynamicMap: null argument");
     *out = nullptr;
     if (desc->value_size == 0) return fail(MXCO_UNSUPPORTED, "DynamicMap: maps only (value_size 4 or 8)");
+    if ((desc->key_size != 4 && desc->key_size != 8) || (desc->value_size != 4 && desc->value_size != 8))
+        return fail(MXCO_UNSUPPORTED, "DynamicMap: keys and payloads of 4 or 8 bytes");
     if (memcmp(empty_key, erased_key, desc->key_size) == 0)
         return fail(MXCO_INVALID_ARGUMENT, "DynamicMap: emptyKey and erasedKey must differ");
     mxco_dm* m = new (std::nothrow) mxco_dm();

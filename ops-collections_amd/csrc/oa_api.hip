@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <string>
 
@@ -85,6 +86,8 @@ uint64_t load_le(const void* p, uint32_t n)
     return v;
 }
 
+// Pair<K, V> (include/pair.h:41): the payload at its natural alignment after
+// the key, the slot aligned to min(16, bit_ceil(sK + sV)).  Sets: the key.
 void pair_layout(uint32_t ks, uint32_t vs, uint32_t* ss, uint32_t* voff)
 {
     if (vs == 0) {
@@ -92,8 +95,15 @@ void pair_layout(uint32_t ks, uint32_t vs, uint32_t* ss, uint32_t* voff)
         *voff = 0;
         return;
     }
-    *ss = (ks == 4 && vs == 4) ? 8 : 16;
-    *voff = (ks == 4 && vs == 4) ? 4 : 8;
+    uint32_t align = 1;
+    while (align < ks + vs) align <<= 1;
+    *ss = std::min<uint32_t>(align, 16);
+    *voff = std::max(ks, vs);
+}
+
+bool wide_layout(const mxco_oa* t)
+{
+    return t->d.key_size >= 4 && (t->d.value_size == 0 || t->d.value_size >= 4);
 }
 
 // Lanes per key when the descriptor leaves it to the library: 4, the best
@@ -194,7 +204,7 @@ mxco_status fill(mxco_oa* t, hipStream_t stream)
     u64x2 pat;
     memcpy(&pat, img, 16);
     const uint64_t bytes = t->cap * t->ss;
-    const unsigned grid = grid_for(bytes / 16, kFillBlocksPerCu);
+    const unsigned grid = std::max(1u, grid_for(bytes / 16, kFillBlocksPerCu));
     oa_fill_kernel<<<grid, 256, 0, stream>>>(t->slots, bytes, pat);
     MXCO_HIP(hipGetLastError(), "Clear");
     return MXCO_OK;
@@ -258,7 +268,7 @@ RegionShape region_shape(const mxco_oa* t)
 bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream_t stream)
 {
     const char* env = getenv("MXCO_OA_REGION_INSERT");
-    if ((env && atoi(env) == 0) || t->d.probing != 0 || a.stencil) return false;
+    if ((env && atoi(env) == 0) || t->d.probing != 0 || a.stencil || !wide_layout(t)) return false;
     if (n < (1ull << 20) || n * 2 < t->cap || n >= 0xffffffffull) return false;
     const char* mb = getenv("MXCO_OA_REGION_INSERT_MIN_BYTES");
     if (t->cap * t->ss < (mb ? strtoull(mb, nullptr, 10) : (256ull << 20))) return false;
@@ -420,10 +430,10 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     if (!desc || !out || !empty_key) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: null argument");
     *out = nullptr;
     const mxco_oa_desc& d = *desc;
-    if (d.key_size != 4 && d.key_size != 8)
-        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: key size must be 4 or 8 bytes");
-    if (d.value_size != 0 && d.value_size != 4 && d.value_size != 8)
-        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: payload size must be 0, 4 or 8 bytes");
+    if (d.key_size != 1 && d.key_size != 2 && d.key_size != 4 && d.key_size != 8)
+        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: key size must be 1, 2, 4 or 8 bytes");
+    if (d.value_size != 0 && d.value_size != 1 && d.value_size != 2 && d.value_size != 4 && d.value_size != 8)
+        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: payload size must be 0, 1, 2, 4 or 8 bytes");
     if (d.value_size && !empty_value) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: null empty value");
     if (d.bucket_size < 1 || d.bucket_size > 1024)
         return fail(MXCO_UNSUPPORTED, "mxco_oa_create: bucket size must be in [1, 1024]");
@@ -432,6 +442,8 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
     if ((d.hash == MXCO_HASH_FMIX_32 && d.key_size != 4) || (d.hash == MXCO_HASH_FMIX_64 && d.key_size != 8))
         return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: fmix hash width differs from the key width");
     if (d.key_kind > MXCO_KEY_FLOAT) return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: unknown key kind");
+    if (d.key_kind == MXCO_KEY_FLOAT && d.key_size == 1)
+        return fail(MXCO_UNSUPPORTED, "mxco_oa_create: floating-point keys are 2 (binary16), 4 or 8 bytes");
     if (d.group_size != 0 && d.group_size != 1 && d.group_size != 2 && d.group_size != 4 && d.group_size != 8)
         return fail(MXCO_INVALID_ARGUMENT, "mxco_oa_create: group size must be 0, 1, 2, 4 or 8");
 
@@ -459,11 +471,14 @@ MXCO_API mxco_status mxco_oa_create(const mxco_oa_desc* desc, uint64_t capacity,
         // chunks.  Operations a variant does not cover use the plain kernel.
         const int m = env_int("MXCO_OA_INFLIGHT", 4);
         t->inflight = (m == 2 || m == 3 || m == 4) ? m : 1;
+        if (!wide_layout(t)) t->inflight = 1;
     }
     t->probe_flags = (uint32_t)env_int("MXCO_OA_PROBE_FLAGS", 0);
     (void)hipGetDevice(&t->device);
 
-    hipError_t e = hipMalloc((void**)&t->slots, cap * t->ss);
+    // rounded up to 16 bytes: the CAS of a 1- or 2-byte slot works on the
+    // aligned 32-bit word that holds it
+    hipError_t e = hipMalloc((void**)&t->slots, (cap * t->ss + 15) & ~15ull);
     if (e != hipSuccess) {
         delete t;
         return hip_fail(e, "mxco_oa_create: slot array");

@@ -1,0 +1,9 @@
+// Open-addressing kernels for the 2-byte key / 8-byte payload slot layout.
+#include "oa_launch.h"
+
+namespace mxco {
+hipError_t launch_oa_k2v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight)
+{
+    return launch_oa_layout<2, 8>(g, op, a, grid, s, inflight);
+}
+}  // namespace mxco

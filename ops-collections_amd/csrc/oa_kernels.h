@@ -65,77 +65,136 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 // Slot layouts: Pair<K,V> with alignas(min(16, bit_ceil(sK+sV)))
 // (include/pair.h:41, detail/pair/pair.inl:20-25); sets store the key only.
-template <int KS, int VS>
-struct Lay;
+// Keys and payloads are 1, 2, 4 or 8 bytes; the payload sits at its natural
+// alignment after the key (offset max(sK, sV)), and the slot is 1 to 16
+// bytes.  Bytes that are neither key nor payload are padding.
+template <int N>
+struct RawOf;
+template <>
+struct RawOf<1> {
+    using T = uint8_t;
+};
+template <>
+struct RawOf<2> {
+    using T = uint16_t;
+};
+template <>
+struct RawOf<4> {
+    using T = uint32_t;
+};
+template <>
+struct RawOf<8> {
+    using T = uint64_t;
+};
+template <>
+struct RawOf<16> {
+    using T = u64x2;
+};
 
-template <>
-struct Lay<4, 4> {  // 8-byte slot {k32, v32}
-    static constexpr int SS = 8;
-    using Raw = uint64_t;
-    static __device__ uint64_t key(Raw r) { return (uint32_t)r; }
-    static __device__ uint64_t val(Raw r) { return r >> 32; }
+constexpr int pow2_at_least(int n)
+{
+    int x = 1;
+    while (x < n) x <<= 1;
+    return x;
+}
+
+constexpr uint64_t low_bytes_mask(int bytes) { return bytes >= 8 ? ~0ull : ((1ull << (8 * bytes)) - 1ull); }
+
+template <int KS, int VS>
+struct Lay {
+    static_assert(KS == 1 || KS == 2 || KS == 4 || KS == 8, "key width");
+    static_assert(VS == 0 || VS == 1 || VS == 2 || VS == 4 || VS == 8, "payload width");
+    static constexpr int SS = VS ? (pow2_at_least(KS + VS) < 16 ? pow2_at_least(KS + VS) : 16) : KS;
+    static constexpr int VOFF = VS ? (KS > VS ? KS : VS) : 0;  // 8 in every 16-byte slot
+    using Raw = typename RawOf<SS>::T;
+    static constexpr uint64_t KMASK = low_bytes_mask(KS);
+    static constexpr uint64_t VMASK = low_bytes_mask(VS);
+    // padding bits of a slot of at most 8 bytes
+    static constexpr uint64_t PADMASK =
+        SS <= 8 ? (low_bytes_mask(SS) & ~KMASK & ~(VS ? (VMASK << (8 * VOFF)) : 0ull)) : 0ull;
+    static __device__ __forceinline__ uint64_t key(Raw r)
+    {
+        if constexpr (SS == 16) return r.x & KMASK;
+        else return (uint64_t)r & KMASK;
+    }
+    static __device__ __forceinline__ uint64_t val(Raw r)
+    {
+        if constexpr (VS == 0) return key(r);
+        else if constexpr (SS == 16) return r.y & VMASK;
+        else return ((uint64_t)r >> (8 * VOFF)) & VMASK;
+    }
+    // A slot of at most 8 bytes holding k and v over the padding of `pad`.
+    static __device__ __forceinline__ uint64_t pack(uint64_t pad, uint64_t k, uint64_t v)
+    {
+        uint64_t r = (pad & PADMASK) | (k & KMASK);
+        if constexpr (VS != 0) r |= (v & VMASK) << (8 * VOFF);
+        return r;
+    }
 };
-template <>
-struct Lay<8, 8> {  // 16-byte slot {k64, v64}
-    static constexpr int SS = 16;
-    using Raw = u64x2;
-    static __device__ uint64_t key(Raw r) { return r.x; }
-    static __device__ uint64_t val(Raw r) { return r.y; }
-};
-template <>
-struct Lay<4, 8> {  // 16-byte slot {k32, pad32, v64}
-    static constexpr int SS = 16;
-    using Raw = u64x2;
-    static __device__ uint64_t key(Raw r) { return (uint32_t)r.x; }
-    static __device__ uint64_t val(Raw r) { return r.y; }
-};
-template <>
-struct Lay<8, 4> {  // 16-byte slot {k64, v32, pad32}
-    static constexpr int SS = 16;
-    using Raw = u64x2;
-    static __device__ uint64_t key(Raw r) { return r.x; }
-    static __device__ uint64_t val(Raw r) { return (uint32_t)r.y; }
-};
-template <>
-struct Lay<4, 0> {  // 4-byte set slot
-    static constexpr int SS = 4;
-    using Raw = uint32_t;
-    static __device__ uint64_t key(Raw r) { return r; }
-    static __device__ uint64_t val(Raw r) { return r; }
-};
-template <>
-struct Lay<8, 0> {  // 8-byte set slot
-    static constexpr int SS = 8;
-    using Raw = uint64_t;
-    static __device__ uint64_t key(Raw r) { return r; }
-    static __device__ uint64_t val(Raw r) { return r; }
-};
+
+// Slot layouts whose bulk Find / Contains use the staged kernels: 4- and
+// 8-byte keys and payloads.  The others (a 1- or 2-byte key or payload) run
+// every operation on the plain kernel.
+template <int KS, int VS>
+constexpr bool kWideLayout = KS >= 4 && (VS == 0 || VS >= 4);
 
 template <int N>
 __device__ __forceinline__ uint64_t load_bits(const uint8_t* p)
 {
-    if constexpr (N == 4) return *(const uint32_t*)p;
+    if constexpr (N == 1) return *p;
+    else if constexpr (N == 2) return *(const uint16_t*)p;
+    else if constexpr (N == 4) return *(const uint32_t*)p;
     else return *(const uint64_t*)p;
 }
 
 template <int N>
 __device__ __forceinline__ void store_bits(uint8_t* p, uint64_t v)
 {
-    if constexpr (N == 4) *(uint32_t*)p = (uint32_t)v;
+    if constexpr (N == 1) *p = (uint8_t)v;
+    else if constexpr (N == 2) *(uint16_t*)p = (uint16_t)v;
+    else if constexpr (N == 4) *(uint32_t*)p = (uint32_t)v;
     else *(uint64_t*)p = v;
+}
+
+// Compare-and-swap of the N = 1 or 2 bytes at p, on the aligned 32-bit word
+// that holds them: the other bytes of the word (a neighbouring slot, or
+// padding) are kept, and a CAS that fails because only they changed is
+// retried (the reference's AtomicCas2Byte, include/utility/atomic_cas_wrap.h:20-56).
+// Returns the N bytes found at p (== expected on success).
+template <int N>
+__device__ __forceinline__ uint64_t cas_narrow(uint8_t* p, uint64_t expected, uint64_t desired)
+{
+    const uintptr_t addr = (uintptr_t)p;
+    unsigned int* word = (unsigned int*)(addr & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(addr & 3u) * 8u;
+    const uint32_t m = (uint32_t)low_bytes_mask(N) << sh;
+    uint32_t cur = *word;
+    for (;;) {
+        const uint32_t mine = (cur & m) >> sh;
+        if (mine != (uint32_t)expected) return mine;
+        const uint32_t want = (cur & ~m) | (((uint32_t)desired << sh) & m);
+        const uint32_t prev = atomicCAS(word, cur, want);
+        if (prev == cur) return expected;
+        cur = prev;  // a neighbouring byte changed: try again on the new word
+    }
 }
 
 template <int N>
 __device__ __forceinline__ uint64_t cas_bits(uint8_t* p, uint64_t expected, uint64_t desired)
 {
-    if constexpr (N == 4) return atomicCAS((unsigned int*)p, (unsigned int)expected, (unsigned int)desired);
+    if constexpr (N <= 2) return cas_narrow<N>(p, expected, desired);
+    else if constexpr (N == 4) return atomicCAS((unsigned int*)p, (unsigned int)expected, (unsigned int)desired);
     else return atomicCAS((unsigned long long*)p, (unsigned long long)expected, (unsigned long long)desired);
 }
 
 template <int N>
 __device__ __forceinline__ uint64_t atomic_load_bits(const uint8_t* p)
 {
-    if constexpr (N == 4)
+    if constexpr (N == 1)
+        return __hip_atomic_load((const uint8_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if constexpr (N == 2)
+        return __hip_atomic_load((const uint16_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if constexpr (N == 4)
         return __hip_atomic_load((const unsigned int*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
         return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -147,11 +206,24 @@ __device__ __forceinline__ uint64_t atomic_load_bits(const uint8_t* p)
 template <int N>
 __device__ __forceinline__ void publish_bits(uint8_t* p, uint64_t v)
 {
-    if constexpr (N == 4)
+    if constexpr (N == 1)
+        __hip_atomic_store((uint8_t*)p, (uint8_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if constexpr (N == 2)
+        __hip_atomic_store((uint16_t*)p, (uint16_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if constexpr (N == 4)
         __hip_atomic_store((unsigned int*)p, (unsigned int)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
         __hip_atomic_store((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// IEEE binary16 equality on raw bits: NaN equals nothing, +0 equals -0.
+__device__ __forceinline__ bool half_eq(uint64_t a, uint64_t b)
+{
+    const bool nan_a = (a & 0x7c00u) == 0x7c00u && (a & 0x03ffu) != 0;
+    const bool nan_b = (b & 0x7c00u) == 0x7c00u && (b & 0x03ffu) != 0;
+    if (nan_a || nan_b) return false;
+    return a == b || ((a | b) & 0x7fffu) == 0;
 }
 
 // KeyEqual = EqualTo<Key> (== on the key type, utility/math_utils.h:67).
@@ -159,8 +231,9 @@ template <int KS>
 __device__ __forceinline__ bool key_eq(uint64_t a, uint64_t b, uint32_t is_float)
 {
     if (is_float) {
-        if constexpr (KS == 4) return __uint_as_float((uint32_t)a) == __uint_as_float((uint32_t)b);
-        else return __longlong_as_double((long long)a) == __longlong_as_double((long long)b);
+        if constexpr (KS == 2) return half_eq(a, b);
+        else if constexpr (KS == 4) return __uint_as_float((uint32_t)a) == __uint_as_float((uint32_t)b);
+        else if constexpr (KS == 8) return __longlong_as_double((long long)a) == __longlong_as_double((long long)b);
     }
     return a == b;
 }
@@ -253,7 +326,9 @@ __device__ __forceinline__ void store_payload(uint8_t* p, uint64_t v, uint32_t m
 {
     if (mode == 1) publish_bits<N>(p, v);
     else if (mode == 2) {
-        if constexpr (N == 4) __builtin_nontemporal_store((uint32_t)v, (uint32_t*)p);
+        if constexpr (N == 1) __builtin_nontemporal_store((uint8_t)v, p);
+        else if constexpr (N == 2) __builtin_nontemporal_store((uint16_t)v, (uint16_t*)p);
+        else if constexpr (N == 4) __builtin_nontemporal_store((uint32_t)v, (uint32_t*)p);
         else __builtin_nontemporal_store(v, (uint64_t*)p);
     } else {
         store_bits<N>(p, v);
@@ -270,12 +345,14 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
     constexpr int SS = L::SS;
-    constexpr int SPL = 128 / SS;  // slots per 128-byte line
+    // slots per window: a 128-byte line; 16 slots (16 or 32 bytes) for 1- and
+    // 2-byte slots, one load per slot
+    constexpr int SPL = SS >= 4 ? 128 / SS : 16;
     constexpr int K = SPL / G;     // slots per lane per window
     constexpr bool kInsertFamily =
         OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
     constexpr int OUTS = VS ? VS : KS;  // Find / InsertAndFind output width
-    constexpr int VOFF = SS == 8 ? 4 : 8;
+    constexpr int VOFF = L::VOFF;
     constexpr uint64_t kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
 
     const uint32_t lane = __lane_id();
@@ -295,17 +372,24 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
     // gid + u*ngroups, gid + (u+U)*ngroups, ...  The window loads of all U
     // walks are issued before any of them is examined.
     const uint64_t gid = first;
-    uint64_t I[U], KEY[U], PVAL[U], PREV[U];
+    // RAW: the input pair as one word (slots of at most 8 bytes), padding
+    // included: the whole-slot CAS stores it as it is.
+    uint64_t I[U], KEY[U], PVAL[U], PREV[U], RAW[U];
     Walk W[U];
     uint32_t WLEN[U];
     Raw S[U][K];
     // Key (and payload) of slot u's next index, loaded one key ahead so a
     // group that finishes a key starts the next walk without a load round
     // trip.
-    uint64_t NKEY[U], NPVAL[U];
+    uint64_t NKEY[U], NPVAL[U], NRAW[U];
     auto prefetch = [&](int u, uint64_t idx) {
         if (idx >= end) return;
-        if constexpr (kInsertFamily) {
+        if constexpr (kInsertFamily && SS <= 8) {
+            const uint64_t raw = load_bits<SS>(a.in + idx * SS);
+            NRAW[u] = raw;
+            NKEY[u] = raw & L::KMASK;
+            if constexpr (VS) NPVAL[u] = (raw >> (8 * VOFF)) & L::VMASK;
+        } else if constexpr (kInsertFamily) {
             NKEY[u] = load_bits<KS>(a.in + idx * SS);
             if constexpr (VS) NPVAL[u] = load_bits<VS>(a.in + idx * SS + VOFF);
         } else {
@@ -320,6 +404,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
         uint64_t& pval = PVAL[u];
         key = NKEY[u];
         if constexpr (kInsertFamily && VS) pval = NPVAL[u];
+        if constexpr (kInsertFamily && SS <= 8) RAW[u] = NRAW[u];
         prefetch(u, idx + (uint64_t)U * ngroups);
         if (!stencil_ok(a, idx)) {
             if (leader) {
@@ -355,6 +440,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
         W[u].done = true;
         NKEY[u] = 0;
         NPVAL[u] = 0;
+        NRAW[u] = 0;
         prefetch(u, gid + (uint64_t)u * ngroups);
         advance(u, gid + (uint64_t)u * ngroups);
     }
@@ -486,16 +572,14 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                         if (winner) {
                             if constexpr (SS <= 8) {
                                 const uint64_t expect = (uint64_t)ws;
-                                uint64_t desired;
-                                if constexpr (VS) desired = (a.erased_key & 0xffffffffull) | (a.empty_val << 32);
-                                else desired = a.erased_key;
+                                const uint64_t desired = L::pack(expect, a.erased_key, a.empty_val);
                                 const uint64_t old = cas_bits<SS>(wslot, expect, desired);
                                 if (old == expect) r = 1;
                                 else if (L::key((Raw)old) == a.erased_key) r = 2;
                             } else {
                                 const uint64_t old = cas_bits<KS>(wslot, L::key(ws), a.erased_key);
                                 if (old == L::key(ws)) {
-                                    store_bits<VS>(wslot + 8, a.empty_val);
+                                    store_bits<VS>(wslot + VOFF, a.empty_val);
                                     r = 1;
                                 } else if (old == a.erased_key) {
                                     r = 2;
@@ -527,15 +611,12 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                         if (winner) {
                             if constexpr (SS <= 8) {
                                 // PackCas / PackCasInsertOrAssign: one CAS of the whole slot
-                                // against (emptyKey, emptyValue) (ref_impl.h:127-147, 234-257).
-                                uint64_t expect, desired;
-                                if constexpr (VS) {
-                                    expect = (ekey & 0xffffffffull) | (a.empty_val << 32);
-                                    desired = (key & 0xffffffffull) | (pval << 32);
-                                } else {
-                                    expect = ekey;
-                                    desired = key;
-                                }
+                                // against (emptyKey, emptyValue) over the slot's own padding
+                                // (PackCasExpected), storing the input pair as it is; slots
+                                // under 4 bytes CAS inside their 32-bit word (NarrowCas)
+                                // (ref_impl.h:109-162, 234-257).
+                                const uint64_t expect = L::pack((uint64_t)ws, ekey, a.empty_val);
+                                const uint64_t desired = RAW[u];
                                 const uint64_t old = cas_bits<SS>(wslot, expect, desired);
                                 if (old == expect) {
                                     r = 1;
@@ -549,12 +630,12 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                                 // then store the payload (ref_impl.h:165-185, 260-279).
                                 const uint64_t old = cas_bits<KS>(wslot, ekey, key);
                                 if (old == ekey) {
-                                    if constexpr (OP == OP_INSERT_AND_FIND) publish_bits<VS>(wslot + 8, pval);
-                                    else store_payload<VS>(wslot + 8, pval, a.store_mode);
+                                    if constexpr (OP == OP_INSERT_AND_FIND) publish_bits<VS>(wslot + VOFF, pval);
+                                    else store_payload<VS>(wslot + VOFF, pval, a.store_mode);
                                     r = 1;
                                 } else if (key_eq<KS>(old, key, a.key_float)) {
                                     r = 2;
-                                    if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<VS>(wslot + 8, pval);
+                                    if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<VS>(wslot + VOFF, pval);
                                 }
                             }
                         }
@@ -582,9 +663,9 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                                     // The window's 16-byte load already holds the final payload
                                     // unless it still read the empty value.
                                     v = seen_window ? seen : a.empty_val;
-                                    if (v == a.empty_val) v = atomic_load_bits<VS>(wslot + 8);
+                                    if (v == a.empty_val) v = atomic_load_bits<VS>(wslot + VOFF);
                                     for (int spin = 0; spin < (1 << 22) && v == a.empty_val; ++spin)
-                                        v = atomic_load_bits<VS>(wslot + 8);
+                                        v = atomic_load_bits<VS>(wslot + VOFF);
                                 } else {
                                     v = seen;
                                 }
@@ -975,8 +1056,10 @@ static __global__ __launch_bounds__(256) void oa_fill_kernel(uint8_t* table, uin
     u64x2* p = (u64x2*)table;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) p[i] = pattern;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint32_t* pw = (const uint32_t*)&pattern;
-        for (uint64_t b = n16 * 16; b < bytes; b += 4) *(uint32_t*)(table + b) = pw[(b % 16) / 4];
+        // the last bytes 15 or fewer, one at a time (1- and 2-byte slots can
+        // leave any count)
+        const uint8_t* pb = (const uint8_t*)&pattern;
+        for (uint64_t b = n16 * 16; b < bytes; ++b) table[b] = pb[b % 16];
     }
 }
 

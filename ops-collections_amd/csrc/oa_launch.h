@@ -12,6 +12,8 @@ template <int KS, int VS, int G>
 hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight)
 {
     const dim3 block(256);
+    if constexpr (!kWideLayout<KS, VS>) inflight = 1;  // 1- and 2-byte fields: the plain kernel only
+    if constexpr (kWideLayout<KS, VS>) {
     if (inflight == 4) {
         // wave-staged outputs: the bulk Find / Contains paths
         switch (op) {
@@ -40,6 +42,7 @@ hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s, int in
         case OP_INSERT: oa_kernel<KS, VS, G, OP_INSERT, 2><<<grid, block, 0, s>>>(a); return hipGetLastError();
         default: break;
         }
+    }
     }
     switch (op) {
     case OP_FIND: oa_kernel<KS, VS, G, OP_FIND><<<grid, block, 0, s>>>(a); break;
@@ -80,6 +83,13 @@ hipError_t launch_oa_k4v8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t
 hipError_t launch_oa_k8v4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
 hipError_t launch_oa_k4(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
 hipError_t launch_oa_k8(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+// layouts with a 1- or 2-byte key or payload (oa_inst_narrow_*.hip)
+#define MXCO_NARROW_LAYOUTS(X) \
+    X(1, 0) X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 0) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(8, 1) X(8, 2)
+#define MXCO_DECLARE_NARROW(K, V) \
+    hipError_t launch_oa_k##K##v##V(int g, int op, const OaArgs& a, dim3 grid, hipStream_t s, int inflight);
+MXCO_NARROW_LAYOUTS(MXCO_DECLARE_NARROW)
+#undef MXCO_DECLARE_NARROW
 
 inline hipError_t launch_oa(int ks, int vs, int g, int op, const OaArgs& a, dim3 grid, hipStream_t s,
                             int inflight = 1)
@@ -90,6 +100,10 @@ inline hipError_t launch_oa(int ks, int vs, int g, int op, const OaArgs& a, dim3
     if (ks == 8 && vs == 4) return launch_oa_k8v4(g, op, a, grid, s, inflight);
     if (ks == 4 && vs == 0) return launch_oa_k4(g, op, a, grid, s, inflight);
     if (ks == 8 && vs == 0) return launch_oa_k8(g, op, a, grid, s, inflight);
+#define MXCO_DISPATCH_NARROW(K, V) \
+    if (ks == K && vs == V) return launch_oa_k##K##v##V(g, op, a, grid, s, inflight);
+    MXCO_NARROW_LAYOUTS(MXCO_DISPATCH_NARROW)
+#undef MXCO_DISPATCH_NARROW
     return hipErrorInvalidValue;
 }
 

@@ -151,9 +151,10 @@ def slot_layout(key_dtype, value_dtype):
     if value_dtype is None:
         return ks, 0
     vs = torch.empty(0, dtype=value_dtype).element_size()
-    if ks == 4 and vs == 4:
-        return 8, 4
-    return 16, 8
+    align = 1
+    while align < ks + vs:
+        align <<= 1
+    return min(align, 16), max(ks, vs)
 
 
 def make_pairs(keys, values, key_dtype=None, value_dtype=None):

@@ -408,10 +408,25 @@ ORC_API uint8_t* orc_oa_slots(orc_oa* t) { return t->slots; }
 /* Sentinel test is bitwise (include/detail/bitwise_compare.h:91-145). */
 static inline int is_empty_key(const orc_oa* t, const uint8_t* k) { return memcmp(k, t->empty_key, t->ks) == 0; }
 
+/* IEEE binary16 ==, by converting both to float (exact for every half). */
+static float half_to_float(uint16_t h)
+{
+    uint32_t sign = (uint32_t)(h >> 15) << 31, exp = (h >> 10) & 31u, man = h & 1023u;
+    float f;
+    if (exp == 31) {
+        uint32_t bits = sign | 0x7f800000u | (man << 13);
+        memcpy(&f, &bits, 4);
+        return f;
+    }
+    f = exp ? ldexpf((float)(man | 1024u), (int)exp - 25) : ldexpf((float)man, -24);
+    return sign ? -f : f;
+}
+
 /* KeyEqual = EqualTo<Key>, operator== (include/utility/math_utils.h:67). */
 static inline int key_equal(const orc_oa* t, const uint8_t* a, const uint8_t* b)
 {
     if (t->key_float) {
+        if (t->ks == 2) { uint16_t x, y; memcpy(&x, a, 2); memcpy(&y, b, 2); return half_to_float(x) == half_to_float(y); }
         if (t->ks == 4) { float x, y; memcpy(&x, a, 4); memcpy(&y, b, 4); return x == y; }
         double x, y; memcpy(&x, a, 8); memcpy(&y, b, 8); return x == y;
     }

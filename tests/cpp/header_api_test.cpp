@@ -223,6 +223,46 @@ int main()
         CHECK(hipFree(dc) == hipSuccess);
         CHECK(hipFree(dout) == hipSuccess);
     }
+    {  // 1- and 2-byte keys: Pair<uint8_t, uint32_t> (3 padding bytes) and an
+       // int16 set whose two slots share one 32-bit CAS word
+        static_assert(sizeof(aclco::Pair<uint8_t, uint32_t>) == 8, "Pair<uint8_t, uint32_t> layout");
+        static_assert(sizeof(aclco::Pair<int16_t, int64_t>) == 16, "Pair<int16_t, int64_t> layout");
+        aclco::StaticMap<uint8_t, uint32_t, aclco::Extent<size_t>, aclco::EqualTo<uint8_t>,
+                         aclco::LinearProbing<aclco::murmurhash3_32<uint8_t>>, aclco::Storage<1>>
+            map(aclco::Extent<size_t>{128}, 255, 0xffffffffu);
+        std::vector<aclco::Pair<uint8_t, uint32_t>> pairs(255);
+        for (int i = 0; i < 255; ++i) pairs[i] = {static_cast<uint8_t>(i), static_cast<uint32_t>(i * 1000)};
+        auto* dp = to_device(pairs);
+        CHECK(map.Insert(dp, pairs.size(), s) == 255 - 128);  // 128 slots
+        std::vector<uint8_t> keys(255);
+        for (int i = 0; i < 255; ++i) keys[i] = static_cast<uint8_t>(i);
+        auto* dk = to_device(keys);
+        CHECK(map.Count(dk, keys.size(), s) == 128);
+        uint32_t* dv;
+        CHECK(hipMalloc(&dv, 255 * sizeof(uint32_t)) == hipSuccess);
+        map.Find(dk, dv, keys.size(), s);
+        auto v = to_host(dv, 255);
+        int found = 0;
+        for (int i = 0; i < 255; ++i) {
+            CHECK(v[i] == 0xffffffffu || v[i] == static_cast<uint32_t>(i * 1000));
+            found += v[i] != 0xffffffffu;
+        }
+        CHECK(found == 128);
+
+        aclco::StaticSet<int16_t, aclco::Extent<size_t>, aclco::EqualTo<int16_t>,
+                         aclco::LinearProbing<aclco::murmurhash3_32<int16_t>>, aclco::Storage<1>>
+            set(aclco::Extent<size_t>{2}, -1);
+        CHECK(set.Capacity() == 2);
+        std::vector<int16_t> both = {2, -2};
+        auto* db = to_device(both);
+        CHECK(set.Insert(db, 1, s) == 0);
+        CHECK(set.Insert(db + 1, 1, s) == 0);
+        CHECK(set.Count(db, 2, s) == 2);
+        CHECK(set.Erase(db + 1, 1, s) == 0);
+        CHECK(set.Count(db, 2, s) == 1);
+        CHECK(hipFree(dp) == hipSuccess && hipFree(dk) == hipSuccess && hipFree(dv) == hipSuccess);
+        CHECK(hipFree(db) == hipSuccess);
+    }
     CHECK(hipStreamDestroy(s) == hipSuccess);
     std::printf("OK\n");
     return 0;

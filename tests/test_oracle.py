@@ -51,7 +51,7 @@ def test_murmur3_published_vectors():
         assert L.orc_murmur3_32(data, len(data), seed) == exp, (data, seed)
 
 
-@pytest.mark.parametrize("n", [0, 1, 3, 4, 7, 8, 15, 16, 31, 32, 33, 64, 100])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 7, 8, 15, 16, 31, 32, 33, 64, 100])
 def test_xxhash_against_library(n):
     rng = np.random.default_rng(n)
     for _ in range(20):
