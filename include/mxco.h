@@ -494,20 +494,39 @@ MXCO_API mxco_status mxco_server_stats(mxco_server* server, uint64_t* ticks, uin
 MXCO_API mxco_status mxco_server_table(mxco_server* server, uint32_t table, mxco_oa** map, mxco_mm** multimap);
 
 /* ---------------------------------------------------------------------- */
-/* BloomFilter (include/bloom_filter.h:37-133): the default policy        */
-/* BloomFilterPolicy<Key, xxhash_64<Key>, uint32_t, 8 words, 8 bits>      */
-/* (include/bloom_filter_policy.h:54-181).  Keys are 4 or 8 bytes         */
+/* BloomFilter (include/bloom_filter.h:37-133) under                      */
+/* BloomFilterPolicy<Key, xxhash_64<Key>, uint32_t, W words, P bits>      */
+/* (include/bloom_filter_policy.h:54-211).  Keys are 4 or 8 bytes         */
 /* (int32/uint32/float or int64/uint64); the hash covers their bytes.     */
 /* ---------------------------------------------------------------------- */
 
 typedef struct mxco_bloom mxco_bloom;
 
-/* BloomFilter(numBlocks, policy, stream) (bloom_filter_lifecycle.inl:15-55):
- * num_blocks 256-bit blocks, cleared; seed is the policy's xxhash_64 seed. */
+/* The policy's parameters that fix the filter bits
+ * (bloom_filter_policy.h:54-98): words_per_block W (a power of two, 1..32),
+ * pattern_bits P (a multiple of W, at most 64 and 32 * W; P / W bits per
+ * word) and the seed of its xxhash_64<Key> hasher.  The Add / Contains
+ * layout parameters of the reference policy change no result and are not
+ * carried.  DefaultFilterPolicy<.., W> is {W, W}; ArrowFilterPolicy {8, 8}. */
+typedef struct mxco_bloom_policy {
+    uint32_t words_per_block;
+    uint32_t pattern_bits;
+    uint64_t seed;
+} mxco_bloom_policy;
+
+/* BloomFilter(numBlocks, policy, allocator, stream)
+ * (bloom_filter_lifecycle.inl:15-62): num_blocks blocks of W words, cleared.
+ * storage: NULL = the library allocates the words and frees them in
+ * destroy; else the caller's device memory of num_blocks * W words (what the
+ * C++ header's Allocator returns), which destroy leaves to the caller. */
+MXCO_API mxco_status mxco_bloom_create_with_policy(uint64_t num_blocks, uint32_t key_size,
+                                                   const mxco_bloom_policy* policy, uint32_t* storage,
+                                                   hipStream_t stream, mxco_bloom** out);
+/* BloomFilter(numBlocks, policy, stream) under the default policy {8, 8, seed}. */
 MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, uint64_t seed, hipStream_t stream,
                                        mxco_bloom** out);
 MXCO_API mxco_status mxco_bloom_destroy(mxco_bloom* filter);
-/* Data() / BlockExtent(): device pointer to the 8*num_blocks words. */
+/* Data() / BlockExtent(): device pointer to the W * num_blocks words. */
 MXCO_API mxco_status mxco_bloom_data(const mxco_bloom* filter, uint32_t** words, uint64_t* num_blocks);
 /* Clear / ClearAsync. */
 MXCO_API mxco_status mxco_bloom_clear(mxco_bloom* filter, hipStream_t stream, int sync);

@@ -13,12 +13,14 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
 
 #include "../mxco.h"
+#include "detail/hash.h"
 
 namespace mxco {
 
@@ -65,10 +67,29 @@ struct xxhash_32 {
     static constexpr mxco_hash id = MXCO_HASH_XXHASH_32;
     constexpr bool operator==(xxhash_32 const&) const noexcept { return true; }
 };
+// xxhash_64 carries a seed: BloomFilterPolicy is built from Hasher{seed}
+// (include/bloom_filter_policy.h:78).  Open-addressing containers use seeds 0
+// and 1 whatever it holds, as the reference's stateless probing does.
 template <typename Key>
 struct xxhash_64 {
+    using ArgumentType = Key;
+    using ResultType = std::uint64_t;
     static constexpr mxco_hash id = MXCO_HASH_XXHASH_64;
-    constexpr bool operator==(xxhash_64 const&) const noexcept { return true; }
+    constexpr xxhash_64(std::uint64_t seed = 0) noexcept : seed_{seed} {}
+    constexpr std::uint64_t Seed() const noexcept { return seed_; }
+    std::uint64_t operator()(Key const& key) const noexcept
+    {
+        static_assert(sizeof(Key) == 1 || sizeof(Key) == 2 || sizeof(Key) == 4 || sizeof(Key) == 8,
+                      "xxhash_64: 1-, 2-, 4- or 8-byte keys");
+        std::uint64_t bits = 0;
+        std::memcpy(&bits, &key, sizeof(Key));
+        return xxhash64<sizeof(Key)>(bits, seed_);
+    }
+    constexpr bool operator==(xxhash_64 const& o) const noexcept { return seed_ == o.seed_; }
+    constexpr bool operator!=(xxhash_64 const& o) const noexcept { return seed_ != o.seed_; }
+
+private:
+    std::uint64_t seed_;
 };
 template <typename Key>
 struct murmurhash3_fmix32 {
@@ -103,6 +124,31 @@ template <typename T>
 struct EqualTo {
     constexpr bool operator()(T const& lhs, T const& rhs) const noexcept { return lhs == rhs; }
     constexpr bool operator==(EqualTo const&) const noexcept { return true; }
+};
+
+// DefaultAllocator<T> (include/utility/allocator.h:20-50): device memory
+// from hipMalloc; Allocate returns nullptr on failure.
+template <typename T>
+class DefaultAllocator {
+public:
+    using ValueType = T;
+    using value_type = T;
+    DefaultAllocator() = default;
+    template <class U>
+    DefaultAllocator(DefaultAllocator<U> const&) noexcept
+    {
+    }
+    ValueType* Allocate(std::size_t n) const
+    {
+        if (n > static_cast<std::size_t>(-1) / sizeof(ValueType)) return nullptr;
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof(ValueType) * n) != hipSuccess) return nullptr;
+        return static_cast<ValueType*>(p);
+    }
+    void Deallocate(ValueType* p)
+    {
+        if (p != nullptr) (void)hipFree(p);
+    }
 };
 
 // Key and payload types compared as floating point: the standard floating

@@ -18,11 +18,6 @@
 
 namespace mxco {
 
-template <typename T>
-struct DefaultAllocator {
-    using ValueType = T;
-};
-
 template <typename Key>
 class RoaringBitmapRef {
     static_assert(std::is_same_v<Key, uint32_t> || std::is_same_v<Key, uint64_t>,

@@ -19,6 +19,12 @@
 //    four random global atomics.
 // Contains reads the block with two 16-byte loads.  Merge / Intersect are
 // word-wise OR / AND streams.
+//
+// Other policies (BloomFilterPolicy<Key, xxhash_64, uint32_t, W, P>, W = 1..32
+// words per block, P pattern bits spread P / W per word over the 64-salt
+// table, include/bloom_filter_policy.h:54-168) run a generic kernel pair:
+// one key per lane, the block read once, atomic ORs only for words that
+// miss bits.
 #include <stdlib.h>
 #include <string.h>
 
@@ -29,6 +35,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "mxco_common.h"
+#include "mxco/detail/bloom_salts.h"
 #include "mxco/detail/hash.h"
 
 using namespace mxco;
@@ -38,15 +45,17 @@ struct mxco_bloom {
     uint64_t num_blocks;
     uint32_t key_size;
     uint64_t seed;
+    uint32_t wpb;           // words per block
+    uint32_t pattern_bits;  // bits set per key
+    bool owns_words;        // false: the caller's storage
     uint8_t* ws;  // partitioned-Add workspace (grown on demand)
     size_t ws_bytes;
 };
 
 namespace {
 
-// Parquet split-block Bloom filter salts (bloom_filter_policy.h:170-178, first eight).
-__constant__ uint32_t kSalts[8] = {0x47b6137bu, 0x44974d91u, 0x8824ad5bu, 0xa2b7289du,
-                                   0x705495c7u, 0x2df1424bu, 0x9efc4947u, 0x5c6bfb31u};
+// Pattern salts of BloomFilterPolicy (include/mxco/detail/bloom_salts.h).
+using mxco::detail::kBloomSalts;
 
 template <int KS>
 __device__ __forceinline__ uint64_t key_at(const uint8_t* keys, uint64_t i)
@@ -61,8 +70,8 @@ __device__ __forceinline__ void locate(uint64_t h, uint32_t nb, uint32_t& blk, u
     const uint32_t lo = (uint32_t)h;
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        const uint32_t a = 1u << ((kSalts[2 * p] * lo) >> 27);
-        const uint32_t b = 1u << ((kSalts[2 * p + 1] * lo) >> 27);
+        const uint32_t a = 1u << ((kBloomSalts[2 * p] * lo) >> 27);
+        const uint32_t b = 1u << ((kBloomSalts[2 * p + 1] * lo) >> 27);
         m[p] = (uint64_t)a | ((uint64_t)b << 32);
     }
 }
@@ -112,6 +121,89 @@ __global__ __launch_bounds__(256) void bloom_contains_kernel(const uint32_t* wor
         out[i] = hit ? 1 : 0;
     }
 }
+
+// Word w's pattern under a policy of `bpw` bits per word: bit
+// (salt[w * bpw + b] * lo) >> 27 for b < bpw (WordPatternImpl,
+// bloom_filter_policy.h:157-168).
+__device__ __forceinline__ uint32_t word_pattern(uint32_t w, uint32_t bpw, uint32_t lo)
+{
+    uint32_t m = 0;
+    for (uint32_t b = 0; b < bpw; ++b) m |= 1u << ((kBloomSalts[w * bpw + b] * lo) >> 27);
+    return m;
+}
+
+// One key per lane: the W words of its block are read once; a word gets an
+// atomic OR only when it misses bits.
+template <int KS, int W>
+__global__ __launch_bounds__(256) void bloom_generic_add_kernel(uint32_t* words, uint32_t nb, uint32_t bpw,
+                                                                const uint8_t* keys, const uint8_t* stencil,
+                                                                uint64_t n, uint64_t seed)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (stencil && stencil[i] == 0) continue;
+        const uint64_t h = xxhash64<KS>(key_at<KS>(keys, i), seed);
+        const uint32_t blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * nb) >> 32);
+        uint32_t* b = words + (uint64_t)blk * W;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t m = word_pattern((uint32_t)w, bpw, (uint32_t)h);
+            if ((b[w] & m) != m) atomicOr(b + w, m);
+        }
+    }
+}
+
+template <int KS, int W>
+__global__ __launch_bounds__(256) void bloom_generic_contains_kernel(const uint32_t* words, uint32_t nb, uint32_t bpw,
+                                                                     const uint8_t* keys, const uint8_t* stencil,
+                                                                     uint8_t* out, uint64_t n, uint64_t seed)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (stencil && stencil[i] == 0) {
+            out[i] = 0;
+            continue;
+        }
+        const uint64_t h = xxhash64<KS>(key_at<KS>(keys, i), seed);
+        const uint32_t blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * nb) >> 32);
+        const uint32_t* b = words + (uint64_t)blk * W;
+        bool hit = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t m = word_pattern((uint32_t)w, bpw, (uint32_t)h);
+            hit &= (b[w] & m) == m;
+        }
+        out[i] = hit ? 1 : 0;
+    }
+}
+
+template <int KS>
+hipError_t launch_generic(bool add, const mxco_bloom* f, const uint8_t* keys, const uint8_t* stencil, uint8_t* out,
+                          uint64_t n, hipStream_t s)
+{
+    const unsigned grid = grid_for(n, 8);
+    const uint32_t nb = (uint32_t)f->num_blocks, bpw = f->pattern_bits / f->wpb;
+#define MXCO_BLOOM_GENERIC(W)                                                                                  \
+    case W:                                                                                                    \
+        if (add) bloom_generic_add_kernel<KS, W><<<grid, 256, 0, s>>>(f->words, nb, bpw, keys, stencil, n, f->seed); \
+        else bloom_generic_contains_kernel<KS, W><<<grid, 256, 0, s>>>(f->words, nb, bpw, keys, stencil, out, n,     \
+                                                                        f->seed);                                   \
+        break;
+    switch (f->wpb) {
+        MXCO_BLOOM_GENERIC(1)
+        MXCO_BLOOM_GENERIC(2)
+        MXCO_BLOOM_GENERIC(4)
+        MXCO_BLOOM_GENERIC(8)
+        MXCO_BLOOM_GENERIC(16)
+        MXCO_BLOOM_GENERIC(32)
+    default: return hipErrorInvalidValue;
+    }
+#undef MXCO_BLOOM_GENERIC
+    return hipGetLastError();
+}
+
+// The default policy (8 words, 8 pattern bits) has the tuned kernels.
+bool default_policy(const mxco_bloom* f) { return f->wpb == 8 && f->pattern_bits == 8; }
 
 constexpr uint32_t kRangeShift = 11;  // 2048 blocks x 32 B = 64 KiB of LDS per range
 constexpr uint32_t kRangeBlocks = 1u << kRangeShift;
@@ -322,8 +414,8 @@ __global__ __launch_bounds__(1024) void bloom_apply_kernel(uint32_t* words, uint
                 unsigned long long* w = (unsigned long long*)(lw + (uint32_t)(v[u] >> 32) * 8);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const unsigned long long m = (unsigned long long)(1u << ((kSalts[2 * q] * lo) >> 27)) |
-                                                 (unsigned long long)(1u << ((kSalts[2 * q + 1] * lo) >> 27)) << 32;
+                    const unsigned long long m = (unsigned long long)(1u << ((kBloomSalts[2 * q] * lo) >> 27)) |
+                                                 (unsigned long long)(1u << ((kBloomSalts[2 * q + 1] * lo) >> 27)) << 32;
                     if ((w[q] & m) != m) atomicOr(w + q, m);
                 }
             }
@@ -339,11 +431,13 @@ __global__ void words_zero_tail(uint32_t* p)
     *p = 0;
 }
 
-// op 0: dst |= src, op 1: dst &= src, op 2: dst = 0.  Blocks are 32 bytes,
-// so the word count is a multiple of 8 and 16-byte vectors cover it exactly.
-__global__ __launch_bounds__(256) void bloom_words_kernel(uint32_t* dst, const uint32_t* src, uint64_t n16, int op)
+// op 0: dst |= src, op 1: dst &= src, op 2: dst = 0, over nwords 32-bit
+// words: 16-byte vectors, then the last nwords % 4 words (blocks of one or
+// two words leave them).
+__global__ __launch_bounds__(256) void bloom_words_kernel(uint32_t* dst, const uint32_t* src, uint64_t nwords, int op)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n16 = nwords / 4;
     u64x2* d = (u64x2*)dst;
     const u64x2* s = (const u64x2*)src;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
@@ -354,6 +448,10 @@ __global__ __launch_bounds__(256) void bloom_words_kernel(uint32_t* dst, const u
             const u64x2 a = d[i], b = s[i];
             d[i] = op == 0 ? (a | b) : (a & b);
         }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < nwords % 4) {
+        const uint64_t j = n16 * 4 + threadIdx.x;
+        dst[j] = op == 2 ? 0u : (op == 0 ? (dst[j] | src[j]) : (dst[j] & src[j]));
     }
 }
 
@@ -468,6 +566,13 @@ mxco_status add_partitioned(mxco_bloom* f, const uint8_t* keys, const uint8_t* s
 
 mxco_status add_direct(mxco_bloom* f, const void* keys, const uint8_t* stencil, uint64_t n, hipStream_t stream)
 {
+    if (!default_policy(f)) {
+        hipError_t e = f->key_size == 4
+                           ? launch_generic<4>(true, f, (const uint8_t*)keys, stencil, nullptr, n, stream)
+                           : launch_generic<8>(true, f, (const uint8_t*)keys, stencil, nullptr, n, stream);
+        MXCO_HIP(e, "BloomFilter::Add");
+        return MXCO_OK;
+    }
     const unsigned grid = grid_for(n, 8);
     if (f->key_size == 4)
         bloom_add_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,
@@ -520,8 +625,9 @@ mxco_status check_filter(const mxco_bloom* f, const char* fn)
 
 mxco_status words_op(mxco_bloom* dst, const mxco_bloom* src, int op, hipStream_t stream, int sync, const char* fn)
 {
-    const uint64_t n16 = dst->num_blocks * 2;
-    bloom_words_kernel<<<grid_for(n16, 8), 256, 0, stream>>>(dst->words, src ? src->words : nullptr, n16, op);
+    const uint64_t nwords = dst->num_blocks * dst->wpb;
+    bloom_words_kernel<<<std::max(1u, grid_for(nwords / 4, 8)), 256, 0, stream>>>(dst->words, src ? src->words : nullptr,
+                                                                                 nwords, op);
     MXCO_HIP(hipGetLastError(), fn);
     if (sync) MXCO_HIP(hipStreamSynchronize(stream), fn);
     return MXCO_OK;
@@ -533,7 +639,7 @@ mxco_status compatible(const mxco_bloom* a, const mxco_bloom* b, const char* fn)
         return fail(MXCO_LOGIC_ERROR, std::string("BloomFilter::") + fn + ": a moved-from filter cannot participate");
     if (a->num_blocks != b->num_blocks)
         return fail(MXCO_INVALID_ARGUMENT, std::string("BloomFilter::") + fn + ": filters have different block extents");
-    if (a->seed != b->seed || a->key_size != b->key_size)
+    if (a->seed != b->seed || a->key_size != b->key_size || a->wpb != b->wpb || a->pattern_bits != b->pattern_bits)
         return fail(MXCO_INVALID_ARGUMENT, std::string("BloomFilter::") + fn + ": filters have different policies");
     return MXCO_OK;
 }
@@ -542,12 +648,20 @@ mxco_status compatible(const mxco_bloom* a, const mxco_bloom* b, const char* fn)
 
 extern "C" {
 
-MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, uint64_t seed, hipStream_t stream,
-                                       mxco_bloom** out)
+MXCO_API mxco_status mxco_bloom_create_with_policy(uint64_t num_blocks, uint32_t key_size,
+                                                   const mxco_bloom_policy* policy, uint32_t* storage,
+                                                   hipStream_t stream, mxco_bloom** out)
 {
-    if (!out) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter: null output");
+    if (!out || !policy) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter: null argument");
     *out = nullptr;
     if (key_size != 4 && key_size != 8) return fail(MXCO_UNSUPPORTED, "BloomFilter: keys must be 4 or 8 bytes");
+    const uint32_t w = policy->words_per_block, p = policy->pattern_bits;
+    // BloomFilterPolicy's static checks (bloom_filter_policy.h:80-98)
+    if (w == 0 || w > 32 || (w & (w - 1)) != 0)
+        return fail(MXCO_INVALID_ARGUMENT, "BloomFilterPolicy: WordsPerBlock must be a power of two of at most 32");
+    if (p < w || p > 64 || p > 32 * w || p % w != 0)
+        return fail(MXCO_INVALID_ARGUMENT,
+                    "BloomFilterPolicy: PatternBits must be a multiple of WordsPerBlock of at most 64");
     if (num_blocks == 0) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter: numBlocks must be greater than zero");
     if (num_blocks > 0xffffffffull)
         return fail(MXCO_LENGTH_ERROR, "BloomFilter: numBlocks exceeds the supported block-index range");
@@ -555,15 +669,22 @@ MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, u
     if (!f) return fail(MXCO_OUT_OF_MEMORY, "BloomFilter: host allocation");
     f->num_blocks = num_blocks;
     f->key_size = key_size;
-    f->seed = seed;
-    hipError_t e = hipMalloc((void**)&f->words, num_blocks * 32);
-    if (e != hipSuccess) {
-        delete f;
-        return hip_fail(e, "BloomFilter: allocation");
+    f->seed = policy->seed;
+    f->wpb = w;
+    f->pattern_bits = p;
+    f->owns_words = storage == nullptr;
+    if (storage) {
+        f->words = storage;
+    } else {
+        hipError_t e = hipMalloc((void**)&f->words, num_blocks * w * 4);
+        if (e != hipSuccess) {
+            delete f;
+            return hip_fail(e, "BloomFilter: allocation");
+        }
     }
     mxco_status st = words_op(f, nullptr, 2, stream, 1, "BloomFilter::Clear");
     if (st != MXCO_OK) {
-        (void)hipFree(f->words);
+        if (f->owns_words) (void)hipFree(f->words);
         delete f;
         return st;
     }
@@ -571,10 +692,17 @@ MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, u
     return ok();
 }
 
+MXCO_API mxco_status mxco_bloom_create(uint64_t num_blocks, uint32_t key_size, uint64_t seed, hipStream_t stream,
+                                       mxco_bloom** out)
+{
+    const mxco_bloom_policy policy = {8, 8, seed};
+    return mxco_bloom_create_with_policy(num_blocks, key_size, &policy, nullptr, stream, out);
+}
+
 MXCO_API mxco_status mxco_bloom_destroy(mxco_bloom* f)
 {
     if (!f) return ok();
-    if (f->words) (void)hipFree(f->words);
+    if (f->words && f->owns_words) (void)hipFree(f->words);
     if (f->ws) (void)hipFree(f->ws);
     delete f;
     return ok();
@@ -602,7 +730,7 @@ MXCO_API mxco_status mxco_bloom_add(mxco_bloom* f, const void* keys, const uint8
     if (mxco_status s = check_filter(f, "Add")) return s;
     if (!keys) return fail(MXCO_INVALID_ARGUMENT, "BloomFilter::Add: keys must not be null when keyNum is non-zero");
     const uint64_t nranges = (f->num_blocks + kRangeBlocks - 1) >> kRangeShift;
-    if (add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
+    if (default_policy(f) && add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
         for (uint64_t done = 0; done < n; done += kPassKeys) {
             const uint64_t m = std::min(kPassKeys, n - done);
             PassLayout L;
@@ -629,7 +757,7 @@ MXCO_API mxco_status mxco_bloom_add_noalloc(mxco_bloom* f, const void* keys, con
         return fail(MXCO_LENGTH_ERROR,
                     "BloomFilter::Add (no-alloc): at most 4,194,304 keys per call (the library's fixed scratch)");
     const uint64_t nranges = (f->num_blocks + kRangeBlocks - 1) >> kRangeShift;
-    if (add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
+    if (default_policy(f) && add_path(n, f->num_blocks) == 1 && nranges <= kMaxRanges) {
         PassLayout L;
         if (mxco_status e = pass_layout(f, n, stream, L)) return e;
         if (L.total > kScratchBytes)
@@ -654,6 +782,13 @@ MXCO_API mxco_status mxco_bloom_contains(const mxco_bloom* f, const void* keys, 
     if (!out)
         return fail(MXCO_INVALID_ARGUMENT,
                     "BloomFilter::Contains: outputValues must not be null when keyNum is non-zero");
+    if (!default_policy(f)) {
+        hipError_t e = f->key_size == 4 ? launch_generic<4>(false, f, (const uint8_t*)keys, stencil, out, n, stream)
+                                        : launch_generic<8>(false, f, (const uint8_t*)keys, stencil, out, n, stream);
+        MXCO_HIP(e, "BloomFilter::Contains");
+        if (sync) MXCO_HIP(hipStreamSynchronize(stream), "BloomFilter::Contains");
+        return ok();
+    }
     const unsigned grid = grid_for(n, 8);
     if (f->key_size == 4)
         bloom_contains_kernel<4><<<grid, 256, 0, stream>>>(f->words, (uint32_t)f->num_blocks, (const uint8_t*)keys,

@@ -382,7 +382,12 @@ class StaticSet(_OpenAddressing):
 # BloomFilter / HyperLogLog
 # ---------------------------------------------------------------------------
 
+class BloomPolicy(ctypes.Structure):
+    _fields_ = [("words_per_block", u32), ("pattern_bits", u32), ("seed", u64)]
+
+
 _sig("mxco_bloom_create", [u64, u32, u64, vp, ctypes.POINTER(vp)])
+_sig("mxco_bloom_create_with_policy", [u64, u32, ctypes.POINTER(BloomPolicy), vp, vp, ctypes.POINTER(vp)])
 _sig("mxco_bloom_destroy", [vp])
 _sig("mxco_bloom_data", [vp, ctypes.POINTER(vp), pu64])
 _sig("mxco_bloom_clear", [vp, vp, i32])
@@ -413,14 +418,25 @@ def _dev_view(ptr, nbytes, dtype=torch.uint8):
 
 
 class BloomFilter:
-    """aclco::BloomFilter<Key> with the default policy (include/bloom_filter.h:37)."""
+    """aclco::BloomFilter<Key, Extent, BloomFilterPolicy<Key, xxhash_64<Key>,
+    uint32_t, W, P>> (include/bloom_filter.h:37, bloom_filter_policy.h:54):
+    the default policy is W = P = 8.  `storage`: a caller-owned device tensor
+    of at least num_blocks * W int32 words to hold the filter (the
+    reference's Allocator), kept alive by this object."""
 
-    def __init__(self, num_blocks, key_dtype=torch.int64, seed=0, stream=None):
+    def __init__(self, num_blocks, key_dtype=torch.int64, seed=0, words_per_block=8, pattern_bits=None,
+                 storage=None, stream=None):
         self.key_dtype = key_dtype
         self.seed = seed
+        self.words_per_block = words_per_block
+        self.pattern_bits = words_per_block if pattern_bits is None else pattern_bits
+        self._storage = storage
+        if storage is not None and storage.numel() * storage.element_size() < num_blocks * words_per_block * 4:
+            raise ValueError("BloomFilter: storage holds fewer than num_blocks * words_per_block words")
         h = vp()
-        check(_lib.mxco_bloom_create(num_blocks, torch.empty(0, dtype=key_dtype).element_size(), seed,
-                                     _stream(stream), ctypes.byref(h)))
+        pol = BloomPolicy(words_per_block, self.pattern_bits, seed)
+        check(_lib.mxco_bloom_create_with_policy(num_blocks, torch.empty(0, dtype=key_dtype).element_size(),
+                                                 ctypes.byref(pol), _ptr(storage), _stream(stream), ctypes.byref(h)))
         self._h = h
         self._nb = num_blocks
 
@@ -431,10 +447,10 @@ class BloomFilter:
         return self._nb
 
     def num_words(self):
-        return self._nb * 8
+        return self._nb * self.words_per_block
 
     def size_bytes(self):
-        return self._nb * 32
+        return self.num_words() * 4
 
     def data(self):
         p, nb = vp(), u64()
@@ -443,7 +459,7 @@ class BloomFilter:
 
     def words(self):
         """Device copy of the filter words (uint32 viewed as int32)."""
-        return _dev_view(self.data(), self._nb * 32, torch.int32)
+        return _dev_view(self.data(), self.size_bytes(), torch.int32)
 
     def clear(self, stream=None, sync=True):
         check(_lib.mxco_bloom_clear(self._h, _stream(stream), int(sync)))

@@ -818,41 +818,82 @@ ORC_API uint64_t orc_oa_mt(orc_oa* t, int op, const void* in, uint64_t n, void* 
 /* 256-bit block, one bit per word, xxhash_64 seed 0).                      */
 /* ------------------------------------------------------------------------ */
 
-/* Parquet SBBF salts (include/bloom_filter_policy.h:170-178, first eight). */
-static const uint32_t k_salts[8] = {0x47b6137bu, 0x44974d91u, 0x8824ad5bu, 0xa2b7289du,
-                                    0x705495c7u, 0x2df1424bu, 0x9efc4947u, 0x5c6bfb31u};
+/* BloomFilterPolicy's 64 pattern salts (include/bloom_filter_policy.h:170-178;
+ * the first eight are the Parquet SBBF salts).  Data: the filter bits
+ * depend on every value; tests check the table by its SHA-256. */
+static const uint32_t k_salts[64] = {
+    0x47b6137bu, 0x44974d91u, 0x8824ad5bu, 0xa2b7289du, 0x705495c7u, 0x2df1424bu, 0x9efc4947u, 0x5c6bfb31u,
+    0xb24bcdffu, 0xb6843d6du, 0x6db04543u, 0x3a12efddu, 0xb0ddd463u, 0x8d22f6e7u, 0xb82f1e53u, 0x7db9f86bu,
+    0xc7afe639u, 0xfb135cd7u, 0x693256e1u, 0x9466d871u, 0x23d3d02fu, 0x6461d049u, 0x66a91621u, 0xbaa3006fu,
+    0x52fb8d99u, 0x3ea88b4fu, 0x0f470cfdu, 0xb1db79a5u, 0x9809fcd1u, 0xbced4445u, 0x2eb7c737u, 0x2cea6803u,
+    0x156f1955u, 0x8813c027u, 0xa26819f9u, 0x4c3b57bdu, 0x7df94487u, 0xb975e769u, 0xb8f20cb5u, 0x5c9e2e77u,
+    0x5fb1735fu, 0x3a6f759bu, 0x3c090923u, 0xfced424du, 0xa187a6a9u, 0x6f070a41u, 0x2c85233bu, 0x7e62258bu,
+    0x2771ef17u, 0x13bbf093u, 0x4ff059e5u, 0xe3ce3d0fu, 0xf1b4789fu, 0x9fbb6173u, 0x6a320cf5u, 0x1be2c481u,
+    0x7ba8222bu, 0x6fd619b3u, 0x7b1bbf0du, 0x8b8993adu, 0x448eca95u, 0x82ab09d9u, 0x2ce53909u, 0x4f548685u};
+
+ORC_API const uint32_t* orc_bloom_salts(void) { return k_salts; }
 
 /* BlockIndex = (hi32 * numBlocks) >> 32 (bloom_filter_policy.h:102-107);
- * word pattern = 1 << ((salt_w * lo32) >> 27) (:157-168). */
-static inline void bloom_locate(uint64_t h, uint64_t num_blocks, uint32_t* blk, uint32_t pat[8])
+ * word w's pattern = OR over b < P/W of 1 << ((salt[w*(P/W) + b] * lo32) >> 27)
+ * (WordPatternImpl, :157-168). */
+static inline uint32_t bloom_block(uint64_t h, uint64_t num_blocks)
 {
-    *blk = (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * (uint32_t)num_blocks) >> 32);
-    uint32_t lo = (uint32_t)h;
-    for (int w = 0; w < 8; ++w) pat[w] = 1u << ((k_salts[PLANT(7) ? (w ^ 1) : w] * lo) >> 27);
+    return (uint32_t)(((uint64_t)(uint32_t)(h >> 32) * (uint32_t)num_blocks) >> 32);
 }
 
+static inline uint32_t bloom_pattern(uint32_t lo, uint32_t w, uint32_t bpw)
+{
+    uint32_t m = 0;
+    for (uint32_t b = 0; b < bpw; ++b) {
+        uint32_t salt = w * bpw + b;
+        if (PLANT(7)) salt ^= 1;
+        m |= 1u << ((k_salts[salt] * lo) >> 27);
+    }
+    return m;
+}
+
+/* Add / Contains under BloomFilterPolicy<Key, xxhash_64, uint32_t, W, P>. */
+ORC_API void orc_bloom_add_policy(uint32_t* words, uint64_t num_blocks, uint32_t wpb, uint32_t pattern_bits,
+                                  const void* keys, uint32_t ks, uint64_t n, uint64_t seed)
+{
+    const uint8_t* k = (const uint8_t*)keys;
+    const uint32_t bpw = pattern_bits / wpb;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t h = orc_xxh64(k + i * ks, ks, seed);
+        uint64_t base = (uint64_t)bloom_block(h, num_blocks) * wpb;
+        for (uint32_t w = 0; w < wpb; ++w) words[base + w] |= bloom_pattern((uint32_t)h, w, bpw);
+    }
+}
+
+ORC_API void orc_bloom_contains_policy(const uint32_t* words, uint64_t num_blocks, uint32_t wpb,
+                                       uint32_t pattern_bits, const void* keys, uint32_t ks, uint64_t n,
+                                       uint64_t seed, uint8_t* out)
+{
+    const uint8_t* k = (const uint8_t*)keys;
+    const uint32_t bpw = pattern_bits / wpb;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t h = orc_xxh64(k + i * ks, ks, seed);
+        uint64_t base = (uint64_t)bloom_block(h, num_blocks) * wpb;
+        int ok = 1;
+        for (uint32_t w = 0; w < wpb; ++w) {
+            uint32_t m = bloom_pattern((uint32_t)h, w, bpw);
+            ok &= (words[base + w] & m) == m;
+        }
+        out[i] = (uint8_t)ok;
+    }
+}
+
+/* The default policy: 8 words, 8 pattern bits. */
 ORC_API void orc_bloom_add(uint32_t* words, uint64_t num_blocks, const void* keys, uint32_t ks, uint64_t n,
                            uint64_t seed)
 {
-    const uint8_t* k = (const uint8_t*)keys;
-    for (uint64_t i = 0; i < n; ++i) {
-        uint32_t blk, pat[8];
-        bloom_locate(orc_xxh64(k + i * ks, ks, seed), num_blocks, &blk, pat);
-        for (int w = 0; w < 8; ++w) words[(uint64_t)blk * 8 + w] |= pat[w];
-    }
+    orc_bloom_add_policy(words, num_blocks, 8, 8, keys, ks, n, seed);
 }
 
 ORC_API void orc_bloom_contains(const uint32_t* words, uint64_t num_blocks, const void* keys, uint32_t ks,
                                 uint64_t n, uint64_t seed, uint8_t* out)
 {
-    const uint8_t* k = (const uint8_t*)keys;
-    for (uint64_t i = 0; i < n; ++i) {
-        uint32_t blk, pat[8];
-        bloom_locate(orc_xxh64(k + i * ks, ks, seed), num_blocks, &blk, pat);
-        int ok = 1;
-        for (int w = 0; w < 8; ++w) ok &= (words[(uint64_t)blk * 8 + w] & pat[w]) == pat[w];
-        out[i] = (uint8_t)ok;
-    }
+    orc_bloom_contains_policy(words, num_blocks, 8, 8, keys, ks, n, seed, out);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -81,6 +81,11 @@ def lib():
         L.orc_bloom_add.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]
         L.orc_bloom_contains.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
                                          vp]
+        L.orc_bloom_add_policy.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                           ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_bloom_contains_policy.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                                ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, vp]
+        L.orc_bloom_salts.restype = ctypes.POINTER(ctypes.c_uint32)
         L.orc_hll_add.argtypes = [vp, ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.c_uint64]
         L.orc_hll_histogram.argtypes = [vp, ctypes.c_uint32, vp]
         L.orc_hll_z.restype = ctypes.c_double
@@ -232,16 +237,23 @@ class OpenAddressing:
         return out
 
 
-def bloom_add(words, num_blocks, keys, seed=0):
+def bloom_add(words, num_blocks, keys, seed=0, words_per_block=8, pattern_bits=None):
     k = np.ascontiguousarray(keys)
-    lib().orc_bloom_add(_ptr(words), num_blocks, _ptr(k), k.itemsize, len(k), seed)
+    p = words_per_block if pattern_bits is None else pattern_bits
+    lib().orc_bloom_add_policy(_ptr(words), num_blocks, words_per_block, p, _ptr(k), k.itemsize, len(k), seed)
 
 
-def bloom_contains(words, num_blocks, keys, seed=0):
+def bloom_contains(words, num_blocks, keys, seed=0, words_per_block=8, pattern_bits=None):
     k = np.ascontiguousarray(keys)
+    p = words_per_block if pattern_bits is None else pattern_bits
     out = np.zeros(len(k), np.uint8)
-    lib().orc_bloom_contains(_ptr(words), num_blocks, _ptr(k), k.itemsize, len(k), seed, _ptr(out))
+    lib().orc_bloom_contains_policy(_ptr(words), num_blocks, words_per_block, p, _ptr(k), k.itemsize, len(k), seed,
+                                    _ptr(out))
     return out.astype(bool)
+
+
+def bloom_salts():
+    return np.ctypeslib.as_array(lib().orc_bloom_salts(), shape=(64,)).copy()
 
 
 def hll_add(regs, p, keys):

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <limits>
+#include <type_traits>
+#include <utility>
 
 #include "mxco/aclco.h"
 
@@ -262,6 +264,57 @@ int main()
         CHECK(set.Count(db, 2, s) == 1);
         CHECK(hipFree(dp) == hipSuccess && hipFree(dk) == hipSuccess && hipFree(dv) == hipSuccess);
         CHECK(hipFree(db) == hipSuccess);
+    }
+    {  // Bloom filter policies of tests/bloom_filter/generalization_test.cpp:30-55
+        using DefaultP = aclco::DefaultFilterPolicy<aclco::xxhash_64<uint32_t>, uint32_t, 8>;
+        using ExpectedDefaultP =
+            aclco::BloomFilterPolicy<uint32_t, aclco::xxhash_64<uint32_t>, uint32_t, 8, 8, 8, 1, 1, 8, false, false>;
+        using Default16 = aclco::DefaultFilterPolicy<aclco::xxhash_64<uint64_t>, uint32_t, 16>;
+        using Expected16 = aclco::BloomFilterPolicy<uint64_t, aclco::xxhash_64<uint64_t>, uint32_t, 16>;
+        using ArrowP = aclco::ArrowFilterPolicy<uint32_t>;
+        static_assert(std::is_same_v<DefaultP, ExpectedDefaultP>);
+        static_assert(std::is_same_v<Default16, Expected16>);
+        static_assert(std::is_same_v<ArrowP, ExpectedDefaultP>);
+        static_assert(aclco::detail::UsesMultiplyHighBlockIndex<DefaultP>::value);
+        using Scalar16 = aclco::BloomFilterPolicy<uint64_t, aclco::xxhash_64<uint64_t>, uint32_t, 8, 16, 1, 8, 1, 8,
+                                                  true, true>;
+        static_assert(Scalar16::patternBits == 16 && Scalar16::bitsPerWord == 2 && Scalar16::conditionalAdd);
+        using Filter = aclco::BloomFilter<uint64_t, aclco::Extent<size_t>, Scalar16>;
+        const size_t nb = 257;
+        Scalar16 policy(typename Scalar16::Hasher{42});
+        Filter f(aclco::Extent<size_t>{nb}, policy, typename Filter::AllocatorType{}, s);
+        CHECK(f.NumWords() == nb * 8 && f.SizeBytes() == nb * 32);
+        std::vector<uint64_t> keys(2051);
+        for (size_t i = 0; i < keys.size(); ++i) keys[i] = i * 17 + 3;
+        auto* dk = to_device(keys);
+        f.Add(dk, keys.size(), s);
+        // the words the policy's own host functions give
+        std::vector<uint32_t> golden(nb * 8, 0);
+        for (uint64_t k : keys) {
+            const uint64_t h = policy.HashKey(k);
+            const uint32_t blk = policy.BlockIndex(static_cast<uint32_t>(h >> 32), nb);
+            for (uint32_t w = 0; w < 8; ++w) golden[blk * 8 + w] |= policy.WordPatternForLane(w, static_cast<uint32_t>(h));
+        }
+        CHECK(to_host(std::as_const(f).Data(), nb * 8) == golden);
+        CHECK(policy.WordPattern<3>(12345u) == policy.WordPatternForLane(3, 12345u));
+        // WordsPerBlock 16 (64-byte blocks), 32 pattern bits, and a caller allocator
+        using P16 = aclco::BloomFilterPolicy<uint64_t, aclco::xxhash_64<uint64_t>, uint32_t, 16, 32>;
+        aclco::BloomFilter<uint64_t, aclco::Extent<size_t>, P16> g(aclco::Extent<size_t>{100}, P16{}, s);
+        g.Add(dk, keys.size(), s);
+        bool* dc;
+        CHECK(hipMalloc(&dc, keys.size()) == hipSuccess);
+        g.Contains(dk, keys.size(), dc, s);
+        auto c = to_host(reinterpret_cast<uint8_t*>(dc), keys.size());
+        for (uint8_t b : c) CHECK(b == 1);
+        bool threw = false;
+        try {
+            aclco::BloomFilter<uint64_t, aclco::Extent<size_t>, P16> h2(aclco::Extent<size_t>{100}, P16{7}, s);
+            g.Merge(h2, s);  // different seeds
+        } catch (std::invalid_argument const&) {
+            threw = true;
+        }
+        CHECK(threw);
+        CHECK(hipFree(dk) == hipSuccess && hipFree(dc) == hipSuccess);
     }
     CHECK(hipStreamDestroy(s) == hipSuccess);
     std::printf("OK\n");

@@ -154,6 +154,95 @@ def test_bloom_large_fpr():
 
 
 # --------------------------------------------------------------------------
+# BloomFilterPolicy<Key, xxhash_64, uint32_t, W, P>
+# --------------------------------------------------------------------------
+
+POLICIES = [(1, 1), (1, 4), (2, 2), (4, 8), (8, 16), (16, 16), (16, 32), (32, 32), (32, 64)]
+
+
+@pytest.mark.parametrize("wpb,pbits", POLICIES)
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64, np.float32])
+def test_bloom_policy_words_and_contains_bit_exact(wpb, pbits, dtype):
+    """The seeded PatternBits=16 case of tests/bloom_filter/generalization_test.cpp:104-146
+    (257 blocks, keys i*17+3, seed 42) and the other W/P combinations."""
+    nb, seed = 257, 42
+    keys = (np.arange(2051, dtype=np.uint64) * 17 + 3).astype(dtype)
+    misses = (np.arange(257, dtype=np.uint64) * 17 + 3 + 1000000).astype(dtype)
+    f = mx().BloomFilter(nb, TD[dtype], seed=seed, words_per_block=wpb, pattern_bits=pbits)
+    f.add(dev(keys))
+    ref = np.zeros(nb * wpb, np.uint32)
+    O.bloom_add(ref, nb, keys, seed, wpb, pbits)
+    np.testing.assert_array_equal(words_of(f), ref)
+    q = np.concatenate([keys, misses])
+    np.testing.assert_array_equal(f.contains(dev(q)).cpu().numpy().astype(bool),
+                                  O.bloom_contains(ref, nb, q, seed, wpb, pbits))
+    # a stencil, the no-alloc entry point, merge and intersect under the same policy
+    st = (np.arange(len(q)) % 3 == 0).astype(np.uint8)
+    g = mx().BloomFilter(nb, TD[dtype], seed=seed, words_per_block=wpb, pattern_bits=pbits)
+    g.add_if(dev(q), dev(st))
+    g2 = mx().BloomFilter(nb, TD[dtype], seed=seed, words_per_block=wpb, pattern_bits=pbits)
+    g2.add_noalloc(dev(q), stencil=dev(st))
+    ref_g = np.zeros(nb * wpb, np.uint32)
+    O.bloom_add(ref_g, nb, q[st == 1], seed, wpb, pbits)
+    np.testing.assert_array_equal(words_of(g), ref_g)
+    np.testing.assert_array_equal(words_of(g2), ref_g)
+    g.merge(f)
+    np.testing.assert_array_equal(words_of(g), ref | ref_g)
+    g2.intersect(f)
+    np.testing.assert_array_equal(words_of(g2), ref & ref_g)
+
+
+def test_bloom_policy_hotspot_and_repeated_keys():
+    """tests/bloom_filter/generalization_test.cpp:148-173: one key 2048 times,
+    then 512 keys of one block."""
+    nb = 257
+    f = mx().BloomFilter(nb, torch.int32, words_per_block=8)
+    rep = np.full(2048, 1234567, np.uint32)
+    f.add(dev(rep))
+    ref = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(ref, nb, rep)
+    np.testing.assert_array_equal(words_of(f), ref)
+    f.clear()
+    cand = np.arange(200000, dtype=np.uint32)
+    import xxhash
+    blk = np.array([((xxhash.xxh64_intdigest(int(c).to_bytes(4, "little"), 0) >> 32) * nb) >> 32 for c in cand])
+    hot = cand[blk == 73][:512]
+    assert len(hot) == 512
+    f.add(dev(hot))
+    ref = np.zeros(nb * 8, np.uint32)
+    O.bloom_add(ref, nb, hot)
+    np.testing.assert_array_equal(words_of(f), ref)
+    assert (ref.reshape(nb, 8).any(axis=1).nonzero()[0] == [73]).all()
+
+
+def test_bloom_caller_storage_and_policy_rules():
+    mxco = mx()
+    nb, wpb = 100, 16
+    store = torch.full((nb * wpb + 8,), 7, dtype=torch.int32, device="cuda")
+    f = mxco.BloomFilter(nb, torch.int64, words_per_block=wpb, pattern_bits=32, storage=store)
+    assert f.data() == store.data_ptr()
+    assert int(store[: nb * wpb].abs().sum().item()) == 0  # cleared
+    assert int(store[nb * wpb:].sum().item()) == 7 * 8  # nothing beyond the filter
+    keys = np.arange(5000, dtype=np.int64)
+    f.add(dev(keys))
+    ref = np.zeros(nb * wpb, np.uint32)
+    O.bloom_add(ref, nb, keys, 0, wpb, 32)
+    np.testing.assert_array_equal(store[: nb * wpb].cpu().numpy().view(np.uint32), ref)
+    del f
+    torch.cuda.synchronize()
+    assert store.sum().item() != 0  # the caller's memory outlives the filter
+    with pytest.raises(ValueError):
+        mxco.BloomFilter(nb, torch.int64, words_per_block=16, storage=store[:10])
+    for w, p in ((3, 3), (8, 12), (64, 64), (1, 33), (8, 72), (0, 0)):
+        with pytest.raises(ValueError):
+            mxco.BloomFilter(16, torch.int64, words_per_block=w, pattern_bits=p)
+    a = mxco.BloomFilter(16, torch.int64, words_per_block=8, pattern_bits=8)
+    b = mxco.BloomFilter(16, torch.int64, words_per_block=8, pattern_bits=16)
+    with pytest.raises(ValueError):
+        a.merge(b)
+
+
+# --------------------------------------------------------------------------
 # HyperLogLog
 # --------------------------------------------------------------------------
 

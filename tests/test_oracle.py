@@ -387,6 +387,55 @@ def _py_bloom_add(words, num_blocks, key_bytes):
         words[blk * 8 + w] |= 1 << bit
 
 
+# SHA-256 of BloomFilterPolicy's 64 salts as little-endian uint32
+# (include/bloom_filter_policy.h:170-178; tests/common/bloom_filter_golden.h:29-37
+# holds the same table).
+BLOOM_SALTS_SHA256 = "a67bd0b9b64efd063db9fa8a0728d0762d9edc3b8716e48b6d1359f301c91101"
+
+
+def test_bloom_salt_table_is_the_references():
+    salts = O.bloom_salts()
+    assert hashlib.sha256(salts.astype("<u4").tobytes()).hexdigest() == BLOOM_SALTS_SHA256
+    assert salts[:8].tolist() == SALTS
+
+
+def _py_bloom_add_policy(words, num_blocks, key_bytes, wpb, pbits, seed, salts):
+    """MakeBloomGolden<Key, W, P> (tests/common/bloom_filter_golden.h:88-146),
+    bit by bit, with python-xxhash as the hash."""
+    h = xxhash.xxh64_intdigest(key_bytes, seed)
+    blk = ((h >> 32) * num_blocks) >> 32
+    lo = h & 0xFFFFFFFF
+    bpw = pbits // wpb
+    for w in range(wpb):
+        for b in range(bpw):
+            bit = ((salts[w * bpw + b] * lo) & 0xFFFFFFFF) >> 27
+            words[blk * wpb + w] |= 1 << bit
+
+
+@pytest.mark.parametrize("wpb,pbits", [(1, 1), (1, 32), (2, 4), (4, 4), (8, 16), (16, 16), (16, 32), (32, 32),
+                                       (32, 64)])
+@pytest.mark.parametrize("dtype", [np.uint32, np.int64])
+def test_bloom_policies_vs_bitwise_restatement(wpb, pbits, dtype):
+    rng = np.random.default_rng(wpb * 100 + pbits)
+    nb, seed = 61, 42
+    salts = [int(x) for x in O.bloom_salts()]
+    keys = rng.integers(0, 2**31, 2500).astype(dtype)
+    words = np.zeros(nb * wpb, np.uint32)
+    O.bloom_add(words, nb, keys, seed, wpb, pbits)
+    ref = [0] * (nb * wpb)
+    for k in keys:
+        _py_bloom_add_policy(ref, nb, k.tobytes(), wpb, pbits, seed, salts)
+    np.testing.assert_array_equal(words, np.array(ref, np.uint32))
+    assert O.bloom_contains(words, nb, keys, seed, wpb, pbits).all()
+    # a query is contained exactly when every word holds its pattern
+    q = rng.integers(2**31, 2**32, 3000).astype(dtype)
+    got = O.bloom_contains(words, nb, q, seed, wpb, pbits)
+    for k, g in zip(q[:300], got[:300]):
+        one = [0] * (nb * wpb)
+        _py_bloom_add_policy(one, nb, k.tobytes(), wpb, pbits, seed, salts)
+        assert g == all((r & m) == m for r, m in zip(words.tolist(), one))
+
+
 @pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint32, np.float32])
 def test_bloom_vs_bitwise_restatement(dtype):
     rng = np.random.default_rng(6)
