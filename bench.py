@@ -295,6 +295,47 @@ def run_hll(torch, mxco, sketches=64, n=1 << 28):
                              "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_key": 8}}
 
 
+def run_ceilings(torch, mxco, reps=5):
+    """Measured ceilings of the containers' access patterns: random 128-,
+    64- and 32-byte reads over an 8 GiB buffer (HBM; the probe walk reads
+    one 128-byte line per window), random 32- and 128-byte reads over a
+    128 MiB buffer (the Bloom filter's size: served from the 256 MB MALL),
+    and a 4 GiB streaming copy.  Median of `reps` timed runs each."""
+    import statistics
+    s = torch.cuda.current_stream()
+    sink = torch.empty(mxco.cu_count() * 8 * 256, dtype=torch.int64, device="cuda")
+    big = torch.ones(8 << 30, dtype=torch.uint8, device="cuda")
+    out = {}
+
+    def timed(fn):
+        ms = []
+        fn()
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            fn()
+            b.record(s)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return statistics.median(ms)
+
+    for name, nbytes, access, n in (("hbm_random_128B", 8 << 30, 128, 1 << 28),
+                                    ("hbm_random_64B", 8 << 30, 64, 1 << 28),
+                                    ("hbm_random_32B", 8 << 30, 32, 1 << 28),
+                                    ("mall_random_32B_128MiB", 128 << 20, 32, 1 << 27),
+                                    ("mall_random_128B_128MiB", 128 << 20, 128, 1 << 27)):
+        ms = timed(lambda: mxco.random_read(big, nbytes, access, n, sink))
+        out[name] = {"ms": round(ms, 4), "gb_s": round(n * access / (ms * 1e-3) / 1e9, 1),
+                     "accesses": n, "buffer_bytes": nbytes}
+    half = 2 << 30
+    ms = timed(lambda: mxco.stream_copy(big, big[half:], half))
+    out["hbm_stream_copy"] = {"ms": round(ms, 4), "gb_s": round(2 * half / (ms * 1e-3) / 1e9, 1),
+                              "bytes_moved": 2 * half}
+    del big, sink
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_roaring(torch, mxco, steps=5):
     """RoaringBitmap Contains of 2^28 uint32 queries on two bitmaps:
     "spec", the RoaringFormatSpec key set of the reference's tests (200,100
@@ -782,12 +823,13 @@ def main():
                  "lines_per_query": r["find_lines"] / r["n_find"], "launch_ms": r["find_ms"],
                  "algorithmic_bytes": find_bytes}
     if r["insert_path"] == "region build":
-        # the bytes the build's passes move: count 16, scatter 32, regroup 48
-        # and apply 16 per pair, plus each region written once (the step's
-        # Clear leaves the table known empty, so regions are not read)
-        build_bytes = 112.0 * r["n_insert"] + 16.0 * r["capacity"]
+        # the bytes the build's passes move: scatter 32, regroup 32 and apply
+        # 16 per pair, plus each region written once (the step's Clear leaves
+        # the table known empty, so regions are not read)
+        build_bytes = 80.0 * r["n_insert"] + 16.0 * r["capacity"]
         build_gbs = build_bytes / (r["insert_ms"] * 1e-3) / 1e9
-        roof_ins = {"kernel": "build_{count,scatter,regroup,apply,overflow}_kernel<8,8> (region build)",
+        roof_ins = {"kernel": "build_bounded_{scatter,regroup}_kernel, build_{apply,overflow,spill}_kernel<8,8> "
+                              "(region build)",
                     "bound": "hbm", "achieved": round(build_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(build_gbs / PEAK_HBM_GBS, 4),
                     "traffic": traffic.get("insert") if traffic else None, "launch_ms": r["insert_ms"],
@@ -856,6 +898,9 @@ def main():
         log("bench: hyperloglog")
         line["hll"] = run_hll(torch, mxco, sketches=args.hll_sketches)
         log("bench: hll %s" % json.dumps(line["hll"]))
+    log("bench: ceilings")
+    line["ceilings"] = run_ceilings(torch, mxco)
+    log("bench: ceilings %s" % json.dumps(line["ceilings"]))
     if not args.no_roaring:
         log("bench: roaring bitmap")
         line["roaring"] = run_roaring(torch, mxco)

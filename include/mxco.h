@@ -584,6 +584,15 @@ MXCO_API mxco_status mxco_hll_estimate(const mxco_hll* sketch, hipStream_t strea
  * the tuner size themselves from it). */
 MXCO_API mxco_status mxco_device_cu_count(int* cus);
 
+/* Measured ceilings (no reference counterpart; measurement entry points).
+ * random_read: n reads of `access` bytes (16, 32, 64 or 128) at hashed,
+ * aligned offsets of the first `bytes` of buf, four in flight per lane
+ * group; sink receives one word per lane (CUs * 8 * 256 words).
+ * stream_copy: dst = src over `bytes` (a multiple of 16). */
+MXCO_API mxco_status mxco_diag_random_read(const void* buf, uint64_t bytes, uint32_t access, uint64_t n,
+                                           uint64_t* sink, uint64_t sink_words, hipStream_t stream);
+MXCO_API mxco_status mxco_diag_stream_copy(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -247,7 +247,6 @@ mxco_status run_build(mxco_batch_plan* p, const void* in, hipStream_t s)
     a.nreg = p->nreg;
     a.maxbins = p->maxbins;
     a.bin_slots = build_region_slots(b->ss);  // a bin is one region
-    a.heavy_limit = 0xffffffffu;
     a.bucket = b->d.bucket_size;
     a.hash = b->d.hash;
     a.key_float = b->d.key_kind == MXCO_KEY_FLOAT;

@@ -24,16 +24,13 @@
 // The walk order and the EMPTY / EQUAL rules are those of the CAS insert, so
 // Find, Contains and the per-item failure counts equal Clear + Insert's.
 //
-// The same kernels serve Insert into one large table (oa_api.hip).  Its
-// regions outnumber what one LDS histogram holds, so the partition has two
-// levels: count / scan / scatter group the pairs by runs of `group` regions,
-// and build_regroup_kernel regroups each run by region.  The apply then loads
-// each region from the table (the table may hold keys and tombstones
-// already, unless it is known to be empty).  An empty region of at most
-// kPlaceMax pairs with integer keys is placed without walks (place_sorted);
-// others walk as above.  A region with more than `heavy_limit` pairs (a
-// skewed stream of duplicates) is left to the CAS walk of build_heavy_kernel:
-// the regroup gives the apply an empty range for it.
+// The apply and overflow kernels also serve Insert into one large table
+// (oa_api.hip), after a two-level partition of its own (the bounded scatter
+// and regroup at the end of this file: the table's regions outnumber what one
+// LDS histogram holds).  The apply then loads each region from the table (the
+// table may hold keys and tombstones already, unless it is known to be
+// empty).  An empty region of at most kPlaceMax pairs with integer keys is
+// placed without walks (place_sorted); others walk as above.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -84,16 +81,14 @@ struct BuildArgs {
     uint32_t* counts;           // per bin; zero again after the scatter
     uint32_t* offsets;          // exclusive sum of the counts (nbins + 1)
     uint32_t* ovc;              // overflow pairs of each region, kept at its offset
-    uint32_t* heavy;            // [0] count, then the regions left to the CAS walk
-    uint32_t* apply_end;        // one-table: end of each region's pairs for the apply (its start for a hot region)
+    uint32_t* apply_end;        // one-table: end of each region's pairs for the apply
     unsigned long long* counters;  // 2 per item: [0] failed inserts (null: not counted)
     uint64_t empty_key, empty_val;
     uint64_t single_n;
     uint32_t nreg;
     uint32_t maxbins;     // most bins of a table with an item (stride of the scatter's LDS arrays)
-    uint32_t bin_slots;   // one-table Insert: slots per bin of count / scatter (a multiple of the region;
+    uint32_t bin_slots;   // one-table Insert: slots per first-level bin (a multiple of the region;
                           // batched plans bin by region)
-    uint32_t heavy_limit; // regroup: regions with more pairs go to build_heavy_kernel
     uint32_t bucket;
     uint32_t hash;
     uint32_t key_float;
@@ -228,16 +223,6 @@ template <int SS>
 constexpr size_t build_scatter_lds(uint32_t maxbins)
 {
     return (size_t)BuildShape<SS>::kTile * SS + (size_t)BuildShape<SS>::kTile * 2 + (size_t)maxbins * 12;
-}
-
-// Most bins of count / scatter with slot size ss: the count kernel's LDS
-// histogram, and the scatter's LDS of 160 KiB less 8 KiB for its static scan
-// storage (4096 for every slot size).
-constexpr uint32_t build_max_bins(uint32_t ss)
-{
-    const uint32_t tile = build_tile_pairs(ss);
-    const uint32_t fit = (163840u - 8192u - tile * ss - tile * 2) / 12;
-    return fit < kBuildMaxBins ? fit : kBuildMaxBins;
 }
 
 // 3. One tile per workgroup: sort its pairs by region in LDS, claim a run in
@@ -682,47 +667,6 @@ __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
     }
 }
 
-// 6. Pairs of the heavy regions: the CAS walk from their first slot, every
-// heavy region spread over the whole grid.
-template <int KS, int VS>
-__global__ __launch_bounds__(256) void build_heavy_kernel(BuildArgs a)
-{
-    using L = Lay<KS, VS>;
-    using Raw = typename L::Raw;
-    const uint32_t nh = a.heavy[0];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint32_t h = 0; h < nh; ++h) {
-        const uint32_t r = a.heavy[1 + h];
-        const uint32_t ti = 0;  // the one-table Insert only
-        const BatchTable t = a.tables[ti];
-        const uint32_t s = a.offsets[r], e = a.offsets[r + 1];
-        const Raw* part = (const Raw*)a.part;
-        uint32_t fails = 0;
-        for (uint64_t j = s + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < e; j += stride) {
-            const Raw p = part[j];
-            const uint64_t key = L::key(p);
-            if (key == a.empty_key) {
-                ++fails;
-                continue;
-            }
-            fails += cas_walk<KS, VS>(a, t, p, key, first_slot<KS>(a, t.nbuckets, key), t.cap);
-        }
-        for (int o = 32; o > 0; o >>= 1) fails += __shfl_xor(fails, o);
-        if (lane == 0) count_failures(a, ti, fails);
-    }
-}
-
-// Two-level partition of one table: workgroup b regroups the pairs of bin b
-// (`group` consecutive regions, part1[offsets1[b] .. offsets1[b + 1]))
-// by region into part2 at the same positions, and writes the regions'
-// offsets (a.offsets, nreg + 1 entries).  Dynamic LDS: build_regroup_lds.
-template <int SS>
-constexpr size_t build_regroup_lds(uint32_t group)
-{
-    return (size_t)BuildShape<SS>::kTile * SS + (size_t)BuildShape<SS>::kTile * 2 + (size_t)group * 16;
-}
-
 // Exclusive prefix sums of cnt[0 .. n) into off[0 .. n), n <= 256, by the 64
 // lanes of wavefront 0 (four entries a lane, then a shuffle scan of the lane
 // sums); the other wavefronts do not call it.
@@ -751,75 +695,282 @@ __device__ __forceinline__ void wave0_offsets(const uint32_t* cnt, uint32_t* off
     }
 }
 
+// ---------------------------------------------------------------------------
+// One-table Insert with bounded bins: no count pass.
+//
+// The pairs go to first-level bins of `group` regions, then to their
+// regions.  Every bin is cut into one slice per XCD, appended to by the
+// workgroups of that XCD (so the partial lines of their runs meet in that
+// XCD's L2), and every slice and region holds at most a fixed number of pairs
+// (capS, capR), set from the batch size with room to spare for a uniform
+// hash, so no pass needs a count of the pairs first.  Pairs
+// that find their slice or region full go to one spill list and are
+// inserted by the CAS walk after every region is written
+// (build_spill_kernel), so the final table is still one that the CAS walk
+// produces for some order of the inserts: the regions' keys first, then
+// their walks past the region's end, then the spilled pairs.  Each pass reads
+// and writes every pair once:
+//   scatter: pairs -> part1 (bin b, slice x at (b * G + x) * capS), tiles
+//            sorted by bin in LDS, a run per bin reserved with one atomic on
+//            the slice's fill count;
+//   regroup: bin b's G slices -> part2 (region r at r * capR), tiles sorted
+//            by region in LDS, runs placed on the bin's LDS cursors;
+//   apply:   region by region, as above (offsets[r] = r * capR).
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kBuildMaxSlices = 8;  // slices per bin: one per XCD (blockIdx.x % 8)
+
+struct SpillArgs {
+    uint32_t* fill;       // pairs reserved in each slice: fill[b * G + x] (nbins1 * G)
+    uint8_t* spill;       // pairs that found no room (at most the batch)
+    uint32_t* nspill;     // their count
+    uint32_t cap_slice;   // capS
+    uint32_t cap_region;  // capR
+    uint32_t slices;      // G
+};
+
+// Slice and region capacities for n pairs over nbins1 bins of `slices`
+// slices and nreg regions.
+__host__ __device__ inline uint32_t build_cap_slice(uint64_t n, uint32_t nbins1, uint32_t slices)
+{
+    const uint64_t q = (n + (uint64_t)nbins1 * slices - 1) / ((uint64_t)nbins1 * slices);
+    return (uint32_t)(q + q / 16 + 1024);
+}
+__host__ __device__ inline uint32_t build_cap_region(uint64_t n, uint32_t nreg)
+{
+    const uint64_t q = (n + nreg - 1) / nreg;
+    return (uint32_t)(q + q / 4 + 256);
+}
+
+// Dynamic LDS of build_bounded_scatter_kernel: the staged tile and a bin id
+// per pair; per bin the tile's count and the run's start in part1 (32-bit),
+// and the bin's first position in the sorted tile, the pairs that fit the
+// slice and the spilled pairs' start in the tile's spill run (16-bit: at most
+// a tile).
+template <int SS>
+constexpr size_t build_bounded_scatter_lds(uint32_t nbins)
+{
+    return (size_t)BuildShape<SS>::kTile * SS + (size_t)nbins * 8 + (size_t)nbins * 6 + (size_t)BuildShape<SS>::kTile * 2;
+}
+
+// Most bins of the bounded scatter with slot size ss: two workgroups per CU.
+constexpr uint32_t build_bounded_max_bins(uint32_t ss)
+{
+    const uint32_t tile = build_tile_pairs(ss);
+    return (81920u - 64u - tile * ss - tile * 2) / 14;
+}
+
+// Scatter of the one-table Insert: persistent workgroups take tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ...; the next tile's pairs are loaded
+// while the current one is sorted and written.
 template <int KS, int VS>
-__global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs a, const uint8_t* part1,
-                                                                      const uint32_t* offsets1, uint32_t group)
+__global__ __launch_bounds__(kBuildThreads) void build_bounded_scatter_kernel(BuildArgs a, SpillArgs sp)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    using S = BuildShape<L::SS>;
+    extern __shared__ __align__(16) uint8_t smem[];
+    Raw* staged = (Raw*)smem;                           // kTile
+    uint32_t* tcnt = (uint32_t*)(staged + S::kTile);    // pairs of the tile in each bin
+    uint32_t* base = tcnt + a.maxbins;                  // run start in part1
+    uint16_t* toff = (uint16_t*)(base + a.maxbins);     // first of each bin in the sorted tile
+    uint16_t* keep = toff + a.maxbins;                  // pairs of the run that fit the bin
+    uint16_t* sbase = keep + a.maxbins;                 // start of the bin's spilled pairs in the tile's spill run
+    uint16_t* rid = sbase + a.maxbins;                  // kTile
+    typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
+    static_assert(sizeof(typename Scan::TempStorage) <= S::kTile * sizeof(Raw), "scan storage exceeds the tile");
+    static_assert(S::kTile <= 65535, "16-bit tile positions");
+    typename Scan::TempStorage& scan_tmp = *reinterpret_cast<typename Scan::TempStorage*>(smem);
+    __shared__ uint32_t s_spill;  // spilled pairs of the tile, then their start in the spill list
+
+    const BatchTable t = a.tables[0];
+    const uint32_t bs = a.bin_slots;
+    const uint32_t nbins = (t.cap + bs - 1) / bs;
+    const uint32_t ntiles = (uint32_t)((a.single_n + S::kTile - 1) / S::kTile);
+    const uint32_t span = (nbins + kBuildThreads - 1) / kBuildThreads;
+    Raw* part = (Raw*)a.part;
+    Raw* spill = (Raw*)sp.spill;
+    Raw p[S::kPer], pn[S::kPer];
+    const uint32_t x = blockIdx.x % sp.slices;
+    auto load_tile = [&](uint32_t tile, Raw (&q)[S::kPer]) {
+        if (tile >= ntiles) return;
+        const uint64_t st = (uint64_t)tile * S::kTile;
+        const uint32_t len = (uint32_t)min<uint64_t>(S::kTile, a.single_n - st);
+        const Raw* src = (const Raw*)a.in + st;
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < len) q[u] = src[j];
+        }
+    };
+    load_tile(blockIdx.x, p);
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t len = (uint32_t)min<uint64_t>(S::kTile, a.single_n - (uint64_t)tile * S::kTile);
+        for (uint32_t j = threadIdx.x; j < nbins; j += blockDim.x) tcnt[j] = 0;
+        if (threadIdx.x == 0) s_spill = 0;
+        __syncthreads();
+        uint32_t bin[S::kPer], rank[S::kPer];
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < len) {
+                bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / bs;
+                rank[u] = atomicAdd(&tcnt[bin[u]], 1u);
+            }
+        }
+        load_tile(tile + gridDim.x, pn);  // in flight during the sort and the writes
+        __syncthreads();
+        uint32_t local = 0;
+        for (uint32_t q = 0; q < span; ++q) {
+            const uint32_t r = threadIdx.x * span + q;
+            if (r < nbins) local += tcnt[r];
+        }
+        uint32_t excl;
+        Scan(scan_tmp).ExclusiveSum(local, excl);
+        for (uint32_t q = 0; q < span; ++q) {
+            const uint32_t r = threadIdx.x * span + q;
+            if (r < nbins) {
+                const uint32_t c = tcnt[r];
+                toff[r] = (uint16_t)excl;
+                excl += c;
+                uint32_t k = 0;
+                if (c) {
+                    const uint64_t f = (uint64_t)r * sp.slices + x;
+                    const uint32_t at = atomicAdd(&sp.fill[f], c);
+                    k = at < sp.cap_slice ? min(c, sp.cap_slice - at) : 0u;
+                    base[r] = (uint32_t)(f * sp.cap_slice + at);
+                    if (k < c) sbase[r] = (uint16_t)atomicAdd(&s_spill, c - k);
+                }
+                keep[r] = (uint16_t)k;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && s_spill) s_spill = atomicAdd(sp.nspill, s_spill);
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kBuildThreads;
+            if (j < len) {
+                const uint32_t pos = toff[bin[u]] + rank[u];
+                staged[pos] = p[u];
+                rid[pos] = (uint16_t)bin[u];
+            }
+        }
+        __syncthreads();
+        const uint32_t sp0 = s_spill;
+        for (uint32_t pos = threadIdx.x; pos < len; pos += blockDim.x) {
+            const uint32_t r = rid[pos];
+            const uint32_t i = pos - toff[r];
+            if (i < keep[r]) part[base[r] + i] = staged[pos];
+            else spill[sp0 + sbase[r] + (i - keep[r])] = staged[pos];
+        }
+        __syncthreads();  // staged, rid and the bin arrays are rewritten by the next tile
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) p[u] = pn[u];
+    }
+}
+
+// Dynamic LDS of build_bounded_regroup_kernel for bins of `group` regions.
+template <int SS>
+constexpr size_t build_bounded_regroup_lds(uint32_t group)
+{
+    return (size_t)BuildShape<SS>::kTile * SS + (size_t)BuildShape<SS>::kTile * 2 + (size_t)group * 16 +
+           (size_t)(kBuildMaxSlices + 1) * 4;
+}
+
+// Regroup of bin b = blockIdx.x (the pairs of its G slices, read in slice
+// order) by region into part2 (region r at r * capR), reading each pair once.
+// Writes the apply's range of every region of the bin: offsets[r] = r * capR,
+// apply_end[r] = offsets[r] + the pairs kept.  Dynamic LDS:
+// build_bounded_regroup_lds, plus G + 1 words of slice starts.
+template <int KS, int VS>
+__global__ __launch_bounds__(kBuildThreads) void build_bounded_regroup_kernel(BuildArgs a, SpillArgs sp,
+                                                                              const uint8_t* part1, uint32_t group)
 {
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
     using S = BuildShape<L::SS>;
     extern __shared__ __align__(16) uint8_t smem[];
     Raw* staged = (Raw*)smem;                          // kTile
-    uint32_t* rcnt = (uint32_t*)(staged + S::kTile);   // group: pairs of each region in the bin
-    uint32_t* rcur = rcnt + group;                     // group: next position of each region
+    uint32_t* rcur = (uint32_t*)(staged + S::kTile);   // group: pairs kept so far in each region
     uint32_t* tcnt = rcur + group;                     // group: pairs of each region in the tile
-    uint32_t* toff = tcnt + group;                     // group: first of each region in the tile
-    uint16_t* rid = (uint16_t*)(toff + group);         // kTile
+    uint32_t* toff = tcnt + group;                     // group: first of each region in the sorted tile
+    uint32_t* pre = toff + group;                      // G + 1: first pair of each slice in the bin
+    uint16_t* keep = (uint16_t*)(pre + kBuildMaxSlices + 1);  // group: pairs of the run that fit the region
+    uint16_t* sbase = keep + group;                    // group: start of its spilled pairs in the tile's spill run
+    uint16_t* rid = sbase + group;                     // kTile
+    __shared__ uint32_t s_spill;
     const BatchTable t = a.tables[0];
     const uint32_t b = blockIdx.x;
     const uint32_t rfirst = b * group;
     const uint32_t nr = min(group, a.nreg - rfirst);
-    const uint32_t s = offsets1[b], e = offsets1[b + 1];
-    const Raw* src = (const Raw*)part1;
+    const uint32_t G = sp.slices;
+    const Raw* src = (const Raw*)part1 + (uint64_t)b * G * sp.cap_slice;
     Raw* dst = (Raw*)a.part;
-    for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) rcnt[j] = 0;
-    __syncthreads();
-    // kPer loads in flight per thread before their histogram updates
-    for (uint32_t j0 = s; j0 < e; j0 += S::kTile) {
-        Raw p[S::kPer];
+    Raw* spill = (Raw*)sp.spill;
+    for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) rcur[j] = 0;
+    if (threadIdx.x < 64) {
+        // slice starts: a wavefront's running sum over G counts
+        uint32_t run = 0;
+        for (uint32_t c0 = 0; c0 < G; c0 += 64) {
+            const uint32_t sl = c0 + threadIdx.x;
+            const uint32_t v = sl < G ? min(sp.fill[(uint64_t)b * G + sl], sp.cap_slice) : 0u;
+            uint32_t incl = v;
 #pragma unroll
-        for (uint32_t u = 0; u < S::kPer; ++u) {
-            const uint32_t j = j0 + threadIdx.x + u * kBuildThreads;
-            if (j < e) p[u] = src[j];
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (threadIdx.x >= (uint32_t)o) incl += y;
+            }
+            if (sl < G) pre[sl] = run + incl - v;
+            run += __shfl(incl, 63);
         }
-#pragma unroll
-        for (uint32_t u = 0; u < S::kPer; ++u) {
-            const uint32_t j = j0 + threadIdx.x + u * kBuildThreads;
-            if (j < e) atomicAdd(&rcnt[first_slot<KS>(a, t.nbuckets, L::key(p[u])) / S::kRegion - rfirst], 1u);
-        }
+        if (threadIdx.x == 0) pre[G] = run;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t o = s;
-        for (uint32_t j = 0; j < nr; ++j) {
-            rcur[j] = o;
-            a.offsets[rfirst + j] = o;
-            const bool hot = rcnt[j] > a.heavy_limit;
-            a.apply_end[rfirst + j] = hot ? o : o + rcnt[j];
-            if (hot) a.heavy[1 + atomicAdd(&a.heavy[0], 1u)] = rfirst + j;
-            o += rcnt[j];
+    const uint32_t n = pre[G];
+    Raw p[S::kPer], pn[S::kPer];
+    // pair j of the bin: the slice whose start is the last one <= j
+    auto load_tile = [&](uint32_t t0, Raw (&q)[S::kPer]) {
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) {
+            const uint32_t j = t0 + threadIdx.x + u * kBuildThreads;
+            if (j < n) {
+                uint32_t lo = 0, hi = G;  // pre[lo] <= j < pre[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (pre[mid] <= j) lo = mid;
+                    else hi = mid;
+                }
+                q[u] = src[(uint64_t)lo * sp.cap_slice + (j - pre[lo])];
+            }
         }
-        if (rfirst + nr == a.nreg) a.offsets[a.nreg] = e;
-    }
-    __syncthreads();
-    for (uint32_t t0 = s; t0 < e; t0 += S::kTile) {
-        const uint32_t tlen = min(S::kTile, e - t0);
+    };
+    load_tile(0, p);
+    for (uint32_t t0 = 0; t0 < n; t0 += S::kTile) {
+        const uint32_t tlen = min(S::kTile, n - t0);
         for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) tcnt[j] = 0;
+        if (threadIdx.x == 0) s_spill = 0;
         __syncthreads();
-        Raw p[S::kPer];
         uint32_t bin[S::kPer], rank[S::kPer];
 #pragma unroll
         for (uint32_t u = 0; u < S::kPer; ++u) {
             const uint32_t j = threadIdx.x + u * kBuildThreads;
             if (j < tlen) {
-                p[u] = src[t0 + j];
                 bin[u] = first_slot<KS>(a, t.nbuckets, L::key(p[u])) / S::kRegion - rfirst;
                 rank[u] = atomicAdd(&tcnt[bin[u]], 1u);
             }
         }
+        load_tile(t0 + S::kTile, pn);
         __syncthreads();
         if (threadIdx.x < 64) wave0_offsets(tcnt, toff, nr);
+        for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) {
+            const uint32_t c = tcnt[j];
+            const uint32_t k = min(c, sp.cap_region - rcur[j]);
+            keep[j] = (uint16_t)k;
+            rcur[j] += k;
+            if (k < c) sbase[j] = (uint16_t)atomicAdd(&s_spill, c - k);
+        }
         __syncthreads();
+        if (threadIdx.x == 0 && s_spill) s_spill = atomicAdd(sp.nspill, s_spill);
 #pragma unroll
         for (uint32_t u = 0; u < S::kPer; ++u) {
             const uint32_t j = threadIdx.x + u * kBuildThreads;
@@ -830,14 +981,46 @@ __global__ __launch_bounds__(kBuildThreads) void build_regroup_kernel(BuildArgs 
             }
         }
         __syncthreads();
+        const uint32_t sp0 = s_spill;
         for (uint32_t pos = threadIdx.x; pos < tlen; pos += blockDim.x) {
             const uint32_t r = rid[pos];
-            dst[rcur[r] + (pos - toff[r])] = staged[pos];
+            const uint32_t i = pos - toff[r];
+            if (i < keep[r]) dst[(rfirst + r) * sp.cap_region + (rcur[r] - keep[r]) + i] = staged[pos];
+            else spill[sp0 + sbase[r] + (i - keep[r])] = staged[pos];
         }
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) rcur[j] += tcnt[j];
-        __syncthreads();
+#pragma unroll
+        for (uint32_t u = 0; u < S::kPer; ++u) p[u] = pn[u];
     }
+    for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) {
+        const uint32_t r = rfirst + j;
+        a.offsets[r] = r * sp.cap_region;
+        a.apply_end[r] = r * sp.cap_region + rcur[j];
+    }
+}
+
+// The spilled pairs: the CAS walk from their first slot, after every region
+// is written and the region walks' overflow is inserted.
+template <int KS, int VS>
+__global__ __launch_bounds__(256) void build_spill_kernel(BuildArgs a, SpillArgs sp)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    const BatchTable t = a.tables[0];
+    const uint32_t n = *sp.nspill;
+    const Raw* spill = (const Raw*)sp.spill;
+    uint32_t fails = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const Raw p = spill[j];
+        const uint64_t key = L::key(p);
+        if (key == a.empty_key) {
+            ++fails;
+            continue;
+        }
+        fails += cas_walk<KS, VS>(a, t, p, key, first_slot<KS>(a, t.nbuckets, key), t.cap);
+    }
+    for (int o = 32; o > 0; o >>= 1) fails += __shfl_xor(fails, o);
+    if ((threadIdx.x & 63) == 0) count_failures(a, 0, fails);
 }
 
 }  // namespace mxco

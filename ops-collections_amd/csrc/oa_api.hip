@@ -20,19 +20,23 @@
 using namespace mxco;
 
 // Workspace of the region-built Insert, allocated on first use and grown
-// with the batch (batch_build.h).
+// with the batch (batch_build.h): the batch's pairs grouped by first-level
+// bin (part1: nbins1 bins of G slices of build_cap_slice pairs), then by
+// region (part2: nreg regions of build_cap_region pairs), and the pairs that
+// find their slice or region full (spill, up to the whole batch).
 struct RegionWs {
-    uint64_t pairs;  // pairs part1 / part2 hold
+    uint64_t part1_pairs, part2_pairs, spill_pairs;  // what the buffers hold
     uint8_t* part1;
     uint8_t* part2;
-    uint32_t* counts1;   // nbins1 + 1
-    uint32_t* offsets1;  // nbins1 + 1
-    uint32_t* offsets2;  // nreg + 1
+    uint8_t* spill;
+    uint32_t* fill;      // nbins1 * kBuildMaxSlices slice counts, then the spill count
+    uint32_t* offsets2;  // nreg: region starts in part2
     uint32_t* ovc;       // nreg
-    uint32_t* heavy;     // nreg + 1
     uint32_t* apply_end; // nreg
     BatchTable* table;   // the table's descriptor
     uint32_t* reg0;      // {0, nbins1}
+    BatchTable table_host;
+    uint32_t reg0_host[2];
     uint32_t nreg, nbins1, group;
     // A captured region build keeps part1 / part2 in its kernel arguments,
     // so once one is captured the buffers are never reallocated: a larger
@@ -230,8 +234,8 @@ void free_region_ws(mxco_oa* t)
     RegionWs* w = t->rws;
     if (!w) return;
     if (w->done) (void)hipEventDestroy(w->done);
-    for (void* p : {(void*)w->part1, (void*)w->part2, (void*)w->counts1, (void*)w->offsets1, (void*)w->offsets2,
-                    (void*)w->ovc, (void*)w->heavy, (void*)w->apply_end, (void*)w->table, (void*)w->reg0})
+    for (void* p : {(void*)w->part1, (void*)w->part2, (void*)w->spill, (void*)w->fill, (void*)w->offsets2,
+                    (void*)w->ovc, (void*)w->apply_end, (void*)w->table, (void*)w->reg0})
         if (p) (void)hipFree(p);
     delete w;
     t->rws = nullptr;
@@ -241,8 +245,10 @@ void free_region_ws(mxco_oa* t)
 // the table (n >= 2^20 and n >= capacity / 2) and the table is larger than
 // the 256 MiB last-level cache (MXCO_OA_REGION_INSERT_MIN_BYTES), under
 // LinearProbing without a stencil; MXCO_OA_REGION_INSERT=0 keeps the CAS
-// walk (both read at each call).  The build moves about 112 bytes per pair
-// plus 32 per slot; the CAS walk about 300 per distinct key at load factor
+// walk (both read at each call).  The build moves 80 bytes per 16-byte pair
+// (read and written by the scatter and by the regroup, read by the apply)
+// plus 16 per slot written (32 when the regions are first read from a table
+// that may hold keys); the CAS walk about 300 per distinct key at load factor
 // 0.8 on a table that misses the cache, less on one that fits it or for a
 // stream of repeated keys, so those keep the walk.  Under stream capture the
 // workspace must already be large enough, and once a build has been captured
@@ -265,6 +271,30 @@ RegionShape region_shape(const mxco_oa* t)
     return r;
 }
 
+// The buffers a region build of n pairs needs, and its slice count.
+struct RegionNeed {
+    uint32_t slices, cap_slice, cap_region;
+    uint64_t part1, part2, spill;
+};
+
+RegionNeed region_need(const mxco_oa* t, uint64_t n)
+{
+    const RegionShape rs = region_shape(t);
+    RegionNeed q;
+    q.slices = kBuildMaxSlices;
+    q.cap_slice = build_cap_slice(n, rs.nbins1, q.slices);
+    q.cap_region = build_cap_region(n, rs.nreg);
+    q.part1 = (uint64_t)rs.nbins1 * q.slices * q.cap_slice;
+    q.part2 = (uint64_t)rs.nreg * q.cap_region;
+    q.spill = n;
+    return q;
+}
+
+bool region_ws_holds(const RegionWs* w, const RegionNeed& q)
+{
+    return w && w->part1_pairs >= q.part1 && w->part2_pairs >= q.part2 && w->spill_pairs >= q.spill;
+}
+
 bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream_t stream)
 {
     const char* env = getenv("MXCO_OA_REGION_INSERT");
@@ -272,14 +302,18 @@ bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream
     if (n < (1ull << 20) || n * 2 < t->cap || n >= 0xffffffffull) return false;
     const char* mb = getenv("MXCO_OA_REGION_INSERT_MIN_BYTES");
     if (t->cap * t->ss < (mb ? strtoull(mb, nullptr, 10) : (256ull << 20))) return false;
-    if (region_shape(t).nbins1 > build_max_bins(t->ss)) return false;  // the scatter's LDS
+    const RegionShape rs = region_shape(t);
+    if (rs.nbins1 > build_bounded_max_bins(t->ss)) return false;  // the scatter's LDS
+    // part1 / part2 positions are 32-bit
+    const RegionNeed q = region_need(t, n);
+    if (q.part1 > 0xffffffffull || q.part2 > 0xffffffffull) return false;
     // under capture the workspace must exist and be large enough; after a
     // capture it never grows
-    if (capturing(stream) || (t->rws && t->rws->captured)) return t->rws && t->rws->pairs >= n;
+    if (capturing(stream) || (t->rws && t->rws->captured)) return region_ws_holds(t->rws, q);
     return true;
 }
 
-hipError_t region_ws(mxco_oa* t, uint64_t n)
+hipError_t region_ws(mxco_oa* t, uint64_t n, hipStream_t stream)
 {
     if (!t->rws) {
         RegionWs* w = new (std::nothrow) RegionWs();
@@ -291,20 +325,21 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         w->nreg = rs.nreg;
         w->group = rs.group;
         w->nbins1 = rs.nbins1;
-        hipError_t e = hipMalloc((void**)&w->counts1, (w->nbins1 + 1ull) * 4);
-        if (e == hipSuccess) e = hipMalloc((void**)&w->offsets1, (w->nbins1 + 1ull) * 4);
-        if (e == hipSuccess) e = hipMalloc((void**)&w->offsets2, (w->nreg + 1ull) * 4);
+        hipError_t e = hipMalloc((void**)&w->fill, ((uint64_t)w->nbins1 * kBuildMaxSlices + 1) * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->offsets2, w->nreg * 4ull);
         if (e == hipSuccess) e = hipMalloc((void**)&w->ovc, w->nreg * 4ull);
-        if (e == hipSuccess) e = hipMalloc((void**)&w->heavy, (w->nreg + 1ull) * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&w->apply_end, w->nreg * 4ull);
         if (e == hipSuccess) e = hipMalloc((void**)&w->table, sizeof(BatchTable));
         if (e == hipSuccess) e = hipMalloc((void**)&w->reg0, 2 * sizeof(uint32_t));
         if (e == hipSuccess) e = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
         if (e == hipSuccess) {
-            BatchTable bt{t->slots, (uint32_t)t->cap, (uint32_t)(t->cap / t->d.bucket_size)};
-            const uint32_t r0[2] = {0u, w->nbins1};
-            e = hipMemcpy(w->table, &bt, sizeof(bt), hipMemcpyHostToDevice);
-            if (e == hipSuccess) e = hipMemcpy(w->reg0, r0, sizeof(r0), hipMemcpyHostToDevice);
+            // stream-ordered from host memory the workspace keeps
+            w->table_host = BatchTable{t->slots, (uint32_t)t->cap, (uint32_t)(t->cap / t->d.bucket_size)};
+            w->reg0_host[0] = 0u;
+            w->reg0_host[1] = w->nbins1;
+            e = hipMemcpyAsync(w->table, &w->table_host, sizeof(BatchTable), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(w->reg0, w->reg0_host, sizeof(w->reg0_host), hipMemcpyHostToDevice, stream);
         }
         if (e != hipSuccess) {
             free_region_ws(t);
@@ -312,17 +347,22 @@ hipError_t region_ws(mxco_oa* t, uint64_t n)
         }
     }
     RegionWs* w = t->rws;
-    if (w->pairs < n) {
+    const RegionNeed q = region_need(t, n);
+    if (!region_ws_holds(w, q)) {
         if (w->captured) return hipErrorInvalidValue;  // a captured build holds these buffers
         // hipFree waits for the device, so no earlier build still reads them
-        if (w->part1) (void)hipFree(w->part1);
-        if (w->part2) (void)hipFree(w->part2);
-        w->part1 = w->part2 = nullptr;
-        w->pairs = 0;
-        hipError_t e = hipMalloc((void**)&w->part1, n * t->ss);
-        if (e == hipSuccess) e = hipMalloc((void**)&w->part2, n * t->ss);
+        for (uint8_t** b : {&w->part1, &w->part2, &w->spill}) {
+            if (*b) (void)hipFree(*b);
+            *b = nullptr;
+        }
+        w->part1_pairs = w->part2_pairs = w->spill_pairs = 0;
+        hipError_t e = hipMalloc((void**)&w->part1, q.part1 * t->ss);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->part2, q.part2 * t->ss);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->spill, q.spill * t->ss);
         if (e != hipSuccess) return e;
-        w->pairs = n;
+        w->part1_pairs = q.part1;
+        w->part2_pairs = q.part2;
+        w->spill_pairs = q.spill;
     }
     return hipSuccess;
 }
@@ -333,27 +373,30 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     constexpr int SS = Lay<KS, VS>::SS;
     constexpr uint32_t R = BuildShape<SS>::kRegion;
     static const hipError_t attr = [] {
-        hipError_t e = hipFuncSetAttribute((const void*)build_scatter_kernel<KS, VS, true>,
+        hipError_t e = hipFuncSetAttribute((const void*)build_bounded_scatter_kernel<KS, VS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)build_scatter_lds<SS>(kBuildMaxBins));
+                                           (int)build_bounded_scatter_lds<SS>(build_bounded_max_bins(SS)));
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)build_regroup_kernel<KS, VS>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)build_regroup_lds<SS>(256));
+            e = hipFuncSetAttribute((const void*)build_bounded_regroup_kernel<KS, VS>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)build_bounded_regroup_lds<SS>(256));
         return e;
     }();
     if (attr != hipSuccess) return attr;
     const RegionWs& w = *t->rws;
-    if (w.group > 256 || w.nbins1 > kBuildMaxBins) return hipErrorInvalidValue;
+    if (w.group > 256 || w.nbins1 > build_bounded_max_bins(SS)) return hipErrorInvalidValue;
     const uint64_t n = oa.n;
-    zero_u32_pair_kernel<<<grid_for(w.nbins1 + 1ull, 1), 256, 0, s>>>(w.counts1, w.nbins1 + 1, w.heavy);
+    const RegionNeed q = region_need(t, n);
+    if (!region_ws_holds(&w, q)) return hipErrorInvalidValue;
+    uint32_t* nspill = w.fill + (uint64_t)w.nbins1 * kBuildMaxSlices;
+    zero_u32_pair_kernel<<<grid_for((uint64_t)w.nbins1 * q.slices, 1), 256, 0, s>>>(w.fill, w.nbins1 * q.slices,
+                                                                                  nspill);
     BuildArgs a1;
     memset(&a1, 0, sizeof(a1));
     a1.in = oa.in;
     a1.part = w.part1;
     a1.tables = w.table;
     a1.reg0 = w.reg0;
-    a1.counts = w.counts1;
-    a1.offsets = w.offsets1;
     a1.counters = oa.counters;
     a1.empty_key = oa.empty_key;
     a1.empty_val = oa.empty_val;
@@ -361,25 +404,28 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
     a1.nreg = w.nbins1;
     a1.maxbins = w.nbins1;
     a1.bin_slots = w.group * R;
-    a1.heavy_limit = 0xffffffffu;
     a1.bucket = oa.bucket;
     a1.hash = oa.hash;
     a1.key_float = oa.key_float;
+    SpillArgs sp;
+    sp.fill = w.fill;
+    sp.spill = w.spill;
+    sp.nspill = nspill;
+    sp.cap_slice = q.cap_slice;
+    sp.cap_region = q.cap_region;
+    sp.slices = q.slices;
     const unsigned ntiles = (unsigned)((n + BuildShape<SS>::kTile - 1) / BuildShape<SS>::kTile);
-    build_count_kernel<KS, VS, true><<<std::min<unsigned>(ntiles, (unsigned)cu_count() * 4), kBuildThreads, 0, s>>>(a1);
-    build_scan_kernel<<<1, 1024, 0, s>>>(w.counts1, w.offsets1, w.nbins1);
-    build_scatter_kernel<KS, VS, true><<<ntiles, kBuildThreads, build_scatter_lds<SS>(w.nbins1), s>>>(a1);
+    const unsigned sgrid = std::min<unsigned>(ntiles, (unsigned)cu_count() * 2);
+    build_bounded_scatter_kernel<KS, VS><<<sgrid, kBuildThreads, build_bounded_scatter_lds<SS>(w.nbins1), s>>>(a1, sp);
     BuildArgs a2 = a1;
     a2.part = w.part2;
     a2.offsets = w.offsets2;
     a2.ovc = w.ovc;
-    a2.heavy = w.heavy;
     a2.apply_end = w.apply_end;
     a2.nreg = w.nreg;
     a2.bin_slots = R;
-    a2.heavy_limit = 2 * R;
-    build_regroup_kernel<KS, VS><<<w.nbins1, kBuildThreads, build_regroup_lds<SS>(w.group), s>>>(a2, w.part1,
-                                                                                               w.offsets1, w.group);
+    build_bounded_regroup_kernel<KS, VS><<<w.nbins1, kBuildThreads, build_bounded_regroup_lds<SS>(w.group), s>>>(
+        a2, sp, w.part1, w.group);
     // an empty table's regions start from the empty slot
     if (table_empty)
         build_apply_kernel<KS, VS, true, false><<<w.nreg, kBuildThreads, 0, s>>>(a2);
@@ -387,7 +433,7 @@ hipError_t launch_region_insert(const mxco_oa* t, const OaArgs& oa, hipStream_t 
         build_apply_kernel<KS, VS, true, true><<<w.nreg, kBuildThreads, 0, s>>>(a2);
     const unsigned ogrid = std::max(1u, std::min<unsigned>((w.nreg + 3) / 4, (unsigned)cu_count() * 8));
     build_overflow_kernel<KS, VS, true><<<ogrid, 256, 0, s>>>(a2);
-    build_heavy_kernel<KS, VS><<<(unsigned)cu_count() * 4, 256, 0, s>>>(a2);
+    build_spill_kernel<KS, VS><<<(unsigned)cu_count() * 4, 256, 0, s>>>(a2, sp);
     return hipGetLastError();
 }
 
@@ -584,7 +630,7 @@ MXCO_API mxco_status mxco_oa_insert(mxco_oa* t, const void* pairs, const mxco_st
         a.counters = t->counters;
     }
     bool region = region_insert_fits(t, n, a, stream);
-    if (region && region_ws(t, n) != hipSuccess) {
+    if (region && region_ws(t, n, stream) != hipSuccess) {
         (void)hipGetLastError();  // no room for the workspace: the CAS walk needs none
         region = false;
     }

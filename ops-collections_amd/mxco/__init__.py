@@ -1180,3 +1180,20 @@ class Server:
         a, b = u64(), u64()
         check(_lib.mxco_server_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+# ---------------------------------------------------------------------------
+# Measured ceilings (random reads, streaming copy)
+# ---------------------------------------------------------------------------
+
+_sig("mxco_diag_random_read", [vp, u64, u32, u64, vp, u64, vp])
+_sig("mxco_diag_stream_copy", [vp, vp, u64, vp])
+
+
+def random_read(buf, nbytes, access, n, sink, stream=None):
+    """n reads of `access` bytes at hashed aligned offsets of buf's first nbytes."""
+    check(_lib.mxco_diag_random_read(_ptr(buf), nbytes, access, n, _ptr(sink), sink.numel(), _stream(stream)))
+
+
+def stream_copy(src, dst, nbytes, stream=None):
+    check(_lib.mxco_diag_stream_copy(_ptr(src), _ptr(dst), nbytes, _stream(stream)))
