@@ -135,6 +135,10 @@ def run_gpu(args, torch, mxco, lf, group, steps, warmup, verify=True):
         exp = torch.where(is_hit, q * 7 + 1, torch.full_like(q, -1))
         res["verified"] = bool(torch.equal(out, exp))
         m.clear()
+        lost, made = m.insert_stats(pairs)  # the CAS walk, counting its compare-and-swaps
+        res["cas_walk_atomics"] = {"cas_made": made, "cas_lost": lost, "retry_rate": round(lost / max(made, 1), 6),
+                                   "made_minus_lost_equals_keys": made - lost == keys.numel()}
+        m.clear()
         res["insert_failed"] = m.insert(pairs)
         ln_f, sl_f = m.probe_stats(q)
         ln_i, sl_i = m.probe_stats(keys)
@@ -201,13 +205,16 @@ def run_zipf_insert(args, torch, mxco, steps=3):
     torch.cuda.synchronize()
     ms = sum(a.elapsed_time(b) for a, b in ev) / steps
     m.clear()
+    lost, made = m.insert_stats(pairs)
+    m.clear()
     not_inserted = m.insert(pairs)
     uk = Zipf.keys(torch.nonzero(seen).flatten())
     ok = not_inserted == n - distinct and bool(torch.equal(m.find(uk), uk * 7 + 1))
     del m, pairs, keys
     torch.cuda.empty_cache()
     return {"draws": n, "distinct": distinct, "s": 1.1, "ms": round(ms, 4), "gkeys_s": round(n / ms / 1e6, 4),
-            "duplicates_reported": not_inserted, "verified": ok}
+            "duplicates_reported": not_inserted, "verified": ok,
+            "cas_walk_atomics": {"cas_made": made, "cas_lost": lost, "retry_rate": round(lost / max(made, 1), 6)}}
 
 
 def run_bloom(torch, mxco, steps=5):
@@ -637,83 +644,151 @@ def run_batched(args, torch, mxco, steps=3, warmup=1):
 def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_requests=3000):
     """64 resident tables (48 StaticMap + 16 StaticMultimap int64, 2^20..2^26
     slots, half full), open-loop Poisson request stream (80 % Find/Contains,
-    15 % Insert, 5 % Retrieve; batches log-uniform 1..2^16 keys, 70 % hit):
-    the continuously batching server against one-at-a-time plain calls."""
+    15 % Insert on the maps, 5 % Retrieve on the multimaps; batches
+    log-uniform 1..2^16 keys, 70 % hit): the continuously batching server
+    against one-at-a-time plain calls, latency percentiles over all requests
+    and per table kind; then the closed loop: every request of the
+    200k-QPS stream submitted at once, keys over the makespan (saturation)."""
     import numpy as np
     from mxco import workloads
     ops = {"insert": mxco.REQ_INSERT, "find": mxco.REQ_FIND, "contains": mxco.REQ_CONTAINS,
            "retrieve": mxco.REQ_MM_RETRIEVE}
+
+    def serve(w, mode, arrivals):
+        srv = mxco.Server()
+        tabs = [srv.add_map(c) if t < w["num_maps"] else srv.add_multimap(c) for t, c in enumerate(w["caps"])]
+        for t, pairs in enumerate(w["init"]):
+            op = mxco.REQ_INSERT if t < w["num_maps"] else mxco.REQ_MM_INSERT
+            srv.wait(srv.submit(tabs[t], op, pairs))
+        outs, reqs = [], []
+        for r in w["requests"]:
+            n = r["n"]
+            if r["op"] == "retrieve":
+                o1 = torch.zeros(9 * n, dtype=torch.int64, device="cuda")
+                o2 = torch.zeros(9 * n, dtype=torch.int64, device="cuda")
+            elif r["op"] == "find":
+                o1, o2 = torch.empty(n, dtype=torch.int64, device="cuda"), None
+            elif r["op"] == "contains":
+                o1, o2 = torch.empty(n, dtype=torch.uint8, device="cuda"), None
+            else:
+                o1 = o2 = None
+            outs.append((o1, o2))
+            reqs.append((tabs[r["table"]], ops[r["op"]], r["input"].data_ptr(), n,
+                         o1.data_ptr() if o1 is not None else None, o2.data_ptr() if o2 is not None else None,
+                         o1.numel() if r["op"] == "retrieve" else 0))
+        torch.cuda.synchronize()
+        results, lat = srv.replay(reqs, arrivals, plain=(mode == "plain"))
+        ticks, folded = srv.stats()
+        srv.close()
+        del srv
+        return results, outs, np.array(lat) / 1e3, ticks, folded
+
+    def same(a, b):
+        ok = a[0] == b[0]
+        for (a1, a2), (b1, b2) in zip(a[1], b[1]):
+            if a1 is not None:
+                ok &= bool(torch.equal(a1, b1))
+            if a2 is not None:
+                ok &= bool(torch.equal(a2, b2))
+        return bool(ok)
+
+    def pct(x, q):
+        return round(float(np.percentile(x, q)), 4) if len(x) else None
+
     out = {}
     for qps in qps_list:
         w = workloads.serving(num_requests=num_requests, qps=qps, seed=5)
-        res = {}
-        outs_by_mode = {}
+        keys = sum(r["n"] for r in w["requests"])
+        mm = np.array([r["op"] == "retrieve" for r in w["requests"]])
+        res, runs = {}, {}
         for mode in ("batched", "plain"):
-            srv = mxco.Server()
-            tabs = [srv.add_map(c) if t < w["num_maps"] else srv.add_multimap(c) for t, c in enumerate(w["caps"])]
-            for t, pairs in enumerate(w["init"]):
-                op = mxco.REQ_INSERT if t < w["num_maps"] else mxco.REQ_MM_INSERT
-                srv.wait(srv.submit(tabs[t], op, pairs))
-            outs, reqs = [], []
-            for r in w["requests"]:
-                n = r["n"]
-                if r["op"] == "retrieve":
-                    o1 = torch.zeros(9 * n, dtype=torch.int64, device="cuda")
-                    o2 = torch.zeros(9 * n, dtype=torch.int64, device="cuda")
-                elif r["op"] == "find":
-                    o1, o2 = torch.empty(n, dtype=torch.int64, device="cuda"), None
-                elif r["op"] == "contains":
-                    o1, o2 = torch.empty(n, dtype=torch.uint8, device="cuda"), None
-                else:
-                    o1 = o2 = None
-                outs.append((o1, o2))
-                reqs.append((tabs[r["table"]], ops[r["op"]], r["input"].data_ptr(), n,
-                             o1.data_ptr() if o1 is not None else None, o2.data_ptr() if o2 is not None else None,
-                             o1.numel() if r["op"] == "retrieve" else 0))
-            torch.cuda.synchronize()
-            results, lat = srv.replay(reqs, w["arrivals_us"], plain=(mode == "plain"))
-            lat = np.array(lat) / 1e3
+            results, outs, lat, ticks, folded = serve(w, mode, w["arrivals_us"])
             done = np.array(w["arrivals_us"]) / 1e3 + lat
-            keys = sum(r["n"] for r in w["requests"])
-            ticks, folded = srv.stats()
-            res[mode] = {"p50_ms": round(float(np.percentile(lat, 50)), 4),
-                         "p99_ms": round(float(np.percentile(lat, 99)), 4),
+            res[mode] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                         "map_p99_ms": pct(lat[~mm], 99), "multimap_p99_ms": pct(lat[mm], 99),
                          "gkeys_s": round(keys / (done.max() * 1e-3) / 1e9, 4),
                          "ticks": ticks, "requests_per_tick": round(folded / max(ticks, 1), 2)}
-            outs_by_mode[mode] = (results, outs)
-            srv.close()
-            del srv
-        same = outs_by_mode["batched"][0] == outs_by_mode["plain"][0]
-        for (a1, a2), (b1, b2) in zip(outs_by_mode["batched"][1], outs_by_mode["plain"][1]):
-            if a1 is not None:
-                same &= bool(torch.equal(a1, b1))
-            if a2 is not None:
-                same &= bool(torch.equal(a2, b2))
-        res["identical_results"] = bool(same)
-        res["keys"] = sum(r["n"] for r in w["requests"])
+            runs[mode] = (results, outs)
+        res["identical_results"] = same(runs["batched"], runs["plain"])
+        res["keys"] = keys
         out["qps_%d" % int(qps)] = res
-        del w, outs_by_mode
+        del runs
+        if qps == qps_list[-1]:
+            # closed loop: the same requests, all arriving at once
+            closed, runs = {"requests": len(w["requests"]), "keys": keys}, {}
+            for mode in ("batched", "plain"):
+                results, outs, lat, ticks, folded = serve(w, mode, [0.0] * len(w["requests"]))
+                closed[mode] = {"makespan_ms": round(float(lat.max()), 3),
+                                "gkeys_s": round(keys / (lat.max() * 1e-3) / 1e9, 4),
+                                "requests_per_tick": round(folded / max(ticks, 1), 2)}
+                runs[mode] = (results, outs)
+            closed["identical_results"] = same(runs["batched"], runs["plain"])
+            out["closed_loop"] = closed
+            del runs
+        del w
         torch.cuda.empty_cache()
     return out
 
 
-def cpu_baseline(args):
-    """The oracle's restatement of the probe/CAS algorithm on the host cores,
-    on bounded samples of the measured workload: Insert of round(0.8 * cap)
-    distinct int64 keys then Find of cap queries (50 % hit) on a 2^27-slot
-    table with T threads, and on a 2^23-slot table with one thread; and the
-    reference tests' std::unordered_map golden path on the 2^23 sample
-    (about 15 s of CPU time together)."""
+def _cpu_description():
+    """What /proc/cpuinfo and sysfs say about the host CPU."""
+    info = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if info:
+                    break  # the first processor's block
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            if k in ("vendor_id", "model name", "cpu family", "model", "stepping", "cache size", "cpu MHz",
+                     "cpu cores", "siblings") and k not in info:
+                info[k] = v
+            if k == "flags":
+                info["flags_count"] = len(v.split())
+    except OSError:
+        pass
+    try:
+        info["max_mhz"] = int(open("/sys/devices/system/cpu/cpu0/cpufreq/cpuinfo_max_freq").read()) // 1000
+    except (OSError, ValueError):
+        pass
+    info["logical_cpus_visible"] = os.cpu_count()
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def cpu_baseline(args, runs=5):
+    """The oracle's restatements of the reference algorithms on the host
+    cores, each leg the median of `runs` timed runs on a bounded sample of
+    the measured workload (about 20 s together):
+      - StaticMap int64: Insert of round(0.8 * cap) distinct keys then Find of
+        cap queries (50 % hit) on a 2^25-slot table, T threads (pthreads,
+        __atomic 64-bit CAS), and on a 2^21-slot table with one thread;
+      - the reference tests' std::unordered_map golden path, 2^22 sample;
+      - BloomFilter: Add of 2^26 keys and Contains of 2^26 (half absent) on
+        2^22 blocks (the 128 MiB filter), T threads;
+      - HyperLogLog p = 15: Add of 2^27 int64 keys, T threads (private
+        registers merged by maximum).
+    T is the host's CPU share for this process: OMP_NUM_THREADS when set
+    (the measurement host sets it to the share of one GPU), else every CPU
+    in the affinity mask."""
+    import statistics
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = args.cpu_threads or (int(share) if share and share.isdigit() else len(os.sched_getaffinity(0)))
 
-    def leg(cap_log2, nthreads):
+    def median_run(fn):
+        """fn() -> (seconds, verified); the median time and all verified."""
+        res = [fn() for _ in range(runs)]
+        return statistics.median(r[0] for r in res), all(r[1] for r in res), [round(r[0], 4) for r in res]
+
+    def map_leg(cap_log2, nthreads):
         cap_req = 1 << cap_log2
         rng = np.random.default_rng(5)
-        t = O.OpenAddressing(np.int64, np.int64, cap_req, -1, -1)
-        n_ins = int(round(0.8 * t.capacity))
+        probe = O.OpenAddressing(np.int64, np.int64, cap_req, -1, -1)
+        n_ins = int(round(0.8 * probe.capacity))
+        del probe
         # distinct keys below 2^62: an odd multiplier is a bijection mod 2^62
         keys = ((np.arange(n_ins, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) & np.uint64((1 << 62) - 1))
         keys = keys.astype(np.int64)
@@ -721,17 +796,21 @@ def cpu_baseline(args):
         q = np.concatenate([keys[rng.integers(0, n_ins, cap_req // 2)],
                             rng.integers(1 << 62, (1 << 63) - 1, cap_req // 2)])
         rng.shuffle(q)
-        t0 = time.perf_counter()
-        failed = t.mt_insert(keys, vals, nthreads)
-        t1 = time.perf_counter()
-        out = t.mt_find(q, nthreads)
-        t2 = time.perf_counter()
-        ok = failed == 0 and np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1))
-        del t
-        return {"value": (n_ins + len(q)) / (t2 - t0) / 1e9, "insert_s": round(t1 - t0, 3),
-                "find_s": round(t2 - t1, 3), "verified": bool(ok),
-                "sample": "Insert of %d keys + Find of %d queries (50%% hit) on a 2^%d-slot table at LF 0.8, "
-                          "%d thread%s" % (n_ins, len(q), cap_log2, nthreads, "s" if nthreads > 1 else "")}
+        exp = np.where(q < (1 << 62), q * 7 + 1, -1)
+
+        def once():
+            t = O.OpenAddressing(np.int64, np.int64, cap_req, -1, -1)
+            t0 = time.perf_counter()
+            failed = t.mt_insert(keys, vals, nthreads)
+            out = t.mt_find(q, nthreads)
+            dt = time.perf_counter() - t0
+            return dt, failed == 0 and np.array_equal(out, exp)
+
+        sec, ok, all_s = median_run(once)
+        return {"value": (n_ins + len(q)) / sec / 1e9, "unit": "Gkeys/s", "cores": nthreads,
+                "median_s": round(sec, 4), "runs_s": all_s, "verified": ok,
+                "sample": "Insert of %d keys + Find of %d queries (50%% hit) on a 2^%d-slot table at LF 0.8"
+                          % (n_ins, len(q), cap_log2)}
 
     def golden_leg(cap_log2):
         """The reference tests' host path: a std::unordered_map filled and
@@ -747,36 +826,73 @@ def cpu_baseline(args):
         q = np.concatenate([keys[rng.integers(0, n_ins, cap_req // 2)],
                             rng.integers(1 << 62, (1 << 63) - 1, cap_req // 2)])
         rng.shuffle(q)
-        out = np.empty_like(q)
-        ins_s, find_s = ctypes.c_double(), ctypes.c_double()
         P = ctypes.c_void_p
-        lib.golden_map_run(P(keys.ctypes.data), P(vals.ctypes.data), ctypes.c_uint64(n_ins), P(q.ctypes.data),
-                           ctypes.c_uint64(len(q)), ctypes.c_int64(-1), P(out.ctypes.data), ctypes.byref(ins_s),
-                           ctypes.byref(find_s))
-        ok = np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1))
-        return {"value": (n_ins + len(q)) / (ins_s.value + find_s.value) / 1e9, "unit": "Gkeys/s", "cores": 1,
-                "insert_s": round(ins_s.value, 3), "find_s": round(find_s.value, 3), "verified": bool(ok),
-                "sample": "std::unordered_map Insert of %d keys + Find of %d queries (50%% hit), one thread"
-                          % (n_ins, len(q))}
 
-    mt = leg(27, threads)
-    st = leg(23, 1)
-    gm = golden_leg(23)
-    model = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+        def once():
+            out = np.empty_like(q)
+            ins_s, find_s = ctypes.c_double(), ctypes.c_double()
+            lib.golden_map_run(P(keys.ctypes.data), P(vals.ctypes.data), ctypes.c_uint64(n_ins),
+                               P(q.ctypes.data), ctypes.c_uint64(len(q)), ctypes.c_int64(-1), P(out.ctypes.data),
+                               ctypes.byref(ins_s), ctypes.byref(find_s))
+            return ins_s.value + find_s.value, bool(np.array_equal(out, np.where(q < (1 << 62), q * 7 + 1, -1)))
+
+        sec, ok, all_s = median_run(once)
+        return {"value": (n_ins + len(q)) / sec / 1e9, "unit": "Gkeys/s", "cores": 1, "median_s": round(sec, 4),
+                "runs_s": all_s, "verified": ok,
+                "sample": "std::unordered_map Insert of %d keys + Find of %d queries (50%% hit)" % (n_ins, len(q))}
+
+    def bloom_leg():
+        nb, n = 1 << 22, 1 << 26
+        rng = np.random.default_rng(21)
+        keys = rng.integers(-(1 << 63), (1 << 63) - 1, n, dtype=np.int64)
+        q = np.concatenate([keys[: n // 2], rng.integers(-(1 << 63), (1 << 63) - 1, n - n // 2, dtype=np.int64)])
+        words = np.zeros(nb * 8, np.uint32)
+
+        def add():
+            words[:] = 0
+            t0 = time.perf_counter()
+            O.bloom_add_mt(words, nb, keys, threads)
+            return time.perf_counter() - t0, True
+
+        def contains():
+            t0 = time.perf_counter()
+            out = O.bloom_contains_mt(words, nb, q, threads)
+            return time.perf_counter() - t0, bool(out[: n // 2].all())
+
+        a_s, a_ok, a_all = median_run(add)
+        c_s, c_ok, c_all = median_run(contains)
+        return {"add_gkeys_s": round(n / a_s / 1e9, 4), "contains_gkeys_s": round(n / c_s / 1e9, 4),
+                "add_median_s": round(a_s, 4), "contains_median_s": round(c_s, 4), "add_runs_s": a_all,
+                "contains_runs_s": c_all, "cores": threads, "verified": a_ok and c_ok,
+                "sample": "BloomFilter of 2^22 blocks: Add of 2^26 keys, Contains of 2^26 (half absent)"}
+
+    def hll_leg():
+        n, p = 1 << 27, 15
+        keys = np.random.default_rng(22).integers(0, 10 ** 8, n, dtype=np.int64)
+
+        def once():
+            regs = np.zeros(1 << p, np.uint8)
+            t0 = time.perf_counter()
+            O.hll_add_mt(regs, p, keys, threads)
+            return time.perf_counter() - t0, bool(regs.any())
+
+        sec, ok, all_s = median_run(once)
+        return {"add_gkeys_s": round(n / sec / 1e9, 4), "median_s": round(sec, 4), "runs_s": all_s,
+                "cores": threads, "verified": ok, "sample": "HyperLogLog p=15 Add of 2^27 int64 keys"}
+
+    mt = map_leg(25, threads)
+    st = map_leg(21, 1)
+    gm = golden_leg(22)
+    bl = bloom_leg()
+    hl = hll_leg()
     return {"value": mt["value"], "unit": "Gkeys/s", "cores": threads, "kind": "port",
-            "sample": "oracle restatement (pthreads, __atomic 64-bit CAS) of StaticMap int64 " + mt["sample"],
-            "insert_s": mt["insert_s"], "find_s": mt["find_s"], "verified": mt["verified"],
-            "single_thread": {"value": st["value"], "unit": "Gkeys/s", "cores": 1, "sample": st["sample"],
-                              "verified": st["verified"]},
-            "golden_unordered_map": gm,
-            "cpu_model": model}
+            "sample": "oracle restatement (pthreads, __atomic 64-bit CAS) of StaticMap int64 " + mt["sample"]
+                      + ", median of %d runs" % runs,
+            "median_s": mt["median_s"], "runs_s": mt["runs_s"], "verified": mt["verified"],
+            "single_thread": st, "golden_unordered_map": gm, "bloom": bl, "hll": hl,
+            "threads_from": "OMP_NUM_THREADS" if (not args.cpu_threads and share and share.isdigit())
+                            else ("--cpu-threads" if args.cpu_threads else "sched_getaffinity"),
+            "cpu": _cpu_description()}
 
 
 def main():
@@ -855,6 +971,7 @@ def main():
         "find_gkeys_s": round(r["n_find"] / (r["find_ms"] * 1e-3) / 1e9, 4),
         "clear_ms": round(r["clear_ms"], 4),
         "insert_path": r["insert_path"],
+        "cas_walk_atomics": r.get("cas_walk_atomics"),
         "verified": r["verified"] and r["insert_failed"] == 0,
         "roofline": roof_ins if dominant_insert else roof_find,
         "roofline_other": roof_find if dominant_insert else roof_ins,

@@ -194,6 +194,13 @@ MXCO_API mxco_status mxco_oa_insert_path(const mxco_oa* table, uint64_t n, hipSt
  * a diagnostic for roofline accounting): *lines = distinct 128-byte lines
  * and *slots = slots the reference's sequential walk reads
  * (open_addressing_ref_impl.h:568-605), summed over the batch.  Synchronous. */
+/* Insert by the CAS walk (never the region build), counting its
+ * compare-and-swaps: *cas_made attempted on empty slots and *cas_lost lost to
+ * another key, after which the walk goes on at the next slot (no reference
+ * counterpart; a diagnostic: the atomic retry rate is lost / made).  The
+ * table receives the pairs as Insert does. */
+MXCO_API mxco_status mxco_oa_insert_stats(mxco_oa* table, const void* pairs, uint64_t n, hipStream_t stream,
+                                          uint64_t* cas_lost, uint64_t* cas_made);
 MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
                                          uint64_t* lines, uint64_t* slots);
 
@@ -436,7 +443,8 @@ typedef struct mxco_server mxco_server;
 typedef struct mxco_server_config {
     uint64_t hbm_reserve_bytes; /* admission keeps at least this much HBM free */
     uint64_t hbm_budget_bytes;  /* 0 = no cap on the tables' total bytes */
-    uint64_t max_tick_keys;     /* keys folded into one tick (0 = 2^24) */
+    uint64_t max_tick_keys;     /* keys folded into one tick (0 = 2^24); a tick also takes at
+                                   most 4096 requests, and one larger request runs alone */
     uint32_t group_size;        /* lanes per key of the batched probes (0 = 4) */
     uint32_t blocks_per_cu;     /* persistent grid (0 = 64) */
     uint8_t empty_key[8];       /* StaticMap sentinels, little-endian */
@@ -482,7 +490,9 @@ MXCO_API mxco_status mxco_server_wait(mxco_server* server, uint64_t ticket, uint
 /* Waits until every submitted request has completed. */
 MXCO_API mxco_status mxco_server_drain(mxco_server* server);
 /* Open-loop replay: submits request i at arrival_us[i] after the call starts,
- * then waits for all; fills per-request results and latencies. */
+ * then waits for all; fills per-request results and latencies (completion
+ * time minus scheduled arrival time, so a late submit counts too).  All
+ * arrival times 0 is the closed loop: the latency maximum is the makespan. */
 MXCO_API mxco_status mxco_server_replay(mxco_server* server, const mxco_request* requests, const double* arrival_us,
                                         uint32_t n, uint64_t* results, double* latency_us);
 /* The same open-loop replay served one request at a time by the plain

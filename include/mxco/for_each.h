@@ -88,7 +88,7 @@ void for_each_launch(const Slot* slots, uint64_t cap, uint32_t bucket, uint32_t 
 {
     if (n == 0 || keys == nullptr) return;
     int cus = 0;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+    (void)mxco_device_cu_count(&cus);  // of the current device
     // CheckedBatchSize (open_addressing_impl.h:534-545)
     const uint64_t limit = 0xffffffffull - (uint64_t)(cus > 0 ? cus : 1) * 1024ull;
     if (n > limit) throw std::length_error("ForEach: batch larger than the kernel index range");

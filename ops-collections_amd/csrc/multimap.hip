@@ -18,9 +18,11 @@
 // per-query value sort use hipCUB.
 #include <string.h>
 
+#include <algorithm>
 #include <hipcub/hipcub.hpp>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "mxco_common.h"
 #include "oa_kernels.h"
@@ -39,6 +41,10 @@ struct mxco_mm {
     uint64_t* host;  // pinned, 4 words
     void* buf[6];
     size_t buf_size[6];
+    // Outgrown scratch buffers, freed by destroy: hipFree waits for the whole
+    // device, so freeing them while other streams run (a server's map ticks)
+    // would stall those streams' requests.
+    std::vector<void*> retired;
 };
 
 namespace {
@@ -286,11 +292,14 @@ __global__ __launch_bounds__(256) void mm_export_kernel(const uint8_t* table, ui
 mxco_status reserve(mxco_mm* m, int which, size_t bytes)
 {
     if (m->buf_size[which] >= bytes) return MXCO_OK;
-    if (m->buf[which]) (void)hipFree(m->buf[which]);
-    m->buf[which] = nullptr;
-    m->buf_size[which] = 0;
-    size_t want = bytes < 4096 ? 4096 : bytes;
-    MXCO_HIP(hipMalloc(&m->buf[which], want), "StaticMultimap: scratch");
+    // at least double: a buffer is replaced O(log) times, and the retired
+    // ones together stay below the live one
+    size_t want = std::max<size_t>({bytes, 2 * m->buf_size[which], 4096});
+    void* p = nullptr;
+    MXCO_HIP(hipMalloc(&p, want), "StaticMultimap: scratch");
+    // the old buffer is kept until destroy (see `retired`)
+    if (m->buf[which]) m->retired.push_back(m->buf[which]);
+    m->buf[which] = p;
     m->buf_size[which] = want;
     return MXCO_OK;
 }
@@ -442,6 +451,7 @@ MXCO_API mxco_status mxco_mm_destroy(mxco_mm* m)
     if (m->host) (void)hipHostFree(m->host);
     for (int i = 0; i < 6; ++i)
         if (m->buf[i]) (void)hipFree(m->buf[i]);
+    for (void* p : m->retired) (void)hipFree(p);
     delete m;
     return ok();
 }

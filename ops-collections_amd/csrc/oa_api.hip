@@ -815,4 +815,23 @@ MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* t, const void* keys, uint64_t 
     return ok();
 }
 
+MXCO_API mxco_status mxco_oa_insert_stats(mxco_oa* t, const void* pairs, uint64_t n, hipStream_t stream,
+                                          uint64_t* cas_lost, uint64_t* cas_made)
+{
+    if (mxco_status s = check_table(t, "insert_stats")) return s;
+    if (!cas_lost || !cas_made) return fail(MXCO_INVALID_ARGUMENT, "insert_stats: null output");
+    *cas_lost = *cas_made = 0;
+    if (n == 0 || !pairs) return ok();
+    if (mxco_status s = check_batch(n, "insert_stats")) return s;
+    t->empty = false;
+    OaArgs a = base_args(t);
+    a.in = (const uint8_t*)pairs;
+    a.n = n;
+    if (mxco_status s = zero_counters(t, stream)) return s;
+    a.counters = t->counters;
+    if (mxco_status s = launch(t, OP_INSERT_STATS, a, stream, "insert_stats")) return s;
+    if (mxco_status s = read_counters(t, stream, cas_lost, cas_made)) return s;
+    return ok();
+}
+
 }  // extern "C"

@@ -34,6 +34,7 @@ enum OaOp : int {
     OP_INSERT_AND_FIND = 6,
     OP_ERASE = 7,
     OP_PROBE_STATS = 8,  // Find's walk, counting 128-byte lines and slots read
+    OP_INSERT_STATS = 9, // Insert's CAS walk, counting compare-and-swaps lost to another key and made
 };
 
 struct OaArgs {
@@ -349,8 +350,8 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
     // 2-byte slots, one load per slot
     constexpr int SPL = SS >= 4 ? 128 / SS : 16;
     constexpr int K = SPL / G;     // slots per lane per window
-    constexpr bool kInsertFamily =
-        OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN || OP == OP_INSERT_AND_FIND;
+    constexpr bool kInsertFamily = OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_ASSIGN ||
+                                   OP == OP_INSERT_AND_FIND || OP == OP_INSERT_STATS;
     constexpr int OUTS = VS ? VS : KS;  // Find / InsertAndFind output width
     constexpr int VOFF = L::VOFF;
     constexpr uint64_t kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
@@ -639,6 +640,12 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                                 }
                             }
                         }
+                        if constexpr (OP == OP_INSERT_STATS) {
+                            if (winner) {
+                                add0 = r == 0;  // lost to another key: the walk goes on
+                                add1 = 1;
+                            }
+                        }
                         if constexpr (G > 1) {
                             const uint64_t m1 = (__ballot(winner && r == 1) >> gbase) & kGroupMask;
                             const uint64_t m2 = (__ballot(winner && r == 2) >> gbase) & kGroupMask;
@@ -712,9 +719,9 @@ template <int OP>
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long cnt0,
                                                unsigned long long cnt1)
 {
-    constexpr bool kTwoCounters = OP == OP_INSERT_OR_ASSIGN || OP == OP_PROBE_STATS;
+    constexpr bool kTwoCounters = OP == OP_INSERT_OR_ASSIGN || OP == OP_PROBE_STATS || OP == OP_INSERT_STATS;
     if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN || OP == OP_COUNT || OP == OP_ERASE ||
-                  OP == OP_PROBE_STATS) {
+                  OP == OP_PROBE_STATS || OP == OP_INSERT_STATS) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             cnt0 += __shfl_xor(cnt0, o);

@@ -52,6 +52,7 @@ hipError_t launch_oa_g(int op, const OaArgs& a, dim3 grid, hipStream_t s, int in
     case OP_INSERT_AND_FIND: oa_kernel<KS, VS, G, OP_INSERT_AND_FIND><<<grid, block, 0, s>>>(a); break;
     case OP_ERASE: oa_kernel<KS, VS, G, OP_ERASE><<<grid, block, 0, s>>>(a); break;
     case OP_PROBE_STATS: oa_kernel<KS, VS, G, OP_PROBE_STATS><<<grid, block, 0, s>>>(a); break;
+    case OP_INSERT_STATS: oa_kernel<KS, VS, G, OP_INSERT_STATS><<<grid, block, 0, s>>>(a); break;
     case OP_INSERT_OR_ASSIGN:
         if constexpr (VS == 0) return hipErrorInvalidValue;
         else oa_kernel<KS, VS, G, OP_INSERT_OR_ASSIGN><<<grid, block, 0, s>>>(a);

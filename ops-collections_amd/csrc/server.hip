@@ -66,6 +66,7 @@ struct Table {
 
 constexpr int kTickSlots = 4;
 constexpr int kMmWorkers = 4;  // StaticMultimap worker threads (and streams)
+constexpr uint32_t kMaxTickRequests = 4096;
 
 struct Tick {
     // One upload per tick: [items | chunks | zeroed insert counters].
@@ -75,6 +76,9 @@ struct Tick {
     unsigned long long* counters_host = nullptr;  // pinned read-back of the insert counters
     uint32_t counters_cap = 0;
     hipEvent_t done = nullptr;
+    // Outgrown buffers, freed by destroy: hipFree waits for the whole device
+    // and would stall every tick and multimap request in flight.
+    std::vector<void*> retired_host, retired_dev;
     std::vector<Req*> reqs;
     std::vector<uint32_t> insert_items;  // request index of each insert item
     bool busy = false;
@@ -109,21 +113,29 @@ namespace {
 
 mxco_status grow_tick(Tick& t, size_t bytes, uint32_t counters)
 {
+    // Sized at create for a full tick; a tick grows only for one request
+    // larger than max_tick_keys.  The slot is idle here (its previous tick
+    // retired), but the old buffers are only freed at destroy.
     if (bytes > t.blob_cap) {
-        if (t.blob_host) (void)hipHostFree(t.blob_host);
-        if (t.blob_dev) (void)hipFree(t.blob_dev);
-        t.blob_host = nullptr;
-        t.blob_dev = nullptr;
         const size_t cap = std::max(bytes, 2 * t.blob_cap);
-        MXCO_HIP(hipHostMalloc((void**)&t.blob_host, cap, 0), "server: tick buffers");
-        MXCO_HIP(hipMalloc((void**)&t.blob_dev, cap), "server: tick buffers");
+        uint8_t *h = nullptr, *d = nullptr;
+        MXCO_HIP(hipHostMalloc((void**)&h, cap, 0), "server: tick buffers");
+        if (hipError_t e = hipMalloc((void**)&d, cap); e != hipSuccess) {
+            (void)hipHostFree(h);
+            return hip_fail(e, "server: tick buffers");
+        }
+        if (t.blob_host) t.retired_host.push_back(t.blob_host);
+        if (t.blob_dev) t.retired_dev.push_back(t.blob_dev);
+        t.blob_host = h;
+        t.blob_dev = d;
         t.blob_cap = cap;
     }
     if (counters > t.counters_cap) {
-        if (t.counters_host) (void)hipHostFree(t.counters_host);
-        t.counters_host = nullptr;
         const uint32_t cap = std::max(counters, 2 * t.counters_cap);
-        MXCO_HIP(hipHostMalloc((void**)&t.counters_host, cap * sizeof(unsigned long long), 0), "server: counters");
+        unsigned long long* h = nullptr;
+        MXCO_HIP(hipHostMalloc((void**)&h, cap * sizeof(unsigned long long), 0), "server: counters");
+        if (t.counters_host) t.retired_host.push_back(t.counters_host);
+        t.counters_host = h;
         t.counters_cap = cap;
     }
     return MXCO_OK;
@@ -155,6 +167,28 @@ uint32_t tick_chunk(uint64_t keys)
     return c;
 }
 
+struct TickLayout {
+    size_t off_chunks, off_counters, bytes;
+};
+
+// One tick's upload: [items | chunks | zeroed insert counters], 16-byte aligned parts.
+TickLayout tick_layout(uint64_t nitems, uint64_t nchunks, uint64_t ninsert)
+{
+    TickLayout l;
+    l.off_chunks = (nitems * sizeof(ServeItem) + 15) & ~size_t(15);
+    l.off_counters = (l.off_chunks + nchunks * sizeof(BatchChunk) + 15) & ~size_t(15);
+    l.bytes = l.off_counters + 2ull * ninsert * sizeof(unsigned long long);
+    return l;
+}
+
+// The most chunks a tick of at most `keys` keys in `nitems` requests cuts:
+// tick_chunk(k) exceeds k / (8 * CUs) unless it is capped at 64 (then
+// k / 64 < 8 * CUs too) or at kChunk; each request adds at most one partial chunk.
+uint64_t tick_chunk_bound(uint64_t keys, uint64_t nitems)
+{
+    return std::max<uint64_t>(8ull * (uint64_t)cu_count(), keys / kChunk + 1) + nitems;
+}
+
 // Runs one tick's requests (already chosen) on the map stream: one upload,
 // one insert launch, one read launch (Find and Contains), one read-back.
 mxco_status launch_tick(mxco_server* s, Tick& t)
@@ -170,9 +204,8 @@ mxco_status launch_tick(mxco_server* s, Tick& t)
     uint32_t nchunks = 0;
     for (Req* r : t.reqs) nchunks += (uint32_t)((r->r.n + csz - 1) / csz);
     const uint32_t ninsert = (uint32_t)ins.size();
-    const size_t off_chunks = (nitems * sizeof(ServeItem) + 15) & ~size_t(15);
-    const size_t off_counters = (off_chunks + nchunks * sizeof(BatchChunk) + 15) & ~size_t(15);
-    const size_t bytes = off_counters + 2ull * ninsert * sizeof(unsigned long long);
+    const TickLayout lay = tick_layout(nitems, nchunks, ninsert);
+    const size_t off_chunks = lay.off_chunks, off_counters = lay.off_counters, bytes = lay.bytes;
     if (mxco_status e = grow_tick(t, bytes, 2 * ninsert + 2)) return e;
     ServeItem* items = (ServeItem*)t.blob_host;
     BatchChunk* chunks = (BatchChunk*)(t.blob_host + off_chunks);
@@ -257,7 +290,8 @@ void dispatcher_loop(mxco_server* s)
             }
             int& ph = phase[tb];
             const bool insert = r->r.op == MXCO_REQ_INSERT;
-            const bool fits = keys + r->r.n <= s->cfg.max_tick_keys || t->reqs.empty();
+            const bool fits =
+                (keys + r->r.n <= s->cfg.max_tick_keys && t->reqs.size() < kMaxTickRequests) || t->reqs.empty();
             if (ph == 3 || !fits || (insert && ph == 2)) {
                 ph = 3;
                 keep.push_back(r);
@@ -410,6 +444,14 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
         (void)mxco_server_destroy(s);
         return hip_fail(e, "server: create");
     }
+    // every tick slot sized for a full tick up front: no allocation on the request path
+    const TickLayout full = tick_layout(kMaxTickRequests, tick_chunk_bound(s->cfg.max_tick_keys, kMaxTickRequests),
+                                        kMaxTickRequests);
+    for (Tick& t : s->ticks)
+        if (mxco_status st = grow_tick(t, full.bytes, 2 * kMaxTickRequests + 2)) {
+            (void)mxco_server_destroy(s);
+            return st;
+        }
     s->dispatcher = std::thread(dispatcher_loop, s);
     s->completer = std::thread(completer_loop, s);
     for (int w = 0; w < kMmWorkers; ++w) s->mm_workers[w] = std::thread(mm_loop, s, w);
@@ -448,6 +490,8 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
         if (t.blob_host) (void)hipHostFree(t.blob_host);
         if (t.blob_dev) (void)hipFree(t.blob_dev);
         if (t.counters_host) (void)hipHostFree(t.counters_host);
+        for (void* p : t.retired_host) (void)hipHostFree(p);
+        for (void* p : t.retired_dev) (void)hipFree(p);
         if (t.done) (void)hipEventDestroy(t.done);
     }
     for (auto& kv : s->reqs) delete kv.second;
@@ -593,6 +637,7 @@ MXCO_API mxco_status mxco_server_replay(mxco_server* s, const mxco_request* reqs
 {
     if (!s || (n && (!reqs || !arrival_us))) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
     std::vector<uint64_t> tickets(n);
+    std::vector<double> late(n);  // submit time - arrival time
     mxco_status first_err = MXCO_OK;
     std::string err_msg;
     const double t0 = now_us();
@@ -603,6 +648,7 @@ MXCO_API mxco_status mxco_server_replay(mxco_server* s, const mxco_request* reqs
             if (due - t > 200) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(due - t - 100)));
             t = now_us();
         }
+        late[i] = t - due;
         if (mxco_status e = mxco_server_submit(s, &reqs[i], &tickets[i])) {
             tickets[i] = 0;
             if (first_err == MXCO_OK) {
@@ -622,7 +668,7 @@ MXCO_API mxco_status mxco_server_replay(mxco_server* s, const mxco_request* reqs
             }
         }
         if (results) results[i] = res;
-        if (latency_us) latency_us[i] = lat;
+        if (latency_us) latency_us[i] = lat + late[i];  // from the arrival time
     }
     if (first_err != MXCO_OK) return fail(first_err, err_msg);
     return ok();

@@ -89,6 +89,7 @@ _sig("mxco_oa_find", [vp, vp, ctypes.POINTER(Stencil), vp, u64, vp, i32])
 _sig("mxco_oa_contains", [vp, vp, ctypes.POINTER(Stencil), vp, u64, vp, i32])
 _sig("mxco_oa_count", [vp, vp, u64, vp, pu64])
 _sig("mxco_oa_erase", [vp, vp, u64, vp, i32, pu64])
+_sig("mxco_oa_insert_stats", [vp, vp, u64, vp, pu64, pu64])
 _sig("mxco_oa_probe_stats", [vp, vp, u64, vp, pu64, pu64])
 
 
@@ -298,6 +299,14 @@ class _OpenAddressing:
         ln, sl = u64(), u64()
         check(_lib.mxco_oa_probe_stats(self._h, _ptr(keys), n, _stream(stream), ctypes.byref(ln), ctypes.byref(sl)))
         return ln.value, sl.value
+
+    def insert_stats(self, values, n=None, stream=None):
+        """Insert by the CAS walk, returning (CAS lost to another key, CAS made)."""
+        n = values.shape[0] if n is None else n
+        lost, made = u64(), u64()
+        check(_lib.mxco_oa_insert_stats(self._h, _ptr(values), n, _stream(stream), ctypes.byref(lost),
+                                        ctypes.byref(made)))
+        return lost.value, made.value
 
     def insert_uses_region_build(self, n, stream=None):
         """Whether an Insert of n pairs takes the region build (a diagnostic)."""

@@ -896,6 +896,49 @@ ORC_API void orc_bloom_contains(const uint32_t* words, uint64_t num_blocks, cons
     orc_bloom_contains_policy(words, num_blocks, 8, 8, keys, ks, n, seed, out);
 }
 
+/* Multithreaded Add / Contains for the CPU baseline: contiguous key ranges
+ * per thread, Add by __atomic OR on the words (only where bits are missing). */
+typedef struct {
+    uint32_t* words; uint64_t num_blocks; uint32_t wpb, bpw; const uint8_t* keys; uint32_t ks;
+    uint64_t begin, end, seed; uint8_t* out;
+} bloom_job;
+
+static void* bloom_worker(void* arg)
+{
+    bloom_job* j = (bloom_job*)arg;
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        uint64_t h = orc_xxh64(j->keys + i * j->ks, j->ks, j->seed);
+        uint32_t* b = j->words + (uint64_t)bloom_block(h, j->num_blocks) * j->wpb;
+        int ok = 1;
+        for (uint32_t w = 0; w < j->wpb; ++w) {
+            uint32_t m = bloom_pattern((uint32_t)h, w, j->bpw);
+            uint32_t cur = __atomic_load_n(b + w, __ATOMIC_RELAXED);
+            if (j->out) ok &= (cur & m) == m;
+            else if ((cur & m) != m) __atomic_fetch_or(b + w, m, __ATOMIC_RELAXED);
+        }
+        if (j->out) j->out[i] = (uint8_t)ok;
+    }
+    return NULL;
+}
+
+/* out == NULL: Add; else Contains into out. */
+ORC_API void orc_bloom_mt(uint32_t* words, uint64_t num_blocks, uint32_t wpb, uint32_t pattern_bits,
+                          const void* keys, uint32_t ks, uint64_t n, uint64_t seed, uint8_t* out, int threads)
+{
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    bloom_job jobs[256];
+    uint64_t per = (n + threads - 1) / threads;
+    for (int i = 0; i < threads; ++i) {
+        bloom_job jb = {words, num_blocks, wpb, pattern_bits / wpb, (const uint8_t*)keys, ks,
+                        per * i < n ? per * i : n, per * (i + 1) < n ? per * (i + 1) : n, seed, out};
+        jobs[i] = jb;
+        pthread_create(&th[i], NULL, bloom_worker, &jobs[i]);
+    }
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+}
+
 /* ------------------------------------------------------------------------ */
 /* HyperLogLog (hyperloglog_ref.h:30-47, hyperloglog.inl:328-345,           */
 /* finalizer.h:46-151).                                                     */
@@ -911,6 +954,39 @@ ORC_API void orc_hll_add(uint8_t* regs, uint32_t p, const void* keys, uint32_t k
         uint8_t rank = (uint8_t)(__builtin_clzll(tail) + (PLANT(8) ? 0 : 1));
         if (regs[idx] < rank) regs[idx] = rank;
     }
+}
+
+/* Multithreaded Add for the CPU baseline: private registers per thread,
+ * merged by register-wise maximum. */
+typedef struct { uint8_t* regs; uint32_t p; const uint8_t* keys; uint32_t ks; uint64_t begin, end; } hll_job;
+
+static void* hll_worker(void* arg)
+{
+    hll_job* j = (hll_job*)arg;
+    orc_hll_add(j->regs, j->p, j->keys + j->begin * j->ks, j->ks, j->end - j->begin);
+    return NULL;
+}
+
+ORC_API void orc_hll_add_mt(uint8_t* regs, uint32_t p, const void* keys, uint32_t ks, uint64_t n, int threads)
+{
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    const uint64_t m = 1ull << p;
+    uint8_t* priv = (uint8_t*)calloc((size_t)threads, m);
+    pthread_t th[256];
+    hll_job jobs[256];
+    uint64_t per = (n + threads - 1) / threads;
+    for (int i = 0; i < threads; ++i) {
+        hll_job jb = {priv + (uint64_t)i * m, p, (const uint8_t*)keys, ks, per * i < n ? per * i : n,
+                      per * (i + 1) < n ? per * (i + 1) : n};
+        jobs[i] = jb;
+        pthread_create(&th[i], NULL, hll_worker, &jobs[i]);
+    }
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    for (int i = 0; i < threads; ++i)
+        for (uint64_t r = 0; r < m; ++r)
+            if (regs[r] < priv[(uint64_t)i * m + r]) regs[r] = priv[(uint64_t)i * m + r];
+    free(priv);
 }
 
 /* Rank histogram over the registers, 65 bins (kernels.h:74-92). */

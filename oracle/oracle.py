@@ -86,6 +86,9 @@ def lib():
         L.orc_bloom_contains_policy.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, vp,
                                                 ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, vp]
         L.orc_bloom_salts.restype = ctypes.POINTER(ctypes.c_uint32)
+        L.orc_bloom_mt.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                   ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
+        L.orc_hll_add_mt.argtypes = [vp, ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int]
         L.orc_hll_add.argtypes = [vp, ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.c_uint64]
         L.orc_hll_histogram.argtypes = [vp, ctypes.c_uint32, vp]
         L.orc_hll_z.restype = ctypes.c_double
@@ -265,3 +268,23 @@ def hll_histogram(regs, p):
     h = np.zeros(65, np.uint32)
     lib().orc_hll_histogram(_ptr(regs), p, _ptr(h))
     return h
+
+
+def bloom_add_mt(words, num_blocks, keys, threads, seed=0, words_per_block=8, pattern_bits=None):
+    k = np.ascontiguousarray(keys)
+    p = words_per_block if pattern_bits is None else pattern_bits
+    lib().orc_bloom_mt(_ptr(words), num_blocks, words_per_block, p, _ptr(k), k.itemsize, len(k), seed, None, threads)
+
+
+def bloom_contains_mt(words, num_blocks, keys, threads, seed=0, words_per_block=8, pattern_bits=None):
+    k = np.ascontiguousarray(keys)
+    p = words_per_block if pattern_bits is None else pattern_bits
+    out = np.zeros(len(k), np.uint8)
+    lib().orc_bloom_mt(_ptr(words), num_blocks, words_per_block, p, _ptr(k), k.itemsize, len(k), seed, _ptr(out),
+                       threads)
+    return out.astype(bool)
+
+
+def hll_add_mt(regs, p, keys, threads):
+    k = np.ascontiguousarray(keys)
+    lib().orc_hll_add_mt(_ptr(regs), p, _ptr(k), k.itemsize, len(k), threads)

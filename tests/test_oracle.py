@@ -436,6 +436,27 @@ def test_bloom_policies_vs_bitwise_restatement(wpb, pbits, dtype):
         assert g == all((r & m) == m for r, m in zip(words.tolist(), one))
 
 
+def test_bloom_and_hll_multithreaded_equal_sequential():
+    """The CPU baseline's threaded Add / Contains give the sequential words,
+    outputs and registers."""
+    rng = np.random.default_rng(31)
+    keys = rng.integers(-2**63, 2**63 - 1, 200000, dtype=np.int64)
+    for wpb, pb in ((8, 8), (16, 32)):
+        a = np.zeros(4096 * wpb, np.uint32)
+        b = np.zeros(4096 * wpb, np.uint32)
+        O.bloom_add(a, 4096, keys, 3, wpb, pb)
+        O.bloom_add_mt(b, 4096, keys, 7, 3, wpb, pb)
+        np.testing.assert_array_equal(a, b)
+        q = np.concatenate([keys[:1000], rng.integers(0, 2**62, 5000)])
+        np.testing.assert_array_equal(O.bloom_contains(a, 4096, q, 3, wpb, pb),
+                                      O.bloom_contains_mt(a, 4096, q, 5, 3, wpb, pb))
+    r1 = np.zeros(1 << 14, np.uint8)
+    r2 = np.zeros(1 << 14, np.uint8)
+    O.hll_add(r1, 14, keys)
+    O.hll_add_mt(r2, 14, keys, 6)
+    np.testing.assert_array_equal(r1, r2)
+
+
 @pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint32, np.float32])
 def test_bloom_vs_bitwise_restatement(dtype):
     rng = np.random.default_rng(6)

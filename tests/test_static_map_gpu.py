@@ -311,6 +311,26 @@ def test_assign_if_null_and_empty():
     assert m.find(dev(k)).cpu().numpy().tolist() == [50, 60]
 
 
+@pytest.mark.parametrize("group", GROUPS)
+def test_insert_stats_counts_compare_and_swaps(group):
+    """The retry-counting Insert: every distinct key lands with exactly one
+    successful compare-and-swap, so made - lost = keys inserted; the table
+    equals Insert's."""
+    rng = np.random.default_rng(23)
+    keys = distinct_keys(rng, 300000, 2**31 - 1, np.int32)
+    dup = np.concatenate([keys, keys[:50000]])
+    rng.shuffle(dup)
+    m = mx().StaticMap(400000, -1, -1, group_size=group)
+    lost, made = m.insert_stats(dev_pairs(dup, dup))
+    assert 0 <= lost < made
+    assert made - lost >= len(keys)  # plus duplicates caught by a lost compare-and-swap
+    assert m.count(dev(keys)) == len(keys)
+    assert np.array_equal(m.find(dev(keys)).cpu().numpy(), keys)
+    m2 = mx().StaticMap(400000, -1, -1, group_size=group)
+    lost, made = m2.insert_stats(dev_pairs(keys, keys))
+    assert made - lost == len(keys)
+
+
 def test_float_keys_use_operator_equal():
     # KeyEqual is == on float (so -0.0 equals +0.0 once the walk reaches it);
     # the sentinel test stays bitwise and the probe hash is over the bytes.
