@@ -720,6 +720,8 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
                 results, outs, lat, ticks, folded = serve(w, mode, [0.0] * len(w["requests"]))
                 closed[mode] = {"makespan_ms": round(float(lat.max()), 3),
                                 "gkeys_s": round(keys / (lat.max() * 1e-3) / 1e9, 4),
+                                "map_makespan_ms": round(float(lat[~mm].max()), 3),
+                                "multimap_makespan_ms": round(float(lat[mm].max()), 3),
                                 "requests_per_tick": round(folded / max(ticks, 1), 2)}
                 runs[mode] = (results, outs)
             closed["identical_results"] = same(runs["batched"], runs["plain"])

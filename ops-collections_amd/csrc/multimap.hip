@@ -158,7 +158,8 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
                                                        const uint8_t* stencil, uint32_t esz, uint32_t pred,
                                                        uint64_t n, uint64_t ek, uint64_t ev, uint8_t* out,
                                                        uint64_t* counts, const uint64_t* offsets, uint8_t* probe_out,
-                                                       uint8_t* value_out, unsigned long long* total)
+                                                       uint8_t* value_out, unsigned long long* total,
+                                                       uint64_t out_capacity, unsigned long long* clear)
 {
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
@@ -166,6 +167,15 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
     constexpr int SPL = 128 / SS;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long local = 0;
+    // Count clears the word the following gather counts long segments in
+    if (MODE == Q_COUNT && clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
+    if constexpr (MODE == Q_GATHER) {
+        // total[1] = the number of matches, for the host; a gather that would
+        // overflow the output writes nothing (the host reports the error)
+        const uint64_t matches_all = offsets[n];
+        if (blockIdx.x == 0 && threadIdx.x == 0) total[1] = matches_all;
+        if (matches_all > out_capacity) return;
+    }
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t key = load_bits<KS>(keys + i * KS);
         const bool sel = selected(stencil, esz, pred, i);
@@ -204,10 +214,10 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
                         const int64_t v = sval<VS>(L::val(s[k]));
                         if (MODE == Q_CONTAINS) stop = true;
                         if (MODE == Q_GATHER) {
-                            // out: scratch for segments sorted afterwards; value_out: final output
+                            // long segments land unsorted and are sorted afterwards
                             store_bits<KS>(probe_out + wpos * KS, key);
                             if (seg_len > kInlineSort) {
-                                store_bits<VS>(out + wpos * VS, (uint64_t)v);
+                                store_bits<VS>(value_out + wpos * VS, (uint64_t)v);
                             } else {
                                 // insertion into the sorted prefix of the segment (ascending, signed)
                                 uint64_t p = wpos;
@@ -236,9 +246,11 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
         }
     }
     if constexpr (MODE == Q_COUNT) {
+        if (total) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
-        if (__lane_id() == 0 && local) atomicAdd(total, local);
+            for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+            if (__lane_id() == 0 && local) atomicAdd(total, local);
+        }
     }
 }
 
@@ -367,7 +379,8 @@ mxco_status set_pred(const mxco_stencil* st, const uint8_t** data, uint32_t* esz
 
 mxco_status query(mxco_mm* m, int mode, const void* keys, const mxco_stencil* st, void* out, uint64_t n,
                   uint64_t* counts, const uint64_t* offsets, void* probe_out, void* value_out,
-                  unsigned long long* total, hipStream_t s)
+                  unsigned long long* total, hipStream_t s, uint64_t out_capacity = 0,
+                  unsigned long long* clear = nullptr)
 {
     const uint8_t* sd;
     uint32_t esz, pred;
@@ -380,23 +393,23 @@ mxco_status query(mxco_mm* m, int mode, const void* keys, const mxco_stencil* st
         mm_query_kernel<K, V, Q_CONTAINS><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz,   \
                                                                pred, n, m->empty_key, m->empty_val,               \
                                                                (uint8_t*)out, nullptr, nullptr, nullptr, nullptr, \
-                                                               nullptr);                                          \
+                                                               nullptr, 0, nullptr);                              \
         break;                                                                                                    \
     case Q_FIND:                                                                                                  \
         mm_query_kernel<K, V, Q_FIND><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz, pred,  \
                                                            n, m->empty_key, m->empty_val, (uint8_t*)out, nullptr, \
-                                                           nullptr, nullptr, nullptr, nullptr);                   \
+                                                           nullptr, nullptr, nullptr, nullptr, 0, nullptr);       \
         break;                                                                                                    \
     case Q_COUNT:                                                                                                 \
         mm_query_kernel<K, V, Q_COUNT><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz, pred, \
                                                             n, m->empty_key, m->empty_val, nullptr, counts,       \
-                                                            nullptr, nullptr, nullptr, total);                    \
+                                                            nullptr, nullptr, nullptr, total, 0, clear);          \
         break;                                                                                                    \
     default:                                                                                                      \
         mm_query_kernel<K, V, Q_GATHER><<<grid, 256, 0, s>>>(m->table, mask, (const uint8_t*)keys, sd, esz,      \
                                                              pred, n, m->empty_key, m->empty_val, (uint8_t*)out,  \
                                                              counts, offsets, (uint8_t*)probe_out,               \
-                                                             (uint8_t*)value_out, total);                         \
+                                                             (uint8_t*)value_out, total, out_capacity, nullptr);  \
     }
     MM_DISPATCH(m, QL);
 #undef QL
@@ -588,32 +601,49 @@ MXCO_API mxco_status mxco_mm_retrieve(mxco_mm* m, const void* keys, uint64_t n, 
     // the list of segments too long to sort in place (2n + 2 entries)
     if (mxco_status e = reserve(m, 0, (2 * n + 2) * sizeof(uint64_t))) return e;
     if (mxco_status e = reserve(m, 4, (n + 1) * sizeof(uint64_t))) return e;
-    if (mxco_status e = reserve(m, 2, sizeof(unsigned long long))) return e;
+    if (mxco_status e = reserve(m, 2, 2 * sizeof(unsigned long long))) return e;
     uint64_t* counts = sbuf<uint64_t>(m, 0);
     uint64_t* offsets = sbuf<uint64_t>(m, 4);
-    unsigned long long* total_dev = sbuf<unsigned long long>(m, 2);
-    MXCO_HIP(hipMemsetAsync(total_dev, 0, 8, s), "StaticMultimap: retrieve");
-    if (mxco_status e = query(m, Q_COUNT, keys, nullptr, nullptr, n, counts, nullptr, nullptr, nullptr, total_dev, s))
+    // res[0]: long segments listed by the gather, res[1]: matches
+    unsigned long long* res = sbuf<unsigned long long>(m, 2);
+    if (!probe_out || !value_out) {
+        // no output: only legal when nothing matches
+        uint64_t total = 0;
+        if (mxco_status e = query(m, Q_COUNT, keys, nullptr, nullptr, n, counts, nullptr, nullptr, nullptr, nullptr,
+                                  s))
+            return e;
+        if (mxco_status e = exclusive_sum(m, counts, offsets, n, s, &total)) return e;
+        if (total > output_capacity)
+            return fail(MXCO_LENGTH_ERROR, "StaticMultimap: output capacity is insufficient");
+        if (total) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
+        return ok();
+    }
+    // count -> scan -> gather back to back, one read-back: the gather checks
+    // the output capacity on the device and writes nothing when it is short.
+    // Segments of up to kInlineSort values are sorted in place in value_out;
+    // longer ones land there unsorted and are listed for a segmented sort.
+    if (mxco_status e = query(m, Q_COUNT, keys, nullptr, nullptr, n, counts, nullptr, nullptr, nullptr, nullptr, s,
+                              0, res))
         return e;
-    uint64_t total = 0;
-    if (mxco_status e = exclusive_sum(m, counts, offsets, n, s, &total)) return e;
-    if (total > output_capacity) return fail(MXCO_LENGTH_ERROR, "StaticMultimap: output capacity is insufficient");
-    if (total == 0) return ok();
-    if (!probe_out || !value_out) return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: null nonempty buffer");
-    // Gather: segments of up to kInlineSort values are sorted in place in
-    // value_out; longer ones go unsorted to scratch and are listed.
-    if (mxco_status e = reserve(m, 1, total * m->vs)) return e;
-    MXCO_HIP(hipMemsetAsync(total_dev, 0, 8, s), "StaticMultimap: retrieve");
-    if (mxco_status e = query(m, Q_GATHER, keys, nullptr, m->buf[1], n, counts, offsets, probe_out, value_out,
-                              total_dev, s))
+    size_t temp = 0;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, counts, offsets, (int64_t)n + 1, s), "scan");
+    if (mxco_status e = reserve(m, 3, temp)) return e;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(m->buf[3], temp, counts, offsets, (int64_t)n + 1, s), "scan");
+    if (mxco_status e = query(m, Q_GATHER, keys, nullptr, nullptr, n, counts, offsets, probe_out, value_out, res, s,
+                              output_capacity))
         return e;
-    MXCO_HIP(hipMemcpyAsync(m->host, total_dev, 8, hipMemcpyDeviceToHost, s), "StaticMultimap: retrieve");
+    MXCO_HIP(hipMemcpyAsync(m->host, res, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "StaticMultimap: retrieve");
     MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: retrieve");
     const int64_t nlong = (int64_t)m->host[0];
+    const uint64_t total = m->host[1];
+    if (total > output_capacity) return fail(MXCO_LENGTH_ERROR, "StaticMultimap: output capacity is insufficient");
     if (nlong > 0) {
+        // the sort reads a copy of the output and writes the listed segments back
+        if (mxco_status e = reserve(m, 1, total * m->vs)) return e;
+        MXCO_HIP(hipMemcpyAsync(m->buf[1], value_out, total * m->vs, hipMemcpyDeviceToDevice, s),
+                 "StaticMultimap: retrieve");
         const uint64_t* begins = counts;
         const uint64_t* ends = counts + n + 1;
-        size_t temp = 0;
         hipError_t he;
         if (m->vs == 4) {
             he = hipcub::DeviceSegmentedSort::SortKeys(nullptr, temp, (const int32_t*)m->buf[1], (int32_t*)value_out,

@@ -19,9 +19,11 @@
 // once (no erase), so a non-empty key read without an atomic is current.
 #include <string.h>
 
+#include <algorithm>
 #include <hipcub/hipcub.hpp>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "mxco_common.h"
 #include "mxco/detail/hash.h"
@@ -40,6 +42,8 @@ struct mxco_ms {
     uint64_t* host;  // pinned, 4 words
     void* buf[3];    // scratch: ranks / counts, scan temp, totals
     size_t buf_size[3];
+    // outgrown scratch, freed by destroy (hipFree waits for the whole device)
+    std::vector<void*> retired;
 };
 
 namespace {
@@ -217,11 +221,12 @@ __global__ __launch_bounds__(256) void ms_retrieve_kernel(const uint8_t* keys, u
 mxco_status reserve(mxco_ms* m, int which, size_t bytes)
 {
     if (m->buf_size[which] >= bytes) return MXCO_OK;
-    if (m->buf[which]) (void)hipFree(m->buf[which]);
-    m->buf[which] = nullptr;
-    m->buf_size[which] = 0;
-    const size_t want = bytes < 4096 ? 4096 : bytes;
-    MXCO_HIP(hipMalloc(&m->buf[which], want), "StaticMultiset: scratch");
+    // at least double, so the retired buffers together stay below the live one
+    const size_t want = std::max<size_t>({bytes, 2 * m->buf_size[which], 4096});
+    void* p = nullptr;
+    MXCO_HIP(hipMalloc(&p, want), "StaticMultiset: scratch");
+    if (m->buf[which]) m->retired.push_back(m->buf[which]);
+    m->buf[which] = p;
     m->buf_size[which] = want;
     return MXCO_OK;
 }
@@ -338,6 +343,7 @@ MXCO_API mxco_status mxco_ms_destroy(mxco_ms* m)
     if (m->host) (void)hipHostFree(m->host);
     for (void* b : m->buf)
         if (b) (void)hipFree(b);
+    for (void* b : m->retired) (void)hipFree(b);
     delete m;
     return ok();
 }

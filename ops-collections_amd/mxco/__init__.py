@@ -37,9 +37,12 @@ def _release(obj, destroy):
     """Frees obj._h with the named C-ABI destructor (no-op at interpreter exit,
     when the module globals are already gone)."""
     h = getattr(obj, "_h", None)
-    if h and _lib is not None:
-        getattr(_lib, destroy)(h)
     obj._h = None
+    try:
+        if h and _lib is not None:
+            getattr(_lib, destroy)(h)
+    except (NameError, AttributeError, TypeError):
+        pass  # interpreter shutdown: the library handle is gone
 
 
 class LengthError(ValueError):
@@ -196,7 +199,7 @@ class _OpenAddressing:
         self._capacity = c.value
         self.slot_size = s.value
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_oa_destroy")
 
     def capacity(self):
@@ -449,7 +452,7 @@ class BloomFilter:
         self._h = h
         self._nb = num_blocks
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_bloom_destroy")
 
     def block_extent(self):
@@ -540,7 +543,7 @@ class HyperLogLog:
         check(_lib.mxco_hll_precision_for_stddev(sd, ctypes.byref(p)))
         return cls(p.value, key_dtype, stream)
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_hll_destroy")
 
     def sketch_size_kb(self):
@@ -607,7 +610,7 @@ class StaticMultimap:
                                   _stream(stream), ctypes.byref(h)))
         self._h = h
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_mm_destroy")
 
     def capacity(self):
@@ -704,7 +707,7 @@ class RoaringBitmap:
                                   _stream(stream), ctypes.byref(h)))
         self._h = h
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_rb_destroy")
 
     def view(self):
@@ -763,7 +766,7 @@ class DynamicMap:
                                   ctypes.byref(h)))
         self._h = h
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_dm_destroy")
 
     def _info(self):
@@ -859,7 +862,7 @@ class StaticMultiset:
                                   _stream(stream), ctypes.byref(h)))
         self._h = h
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_ms_destroy")
 
     def capacity(self):
@@ -967,7 +970,7 @@ class BatchPlan:
         check(_lib.mxco_batch_plan_create(batch._h, op, arr, len(items), group_size, ctypes.byref(h)))
         self._h = h
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_batch_plan_destroy")
 
     def run(self, inp, out=None, out2=None, stream=None):
@@ -1012,7 +1015,7 @@ class BatchedStaticMap:
         self._h = h
         self.num_tables = len(capacities)
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "mxco_batch_destroy")
 
     def capacity(self, t):
@@ -1069,10 +1072,13 @@ class Graph:
             raise RuntimeError("hipGraphLaunch failed")
 
     def __del__(self):
-        if getattr(self, "_exec", None):
-            _hip.hipGraphExecDestroy(self._exec)
-            _hip.hipGraphDestroy(self._graph)
-            self._exec = None
+        ex, self._exec = getattr(self, "_exec", None), None
+        try:
+            if ex:
+                _hip.hipGraphExecDestroy(ex)
+                _hip.hipGraphDestroy(self._graph)
+        except (NameError, AttributeError, TypeError):
+            pass  # interpreter shutdown: the HIP runtime handle is gone
 
 
 # ---------------------------------------------------------------------------
@@ -1137,8 +1143,8 @@ class Server:
             check(_lib.mxco_server_destroy(self._h))
             self._h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _release=_release):
+        _release(self, "mxco_server_destroy")
 
     def add_map(self, capacity):
         t = u32()

@@ -118,3 +118,13 @@ def test_retrieve_long_and_short_segments(kd, vd):
     ep, ev = ref.retrieve(q)
     np.testing.assert_array_equal(po.cpu().numpy(), ep)
     np.testing.assert_array_equal(vo.cpu().numpy(), ev)
+    # calls of other sizes reuse (and grow) the scratch: a smaller one, then
+    # a larger one with an output larger than needed
+    for q2 in (np.array([15, 11], kd), np.tile(np.array([16, 14, 99, 12], kd), 50)):
+        total2 = ref.count(q2)
+        po2 = torch.empty(total2 + 7, dtype=TD[kd], device="cuda")
+        vo2 = torch.empty(total2 + 7, dtype=TD[vd], device="cuda")
+        assert mm.retrieve(dev(q2), po2, vo2) == total2
+        ep2, ev2 = ref.retrieve(q2)
+        np.testing.assert_array_equal(po2[:total2].cpu().numpy(), ep2)
+        np.testing.assert_array_equal(vo2[:total2].cpu().numpy(), ev2)
