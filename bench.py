@@ -556,7 +556,9 @@ def run_batched(args, torch, mxco, steps=3, warmup=1):
     (a BUILD plan -- Clear + Insert built region by region -- and one
     persistent Find launch) against 1024 plain per-table Clear/Insert/Find
     calls.  The graph of Clear + the persistent CAS Insert + Find is timed
-    beside it (insert_plan_ms)."""
+    beside it (insert_plan_ms, against the plain CAS calls), and the plain
+    calls again with the region build open to every table large enough
+    (plain_region_ms, against the BUILD plan)."""
     from mxco import workloads
     w = workloads.batched_tables(1024, (10, 22), (0.5, 0.85), (6, 20), seed=4,
                                  valid_extent=lambda c: mxco.valid_extent(c, 5, "linear"))
@@ -618,25 +620,48 @@ def run_batched(args, torch, mxco, steps=3, warmup=1):
             m.find(qv, out=ov)
         return failed
 
-    for _ in range(warmup):
-        plain_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    failed = 0
-    for _ in range(steps):
-        failed += plain_step()
-    torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t0) / steps
-    ok_p = failed == 0 and bool(torch.equal(out_p, w["expected"]))
+    def timed_plain():
+        out_p.fill_(0)
+        for _ in range(warmup):
+            plain_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        failed = 0
+        for _ in range(steps):
+            failed += plain_step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        return t, failed == 0 and bool(torch.equal(out_p, w["expected"]))
+
+    # the plain calls as they come (CAS walk: these tables are below the
+    # region build's size threshold), then with the region build opened to
+    # every table (it still needs n >= 2^20 and n >= capacity / 2), the
+    # same insert algorithm the BUILD plan runs
+    t_plain, ok_p = timed_plain()
+    key = "MXCO_OA_REGION_INSERT_MIN_BYTES"
+    saved = os.environ.get(key)
+    os.environ[key] = "0"
+    try:
+        t_plain_region, ok_pr = timed_plain()
+    finally:
+        if saved is None:
+            del os.environ[key]
+        else:
+            os.environ[key] = saved
+    region_tables = sum(1 for t in range(T) if w["n_ins"][t] >= (1 << 20) and 2 * w["n_ins"][t] >= w["caps"][t])
     keys = sum(w["n_ins"]) + sum(w["n_find"])
-    cfg = {"insert": ins.config(), "find": fnd.config()}
+    cfg = {name: {"group": pl.config()[0], "blocks_per_cu": pl.config()[1], "block": pl.launch_shape()[0],
+                  "wave_staged": pl.launch_shape()[1] if name == "find" else None}
+           for name, pl in (("insert", ins), ("find", fnd))}
     del maps, b, ins, bld, fnd
     torch.cuda.empty_cache()
     return {"tables": T, "slots": sum(w["caps"]), "inserts": sum(w["n_ins"]), "finds": sum(w["n_find"]),
             "batched_ms": round(t_batched * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
             "batched_gkeys_s": round(keys / t_batched / 1e9, 4), "plain_gkeys_s": round(keys / t_plain / 1e9, 4),
-            "speedup": round(t_plain / t_batched, 3), "identical_results": ok_b and ok_p,
+            "speedup": round(t_plain / t_batched, 3), "identical_results": ok_b and ok_p and ok_pr,
             "insert_plan_ms": round(t_cas * 1e3, 3), "insert_plan_speedup": round(t_plain / t_cas, 3),
+            "plain_region_ms": round(t_plain_region * 1e3, 3), "plain_region_tables": region_tables,
+            "speedup_vs_plain_region": round(t_plain_region / t_batched, 3),
             "tuned_config": cfg,
             "tune_ms": {k: round(v, 3) for k, v in tuned.items()}}
 

@@ -250,15 +250,22 @@ MXCO_API mxco_status mxco_batch_run(mxco_batch_plan* plan, const void* in, void*
                                     hipStream_t stream);
 /* Per-item failure counts of the last insert run (synchronous). */
 MXCO_API mxco_status mxco_batch_failures(mxco_batch_plan* plan, hipStream_t stream, uint64_t* failed_per_item);
-/* Autotuner: times the plan under every lane-group width x grid size and
- * keeps the fastest (results do not depend on the choice).  It runs the plan
- * several times; an insert plan's tables are cleared before each timing and
- * are left as one run leaves them, so tune an insert plan before the find
- * plans that read its tables. */
+/* Autotuner: times the plan under every lane-group width (1/2/4/8) x
+ * workgroups per CU x threads per workgroup (64/128/256), and for Find /
+ * Contains plans also the wave-staged kernel (256 threads), and keeps the
+ * fastest (results do not depend on the choice).  It runs the plan several
+ * times; an insert plan's tables are cleared before each timing and are left
+ * as one run leaves them, so tune an insert plan before the find plans that
+ * read its tables.  A BUILD plan is only timed. */
 MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* plan, const void* in, void* out, uint8_t* out2,
                                           hipStream_t stream, float* best_ms);
 MXCO_API mxco_status mxco_batch_plan_config(mxco_batch_plan* plan, uint32_t* group_size, uint32_t* blocks_per_cu);
 MXCO_API mxco_status mxco_batch_plan_set_config(mxco_batch_plan* plan, uint32_t group_size, uint32_t blocks_per_cu);
+/* Threads per workgroup of the plan's probe kernel (64, 128, 256) and, for
+ * Find / Contains, whether it is the wave-staged kernel (which takes 256). */
+MXCO_API mxco_status mxco_batch_plan_launch_shape(mxco_batch_plan* plan, uint32_t* block_size, uint32_t* wave_staged);
+MXCO_API mxco_status mxco_batch_plan_set_launch_shape(mxco_batch_plan* plan, uint32_t block_size,
+                                                      uint32_t wave_staged);
 
 /* ---------------------------------------------------------------------- */
 /* StaticMultimap (include/static_multimap.h:88-556): signed int32/int64  */

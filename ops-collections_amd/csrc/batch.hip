@@ -31,23 +31,24 @@ namespace mxco {
 template <int KS, int VS, int G, typename Item>
 static hipError_t launch_batch_g(int op, const OaArgs& a, const BatchTable* tables, const Item* items,
                                  const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters,
-                                 unsigned grid, hipStream_t s)
+                                 unsigned grid, hipStream_t s, unsigned block_threads, bool wave_staged)
 {
-    const dim3 block(256);
+    const dim3 block(block_threads);
     if constexpr (std::is_same<Item, mxco_batch_item>::value) {
-        // plans: Find / Contains with wave-staged outputs (MXCO_BATCH_WAVE_STAGED=0 selects the plain kernel)
-        static const int staged = env_int("MXCO_BATCH_WAVE_STAGED", 1);
-        if (staged && (op == OP_FIND || op == OP_CONTAINS)) {
+        // plans: Find / Contains with wave-staged outputs, 256 threads (MXCO_BATCH_WAVE_STAGED=0 or a
+        // plan shape without it selects the plain kernel)
+        static const int staged_env = env_int("MXCO_BATCH_WAVE_STAGED", 1);
+        if (staged_env && wave_staged && (op == OP_FIND || op == OP_CONTAINS)) {
             // a chunk is 16 wave sub-chunks, 4 wavefronts per workgroup: a
             // grid capped below the chunk count stays as tuned, a smaller one
             // grows to one workgroup per 4 sub-chunks
             const unsigned sg = grid >= nchunks ? nchunks * (kChunk / kWaveChunk / 4) : grid;
             if (op == OP_FIND)
-                oa_batch_wave_staged_kernel<KS, VS, G, OP_FIND><<<sg, block, 0, s>>>(a, tables, items, chunks,
-                                                                                     nchunks);
+                oa_batch_wave_staged_kernel<KS, VS, G, OP_FIND><<<sg, 256, 0, s>>>(a, tables, items, chunks,
+                                                                                   nchunks);
             else
-                oa_batch_wave_staged_kernel<KS, VS, G, OP_CONTAINS><<<sg, block, 0, s>>>(a, tables, items, chunks,
-                                                                                         nchunks);
+                oa_batch_wave_staged_kernel<KS, VS, G, OP_CONTAINS><<<sg, 256, 0, s>>>(a, tables, items, chunks,
+                                                                                       nchunks);
             return hipGetLastError();
         }
     }
@@ -71,26 +72,29 @@ static hipError_t launch_batch_g(int op, const OaArgs& a, const BatchTable* tabl
 template <int KS, int VS, typename Item>
 static hipError_t launch_batch_l(int g, int op, const OaArgs& a, const BatchTable* tables, const Item* items,
                                  const BatchChunk* chunks, uint32_t nchunks, unsigned long long* counters,
-                                 unsigned grid, hipStream_t s)
+                                 unsigned grid, hipStream_t s, unsigned block, bool staged)
 {
+#define LG(G) launch_batch_g<KS, VS, G, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s, block, staged)
     switch (g) {
-    case 1: return launch_batch_g<KS, VS, 1, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
-    case 2: return launch_batch_g<KS, VS, 2, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
-    case 4: return launch_batch_g<KS, VS, 4, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
-    case 8: return launch_batch_g<KS, VS, 8, Item>(op, a, tables, items, chunks, nchunks, counters, grid, s);
+    case 1: return LG(1);
+    case 2: return LG(2);
+    case 4: return LG(4);
+    case 8: return LG(8);
     default: return hipErrorInvalidValue;
     }
+#undef LG
 }
 
 hipError_t launch_batch_any(int ks, int vs, int g, int op, bool serve, const OaArgs& a, const BatchTable* tables,
                             const void* items, const BatchChunk* chunks, uint32_t nchunks,
-                            unsigned long long* counters, unsigned grid, hipStream_t s)
+                            unsigned long long* counters, unsigned grid, hipStream_t s, unsigned block, bool staged)
 {
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
 #define LB(K, V)                                                                                              \
     return serve ? launch_batch_l<K, V, ServeItem>(g, op, a, tables, (const ServeItem*)items, chunks, nchunks, \
-                                                   counters, grid, s)                                         \
+                                                   counters, grid, s, block, staged)                          \
                  : launch_batch_l<K, V, mxco_batch_item>(g, op, a, tables, (const mxco_batch_item*)items,     \
-                                                         chunks, nchunks, counters, grid, s)
+                                                         chunks, nchunks, counters, grid, s, block, staged)
     if (ks == 4 && vs == 4) LB(4, 4);
     if (ks == 8 && vs == 8) LB(8, 8);
     if (ks == 4 && vs == 8) LB(4, 8);
@@ -142,6 +146,8 @@ struct mxco_batch_plan {
     unsigned long long* host_counters;
     int group;
     unsigned blocks_per_cu;
+    unsigned block = 256;     // threads per workgroup of the probe kernel
+    bool wave_staged = true;  // Find / Contains: the wave-staged kernel (256 threads)
     // MXCO_BATCH_BUILD
     bool build;   // Clear of every table, then this insert plan
     bool region;  // ... by the region build (otherwise Clear + the insert kernel)
@@ -173,11 +179,22 @@ OaArgs batch_args(const mxco_batch* b)
     return a;
 }
 
-hipError_t dispatch(const mxco_batch* b, int g, int op, const OaArgs& a, const mxco_batch_plan* p, unsigned grid,
-                    hipStream_t s)
+// A probe launch shape of a plan: lanes per key, workgroups per CU,
+// threads per workgroup, and (Find / Contains) the wave-staged kernel.
+struct Shape {
+    int group;
+    unsigned bpc, block;
+    bool staged;
+};
+
+Shape plan_shape(const mxco_batch_plan* p) { return {p->group, p->blocks_per_cu, p->block, p->wave_staged}; }
+
+hipError_t dispatch(const mxco_batch* b, int op, const OaArgs& a, const mxco_batch_plan* p, unsigned grid,
+                    const Shape& sh, hipStream_t s)
 {
-    return launch_batch_any(b->d.key_size, b->d.value_size, g, op, false, a, b->tables_dev, p->items_dev,
-                            p->chunks_dev, p->nchunks, op == OP_INSERT ? p->counters : nullptr, grid, s);
+    return launch_batch_any(b->d.key_size, b->d.value_size, sh.group, op, false, a, b->tables_dev, p->items_dev,
+                            p->chunks_dev, p->nchunks, op == OP_INSERT ? p->counters : nullptr, grid, s, sh.block,
+                            sh.staged);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, uint64_t n)
@@ -214,8 +231,7 @@ hipError_t launch_build_any(int ks, int vs, const BuildArgs& a, uint32_t ntiles,
     return hipErrorInvalidValue;
 }
 
-mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, int g,
-                     unsigned bpc);
+mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, const Shape& sh);
 
 // MXCO_BATCH_BUILD: Clear + Insert (capture-safe: kernels only).
 mxco_status run_build(mxco_batch_plan* p, const void* in, hipStream_t s)
@@ -223,7 +239,7 @@ mxco_status run_build(mxco_batch_plan* p, const void* in, hipStream_t s)
     mxco_batch* b = p->batch;
     if (!p->region) {
         if (mxco_status st = mxco_batch_clear(b, s)) return st;
-        return run_plan(p, in, nullptr, nullptr, s, p->group, p->blocks_per_cu);
+        return run_plan(p, in, nullptr, nullptr, s, plan_shape(p));
     }
     zero_u64_kernel<<<grid_for(2ull * p->nitems, 1), 256, 0, s>>>(p->counters, 2ull * p->nitems);
     zero_u32_kernel<<<grid_for(p->nreg + 1ull, 1), 256, 0, s>>>(p->counts, p->nreg + 1ull);
@@ -313,8 +329,7 @@ hipError_t prepare_build(mxco_batch_plan* p, const mxco_batch_item* items)
     return e;
 }
 
-mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, int g,
-                     unsigned bpc)
+mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2, hipStream_t s, const Shape& sh)
 {
     if (p->op == OP_INSERT) {
         zero_u64_kernel<<<grid_for(2ull * p->nitems, 1), 256, 0, s>>>(p->counters, 2ull * p->nitems);
@@ -324,8 +339,8 @@ mxco_status run_plan(mxco_batch_plan* p, const void* in, void* out, uint8_t* out
     a.in = (const uint8_t*)in;
     a.out = (uint8_t*)out;
     a.out2 = out2;
-    const unsigned grid = std::max(1u, std::min<unsigned>(p->nchunks, (unsigned)cu_count() * bpc));
-    MXCO_HIP(dispatch(p->batch, g, p->op, a, p, grid, s), "batch: launch");
+    const unsigned grid = std::max(1u, std::min<unsigned>(p->nchunks, (unsigned)cu_count() * sh.bpc));
+    MXCO_HIP(dispatch(p->batch, p->op, a, p, grid, sh, s), "batch: launch");
     return MXCO_OK;
 }
 
@@ -489,7 +504,7 @@ MXCO_API mxco_status mxco_batch_run(mxco_batch_plan* p, const void* in, void* ou
         if (mxco_status s = run_build(p, in, stream)) return s;
         return ok();
     }
-    if (mxco_status s = run_plan(p, in, out, out2, stream, p->group, p->blocks_per_cu)) return s;
+    if (mxco_status s = run_plan(p, in, out, out2, stream, plan_shape(p))) return s;
     return ok();
 }
 
@@ -513,8 +528,8 @@ MXCO_API mxco_status mxco_batch_plan_config(mxco_batch_plan* p, uint32_t* group_
     return ok();
 }
 
-// Times the plan under every lane-group width and grid size and keeps the
-// fastest.  The run must be repeatable on the given buffers: a Find or
+// Times the plan under every launch shape (lane-group width, grid size,
+// workgroup size, Find / Contains kernel) and keeps the fastest.  The run must be repeatable on the given buffers: a Find or
 // Contains plan always is; an insert plan is timed on a cleared table
 // arena each time and left cleared.
 MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, void* out, uint8_t* out2,
@@ -543,49 +558,56 @@ MXCO_API mxco_status mxco_batch_plan_tune(mxco_batch_plan* p, const void* in, vo
         if (best_ms) *best_ms = total / 2;
         return ok();
     }
+    // the grid: lanes per key x workgroups per CU x threads per workgroup
+    // (64 / 128 / 256) for the plain probe kernel, and for a Find / Contains
+    // plan also lanes x workgroups for the wave-staged kernel (256 threads)
     static const int groups[] = {1, 2, 4, 8};
     static const unsigned bpcs[] = {8, 32, 64, 128, 256, 512};
+    static const unsigned blocks[] = {64, 128, 256};
+    std::vector<Shape> grid;
+    for (int g : groups)
+        for (unsigned bpc : bpcs) {
+            if (p->op != OP_INSERT) grid.push_back({g, bpc, 256, true});
+            for (unsigned blk : blocks) grid.push_back({g, bpc, blk, false});
+        }
     hipEvent_t e0, e1;
     MXCO_HIP(hipEventCreate(&e0), "tune");
     MXCO_HIP(hipEventCreate(&e1), "tune");
     float best = 1e30f;
-    int bg = p->group;
-    unsigned bb = p->blocks_per_cu;
+    Shape bs = plan_shape(p);
     mxco_status st = MXCO_OK;
-    for (int g : groups) {
-        for (unsigned bpc : bpcs) {
-            float total = 0;
-            for (int rep = 0; rep < 3 && st == MXCO_OK; ++rep) {
-                if (p->op == OP_INSERT) st = mxco_batch_clear(p->batch, stream);
-                if (st != MXCO_OK) break;
-                if (hipEventRecord(e0, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune: event");
-                if (st == MXCO_OK) st = run_plan(p, in, out, out2, stream, g, bpc);
-                if (st == MXCO_OK && hipEventRecord(e1, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
-                if (st == MXCO_OK && hipEventSynchronize(e1) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
-                float ms = 0;
-                if (st == MXCO_OK) (void)hipEventElapsedTime(&ms, e0, e1);
-                if (rep > 0) total += ms;  // first repetition warms up
-            }
+    for (const Shape& sh : grid) {
+        float total = 0;
+        for (int rep = 0; rep < 3 && st == MXCO_OK; ++rep) {
+            if (p->op == OP_INSERT) st = mxco_batch_clear(p->batch, stream);
             if (st != MXCO_OK) break;
-            if (total < best) {
-                best = total;
-                bg = g;
-                bb = bpc;
-            }
+            if (hipEventRecord(e0, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune: event");
+            if (st == MXCO_OK) st = run_plan(p, in, out, out2, stream, sh);
+            if (st == MXCO_OK && hipEventRecord(e1, stream) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
+            if (st == MXCO_OK && hipEventSynchronize(e1) != hipSuccess) st = fail(MXCO_RUNTIME_ERROR, "tune");
+            float ms = 0;
+            if (st == MXCO_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) total += ms;  // first repetition warms up
         }
         if (st != MXCO_OK) break;
+        if (total < best) {
+            best = total;
+            bs = sh;
+        }
     }
     // Leave the tables as one run() of the plan leaves them (an insert plan
     // tuned above ran several times), so a find plan tuned next sees the
     // filled tables.
     if (p->op == OP_INSERT && st == MXCO_OK) st = mxco_batch_clear(p->batch, stream);
-    if (p->op == OP_INSERT && st == MXCO_OK) st = run_plan(p, in, out, out2, stream, bg, bb);
+    if (p->op == OP_INSERT && st == MXCO_OK) st = run_plan(p, in, out, out2, stream, bs);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != MXCO_OK) return st;
     MXCO_HIP(hipStreamSynchronize(stream), "tune");
-    p->group = bg;
-    p->blocks_per_cu = bb;
+    p->group = bs.group;
+    p->blocks_per_cu = bs.bpc;
+    p->block = bs.block;
+    p->wave_staged = bs.staged;
     if (best_ms) *best_ms = best / 2;
     return ok();
 }
@@ -596,6 +618,25 @@ MXCO_API mxco_status mxco_batch_plan_set_config(mxco_batch_plan* p, uint32_t gro
         return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_set_config: bad arguments");
     p->group = (int)group_size;
     p->blocks_per_cu = blocks_per_cu;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_plan_launch_shape(mxco_batch_plan* p, uint32_t* block_size, uint32_t* wave_staged)
+{
+    if (!p) return fail(MXCO_INVALID_ARGUMENT, "mxco_batch_plan_launch_shape: null plan");
+    if (block_size) *block_size = p->block;
+    if (wave_staged) *wave_staged = p->wave_staged ? 1u : 0u;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_batch_plan_set_launch_shape(mxco_batch_plan* p, uint32_t block_size, uint32_t wave_staged)
+{
+    if (!p || (block_size != 64 && block_size != 128 && block_size != 256) || wave_staged > 1 ||
+        (wave_staged && block_size != 256))
+        return fail(MXCO_INVALID_ARGUMENT,
+                    "mxco_batch_plan_set_launch_shape: 64, 128 or 256 threads; the wave-staged kernel takes 256");
+    p->block = block_size;
+    p->wave_staged = wave_staged != 0;
     return ok();
 }
 

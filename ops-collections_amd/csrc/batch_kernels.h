@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void oa_batch_wave_staged_kernel(OaArgs base, 
 // `serve` selects ServeItem items instead of mxco_batch_item.
 hipError_t launch_batch_any(int ks, int vs, int g, int op, bool serve, const OaArgs& a, const BatchTable* tables,
                             const void* items, const BatchChunk* chunks, uint32_t nchunks,
-                            unsigned long long* counters, unsigned grid, hipStream_t s);
+                            unsigned long long* counters, unsigned grid, hipStream_t s, unsigned block = 256,
+                            bool staged = true);
 
 }  // namespace mxco

@@ -17,8 +17,8 @@
 //  - a completion thread retires ticks in order (event wait), stamps each
 //    request's completion time and publishes its result.
 // Per table, requests take effect in arrival order: a tick takes a table's
-// requests while they are inserts followed by reads, so every result equals
-// the result of the same requests run one by one.  New tables are admitted
+// requests while they are one insert followed by reads, so every result
+// equals the result of the same requests run one by one.  New tables are admitted
 // while free HBM stays above the configured reserve.
 #include <string.h>
 
@@ -277,7 +277,9 @@ void dispatcher_loop(mxco_server* s)
                 break;
             }
         if (!t) break;
-        // Choose the tick's requests: per table, inserts followed by reads.
+        // Choose the tick's requests: per table, at most one insert followed
+        // by reads (two inserts in one launch would race on a shared key,
+        // where arrival order says the earlier one wins).
         std::unordered_map<uint32_t, int> phase;  // 1 inserting, 2 reading, 3 blocked
         std::deque<Req*> keep;
         uint64_t keys = 0;
@@ -292,7 +294,7 @@ void dispatcher_loop(mxco_server* s)
             const bool insert = r->r.op == MXCO_REQ_INSERT;
             const bool fits =
                 (keys + r->r.n <= s->cfg.max_tick_keys && t->reqs.size() < kMaxTickRequests) || t->reqs.empty();
-            if (ph == 3 || !fits || (insert && ph == 2)) {
+            if (ph == 3 || !fits || (insert && ph != 0)) {
                 ph = 3;
                 keep.push_back(r);
                 continue;

@@ -956,6 +956,8 @@ _sig("mxco_batch_failures", [vp, vp, pu64])
 _sig("mxco_batch_plan_tune", [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_float)])
 _sig("mxco_batch_plan_config", [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)])
 _sig("mxco_batch_plan_set_config", [vp, u32, u32])
+_sig("mxco_batch_plan_launch_shape", [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)])
+_sig("mxco_batch_plan_set_launch_shape", [vp, u32, u32])
 
 
 class BatchPlan:
@@ -988,12 +990,22 @@ class BatchPlan:
         return ms.value
 
     def config(self):
+        """(lanes per key, workgroups per CU)."""
         g, b = u32(), u32()
         check(_lib.mxco_batch_plan_config(self._h, ctypes.byref(g), ctypes.byref(b)))
         return g.value, b.value
 
     def set_config(self, group_size, blocks_per_cu):
         check(_lib.mxco_batch_plan_set_config(self._h, group_size, blocks_per_cu))
+
+    def launch_shape(self):
+        """(threads per workgroup, wave-staged Find / Contains kernel)."""
+        blk, st = u32(), u32()
+        check(_lib.mxco_batch_plan_launch_shape(self._h, ctypes.byref(blk), ctypes.byref(st)))
+        return blk.value, bool(st.value)
+
+    def set_launch_shape(self, block_size, wave_staged):
+        check(_lib.mxco_batch_plan_set_launch_shape(self._h, block_size, int(wave_staged)))
 
 
 class BatchedStaticMap:

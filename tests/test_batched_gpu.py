@@ -109,6 +109,37 @@ def test_tuner_keeps_results():
     fnd.tune(w["queries"], out)
     g, bpc = fnd.config()
     assert g in (1, 2, 4, 8) and bpc > 0
+    blk, staged = fnd.launch_shape()
+    assert blk in (64, 128, 256) and (blk == 256 or not staged)
+    assert ins.launch_shape()[0] in (64, 128, 256)
     out.fill_(0)
     fnd.run(w["queries"], out)
     assert torch.equal(out, w["expected"])
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_every_launch_shape_gives_the_same_tables(block):
+    """Insert and Find plans under 64-, 128- and 256-thread workgroups (the
+    plain probe kernel, and the wave-staged Find at 256) with each lane-group
+    width: the same failures and Find outputs."""
+    import mxco
+    w = workload(48)
+    T = len(w["caps"])
+    for g in (1, 2, 4, 8):
+        b = mxco.BatchedStaticMap(w["caps"], -1, -1)
+        ins = b.plan(mxco.BATCH_INSERT, [(t, w["ins_off"][t], w["n_ins"][t]) for t in range(T)])
+        fnd = b.plan(mxco.BATCH_FIND, [(t, w["find_off"][t], w["n_find"][t]) for t in range(T)])
+        ins.set_config(g, 4)
+        ins.set_launch_shape(block, False)
+        ins.run(w["pairs"])
+        assert ins.failures() == [0] * T
+        for staged in ((False, True) if block == 256 else (False,)):
+            fnd.set_config(g, 4)
+            fnd.set_launch_shape(block, staged)
+            out = torch.zeros_like(w["queries"])
+            fnd.run(w["queries"], out)
+            assert torch.equal(out, w["expected"]), (g, block, staged)
+    with pytest.raises(ValueError):
+        ins.set_launch_shape(96, False)
+    with pytest.raises(ValueError):
+        fnd.set_launch_shape(128, True)

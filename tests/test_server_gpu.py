@@ -81,9 +81,9 @@ def test_batched_serving_equals_one_by_one(qps):
     s2.close()
 
 
-def test_find_results_against_construction():
-    """Map finds return key*7+1 for present keys and -1 for absent ones."""
-    import mxco
+def test_find_results_against_closed_form():
+    """Map finds return key*7+1 for present keys and -1 for absent ones, the
+    workload's closed form (a query's hit flag is fixed when it is drawn)."""
     from mxco import workloads
     w = workloads.serving(num_maps=3, num_multimaps=1, slots_log2=(12, 13), num_requests=60, qps=100000.0,
                           batch_log2=(0, 9), seed=3)
@@ -93,13 +93,44 @@ def test_find_results_against_construction():
     o = outputs(w)
     s.replay(plan(w, t, o), w["arrivals_us"])
     for req, (a, _) in zip(w["requests"], o):
-        ref = mxco.StaticMap(w["caps"][req["table"]], -1, -1, torch.int64, torch.int64)
-        ref.insert(w["init"][req["table"]])
+        hit = req["hit"]
         if req["op"] == "find":
-            assert torch.equal(a, ref.find(req["input"]))
+            assert torch.equal(a, torch.where(hit, req["input"] * 7 + 1, torch.full_like(a, -1)))
         else:
-            assert torch.equal(a.bool(), ref.contains(req["input"]))
+            assert torch.equal(a.bool(), hit)
     s.close()
+
+
+def test_same_table_inserts_keep_arrival_order():
+    """Insert requests on one table that arrive together, with keys shared
+    between requests and different values: the earliest request's value
+    stays (as one by one), and each request's result equals the plain
+    path's; a find of every key reads the first writer's value."""
+    import mxco
+    rng = np.random.default_rng(17)
+    space, nreq, per = 1500, 24, 300
+    reqs, inputs, expect = [], [], {}
+    for r in range(nreq):
+        k = rng.choice(space, per, replace=False).astype(np.int64)  # distinct within a request
+        v = k * 1000 + r
+        for kk, vv in zip(k.tolist(), v.tolist()):
+            expect.setdefault(kk, vv)
+        inputs.append(torch.from_numpy(np.stack([k, v], 1)).cuda().contiguous())
+    q = torch.arange(space, dtype=torch.int64, device="cuda")
+    results = []
+    for plain in (False, True):
+        s = mxco.Server()
+        t = s.add_map(4096)
+        outs = torch.full((space,), 7, dtype=torch.int64, device="cuda")
+        plan_ = [(t, mxco.REQ_INSERT, x.data_ptr(), per, None, None, 0) for x in inputs]
+        plan_.append((t, mxco.REQ_FIND, q.data_ptr(), space, outs.data_ptr(), None, 0))
+        res, _ = s.replay(plan_, [0.0] * len(plan_), plain=plain)  # all at once
+        results.append((res, outs.cpu().numpy()))
+        s.close()
+    assert results[0][0] == results[1][0]
+    exp = np.array([expect.get(i, -1) for i in range(space)], np.int64)
+    np.testing.assert_array_equal(results[0][1], exp)
+    np.testing.assert_array_equal(results[1][1], exp)
 
 
 def test_admission_by_budget():
