@@ -194,6 +194,8 @@ MXCO_API mxco_status mxco_oa_insert_path(const mxco_oa* table, uint64_t n, hipSt
  * a diagnostic for roofline accounting): *lines = distinct 128-byte lines
  * and *slots = slots the reference's sequential walk reads
  * (open_addressing_ref_impl.h:568-605), summed over the batch.  Synchronous. */
+MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
+                                         uint64_t* lines, uint64_t* slots);
 /* Insert by the CAS walk (never the region build), counting its
  * compare-and-swaps: *cas_made attempted on empty slots and *cas_lost lost to
  * another key, after which the walk goes on at the next slot (no reference
@@ -201,8 +203,22 @@ MXCO_API mxco_status mxco_oa_insert_path(const mxco_oa* table, uint64_t n, hipSt
  * table receives the pairs as Insert does. */
 MXCO_API mxco_status mxco_oa_insert_stats(mxco_oa* table, const void* pairs, uint64_t n, hipStream_t stream,
                                           uint64_t* cas_lost, uint64_t* cas_made);
-MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* table, const void* keys, uint64_t n, hipStream_t stream,
-                                         uint64_t* lines, uint64_t* slots);
+
+/* Caller-compiled probe kernels: the reference default-constructs any
+ * stateless KeyEqual and hasher type on the device
+ * (include/detail/open_addressing/kernels.h:64-66).  A caller's types exist
+ * only in its own translation unit, so the C++ header instantiates the probe
+ * kernels there (include/mxco/detail/caller_probe.h) and registers this
+ * launcher on the table: every later probing operation calls it with the
+ * operation code, the prepared kernel arguments (struct OaArgs of
+ * include/mxco/detail/oa_kernels.h), a grid for kCallerGroup lanes per key,
+ * the block size and the stream, and it returns a hipError_t.  Argument
+ * checks, counters and synchronization stay in the library; the region
+ * build is never taken for such a table.  A null launcher restores the
+ * library's kernels. */
+typedef int (*mxco_oa_probe_launcher)(int op, const void* kernel_args, uint32_t grid, uint32_t block,
+                                      hipStream_t stream);
+MXCO_API mxco_status mxco_oa_set_probe_launcher(mxco_oa* table, mxco_oa_probe_launcher launch);
 
 /* ---------------------------------------------------------------------- */
 /* Batched tables (no single reference counterpart: the batched form of   */

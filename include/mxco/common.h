@@ -6,9 +6,11 @@
 // that code written against them compiles after changing the include and
 // namespace (include/extent.h:20-99, include/hash_functions.h:17-30,
 // include/probing_scheme.h:37-122, include/storage.h:16-20,
-// include/pair.h:41, include/utility/math_utils.h:67).  The template
-// arguments select a configuration of the compiled kernels; they carry no
-// device code of their own.
+// include/pair.h:41, include/utility/math_utils.h:67).  The library's own
+// hashers, EqualTo and the built-in predicates select a configuration of the
+// compiled kernels; a caller's KeyEqual, hasher or predicate type has its
+// kernels compiled in the caller's translation unit instead
+// (detail/caller_probe.h, HIP compiler required).
 #pragma once
 
 #include <cstddef>
@@ -102,20 +104,39 @@ struct murmurhash3_fmix64 {
     constexpr bool operator==(murmurhash3_fmix64 const&) const noexcept { return true; }
 };
 
-// Probing schemes (include/probing_scheme.h:37-122).
+namespace detail {
+// A hasher of the library (a compiled hash family) carries its id.
+template <typename H, typename = void>
+struct HasHashId : std::false_type {};
+template <typename H>
+struct HasHashId<H, std::void_t<decltype(H::id)>> : std::true_type {};
+template <typename H>
+constexpr mxco_hash hash_id_or_default()
+{
+    if constexpr (HasHashId<H>::value) return H::id;
+    else return MXCO_HASH_MURMUR3_32;  // unused: the caller's kernels hash
+}
+}  // namespace detail
+
+// Probing schemes (include/probing_scheme.h:37-122).  Any hasher type works:
+// the library's own select the compiled kernels, a caller's runs in kernels
+// compiled with it.  `compiled` tells which.
 template <typename Hash>
 struct LinearProbing {
     using Hasher = Hash;
     static constexpr mxco_probing id = MXCO_PROBING_LINEAR;
-    static constexpr mxco_hash hash = Hash::id;
+    static constexpr bool compiled = detail::HasHashId<Hash>::value;
+    static constexpr mxco_hash hash = detail::hash_id_or_default<Hash>();
     constexpr bool operator==(LinearProbing const&) const noexcept { return true; }
 };
 
 template <typename Hash1, typename Hash2 = Hash1>
 struct DoubleHashing {
-    static_assert(Hash1::id == Hash2::id, "DoubleHashing: both hashes must be of one family (seeds 0 and 1)");
     static constexpr mxco_probing id = MXCO_PROBING_DOUBLE;
-    static constexpr mxco_hash hash = Hash1::id;
+    // the compiled kernels hash with one family at seeds 0 and 1
+    static constexpr bool compiled = detail::HasHashId<Hash1>::value && detail::HasHashId<Hash2>::value &&
+                                     detail::hash_id_or_default<Hash1>() == detail::hash_id_or_default<Hash2>();
+    static constexpr mxco_hash hash = detail::hash_id_or_default<Hash1>();
     constexpr bool operator==(DoubleHashing const&) const noexcept { return true; }
 };
 
@@ -183,7 +204,9 @@ struct alignas(detail::pair_alignment(sizeof(First), sizeof(Second))) Pair {
 };
 
 // Stencil predicates.  The reference default-constructs a predicate type on
-// the device; the compiled kernels provide these four.
+// the device; the compiled kernels provide these four, and any other
+// predicate type is evaluated by a kernel compiled with it
+// (detail/caller_probe.h).
 struct AlwaysTrue {
     static constexpr mxco_pred id = MXCO_PRED_ALWAYS;
 };
@@ -222,16 +245,60 @@ inline void check(mxco_status s)
 template <typename StencilT, typename Predicate>
 mxco_stencil make_stencil(StencilT const* stencil)
 {
-    static_assert(HasPredId<Predicate>::value,
-                  "Predicate must be one of mxco::AlwaysTrue, IsNonZero, IsOdd, IsEven");
+    static_assert(HasPredId<Predicate>::value, "a built-in predicate: AlwaysTrue, IsNonZero, IsOdd, IsEven");
     static_assert(sizeof(StencilT) == 1 || sizeof(StencilT) == 4 || sizeof(StencilT) == 8,
-                  "stencil elements must be 1, 4 or 8 bytes");
+                  "stencil elements of the built-in predicates are 1, 4 or 8 bytes");
     mxco_stencil s;
     s.data = stencil;
     s.elem_size = sizeof(StencilT);
     s.pred = Predicate::id;
     return s;
 }
+
+// The stencil an operation runs with: a built-in predicate goes to the
+// compiled kernels as it is; the caller's (detail/caller_probe.h) is
+// evaluated first into one byte per input.
+template <typename StencilT, typename Predicate, bool Builtin = HasPredId<Predicate>::value>
+class Selection;
+
+template <typename StencilT, typename Predicate>
+class Selection<StencilT, Predicate, true> {
+public:
+    Selection(StencilT const* stencil, Predicate const&, uint64_t, hipStream_t)
+        : s_(make_stencil<StencilT, Predicate>(stencil))
+    {
+    }
+    const mxco_stencil* get() const noexcept { return &s_; }
+
+private:
+    mxco_stencil s_;
+};
+
+#if !defined(__HIPCC__)
+template <typename StencilT, typename Predicate>
+class Selection<StencilT, Predicate, false> {
+    static_assert(sizeof(Predicate) == 0,
+                  "a caller-defined predicate runs on the device: compile this translation unit with hipcc");
+};
+#endif
+
+// The reference rejects a KeyEqual or probing scheme that differs from its
+// default-constructed value (open_addressing_impl.h:527-532), where the
+// type compares.
+template <typename T>
+auto equals_default(T const& x, int) -> decltype(static_cast<bool>(x == T{}))
+{
+    return static_cast<bool>(x == T{});
+}
+template <typename T>
+bool equals_default(T const&, long)
+{
+    return true;
+}
+
+// Whether a (KeyEqual, ProbingScheme) pair runs on the compiled kernels.
+template <typename Key, typename KeyEqual, typename ProbingScheme>
+inline constexpr bool kCompiledProbe = std::is_same_v<KeyEqual, EqualTo<Key>> && ProbingScheme::compiled;
 
 }  // namespace detail
 }  // namespace mxco

@@ -254,6 +254,41 @@ __device__ __forceinline__ bool stencil_ok(const OaArgs& a, uint64_t i)
     }
 }
 
+// The probe hashes, key equality and sentinel test of a walk.  The compiled
+// containers take them from OaArgs at run time (hash family, float keys);
+// a header-only instantiation supplies the caller's functor types instead
+// (include/mxco/detail/caller_probe.h).  Every hash is reduced to the
+// 32-bit probe size type, as SanitizeHash's cast does
+// (include/utility/math_utils.h:19-29, bucket_storage_ref.h:21-29).
+struct BuiltinFns {
+    template <int KS>
+    static __device__ __forceinline__ uint32_t hash1(const OaArgs& a, uint64_t key)
+    {
+        return probe_hash<KS>(a.hash, key, 0u);
+    }
+    template <int KS>
+    static __device__ __forceinline__ uint32_t hash2(const OaArgs& a, uint64_t key)
+    {
+        return probe_hash<KS>(a.hash, key, 1u);
+    }
+    // KeyEqual(probe key, slot key)
+    template <int KS>
+    static __device__ __forceinline__ bool equal(const OaArgs& a, uint64_t slot_key, uint64_t key)
+    {
+        return key_eq<KS>(slot_key, key, a.key_float);
+    }
+    // EqualWrapper::IsSentinelValue: bitwise for integers and floating point
+    template <int KS>
+    static __device__ __forceinline__ bool sentinel(uint64_t key, uint64_t sentinel_key)
+    {
+        return key == sentinel_key;
+    }
+};
+
+// Lanes per key of the kernels a caller instantiates with its own functor
+// types (the library's grid for them assumes this width).
+constexpr int kCallerGroup = 4;
+
 // The probe walk of one key.  All fields are uniform across the key's group.
 struct Walk {
     uint32_t pos;     // next slot in walk order
@@ -264,11 +299,11 @@ struct Walk {
     bool done;
 };
 
-template <int KS>
+template <int KS, class Fns = BuiltinFns>
 __device__ __forceinline__ Walk make_walk(const OaArgs& a, uint64_t key)
 {
     Walk w;
-    const uint32_t h1 = probe_hash<KS>(a.hash, key, 0u);
+    const uint32_t h1 = Fns::template hash1<KS>(a, key);
     const uint32_t start = (h1 % a.nbuckets) * a.bucket;
     w.pos = start;
     w.bucket = start;
@@ -276,7 +311,7 @@ __device__ __forceinline__ Walk make_walk(const OaArgs& a, uint64_t key)
     w.left = a.cap;
     w.done = false;
     if (a.probing) {
-        const uint32_t h2 = probe_hash<KS>(a.hash, key, 1u);
+        const uint32_t h2 = Fns::template hash2<KS>(a, key);
         w.step = (h2 % (a.nbuckets - 1u) + 1u) * a.bucket;
     } else {
         w.step = a.bucket;
@@ -339,7 +374,7 @@ __device__ __forceinline__ void store_payload(uint8_t* p, uint64_t v, uint32_t m
 // Probes keys first, first + stride, ... < end of one table for the calling
 // lane group and accumulates the per-lane counters.  Every lane group of the
 // wavefront calls it with its own `first`; `end` and `stride` are uniform.
-template <int KS, int VS, int G, int OP, int U = 1>
+template <int KS, int VS, int G, int OP, int U = 1, class Fns = BuiltinFns>
 __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint64_t end, uint64_t stride,
                                            unsigned long long& cnt0, unsigned long long& cnt1)
 {
@@ -416,7 +451,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
         }
         if constexpr (kInsertFamily) {
             // The reserved empty key is never inserted (open_addressing_ref_impl.h:494-496).
-            if (key == ekey) {
+            if (Fns::template sentinel<KS>(key, ekey)) {
                 if (leader) {
                     if constexpr (OP == OP_INSERT || OP == OP_INSERT_OR_ASSIGN) cnt0++;
                     if constexpr (OP == OP_INSERT_AND_FIND) {
@@ -427,7 +462,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                 return false;
             }
         }
-        W[u] = make_walk<KS>(a, key);
+        W[u] = make_walk<KS, Fns>(a, key);
         PREV[u] = ~0ull;
         return true;
     };
@@ -499,8 +534,8 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                 const uint32_t off = gl + k * G;
                 if (off < wlen) {
                     const uint64_t sk = L::key(s[k]);
-                    const bool em = sk == ekey;
-                    const bool eq = !em && key_eq<KS>(sk, key, a.key_float);
+                    const bool em = Fns::template sentinel<KS>(sk, ekey);
+                    const bool eq = !em && Fns::template equal<KS>(a, sk, key);
                     bool ev;
                     if constexpr (OP == OP_ERASE) ev = eq || (em && a.has_tombstone);
                     else ev = eq || em;
@@ -621,7 +656,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                                 const uint64_t old = cas_bits<SS>(wslot, expect, desired);
                                 if (old == expect) {
                                     r = 1;
-                                } else if (key_eq<KS>(L::key((Raw)old), key, a.key_float)) {
+                                } else if (Fns::template equal<KS>(a, L::key((Raw)old), key)) {
                                     r = 2;
                                     seen = VS ? L::val((Raw)old) : L::key((Raw)old);
                                     if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<SS>(wslot, desired);
@@ -634,7 +669,7 @@ __device__ __forceinline__ void oa_process(const OaArgs& a, uint64_t first, uint
                                     if constexpr (OP == OP_INSERT_AND_FIND) publish_bits<VS>(wslot + VOFF, pval);
                                     else store_payload<VS>(wslot + VOFF, pval, a.store_mode);
                                     r = 1;
-                                } else if (key_eq<KS>(old, key, a.key_float)) {
+                                } else if (Fns::template equal<KS>(a, old, key)) {
                                     r = 2;
                                     if constexpr (OP == OP_INSERT_OR_ASSIGN) store_bits<VS>(wslot + VOFF, pval);
                                 }
@@ -736,13 +771,13 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
     }
 }
 
-template <int KS, int VS, int G, int OP, int U = 1>
+template <int KS, int VS, int G, int OP, int U = 1, class Fns = BuiltinFns>
 __global__ __launch_bounds__(256) void oa_kernel(OaArgs a)
 {
     unsigned long long cnt0 = 0, cnt1 = 0;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
-    oa_process<KS, VS, G, OP, U>(a, gid, a.n, ngroups, cnt0, cnt1);
+    oa_process<KS, VS, G, OP, U, Fns>(a, gid, a.n, ngroups, cnt0, cnt1);
     if (a.counters) flush_counters<OP>(counter_slot(a.counters), cnt0, cnt1);
 }
 

@@ -9,7 +9,9 @@
 // caller's device code, so it is compiled into the caller's translation unit:
 // this header needs the HIP compiler.  CallbackOp provides a device
 // constructor taking the callbackArgs device pointer (uint8_t*) and a device
-// operator() taking the slot (Pair<Key, T> for a map, Key for a set).
+// operator() taking the slot (Pair<Key, T> for a map, Key for a set).  The
+// walk hashes and compares as the container's probe kernels do (the
+// library's hash families and EqualTo, or the caller's types).
 #pragma once
 
 #if !defined(__HIPCC__)
@@ -22,7 +24,7 @@
 #include <stdexcept>
 
 #include "common.h"
-#include "detail/hash.h"
+#include "detail/oa_kernels.h"
 
 namespace mxco {
 namespace detail {
@@ -30,15 +32,9 @@ namespace detail {
 template <class Key>
 __device__ __forceinline__ uint64_t key_bits(Key k)
 {
-    if constexpr (sizeof(Key) == 4) {
-        uint32_t b;
-        __builtin_memcpy(&b, &k, 4);
-        return b;
-    } else {
-        uint64_t b;
-        __builtin_memcpy(&b, &k, 8);
-        return b;
-    }
+    uint64_t b = 0;
+    __builtin_memcpy(&b, &k, sizeof(Key));
+    return b;
 }
 
 template <class Slot>
@@ -48,31 +44,31 @@ __device__ __forceinline__ auto slot_key(const Slot& s)
     else return s.first;
 }
 
-template <class Key, class Slot, class CallbackOp>
-__global__ __launch_bounds__(256) void for_each_kernel(const Slot* slots, uint32_t cap, uint32_t bucket,
-                                                       uint32_t probing, uint32_t hash, const Key* keys, uint32_t n,
-                                                       Key empty, uint8_t* args)
+// `a` carries the table shape, the probing, the hash family and the empty key.
+template <class Key, class Slot, class CallbackOp, class Fns>
+__global__ __launch_bounds__(256) void for_each_kernel(const Slot* slots, OaArgs a, const Key* keys, uint32_t n,
+                                                       uint8_t* args)
 {
+    constexpr int KS = (int)sizeof(Key);
     CallbackOp op(args);
-    const uint32_t nb = cap / bucket;
-    const uint64_t ebits = key_bits(empty);
+    const uint32_t cap = a.cap, bucket = a.bucket, nb = a.nbuckets;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const Key key = keys[i];
         const uint64_t kb = key_bits(key);
         // LinearProbing / DoubleHashing::MakeIterator (probing_scheme_impl.inl:118-128, :177-189)
-        uint32_t idx = (probe_hash<sizeof(Key)>(hash, kb, 0u) % nb) * bucket;
-        const uint32_t step = probing ? (probe_hash<sizeof(Key)>(hash, kb, 1u) % (nb - 1u) + 1u) * bucket : bucket;
+        uint32_t idx = (Fns::template hash1<KS>(a, kb) % nb) * bucket;
+        const uint32_t step = a.probing ? (Fns::template hash2<KS>(a, kb) % (nb - 1u) + 1u) * bucket : bucket;
         const uint32_t init = idx;
         bool done = false;
         while (!done) {
             for (uint32_t b = 0; b < bucket; ++b) {
                 const Slot s = slots[idx + b];
-                const Key sk = slot_key(s);
-                if (key_bits(sk) == ebits) {  // EMPTY (bitwise sentinel): the walk ends
+                const uint64_t sk = key_bits(slot_key(s));
+                if (Fns::template sentinel<KS>(sk, a.empty_key)) {  // EMPTY: the walk ends
                     done = true;
                     break;
                 }
-                if (sk == key) op(s);  // EQUAL (KeyEqual is ==)
+                if (Fns::template equal<KS>(a, sk, kb)) op(s);  // EQUAL
             }
             if (done) break;
             // ProbingIterator::operator++ (probing_scheme_impl.inl:76-84)
@@ -82,7 +78,7 @@ __global__ __launch_bounds__(256) void for_each_kernel(const Slot* slots, uint32
     }
 }
 
-template <class Key, class Slot, class CallbackOp>
+template <class Key, class Slot, class CallbackOp, class Fns = BuiltinFns>
 void for_each_launch(const Slot* slots, uint64_t cap, uint32_t bucket, uint32_t probing, uint32_t hash,
                      const void* keys, uint64_t n, Key empty, void* args, hipStream_t stream)
 {
@@ -92,10 +88,19 @@ void for_each_launch(const Slot* slots, uint64_t cap, uint32_t bucket, uint32_t 
     // CheckedBatchSize (open_addressing_impl.h:534-545)
     const uint64_t limit = 0xffffffffull - (uint64_t)(cus > 0 ? cus : 1) * 1024ull;
     if (n > limit) throw std::length_error("ForEach: batch larger than the kernel index range");
+    OaArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.cap = (uint32_t)cap;
+    a.bucket = bucket;
+    a.nbuckets = (uint32_t)(cap / bucket);
+    a.probing = probing;
+    a.hash = hash;
+    a.key_float = IsFloatKey<Key>::value ? 1u : 0u;
+    std::memcpy(&a.empty_key, &empty, sizeof(Key));
     const uint64_t blocks = (n + 255) / 256;
     const unsigned grid = (unsigned)(blocks < (uint64_t)cus * 8 ? blocks : (uint64_t)cus * 8);
-    for_each_kernel<Key, Slot, CallbackOp><<<grid ? grid : 1, 256, 0, stream>>>(
-        slots, (uint32_t)cap, bucket, probing, hash, (const Key*)keys, (uint32_t)n, empty, (uint8_t*)args);
+    for_each_kernel<Key, Slot, CallbackOp, Fns><<<grid ? grid : 1, 256, 0, stream>>>(slots, a, (const Key*)keys,
+                                                                                     (uint32_t)n, (uint8_t*)args);
     if (hipGetLastError() != hipSuccess) throw std::runtime_error("ForEach: kernel launch failed");
 }
 

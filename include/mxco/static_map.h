@@ -3,7 +3,9 @@
 // Same template parameters, same member functions and argument order; the
 // stream argument is a hipStream_t.  Every call forwards to the C ABI of
 // libmxco.so (include/mxco.h) and rethrows its status as the reference's
-// exception type.
+// exception type.  A caller-defined KeyEqual, hasher or predicate type runs
+// in probe kernels compiled in the caller's translation unit
+// (detail/caller_probe.h), which the library calls; that needs hipcc.
 #pragma once
 
 #include <memory>
@@ -11,6 +13,7 @@
 
 #include "common.h"
 #if defined(__HIPCC__)
+#include "detail/caller_probe.h"
 #include "for_each.h"
 #endif
 
@@ -26,7 +29,11 @@ class StaticMap {
                   "keys must be 1, 2, 4 or 8 bytes");
     static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8,
                   "payloads must be 1, 2, 4 or 8 bytes");
-    static_assert(std::is_same_v<KeyEqual, mxco::EqualTo<Key>>, "KeyEqual must be mxco::EqualTo<Key>");
+    static constexpr bool kCompiledProbe = detail::kCompiledProbe<Key, KeyEqual, ProbingScheme>;
+#if !defined(__HIPCC__)
+    static_assert(kCompiledProbe,
+                  "a caller-defined KeyEqual or hasher runs in kernels compiled with it: use hipcc for this unit");
+#endif
 
 public:
     static constexpr auto bucketSize = Storage::bucketSize;
@@ -41,10 +48,12 @@ public:
     StaticMap& operator=(StaticMap&&) = default;
     ~StaticMap() = default;
 
-    StaticMap(Extent capacity, Key emptyKey, T emptyValue, KeyEqual const& = {}, ProbingScheme const& = {},
-              Storage = {}, hipStream_t stream = nullptr)
+    StaticMap(Extent capacity, Key emptyKey, T emptyValue, KeyEqual const& pred = {},
+              ProbingScheme const& probing = {}, Storage = {}, hipStream_t stream = nullptr)
         : emptyKey_{emptyKey}, emptyValue_{emptyValue}
     {
+        if (!detail::equals_default(pred, 0) || !detail::equals_default(probing, 0))
+            throw std::invalid_argument("StaticMap: stateful KeyEqual or probing scheme");
         mxco_oa_desc d{};
         d.key_size = sizeof(Key);
         d.value_size = sizeof(T);
@@ -56,6 +65,11 @@ public:
         mxco_oa* h = nullptr;
         detail::check(mxco_oa_create(&d, static_cast<SizeType>(capacity), &emptyKey, &emptyValue, stream, &h));
         impl_.reset(h);
+#if defined(__HIPCC__)
+        if constexpr (!kCompiledProbe)
+            detail::check(mxco_oa_set_probe_launcher(
+                h, &detail::caller_launch<Key, T, typename detail::CallerFnsOf<Key, KeyEqual, ProbingScheme>::type>));
+#endif
     }
 
     void SetErasedKey(Key erasedKey, hipStream_t stream) { detail::check(mxco_oa_set_erased_key(get(), &erasedKey, stream)); }
@@ -90,24 +104,24 @@ public:
     template <typename StencilT, typename Predicate>
     void AssignIfAsync(void* values, StencilT* stencil, Extent valueNum, hipStream_t stream)
     {
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_assign_if(get(), values, &s, static_cast<SizeType>(valueNum), stream, 0));
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(valueNum), stream);
+        detail::check(mxco_oa_assign_if(get(), values, s.get(), static_cast<SizeType>(valueNum), stream, 0));
     }
     template <typename StencilT, typename Predicate>
     SizeType InsertIf(void* values, StencilT* stencil, Extent valueNum, hipStream_t stream)
     {
         if (static_cast<SizeType>(valueNum) != 0 && stencil == nullptr) return static_cast<SizeType>(valueNum);
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(valueNum), stream);
         uint64_t failed = 0;
-        detail::check(mxco_oa_insert(get(), values, &s, static_cast<SizeType>(valueNum), stream, 1, &failed));
+        detail::check(mxco_oa_insert(get(), values, s.get(), static_cast<SizeType>(valueNum), stream, 1, &failed));
         return static_cast<SizeType>(failed);
     }
     template <typename StencilT, typename Predicate>
     void InsertIfAsync(void* values, StencilT* stencil, Extent valueNum, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_insert(get(), values, &s, static_cast<SizeType>(valueNum), stream, 0, nullptr));
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(valueNum), stream);
+        detail::check(mxco_oa_insert(get(), values, s.get(), static_cast<SizeType>(valueNum), stream, 0, nullptr));
     }
     void InsertAndFind(void* values, void* outputFind, void* outputInsert, Extent valueNum, hipStream_t stream)
     {
@@ -143,15 +157,15 @@ public:
     void FindIf(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_find(get(), keys, &s, outputValues, static_cast<SizeType>(keyNum), stream, 1));
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(keyNum), stream);
+        detail::check(mxco_oa_find(get(), keys, s.get(), outputValues, static_cast<SizeType>(keyNum), stream, 1));
     }
     template <typename StencilT, typename Predicate>
     void FindIfAsync(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_find(get(), keys, &s, outputValues, static_cast<SizeType>(keyNum), stream, 0));
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(keyNum), stream);
+        detail::check(mxco_oa_find(get(), keys, s.get(), outputValues, static_cast<SizeType>(keyNum), stream, 0));
     }
     void Contains(void* keys, void* outputValues, Extent keyNum, hipStream_t stream)
     {
@@ -167,16 +181,16 @@ public:
     void ContainsIf(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_contains(get(), keys, &s, static_cast<uint8_t*>(outputValues),
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(keyNum), stream);
+        detail::check(mxco_oa_contains(get(), keys, s.get(), static_cast<uint8_t*>(outputValues),
                                        static_cast<SizeType>(keyNum), stream, 1));
     }
     template <typename StencilT, typename Predicate>
     void ContainsIfAsync(void* keys, StencilT* stencil, void* outputValues, Extent keyNum, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_contains(get(), keys, &s, static_cast<uint8_t*>(outputValues),
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(keyNum), stream);
+        detail::check(mxco_oa_contains(get(), keys, s.get(), static_cast<uint8_t*>(outputValues),
                                        static_cast<SizeType>(keyNum), stream, 0));
     }
     SizeType Count(void* keys, Extent keyNum, hipStream_t stream)
@@ -205,9 +219,12 @@ public:
     template <typename CallbackOp>
     void ForEachAsync(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)
     {
-        detail::for_each_launch<Key, ValueType, CallbackOp>(Data(), Capacity(), static_cast<uint32_t>(bucketSize),
-                                                     ProbingScheme::id, ProbingScheme::hash, keys,
-                                                     static_cast<SizeType>(keyNum), emptyKey_, callbackArgs, stream);
+        using Fns = std::conditional_t<kCompiledProbe, mxco::BuiltinFns,
+                                       typename detail::CallerFnsOf<Key, KeyEqual, ProbingScheme>::type>;
+        detail::for_each_launch<Key, ValueType, CallbackOp, Fns>(Data(), Capacity(), static_cast<uint32_t>(bucketSize),
+                                                               ProbingScheme::id, ProbingScheme::hash, keys,
+                                                               static_cast<SizeType>(keyNum), emptyKey_, callbackArgs,
+                                                               stream);
     }
     template <typename CallbackOp>
     void ForEach(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)

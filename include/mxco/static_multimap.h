@@ -13,6 +13,9 @@
 #include <utility>
 
 #include "common.h"
+#if defined(__HIPCC__)
+#include "detail/caller_probe.h"
+#endif
 
 namespace mxco {
 
@@ -69,9 +72,9 @@ public:
     template <class Stencil, class Predicate>
     SizeType InsertIf(void* pairs, Stencil* stencil, ExtentType n, hipStream_t stream = nullptr)
     {
-        mxco_stencil s = detail::make_stencil<Stencil, Predicate>(stencil);
+        detail::Selection<Stencil, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
         uint64_t failed = 0;
-        detail::check(mxco_mm_insert(get(), pairs, &s, static_cast<SizeType>(n), stream, &failed));
+        detail::check(mxco_mm_insert(get(), pairs, s.get(), static_cast<SizeType>(n), stream, &failed));
         return failed;
     }
 
@@ -84,9 +87,9 @@ public:
     template <class Stencil, class Predicate>
     void ContainsIf(void* keys, Stencil* stencil, void* output, ExtentType n, hipStream_t stream = nullptr) const
     {
-        mxco_stencil s = detail::make_stencil<Stencil, Predicate>(stencil);
+        detail::Selection<Stencil, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
         detail::check(
-            mxco_mm_contains(get(), keys, &s, static_cast<uint8_t*>(output), static_cast<SizeType>(n), stream));
+            mxco_mm_contains(get(), keys, s.get(), static_cast<uint8_t*>(output), static_cast<SizeType>(n), stream));
     }
 
     // Smallest value stored for each key, or the empty value.
@@ -98,8 +101,8 @@ public:
     template <class Stencil, class Predicate>
     void FindIf(void* keys, Stencil* stencil, void* output, ExtentType n, hipStream_t stream = nullptr) const
     {
-        mxco_stencil s = detail::make_stencil<Stencil, Predicate>(stencil);
-        detail::check(mxco_mm_find(get(), keys, &s, output, static_cast<SizeType>(n), stream));
+        detail::Selection<Stencil, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
+        detail::check(mxco_mm_find(get(), keys, s.get(), output, static_cast<SizeType>(n), stream));
     }
 
     uint64_t Count(void* keys, ExtentType count, hipStream_t stream = nullptr) const

@@ -8,13 +8,21 @@
 // pointer for a nonempty operation), std::length_error (Retrieve /
 // RetrieveAll outputs too small, nothing written), std::overflow_error
 // (Count / Retrieve when n * Size() may exceed 64 bits), std::logic_error
-// (moved-from object).  Predicates are the built-in stencil predicates.
+// (moved-from object).  Predicates are the built-in stencil predicates or,
+// compiled with hipcc, any predicate object (stateless or not, as the
+// reference's InsertIf / ContainsIf / FindIf take it by value).  CountEach /
+// CountEachOuter with an explicit equality and hash require them to agree
+// with exact key equality and the container's hasher, as the reference's
+// do; they give the plain counts.
 #pragma once
 
 #include <memory>
 #include <utility>
 
 #include "common.h"
+#if defined(__HIPCC__)
+#include "detail/caller_probe.h"
+#endif
 
 namespace mxco {
 
@@ -26,6 +34,8 @@ class StaticMultiset {
 public:
     using KeyType = Key;
     using SizeType = uint64_t;
+    // the probe hash of the slots (include/static_multiset.h:45)
+    using Hasher = std::conditional_t<sizeof(Key) == 4, murmurhash3_fmix32<Key>, murmurhash3_fmix64<Key>>;
     using Extent = mxco::Extent<std::size_t>;
 
     StaticMultiset(Extent capacity, Key emptyKey, hipStream_t stream = nullptr)
@@ -66,13 +76,13 @@ public:
         return InsertIf(keys, n, stencil, Predicate{}, stream);
     }
     template <typename Stencil, typename Predicate>
-    SizeType InsertIf(void* keys, Extent n, Stencil* stencil, Predicate, hipStream_t stream = nullptr)
+    SizeType InsertIf(void* keys, Extent n, Stencil* stencil, Predicate pred, hipStream_t stream = nullptr)
     {
         uint64_t failed = 0;
         if (static_cast<std::size_t>(n) && keys && !stencil)
             throw std::invalid_argument("StaticMultiset: null pointer for nonempty operation");
-        mxco_stencil s = detail::make_stencil<Stencil, Predicate>(stencil);
-        detail::check(mxco_ms_insert(get(), keys, &s, static_cast<std::size_t>(n), stream, &failed));
+        detail::Selection<Stencil, Predicate> s(stencil, pred, keys ? static_cast<std::size_t>(n) : 0, stream);
+        detail::check(mxco_ms_insert(get(), keys, s.get(), static_cast<std::size_t>(n), stream, &failed));
         return failed;
     }
 
@@ -92,10 +102,11 @@ public:
         ContainsIf(keys, n, stencil, Predicate{}, output, stream);
     }
     template <typename Stencil, typename Predicate>
-    void ContainsIf(void* keys, Extent n, Stencil* stencil, Predicate, void* output, hipStream_t stream = nullptr)
+    void ContainsIf(void* keys, Extent n, Stencil* stencil, Predicate pred, void* output,
+                    hipStream_t stream = nullptr)
     {
-        mxco_stencil s = conditional<Stencil, Predicate>(stencil, n);
-        query(MXCO_MS_CONTAINS, keys, &s, output, n, stream);
+        detail::Selection<Stencil, Predicate> s = conditional(stencil, pred, n, stream);
+        query(MXCO_MS_CONTAINS, keys, s.get(), output, n, stream);
     }
     template <typename Stencil, typename Predicate>
     void FindIf(void* keys, Stencil* stencil, void* output, Extent n, hipStream_t stream = nullptr)
@@ -103,10 +114,10 @@ public:
         FindIf(keys, n, stencil, Predicate{}, output, stream);
     }
     template <typename Stencil, typename Predicate>
-    void FindIf(void* keys, Extent n, Stencil* stencil, Predicate, void* output, hipStream_t stream = nullptr)
+    void FindIf(void* keys, Extent n, Stencil* stencil, Predicate pred, void* output, hipStream_t stream = nullptr)
     {
-        mxco_stencil s = conditional<Stencil, Predicate>(stencil, n);
-        query(MXCO_MS_FIND, keys, &s, output, n, stream);
+        detail::Selection<Stencil, Predicate> s = conditional(stencil, pred, n, stream);
+        query(MXCO_MS_FIND, keys, s.get(), output, n, stream);
     }
     // uint64_t multiplicity per query (CountEachOuter writes 1 for absent keys).
     void CountEach(void* keys, void* output, Extent n, hipStream_t stream = nullptr)
@@ -119,6 +130,18 @@ public:
         query(MXCO_MS_COUNT_EACH_OUTER, keys, nullptr, output, n, stream);
     }
     void CountEachOuter(void* keys, Extent n, void* output, hipStream_t stream = nullptr)
+    {
+        CountEachOuter(keys, output, n, stream);
+    }
+    // With an equality and probe hash compatible with the construction policy
+    // (include/static_multiset.h:264-288): the same counts.
+    template <typename Equal, typename Hash>
+    void CountEach(void* keys, Extent n, Equal, Hash, void* output, hipStream_t stream = nullptr)
+    {
+        CountEach(keys, output, n, stream);
+    }
+    template <typename Equal, typename Hash>
+    void CountEachOuter(void* keys, Extent n, Equal, Hash, void* output, hipStream_t stream = nullptr)
     {
         CountEachOuter(keys, output, n, stream);
     }
@@ -154,11 +177,12 @@ private:
         return h_.get();
     }
     template <typename Stencil, typename Predicate>
-    static mxco_stencil conditional(Stencil* stencil, Extent n)
+    static detail::Selection<Stencil, Predicate> conditional(Stencil* stencil, Predicate const& pred, Extent n,
+                                                             hipStream_t stream)
     {
         if (static_cast<std::size_t>(n) && !stencil)
             throw std::invalid_argument("StaticMultiset: null pointer for nonempty operation");
-        return detail::make_stencil<Stencil, Predicate>(stencil);
+        return detail::Selection<Stencil, Predicate>(stencil, pred, static_cast<std::size_t>(n), stream);
     }
     void query(int kind, void* keys, const mxco_stencil* s, void* output, Extent n, hipStream_t stream)
     {

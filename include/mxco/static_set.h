@@ -1,12 +1,15 @@
 // mxco::StaticSet -- drop-in for aclco::StaticSet (include/static_set.h:27-525).
 // Defaults: DoubleHashing<xxhash_32<Key>>, Storage<5>.  Find returns the
-// stored key or the empty key.
+// stored key or the empty key.  A caller-defined KeyEqual, hasher or
+// predicate type runs in kernels compiled in the caller's translation unit
+// (detail/caller_probe.h, hipcc).
 #pragma once
 
 #include <memory>
 
 #include "common.h"
 #if defined(__HIPCC__)
+#include "detail/caller_probe.h"
 #include "for_each.h"
 #endif
 
@@ -20,7 +23,11 @@ template <class Key, class Extent = mxco::Extent<std::size_t>, class KeyEqual = 
 class StaticSet {
     static_assert(sizeof(Key) == 1 || sizeof(Key) == 2 || sizeof(Key) == 4 || sizeof(Key) == 8,
                   "keys must be 1, 2, 4 or 8 bytes");
-    static_assert(std::is_same_v<KeyEqual, mxco::EqualTo<Key>>, "KeyEqual must be mxco::EqualTo<Key>");
+    static constexpr bool kCompiledProbe = detail::kCompiledProbe<Key, KeyEqual, ProbingScheme>;
+#if !defined(__HIPCC__)
+    static_assert(kCompiledProbe,
+                  "a caller-defined KeyEqual or hasher runs in kernels compiled with it: use hipcc for this unit");
+#endif
 
 public:
     static constexpr auto bucketSize = Storage::bucketSize;
@@ -33,10 +40,12 @@ public:
     StaticSet(StaticSet&&) = default;
     StaticSet& operator=(StaticSet&&) = default;
 
-    StaticSet(Extent capacity, Key emptyKey, KeyEqual const& = {}, ProbingScheme const& = {}, Storage = {},
-              hipStream_t stream = nullptr)
+    StaticSet(Extent capacity, Key emptyKey, KeyEqual const& pred = {}, ProbingScheme const& probing = {},
+              Storage = {}, hipStream_t stream = nullptr)
         : emptyKey_{emptyKey}
     {
+        if (!detail::equals_default(pred, 0) || !detail::equals_default(probing, 0))
+            throw std::invalid_argument("StaticSet: stateful KeyEqual or probing scheme");
         mxco_oa_desc d{};
         d.key_size = sizeof(Key);
         d.value_size = 0;
@@ -47,6 +56,11 @@ public:
         mxco_oa* h = nullptr;
         detail::check(mxco_oa_create(&d, static_cast<SizeType>(capacity), &emptyKey, nullptr, stream, &h));
         impl_.reset(h);
+#if defined(__HIPCC__)
+        if constexpr (!kCompiledProbe)
+            detail::check(mxco_oa_set_probe_launcher(
+                h, &detail::caller_launch<Key, void, typename detail::CallerFnsOf<Key, KeyEqual, ProbingScheme>::type>));
+#endif
     }
 
     void SetErasedKey(Key erasedKey, hipStream_t stream) { detail::check(mxco_oa_set_erased_key(get(), &erasedKey, stream)); }
@@ -64,17 +78,17 @@ public:
     SizeType InsertIf(void* keys, StencilT* stencil, Extent n, hipStream_t stream)
     {
         if (static_cast<SizeType>(n) != 0 && stencil == nullptr) return static_cast<SizeType>(n);
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
         uint64_t f = 0;
-        detail::check(mxco_oa_insert(get(), keys, &s, static_cast<SizeType>(n), stream, 1, &f));
+        detail::check(mxco_oa_insert(get(), keys, s.get(), static_cast<SizeType>(n), stream, 1, &f));
         return static_cast<SizeType>(f);
     }
     template <typename StencilT, typename Predicate>
     void InsertIfAsync(void* keys, StencilT* stencil, Extent n, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_insert(get(), keys, &s, static_cast<SizeType>(n), stream, 0, nullptr));
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
+        detail::check(mxco_oa_insert(get(), keys, s.get(), static_cast<SizeType>(n), stream, 0, nullptr));
     }
     void InsertAndFind(void* keys, void* outputFind, void* outputInsert, Extent n, hipStream_t stream)
     {
@@ -110,8 +124,8 @@ public:
     void FindIf(void* keys, StencilT* stencil, void* out, Extent n, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
-        detail::check(mxco_oa_find(get(), keys, &s, out, static_cast<SizeType>(n), stream, 1));
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
+        detail::check(mxco_oa_find(get(), keys, s.get(), out, static_cast<SizeType>(n), stream, 1));
     }
     void Contains(void* keys, void* out, Extent n, hipStream_t stream)
     {
@@ -127,9 +141,9 @@ public:
     void ContainsIf(void* keys, StencilT* stencil, void* out, Extent n, hipStream_t stream)
     {
         if (stencil == nullptr) return;
-        mxco_stencil s = detail::make_stencil<StencilT, Predicate>(stencil);
+        detail::Selection<StencilT, Predicate> s(stencil, Predicate{}, static_cast<SizeType>(n), stream);
         detail::check(
-            mxco_oa_contains(get(), keys, &s, static_cast<uint8_t*>(out), static_cast<SizeType>(n), stream, 1));
+            mxco_oa_contains(get(), keys, s.get(), static_cast<uint8_t*>(out), static_cast<SizeType>(n), stream, 1));
     }
     SizeType Count(void* keys, Extent n, hipStream_t stream)
     {
@@ -156,9 +170,12 @@ public:
     template <typename CallbackOp>
     void ForEachAsync(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)
     {
-        detail::for_each_launch<Key, Key, CallbackOp>(Data(), Capacity(), static_cast<uint32_t>(bucketSize),
-                                                     ProbingScheme::id, ProbingScheme::hash, keys,
-                                                     static_cast<SizeType>(keyNum), emptyKey_, callbackArgs, stream);
+        using Fns = std::conditional_t<kCompiledProbe, mxco::BuiltinFns,
+                                       typename detail::CallerFnsOf<Key, KeyEqual, ProbingScheme>::type>;
+        detail::for_each_launch<Key, Key, CallbackOp, Fns>(Data(), Capacity(), static_cast<uint32_t>(bucketSize),
+                                                          ProbingScheme::id, ProbingScheme::hash, keys,
+                                                          static_cast<SizeType>(keyNum), emptyKey_, callbackArgs,
+                                                          stream);
     }
     template <typename CallbackOp>
     void ForEach(void* keys, Extent keyNum, void* callbackArgs, hipStream_t stream)

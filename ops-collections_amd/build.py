@@ -50,10 +50,10 @@ def build(jobs=None, verbose=False):
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
     # C++ header-API check programs (run by tests/test_header_api_gpu.py): the
-    # host-only header use, then ForEach and RoaringBitmapRef, which run in
-    # the caller's device code.
+    # host-only header use, then ForEach, RoaringBitmapRef and the caller's
+    # functor types, which run in the caller's device code.
     for name, src in (("header_api_test", "header_api_test.cpp"), ("for_each_test", "for_each_test.hip"),
-                      ("roaring_ref_test", "roaring_ref_test.hip")):
+                      ("roaring_ref_test", "roaring_ref_test.hip"), ("caller_functor_test", "caller_functor_test.hip")):
         test_src = os.path.join(HERE, "..", "tests", "cpp", src)
         if not os.path.exists(test_src):
             continue

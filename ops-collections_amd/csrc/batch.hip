@@ -22,7 +22,7 @@
 #include "mxco_common.h"
 #include "batch_build.h"
 #include "batch_kernels.h"
-#include "oa_kernels.h"
+#include "mxco/detail/oa_kernels.h"
 
 using namespace mxco;
 

@@ -38,7 +38,7 @@
 #include <type_traits>
 
 #include "batch_kernels.h"
-#include "oa_kernels.h"
+#include "mxco/detail/oa_kernels.h"
 
 namespace mxco {
 

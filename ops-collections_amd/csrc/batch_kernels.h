@@ -7,7 +7,7 @@
 #include <type_traits>
 
 #include "../../include/mxco.h"
-#include "oa_kernels.h"
+#include "mxco/detail/oa_kernels.h"
 
 namespace mxco {
 

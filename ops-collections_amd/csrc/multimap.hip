@@ -25,7 +25,7 @@
 #include <vector>
 
 #include "mxco_common.h"
-#include "oa_kernels.h"
+#include "mxco/detail/oa_kernels.h"
 
 using namespace mxco;
 

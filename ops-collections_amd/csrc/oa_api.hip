@@ -68,6 +68,9 @@ struct mxco_oa {
     uint32_t store_mode;
     int inflight;
     uint32_t probe_flags;
+    // Probe kernels compiled in the caller's translation unit (its own
+    // KeyEqual / hasher types); null: the library's kernels.
+    mxco_oa_probe_launcher caller_launch;
     RegionWs* rws;
     // No key can be in the slots: set by creation and by an eager Clear,
     // cleared by every call that can add keys (Insert, InsertOrAssign,
@@ -167,6 +170,12 @@ mxco_status set_stencil(OaArgs& a, const mxco_stencil* st, const char* fn)
 
 mxco_status launch(const mxco_oa* t, int op, OaArgs& a, hipStream_t stream, const char* fn)
 {
+    if (t->caller_launch) {
+        const unsigned grid = grid_for(a.n * (uint64_t)kCallerGroup, t->blocks_per_cu);
+        const hipError_t e = (hipError_t)t->caller_launch(op, &a, grid, 256, stream);
+        if (e != hipSuccess) return hip_fail(e, fn);
+        return MXCO_OK;
+    }
     const unsigned grid = grid_for(a.n * (uint64_t)t->group, t->blocks_per_cu);
     hipError_t e = launch_oa(t->d.key_size, t->d.value_size, t->group, op, a, dim3(grid), stream, t->inflight);
     if (e != hipSuccess) return hip_fail(e, fn);
@@ -298,7 +307,8 @@ bool region_ws_holds(const RegionWs* w, const RegionNeed& q)
 bool region_insert_fits(const mxco_oa* t, uint64_t n, const OaArgs& a, hipStream_t stream)
 {
     const char* env = getenv("MXCO_OA_REGION_INSERT");
-    if ((env && atoi(env) == 0) || t->d.probing != 0 || a.stencil || !wide_layout(t)) return false;
+    if ((env && atoi(env) == 0) || t->d.probing != 0 || a.stencil || !wide_layout(t) || t->caller_launch)
+        return false;
     if (n < (1ull << 20) || n * 2 < t->cap || n >= 0xffffffffull) return false;
     const char* mb = getenv("MXCO_OA_REGION_INSERT_MIN_BYTES");
     if (t->cap * t->ss < (mb ? strtoull(mb, nullptr, 10) : (256ull << 20))) return false;
@@ -812,6 +822,13 @@ MXCO_API mxco_status mxco_oa_probe_stats(mxco_oa* t, const void* keys, uint64_t 
     a.counters = t->counters;
     if (mxco_status s = launch(t, OP_PROBE_STATS, a, stream, "probe_stats")) return s;
     if (mxco_status s = read_counters(t, stream, lines, slots)) return s;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_oa_set_probe_launcher(mxco_oa* t, mxco_oa_probe_launcher launch)
+{
+    if (mxco_status s = check_table(t, "set_probe_launcher")) return s;
+    t->caller_launch = launch;
     return ok();
 }
 

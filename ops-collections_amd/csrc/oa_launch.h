@@ -4,7 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "oa_kernels.h"
+#include "mxco/detail/oa_kernels.h"
 
 namespace mxco {
 
