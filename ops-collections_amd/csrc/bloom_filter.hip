@@ -405,6 +405,10 @@ __global__ __launch_bounds__(1024) void bloom_apply_kernel(uint32_t* words, uint
                 const uint32_t k = k0 + u * blockDim.x;
                 v[u] = k < e ? entries[k] : 0;
             }
+            // Lane l visits the block's 64-bit words in the order l, l+1, ...
+            // (mod 4): blocks start at multiples of 8 words, so in a fixed
+            // order every lane of a wavefront would hit the same few banks.
+            const uint32_t rot = threadIdx.x & 3u;
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 if (k0 + u * blockDim.x >= e) break;
@@ -412,11 +416,16 @@ __global__ __launch_bounds__(1024) void bloom_apply_kernel(uint32_t* words, uint
                 // the block's 8 words as 4 64-bit words: one LDS atomic per
                 // pair of words, none when the bits are already set
                 unsigned long long* w = (unsigned long long*)(lw + (uint32_t)(v[u] >> 32) * 8);
+                unsigned long long m[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    m[q] = (unsigned long long)(1u << ((kBloomSalts[2 * q] * lo) >> 27)) |
+                           (unsigned long long)(1u << ((kBloomSalts[2 * q + 1] * lo) >> 27)) << 32;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const unsigned long long m = (unsigned long long)(1u << ((kBloomSalts[2 * q] * lo) >> 27)) |
-                                                 (unsigned long long)(1u << ((kBloomSalts[2 * q + 1] * lo) >> 27)) << 32;
-                    if ((w[q] & m) != m) atomicOr(w + q, m);
+                    const uint32_t qq = (q + rot) & 3u;
+                    const unsigned long long mq = qq == 0 ? m[0] : (qq == 1 ? m[1] : (qq == 2 ? m[2] : m[3]));
+                    if ((w[qq] & mq) != mq) atomicOr(w + qq, mq);
                 }
             }
         }
