@@ -619,8 +619,9 @@ MXCO_API mxco_status mxco_device_cu_count(int* cus);
 
 /* Measured ceilings (no reference counterpart; measurement entry points).
  * random_read: n reads of `access` bytes (16, 32, 64 or 128) at hashed,
- * aligned offsets of the first `bytes` of buf, four in flight per lane
- * group; sink receives one word per lane (CUs * 8 * 256 words).
+ * aligned offsets within the largest power-of-two number of `access`-byte
+ * units in the first `bytes` of buf, eight in flight per lane group; sink
+ * receives one word per lane (CUs * 8 * 256 words).
  * stream_copy: dst = src over `bytes` (a multiple of 16). */
 MXCO_API mxco_status mxco_diag_random_read(const void* buf, uint64_t bytes, uint32_t access, uint64_t n,
                                            uint64_t* sink, uint64_t sink_words, hipStream_t stream);

@@ -7,8 +7,10 @@
 //     buffer is the Bloom filter's block read, served by the MALL;
 //   - a streaming copy, the pattern of the region build and the fill.
 // Reads are spread over lane groups of access/16 lanes (one 16-byte load
-// each), four accesses in flight per group, offsets from a multiplicative
-// hash of the access index; each lane folds what it reads into one word
+// each), eight accesses in flight per group, offsets from a multiplicative
+// hash of the access index masked to the largest power-of-two count of units
+// that fits the buffer (no division, so the address arithmetic stays a few
+// VALU instructions per access); each lane folds what it reads into one word
 // written once, so no load is dead.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,10 +24,10 @@ namespace {
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 template <int LANES>
-__global__ __launch_bounds__(256) void random_read_kernel(const uint8_t* buf, uint64_t units, uint64_t n,
+__global__ __launch_bounds__(256) void random_read_kernel(const uint8_t* buf, uint64_t unit_mask, uint64_t n,
                                                           uint64_t* sink)
 {
-    constexpr int U = 4;
+    constexpr int U = 8;
     const uint32_t lane = threadIdx.x % LANES;
     const uint64_t group = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
     const uint64_t groups = ((uint64_t)gridDim.x * blockDim.x) / LANES;
@@ -36,7 +38,7 @@ __global__ __launch_bounds__(256) void random_read_kernel(const uint8_t* buf, ui
         for (int u = 0; u < U; ++u) {
             const uint64_t i = i0 + (uint64_t)u * groups;
             if (i < n) {
-                const uint64_t unit = ((i * 0x9E3779B97F4A7C15ull) >> 17) % units;
+                const uint64_t unit = ((i * 0x9E3779B97F4A7C15ull) >> 17) & unit_mask;
                 v[u] = *(const u64x2*)(buf + unit * (16 * LANES) + lane * 16);
             } else {
                 v[u] = u64x2{0, 0};
@@ -68,13 +70,15 @@ MXCO_API mxco_status mxco_diag_random_read(const void* buf, uint64_t bytes, uint
         return fail(MXCO_INVALID_ARGUMENT, "random_read: buffer not aligned to the access or too small");
     const unsigned grid = (unsigned)cu_count() * 8;
     if (sink_words < (uint64_t)grid * 256) return fail(MXCO_INVALID_ARGUMENT, "random_read: sink too small");
-    const uint64_t units = bytes / access;
+    uint64_t units = 1;  // the largest power of two of whole units in the buffer
+    while (units * 2 <= bytes / access) units *= 2;
+    const uint64_t mask = units - 1;
     const uint8_t* b = (const uint8_t*)buf;
     switch (access) {
-    case 16: random_read_kernel<1><<<grid, 256, 0, stream>>>(b, units, n, sink); break;
-    case 32: random_read_kernel<2><<<grid, 256, 0, stream>>>(b, units, n, sink); break;
-    case 64: random_read_kernel<4><<<grid, 256, 0, stream>>>(b, units, n, sink); break;
-    default: random_read_kernel<8><<<grid, 256, 0, stream>>>(b, units, n, sink); break;
+    case 16: random_read_kernel<1><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
+    case 32: random_read_kernel<2><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
+    case 64: random_read_kernel<4><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
+    default: random_read_kernel<8><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
     }
     MXCO_HIP(hipGetLastError(), "random_read");
     return ok();
