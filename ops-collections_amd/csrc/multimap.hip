@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "multimap_batch.h"
 #include "mxco_common.h"
 #include "mxco/detail/oa_kernels.h"
 
@@ -151,8 +152,69 @@ enum { Q_CONTAINS = 0, Q_FIND = 1, Q_COUNT = 2, Q_GATHER = 3 };
 // scratch copy and a segmented sort.
 constexpr uint64_t kInlineSort = 64;
 
-// One lane per query; the walk covers the run of occupied slots from the
-// key's start slot, one line-remainder window per round trip.
+// One query's walk: the run of occupied slots from the key's start slot, one
+// line-remainder window per round trip.  Returns the matches; Find keeps the
+// smallest value in *best (any = a match was seen), Contains stops at the
+// first match, Gather writes the matches to positions wpos, wpos + 1, ... of
+// the output (inserted into sorted place within [seg_off, seg_off + seg_len)
+// when seg_len <= kInlineSort, unsorted otherwise).
+template <int KS, int VS, int MODE>
+__device__ __forceinline__ uint64_t mm_walk(const uint8_t* table, uint64_t mask, uint64_t key, uint64_t ek,
+                                            int64_t& best, bool& any, uint64_t wpos, uint64_t seg_off,
+                                            uint64_t seg_len, uint8_t* probe_out, uint8_t* value_out)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr int SS = L::SS;
+    constexpr int SPL = 128 / SS;
+    uint64_t matches = 0;
+    uint64_t pos = mm_start<KS, VS>(key, mask);
+    uint64_t left = mask + 1;
+    bool stop = false;
+    while (!stop && left) {
+        uint32_t wlen = SPL - (uint32_t)(pos & (SPL - 1));
+        if (wlen > left) wlen = (uint32_t)left;
+        if ((mask + 1) - pos < wlen) wlen = (uint32_t)((mask + 1) - pos);
+        Raw s[SPL];
+#pragma unroll
+        for (int k = 0; k < SPL; ++k)
+            if ((uint32_t)k < wlen) s[k] = *(const Raw*)(table + (pos + k) * SS);
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            if ((uint32_t)k >= wlen || stop) continue;
+            const uint64_t sk = L::key(s[k]);
+            if (sk == ek) {
+                stop = true;
+            } else if (sk == key) {
+                const int64_t v = sval<VS>(L::val(s[k]));
+                if (MODE == Q_CONTAINS) stop = true;
+                if (MODE == Q_GATHER) {
+                    store_bits<KS>(probe_out + wpos * KS, key);
+                    if (seg_len > kInlineSort) {
+                        store_bits<VS>(value_out + wpos * VS, (uint64_t)v);
+                    } else {
+                        // insertion into the sorted prefix of the segment (ascending, signed)
+                        uint64_t p = wpos;
+                        while (p > seg_off && sval<VS>(load_bits<VS>(value_out + (p - 1) * VS)) > v) {
+                            store_bits<VS>(value_out + p * VS, load_bits<VS>(value_out + (p - 1) * VS));
+                            --p;
+                        }
+                        store_bits<VS>(value_out + p * VS, (uint64_t)v);
+                    }
+                    ++wpos;
+                }
+                best = (!any || v < best) ? v : best;
+                any = true;
+                ++matches;
+            }
+        }
+        left -= wlen;
+        pos = (pos + wlen) & mask;
+    }
+    return matches;
+}
+
+// One lane per query.
 template <int KS, int VS, int MODE>
 __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uint64_t mask, const uint8_t* keys,
                                                        const uint8_t* stencil, uint32_t esz, uint32_t pred,
@@ -161,10 +223,6 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
                                                        uint8_t* value_out, unsigned long long* total,
                                                        uint64_t out_capacity, unsigned long long* clear)
 {
-    using L = Lay<KS, VS>;
-    using Raw = typename L::Raw;
-    constexpr int SS = L::SS;
-    constexpr int SPL = 128 / SS;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long local = 0;
     // Count clears the word the following gather counts long segments in
@@ -182,8 +240,7 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
         uint64_t matches = 0;
         int64_t best = 0;
         bool any = false;
-        uint64_t wpos = (MODE == Q_GATHER) ? offsets[i] : 0;
-        const uint64_t seg_off = wpos;
+        const uint64_t seg_off = (MODE == Q_GATHER) ? offsets[i] : 0;
         const uint64_t seg_len = (MODE == Q_GATHER) ? offsets[i + 1] - offsets[i] : 0;
         if (MODE == Q_GATHER && seg_len > kInlineSort) {
             // listed for the segmented sort: begins in counts[0..n],
@@ -192,52 +249,9 @@ __global__ __launch_bounds__(256) void mm_query_kernel(const uint8_t* table, uin
             counts[j] = seg_off;
             counts[n + 1 + j] = seg_off + seg_len;
         }
-        if (sel && key != ek) {
-            uint64_t pos = mm_start<KS, VS>(key, mask);
-            uint64_t left = mask + 1;
-            bool stop = false;
-            while (!stop && left) {
-                uint32_t wlen = SPL - (uint32_t)(pos & (SPL - 1));
-                if (wlen > left) wlen = (uint32_t)left;
-                if ((mask + 1) - pos < wlen) wlen = (uint32_t)((mask + 1) - pos);
-                Raw s[SPL];
-#pragma unroll
-                for (int k = 0; k < SPL; ++k)
-                    if ((uint32_t)k < wlen) s[k] = *(const Raw*)(table + (pos + k) * SS);
-#pragma unroll
-                for (int k = 0; k < SPL; ++k) {
-                    if ((uint32_t)k >= wlen || stop) continue;
-                    const uint64_t sk = L::key(s[k]);
-                    if (sk == ek) {
-                        stop = true;
-                    } else if (sk == key) {
-                        const int64_t v = sval<VS>(L::val(s[k]));
-                        if (MODE == Q_CONTAINS) stop = true;
-                        if (MODE == Q_GATHER) {
-                            // long segments land unsorted and are sorted afterwards
-                            store_bits<KS>(probe_out + wpos * KS, key);
-                            if (seg_len > kInlineSort) {
-                                store_bits<VS>(value_out + wpos * VS, (uint64_t)v);
-                            } else {
-                                // insertion into the sorted prefix of the segment (ascending, signed)
-                                uint64_t p = wpos;
-                                while (p > seg_off && sval<VS>(load_bits<VS>(value_out + (p - 1) * VS)) > v) {
-                                    store_bits<VS>(value_out + p * VS, load_bits<VS>(value_out + (p - 1) * VS));
-                                    --p;
-                                }
-                                store_bits<VS>(value_out + p * VS, (uint64_t)v);
-                            }
-                            ++wpos;
-                        }
-                        best = (!any || v < best) ? v : best;
-                        any = true;
-                        ++matches;
-                    }
-                }
-                left -= wlen;
-                pos = (pos + wlen) & mask;
-            }
-        }
+        if (sel && key != ek)
+            matches = mm_walk<KS, VS, MODE>(table, mask, key, ek, best, any, seg_off, seg_off, seg_len, probe_out,
+                                            value_out);
         if constexpr (MODE == Q_CONTAINS) out[i] = any ? 1 : 0;
         if constexpr (MODE == Q_FIND) store_bits<VS>(out + i * VS, any ? (uint64_t)best : ev);
         if constexpr (MODE == Q_COUNT) {
@@ -700,3 +714,296 @@ MXCO_API mxco_status mxco_mm_retrieve_all(mxco_mm* m, void* key_out, void* value
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched reads over many tables (multimap_batch.h)
+// ---------------------------------------------------------------------------
+namespace mxco {
+
+namespace {
+
+struct MmBatchItem {
+    const uint8_t* table;
+    const uint8_t* keys;
+    uint8_t* out;   // Find: values; Retrieve: probe keys
+    uint8_t* out2;  // Retrieve: values
+    uint64_t mask, ek, ev;
+    uint64_t n;
+    uint64_t base;      // Count / Retrieve: the item's first entry of the counts and offsets
+    uint64_t capacity;  // Retrieve
+    uint32_t mode;      // Q_FIND, Q_COUNT (Count) or Q_GATHER (Retrieve)
+    uint32_t pad;
+};
+
+struct MmBatchChunk {
+    uint32_t item, len;
+    uint64_t start;
+};
+
+constexpr uint32_t kMmChunk = 1024;
+
+// Walk pass: Find writes its values; Count and Retrieve write each query's
+// match count at counts[base + i].
+__global__ __launch_bounds__(256) void mm_batch_walk_kernel(const MmBatchItem* items, const MmBatchChunk* chunks,
+                                                            uint32_t nchunks, uint64_t* counts)
+{
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const MmBatchChunk ch = chunks[c];
+        const MmBatchItem it = items[ch.item];
+        for (uint32_t j = threadIdx.x; j < ch.len; j += blockDim.x) {
+            const uint64_t i = ch.start + j;
+            const uint64_t key = load_bits<8>(it.keys + i * 8);
+            int64_t best = 0;
+            bool any = false;
+            if (it.mode == Q_FIND) {
+                if (key != it.ek)
+                    (void)mm_walk<8, 8, Q_FIND>(it.table, it.mask, key, it.ek, best, any, 0, 0, 0, nullptr, nullptr);
+                store_bits<8>(it.out + i * 8, any ? (uint64_t)best : it.ev);
+            } else {
+                uint64_t m = 0;
+                if (key != it.ek)
+                    m = mm_walk<8, 8, Q_COUNT>(it.table, it.mask, key, it.ek, best, any, 0, 0, 0, nullptr, nullptr);
+                counts[it.base + i] = m;
+            }
+        }
+    }
+}
+
+// res[4 r] = matches of item r (Count / Retrieve); res[4 r + 1] = 0, the
+// long-segment count the gather raises.
+__global__ __launch_bounds__(256) void mm_batch_totals_kernel(const MmBatchItem* items, uint32_t nitems,
+                                                              const uint64_t* offsets, uint64_t* res)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nitems) return;
+    const MmBatchItem it = items[r];
+    res[4 * r] = it.mode == Q_FIND ? 0 : offsets[it.base + it.n] - offsets[it.base];
+    res[4 * r + 1] = 0;
+}
+
+// Gather pass of the Retrieves (chunks of Retrieve items only): matches at
+// their offsets within the item's own output; an item whose matches exceed
+// its capacity writes nothing.  Segments longer than kInlineSort land
+// unsorted and are listed: begins at longs[2 base + k], ends at
+// longs[2 base + n + k].
+__global__ __launch_bounds__(256) void mm_batch_gather_kernel(const MmBatchItem* items, const MmBatchChunk* chunks,
+                                                              uint32_t nchunks, const uint64_t* offsets,
+                                                              unsigned long long* res, uint64_t* longs)
+{
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const MmBatchChunk ch = chunks[c];
+        const MmBatchItem it = items[ch.item];
+        const uint64_t first = offsets[it.base];
+        if (offsets[it.base + it.n] - first > it.capacity) continue;
+        for (uint32_t j = threadIdx.x; j < ch.len; j += blockDim.x) {
+            const uint64_t i = ch.start + j;
+            const uint64_t key = load_bits<8>(it.keys + i * 8);
+            const uint64_t seg_off = offsets[it.base + i] - first;
+            const uint64_t seg_len = offsets[it.base + i + 1] - offsets[it.base + i];
+            if (seg_len == 0) continue;
+            if (seg_len > kInlineSort) {
+                const unsigned long long k = atomicAdd(&res[4 * ch.item + 1], 1ull);
+                longs[2 * it.base + k] = seg_off;
+                longs[2 * it.base + it.n + k] = seg_off + seg_len;
+            }
+            int64_t best = 0;
+            bool any = false;
+            (void)mm_walk<8, 8, Q_GATHER>(it.table, it.mask, key, it.ek, best, any, seg_off, seg_off, seg_len,
+                                          it.out, it.out2);
+        }
+    }
+}
+
+// A device or pinned buffer that grows by doubling and keeps what it outgrew.
+struct Grow {
+    void* p = nullptr;
+    size_t cap = 0;
+    bool pinned = false;
+    std::vector<void*> retired;
+    hipError_t need(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        const size_t want = std::max<size_t>({bytes, 2 * cap, 4096});
+        void* q = nullptr;
+        hipError_t e = pinned ? hipHostMalloc(&q, want, 0) : hipMalloc(&q, want);
+        if (e != hipSuccess) return e;
+        if (p) retired.push_back(p);
+        p = q;
+        cap = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        for (void* q : retired) (void)(pinned ? hipHostFree(q) : hipFree(q));
+        if (p) (void)(pinned ? hipHostFree(p) : hipFree(p));
+        retired.clear();
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct MmBatchWs {
+    Grow host_up{nullptr, 0, true, {}};  // items + walk chunks + gather chunks
+    Grow host_res{nullptr, 0, true, {}};
+    Grow dev_up, counts, offsets, longs, res, scan_tmp, sort_tmp, sort_copy;
+};
+
+MmBatchWs* mm_batch_ws_create() { return new (std::nothrow) MmBatchWs(); }
+
+mxco_status mm_batch_ws_reserve(MmBatchWs* ws, uint32_t reads, uint64_t keys)
+{
+    const uint64_t chunks = 2 * (keys / kMmChunk + reads);
+    const size_t up_b = reads * sizeof(MmBatchItem) + chunks * sizeof(MmBatchChunk) + 32;
+    MXCO_HIP(ws->host_up.need(up_b), "multimap batch: staging");
+    MXCO_HIP(ws->dev_up.need(up_b), "multimap batch: upload");
+    MXCO_HIP(ws->counts.need((keys + 1) * 8), "multimap batch: counts");
+    MXCO_HIP(ws->offsets.need((keys + 1) * 8), "multimap batch: offsets");
+    MXCO_HIP(ws->longs.need(2 * keys * 8), "multimap batch: segments");
+    MXCO_HIP(ws->res.need(4ull * reads * 8), "multimap batch: results");
+    MXCO_HIP(ws->host_res.need(4ull * reads * 8), "multimap batch: results");
+    size_t temp = 0;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                              (int64_t)keys + 1, (hipStream_t)0),
+             "scan");
+    MXCO_HIP(ws->scan_tmp.need(temp), "multimap batch: scan");
+    return MXCO_OK;
+}
+
+void mm_batch_ws_destroy(MmBatchWs* ws)
+{
+    if (!ws) return;
+    for (Grow* g : {&ws->host_up, &ws->host_res, &ws->dev_up, &ws->counts, &ws->offsets, &ws->longs, &ws->res,
+                    &ws->scan_tmp, &ws->sort_tmp, &ws->sort_copy})
+        g->release();
+    delete ws;
+}
+
+mxco_status mm_batch_reads(MmRead* reads, uint32_t n, MmBatchWs* ws, hipStream_t s)
+{
+    if (n == 0) return MXCO_OK;
+    std::vector<MmBatchItem> items(n);
+    std::vector<MmBatchChunk> walk, gather;
+    uint64_t L = 0;  // counts of Count and Retrieve items, concatenated
+    for (uint32_t r = 0; r < n; ++r) {
+        MmRead& q = reads[r];
+        q.result = 0;
+        q.status = MXCO_OK;
+        const mxco_mm* m = q.table;
+        if (!m || !m->table || m->ks != 8 || m->vs != 8) return fail(MXCO_INVALID_ARGUMENT, "multimap batch: table");
+        MmBatchItem& it = items[r];
+        memset(&it, 0, sizeof(it));
+        it.table = m->table;
+        it.keys = (const uint8_t*)q.keys;
+        it.out = (uint8_t*)q.out;
+        it.out2 = (uint8_t*)q.out2;
+        it.mask = m->slots_pow2 - 1;
+        it.ek = m->empty_key;
+        it.ev = m->empty_val;
+        it.n = q.keys ? q.n : 0;
+        it.capacity = q.out_capacity;
+        if (q.op == MXCO_REQ_MM_FIND) {
+            it.mode = Q_FIND;
+            if (it.n && !q.out) return fail(MXCO_INVALID_ARGUMENT, "multimap batch: null output");
+        } else if (q.op == MXCO_REQ_MM_COUNT) {
+            it.mode = Q_COUNT;
+        } else if (q.op == MXCO_REQ_MM_RETRIEVE) {
+            it.mode = Q_GATHER;
+        } else {
+            return fail(MXCO_INVALID_ARGUMENT, "multimap batch: not a read");
+        }
+        if (it.mode != Q_FIND) {
+            it.base = L;
+            L += it.n;
+        }
+        for (uint64_t st = 0; st < it.n; st += kMmChunk) {
+            const MmBatchChunk ch{r, (uint32_t)std::min<uint64_t>(kMmChunk, it.n - st), st};
+            walk.push_back(ch);
+            if (it.mode == Q_GATHER) gather.push_back(ch);
+        }
+    }
+    const size_t items_b = (n * sizeof(MmBatchItem) + 15) & ~size_t(15);
+    const size_t walk_b = (walk.size() * sizeof(MmBatchChunk) + 15) & ~size_t(15);
+    const size_t up_b = items_b + walk_b + gather.size() * sizeof(MmBatchChunk);
+    MXCO_HIP(ws->host_up.need(up_b), "multimap batch: staging");
+    MXCO_HIP(ws->dev_up.need(up_b), "multimap batch: upload");
+    MXCO_HIP(ws->counts.need((L + 1) * 8), "multimap batch: counts");
+    MXCO_HIP(ws->offsets.need((L + 1) * 8), "multimap batch: offsets");
+    MXCO_HIP(ws->longs.need(std::max<uint64_t>(1, 2 * L) * 8), "multimap batch: segments");
+    MXCO_HIP(ws->res.need(4ull * n * 8), "multimap batch: results");
+    MXCO_HIP(ws->host_res.need(4ull * n * 8), "multimap batch: results");
+    uint8_t* hu = (uint8_t*)ws->host_up.p;
+    memcpy(hu, items.data(), n * sizeof(MmBatchItem));
+    if (!walk.empty()) memcpy(hu + items_b, walk.data(), walk.size() * sizeof(MmBatchChunk));
+    if (!gather.empty()) memcpy(hu + items_b + walk_b, gather.data(), gather.size() * sizeof(MmBatchChunk));
+    uint8_t* du = (uint8_t*)ws->dev_up.p;
+    const MmBatchItem* ditems = (const MmBatchItem*)du;
+    const MmBatchChunk* dwalk = (const MmBatchChunk*)(du + items_b);
+    const MmBatchChunk* dgather = (const MmBatchChunk*)(du + items_b + walk_b);
+    uint64_t* counts = (uint64_t*)ws->counts.p;
+    uint64_t* offsets = (uint64_t*)ws->offsets.p;
+    uint64_t* res = (uint64_t*)ws->res.p;
+    MXCO_HIP(hipMemcpyAsync(du, hu, up_b, hipMemcpyHostToDevice, s), "multimap batch: upload");
+    const unsigned cap = (unsigned)cu_count() * 16;
+    if (!walk.empty())
+        mm_batch_walk_kernel<<<std::min<unsigned>((unsigned)walk.size(), cap), 256, 0, s>>>(
+            ditems, dwalk, (uint32_t)walk.size(), counts);
+    MXCO_HIP(hipGetLastError(), "multimap batch: walk");
+    // offsets over the L counts: offsets[L] is the total (counts[L] is read
+    // but does not change the result)
+    size_t temp = 0;
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, counts, offsets, (int64_t)L + 1, s), "scan");
+    MXCO_HIP(ws->scan_tmp.need(temp), "multimap batch: scan");
+    MXCO_HIP(hipcub::DeviceScan::ExclusiveSum(ws->scan_tmp.p, temp, counts, offsets, (int64_t)L + 1, s), "scan");
+    mm_batch_totals_kernel<<<(n + 255) / 256, 256, 0, s>>>(ditems, n, offsets, res);
+    if (!gather.empty())
+        mm_batch_gather_kernel<<<std::min<unsigned>((unsigned)gather.size(), cap), 256, 0, s>>>(
+            ditems, dgather, (uint32_t)gather.size(), offsets, (unsigned long long*)res, (uint64_t*)ws->longs.p);
+    MXCO_HIP(hipGetLastError(), "multimap batch: gather");
+    uint64_t* hres = (uint64_t*)ws->host_res.p;
+    MXCO_HIP(hipMemcpyAsync(hres, res, 4ull * n * 8, hipMemcpyDeviceToHost, s), "multimap batch: results");
+    MXCO_HIP(hipStreamSynchronize(s), "multimap batch");
+    bool sorted_any = false;
+    for (uint32_t r = 0; r < n; ++r) {
+        MmRead& q = reads[r];
+        const MmBatchItem& it = items[r];
+        if (it.mode == Q_FIND) continue;
+        const uint64_t total = hres[4 * r];
+        if (it.mode == Q_COUNT) {
+            q.result = total;
+            continue;
+        }
+        if (total > it.capacity) {
+            q.status = MXCO_LENGTH_ERROR;
+            continue;
+        }
+        if (total && (!q.out || !q.out2)) {
+            q.status = MXCO_INVALID_ARGUMENT;
+            continue;
+        }
+        q.result = total;
+        const int64_t nlong = (int64_t)hres[4 * r + 1];
+        if (nlong > 0) {
+            // sort the listed segments from a copy of the values back into the output
+            MXCO_HIP(ws->sort_copy.need(total * 8), "multimap batch: sort");
+            MXCO_HIP(hipMemcpyAsync(ws->sort_copy.p, q.out2, total * 8, hipMemcpyDeviceToDevice, s), "sort");
+            const uint64_t* begins = (const uint64_t*)ws->longs.p + 2 * it.base;
+            const uint64_t* ends = begins + it.n;
+            size_t stemp = 0;
+            MXCO_HIP(hipcub::DeviceSegmentedSort::SortKeys(nullptr, stemp, (const int64_t*)ws->sort_copy.p,
+                                                          (int64_t*)q.out2, (int64_t)total, nlong, begins, ends, s),
+                     "multimap batch: sort");
+            MXCO_HIP(ws->sort_tmp.need(stemp), "multimap batch: sort");
+            MXCO_HIP(hipcub::DeviceSegmentedSort::SortKeys(ws->sort_tmp.p, stemp, (const int64_t*)ws->sort_copy.p,
+                                                          (int64_t*)q.out2, (int64_t)total, nlong, begins, ends, s),
+                     "multimap batch: sort");
+            // the copy and the sort's scratch are reused by the next item's sort: in stream order
+            sorted_any = true;
+        }
+    }
+    if (sorted_any) MXCO_HIP(hipStreamSynchronize(s), "multimap batch: sort");
+    return MXCO_OK;
+}
+
+}  // namespace mxco

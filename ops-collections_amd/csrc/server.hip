@@ -10,10 +10,10 @@
 //    probe kernel), queued on the map stream without waiting for earlier
 //    ticks -- requests that arrive while a tick runs are admitted into the
 //    next one as tick buffers free up;
-//  - StaticMultimap requests run on kMmWorkers worker threads, each with
-//    its own stream, overlapping the map ticks and each other on the device;
-//    a table's requests all go to worker `table % kMmWorkers`, in arrival
-//    order;
+//  - StaticMultimap requests run on their own thread and stream, in ticks
+//    of their own that overlap the map ticks on the device: per table at
+//    most one insert (run as StaticMultimap::Insert) and then reads, and the
+//    tick's reads of every table run as one batched pass (multimap_batch.h);
 //  - a completion thread retires ticks in order (event wait), stamps each
 //    request's completion time and publishes its result.
 // Per table, requests take effect in arrival order: a tick takes a table's
@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "batch_kernels.h"
+#include "multimap_batch.h"
 #include "mxco_common.h"
 
 using namespace mxco;
@@ -65,8 +66,9 @@ struct Table {
 };
 
 constexpr int kTickSlots = 4;
-constexpr int kMmWorkers = 4;  // StaticMultimap worker threads (and streams)
+constexpr int kMmWorkers = 1;  // StaticMultimap tick threads (and streams)
 constexpr uint32_t kMaxTickRequests = 4096;
+constexpr uint64_t kMaxMmTickKeys = 1ull << 22;  // multimap queries per tick (one larger request runs alone)
 
 struct Tick {
     // One upload per tick: [items | chunks | zeroed insert counters].
@@ -102,6 +104,7 @@ struct mxco_server {
     Tick ticks[kTickSlots];
     std::deque<Tick*> inflight;
     hipStream_t map_stream = nullptr, mm_streams[kMmWorkers] = {};
+    MmBatchWs* mm_ws[kMmWorkers] = {};
     std::thread dispatcher, completer, mm_workers[kMmWorkers];
     bool stop = false;
     std::string worker_error;
@@ -356,36 +359,77 @@ void completer_loop(mxco_server* s)
     }
 }
 
-// Multimap requests of worker w's tables, in arrival order, on its stream.
+// Multimap ticks of worker w's tables, on its stream.  A tick takes, per
+// table, at most one insert followed by reads (a second insert, or an insert
+// after a read, waits for the next tick): the inserts run one by one, then
+// all reads of the tick as one batched pass, so per table every request
+// takes effect in arrival order.
 void mm_loop(mxco_server* s, int w)
 {
     (void)hipSetDevice(s->device);
     hipStream_t stream = s->mm_streams[w];
     std::deque<Req*>& queue = s->mm_pending[w];
     std::unique_lock<std::mutex> lk(s->mu);
+    std::vector<Req*> ins, rd;
+    std::vector<mxco_mm*> ins_mm, rd_mm;  // taken under the lock (tables may grow meanwhile)
+    std::vector<MmRead> reads;
     for (;;) {
         s->cv_mm.wait(lk, [&] { return s->stop || !queue.empty(); });
         if (queue.empty()) {
             if (s->stop) break;
             continue;
         }
-        Req* r = queue.front();
-        queue.pop_front();
-        mxco_mm* mm = s->tables[r->r.table].mm;
-        lk.unlock();
-        uint64_t result = 0;
-        mxco_status st = MXCO_OK;
-        switch (r->r.op) {
-        case MXCO_REQ_MM_INSERT: st = mxco_mm_insert(mm, r->r.in, nullptr, r->r.n, stream, &result); break;
-        case MXCO_REQ_MM_FIND: st = mxco_mm_find(mm, r->r.in, nullptr, r->r.out, r->r.n, stream); break;
-        case MXCO_REQ_MM_COUNT: st = mxco_mm_count(mm, r->r.in, r->r.n, stream, &result); break;
-        case MXCO_REQ_MM_RETRIEVE:
-            st = mxco_mm_retrieve(mm, r->r.in, r->r.n, r->r.out, r->r.out2, r->r.out_capacity, stream, &result);
-            break;
-        default: st = fail(MXCO_INVALID_ARGUMENT, "server: unknown request");
+        std::unordered_map<uint32_t, int> phase;  // 1 inserting, 2 reading, 3 blocked
+        std::deque<Req*> keep;
+        ins.clear();
+        rd.clear();
+        uint64_t keys = 0;
+        for (Req* r : queue) {
+            int& ph = phase[r->r.table];
+            const bool insert = r->r.op == MXCO_REQ_MM_INSERT;
+            const bool full = ins.size() + rd.size() >= kMaxTickRequests ||
+                              (!insert && !rd.empty() && keys + r->r.n > kMaxMmTickKeys);
+            if (ph == 3 || (insert && ph != 0) || full) {
+                ph = 3;
+                keep.push_back(r);
+                continue;
+            }
+            ph = insert ? 1 : 2;
+            (insert ? ins : rd).push_back(r);
+            if (!insert) keys += r->r.n;
         }
+        queue.swap(keep);
+        ins_mm.clear();
+        rd_mm.clear();
+        for (Req* r : ins) ins_mm.push_back(s->tables[r->r.table].mm);
+        for (Req* r : rd) rd_mm.push_back(s->tables[r->r.table].mm);
+        lk.unlock();
+        std::vector<std::pair<uint64_t, mxco_status>> ins_res(ins.size());
+        for (size_t k = 0; k < ins.size(); ++k) {
+            const mxco_request& q = ins[k]->r;
+            uint64_t result = 0;
+            const mxco_status st = mxco_mm_insert(ins_mm[k], q.in, nullptr, q.n, stream, &result);
+            ins_res[k] = {result, st};
+        }
+        reads.resize(rd.size());
+        for (size_t k = 0; k < rd.size(); ++k) {
+            const mxco_request& q = rd[k]->r;
+            reads[k] = MmRead{rd_mm[k], q.op, q.in, q.n, q.out, q.out2, q.out_capacity, 0, MXCO_OK};
+        }
+        const mxco_status bst = mm_batch_reads(reads.data(), (uint32_t)reads.size(), s->mm_ws[w], stream);
+        const std::string berr = bst == MXCO_OK ? std::string() : mxco_last_error();
         lk.lock();
-        finish(s, r, result, st);
+        for (size_t k = 0; k < ins.size(); ++k) {
+            if (ins_res[k].second != MXCO_OK) s->worker_error = "StaticMultimap insert failed";
+            finish(s, ins[k], ins_res[k].first, ins_res[k].second);
+        }
+        for (size_t k = 0; k < rd.size(); ++k) {
+            mxco_status st = bst != MXCO_OK ? bst : reads[k].status;
+            if (bst != MXCO_OK) s->worker_error = berr;
+            else if (st == MXCO_LENGTH_ERROR) s->worker_error = "StaticMultimap: output capacity is insufficient";
+            else if (st != MXCO_OK) s->worker_error = "StaticMultimap: null nonempty buffer";
+            finish(s, rd[k], reads[k].result, st);
+        }
         s->cv_done.notify_all();
     }
 }
@@ -439,8 +483,13 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
     s->tables_cap = 4096;
     hipError_t e = hipMalloc((void**)&s->tables_dev, s->tables_cap * sizeof(BatchTable));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->map_stream, hipStreamNonBlocking);
-    for (int w = 0; w < kMmWorkers && e == hipSuccess; ++w)
+    for (int w = 0; w < kMmWorkers && e == hipSuccess; ++w) {
         e = hipStreamCreateWithFlags(&s->mm_streams[w], hipStreamNonBlocking);
+        if (e == hipSuccess && !(s->mm_ws[w] = mm_batch_ws_create())) e = hipErrorOutOfMemory;
+    }
+    // multimap tick buffers sized up front: no allocation on the request path
+    for (int w = 0; w < kMmWorkers && e == hipSuccess; ++w)
+        if (mm_batch_ws_reserve(s->mm_ws[w], kMaxTickRequests, kMaxMmTickKeys) != MXCO_OK) e = hipErrorOutOfMemory;
     for (int i = 0; i < kTickSlots && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&s->ticks[i].done, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)mxco_server_destroy(s);
@@ -501,6 +550,7 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
     if (s->map_stream) (void)hipStreamDestroy(s->map_stream);
     for (hipStream_t st : s->mm_streams)
         if (st) (void)hipStreamDestroy(st);
+    for (MmBatchWs* ws : s->mm_ws) mm_batch_ws_destroy(ws);
     delete s;
     return ok();
 }

@@ -133,6 +133,98 @@ def test_same_table_inserts_keep_arrival_order():
     np.testing.assert_array_equal(results[1][1], exp)
 
 
+def _mm_pairs(rng, table_salt, nkeys, mult):
+    """Pairs over keys 0..nkeys-1 (+ salt << 32), key k repeated mult(k)
+    times with distinct values, shuffled."""
+    ks, vs = [], []
+    for k in range(nkeys):
+        for c in range(mult(k)):
+            ks.append(k + (table_salt << 32))
+            vs.append(int(rng.integers(-10**9, 10**9)))
+    perm = rng.permutation(len(ks))
+    k = np.array(ks, np.int64)[perm]
+    v = np.array(vs, np.int64)[perm]
+    return torch.from_numpy(np.stack([k, v], 1)).cuda().contiguous(), k, v
+
+
+def test_multimap_ticks_equal_plain_and_counts():
+    """Multimap requests arriving together (inserts, Find, Count, Retrieve
+    with segments of up to 300 values, on three tables): the batched ticks
+    give the plain one-by-one results; counts equal the host's multiplicities
+    at each point of the arrival order."""
+    import mxco
+    rng = np.random.default_rng(29)
+    mult_a = lambda k: 1 + (k % 5) + (300 if k % 97 == 0 else 0)
+    mult_b = lambda k: 1 + (k % 3)
+    batches = []
+    for t in range(3):
+        a = _mm_pairs(rng, t, 400, mult_a)
+        b = _mm_pairs(rng, t, 500, mult_b)  # overlaps a's keys 0..399 and adds 400..499
+        batches.append((a, b))
+    q = [torch.from_numpy(np.arange(-3, 520, dtype=np.int64) + (t << 32)).cuda() for t in range(3)]
+    nq = q[0].numel()
+    results = []
+    for plain in (False, True):
+        s = mxco.Server()
+        tabs = [s.add_multimap(1 << 14) for _ in range(3)]
+        outs, plan_ = [], []
+        for phase in range(2):
+            for t in range(3):
+                pairs = batches[t][phase][0]
+                plan_.append((tabs[t], mxco.REQ_MM_INSERT, pairs.data_ptr(), pairs.shape[0], None, None, 0))
+            for t in range(3):
+                f = torch.full((nq,), 7, dtype=torch.int64, device="cuda")
+                cap = 9 * nq + 400 * 3
+                po = torch.zeros(cap, dtype=torch.int64, device="cuda")
+                vo = torch.zeros(cap, dtype=torch.int64, device="cuda")
+                outs.append((f, po, vo))
+                plan_.append((tabs[t], mxco.REQ_MM_FIND, q[t].data_ptr(), nq, f.data_ptr(), None, 0))
+                plan_.append((tabs[t], mxco.REQ_MM_COUNT, q[t].data_ptr(), nq, None, None, 0))
+                plan_.append((tabs[t], mxco.REQ_MM_RETRIEVE, q[t].data_ptr(), nq, po.data_ptr(), vo.data_ptr(), cap))
+        res, _ = s.replay(plan_, [0.0] * len(plan_), plain=plain)
+        results.append((res, [tuple(x.cpu().numpy() for x in o) for o in outs]))
+        s.close()
+    assert results[0][0] == results[1][0]
+    for a, b in zip(results[0][1], results[1][1]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    # counts: phase 0 after batch a, phase 1 after a + b
+    res = results[0][0]
+    per_phase = 3 + 3 * 3
+    for phase in range(2):
+        for t in range(3):
+            count = res[per_phase * phase + 3 + 3 * t + 1]
+            expect = sum(mult_a(k) for k in range(400))
+            if phase == 1:
+                expect += sum(mult_b(k) for k in range(500))
+            assert count == expect, (phase, t)
+
+
+def test_multimap_retrieve_too_small_fails_alone():
+    """A served Retrieve whose matches exceed its capacity raises
+    std::length_error's counterpart and writes nothing; the other requests
+    of its tick complete."""
+    import mxco
+    s = mxco.Server()
+    t = s.add_multimap(1024)
+    pairs = torch.tensor([[5, 1], [5, 2], [5, 3], [6, 4]], dtype=torch.int64, device="cuda")
+    s.wait(s.submit(t, mxco.REQ_MM_INSERT, pairs))
+    k5 = torch.tensor([5], dtype=torch.int64, device="cuda")
+    k6 = torch.tensor([6], dtype=torch.int64, device="cuda")
+    po = torch.full((2,), 9, dtype=torch.int64, device="cuda")
+    vo = torch.full((2,), 9, dtype=torch.int64, device="cuda")
+    po6 = torch.zeros(2, dtype=torch.int64, device="cuda")
+    vo6 = torch.zeros(2, dtype=torch.int64, device="cuda")
+    bad = s.submit(t, mxco.REQ_MM_RETRIEVE, k5, out=po, out2=vo, out_capacity=2)
+    good = s.submit(t, mxco.REQ_MM_RETRIEVE, k6, out=po6, out2=vo6, out_capacity=2)
+    with pytest.raises(mxco.LengthError):
+        s.wait(bad)
+    assert s.wait(good)[0] == 1
+    assert po.cpu().tolist() == [9, 9] and vo.cpu().tolist() == [9, 9]
+    assert po6.cpu().tolist()[0] == 6 and vo6.cpu().tolist()[0] == 4
+    s.close()
+
+
 def test_admission_by_budget():
     import mxco
     s = mxco.Server(hbm_budget_bytes=64 << 20)
