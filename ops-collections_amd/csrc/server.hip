@@ -160,11 +160,13 @@ OaArgs map_args(const mxco_server* s)
 
 bool is_map_op(uint32_t op) { return op == MXCO_REQ_INSERT || op == MXCO_REQ_FIND || op == MXCO_REQ_CONTAINS; }
 
-// Keys per chunk: small ticks are cut finer so that they still spread over
-// many workgroups; large ticks use the batched path's chunk.
+// Keys per chunk of one launch of a tick: about 16 chunks per CU, so that a
+// launch of a few hundred thousand keys still spreads over every CU with a
+// few keys per lane group (each a chain of dependent line reads); at least
+// 64, at most the batched path's chunk.
 uint32_t tick_chunk(uint64_t keys)
 {
-    const uint64_t target = keys / (4ull * (uint64_t)cu_count());
+    const uint64_t target = keys / (16ull * (uint64_t)cu_count());
     uint32_t c = 64;
     while (c < kChunk && c * 2 <= target) c *= 2;
     return c;
@@ -185,11 +187,12 @@ TickLayout tick_layout(uint64_t nitems, uint64_t nchunks, uint64_t ninsert)
 }
 
 // The most chunks a tick of at most `keys` keys in `nitems` requests cuts:
-// tick_chunk(k) exceeds k / (8 * CUs) unless it is capped at 64 (then
-// k / 64 < 8 * CUs too) or at kChunk; each request adds at most one partial chunk.
+// each of its two launches cuts k keys into chunks of tick_chunk(k), which
+// exceeds k / (32 * CUs) unless it is capped at 64 (then k / 64 < 32 * CUs
+// too) or at kChunk; each request adds at most one partial chunk.
 uint64_t tick_chunk_bound(uint64_t keys, uint64_t nitems)
 {
-    return std::max<uint64_t>(8ull * (uint64_t)cu_count(), keys / kChunk + 1) + nitems;
+    return 2 * std::max<uint64_t>(32ull * (uint64_t)cu_count(), keys / kChunk + 1) + nitems;
 }
 
 // Runs one tick's requests (already chosen) on the map stream: one upload,
@@ -197,15 +200,20 @@ uint64_t tick_chunk_bound(uint64_t keys, uint64_t nitems)
 mxco_status launch_tick(mxco_server* s, Tick& t)
 {
     std::vector<uint32_t> ins, rd;
-    uint64_t keys = 0;
+    uint64_t ins_keys = 0, rd_keys = 0;
     for (uint32_t i = 0; i < t.reqs.size(); ++i) {
-        (t.reqs[i]->r.op == MXCO_REQ_INSERT ? ins : rd).push_back(i);
-        keys += t.reqs[i]->r.n;
+        const bool insert = t.reqs[i]->r.op == MXCO_REQ_INSERT;
+        (insert ? ins : rd).push_back(i);
+        (insert ? ins_keys : rd_keys) += t.reqs[i]->r.n;
     }
-    const uint32_t csz = tick_chunk(keys);
+    // each launch cut by its own key count
+    const uint32_t csz_ins = tick_chunk(ins_keys), csz_rd = tick_chunk(rd_keys);
     const uint32_t nitems = (uint32_t)t.reqs.size();
     uint32_t nchunks = 0;
-    for (Req* r : t.reqs) nchunks += (uint32_t)((r->r.n + csz - 1) / csz);
+    for (Req* r : t.reqs) {
+        const uint32_t csz = r->r.op == MXCO_REQ_INSERT ? csz_ins : csz_rd;
+        nchunks += (uint32_t)((r->r.n + csz - 1) / csz);
+    }
     const uint32_t ninsert = (uint32_t)ins.size();
     const TickLayout lay = tick_layout(nitems, nchunks, ninsert);
     const size_t off_chunks = lay.off_chunks, off_counters = lay.off_counters, bytes = lay.bytes;
@@ -217,6 +225,7 @@ mxco_status launch_tick(mxco_server* s, Tick& t)
     t.insert_items.clear();
     for (int k = 0; k < 2; ++k) {
         if (k == 1) first_read = chunk;
+        const uint32_t csz = k == 0 ? csz_ins : csz_rd;
         for (uint32_t ri : (k == 0 ? ins : rd)) {
             const mxco_request& r = t.reqs[ri]->r;
             items[item] = {r.table, r.op, (const uint8_t*)r.in, (uint8_t*)r.out, r.n};
