@@ -671,16 +671,20 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
     slots, half full), open-loop Poisson request stream (80 % Find/Contains,
     15 % Insert on the maps, 5 % Retrieve on the multimaps; batches
     log-uniform 1..2^16 keys, 70 % hit): the continuously batching server
-    against one-at-a-time plain calls, latency percentiles over all requests
-    and per table kind; then the closed loop: every request of the
-    200k-QPS stream submitted at once, keys over the makespan (saturation)."""
+    (StaticMap ticks fed to the resident kernel), the same server with one
+    insert and one read launch per tick, and one-at-a-time plain calls;
+    latency percentiles over all requests and per table kind; then the closed
+    loop: every request of the 200k-QPS stream submitted at once, keys over
+    the makespan (saturation)."""
     import numpy as np
     from mxco import workloads
     ops = {"insert": mxco.REQ_INSERT, "find": mxco.REQ_FIND, "contains": mxco.REQ_CONTAINS,
            "retrieve": mxco.REQ_MM_RETRIEVE}
 
     def serve(w, mode, arrivals):
-        srv = mxco.Server()
+        # "batched": the default server (StaticMap ticks in the resident kernel);
+        # "launched": one insert and one read launch per tick; "plain": one by one
+        srv = mxco.Server(resident=(mode != "launched"))
         tabs = [srv.add_map(c) if t < w["num_maps"] else srv.add_multimap(c) for t, c in enumerate(w["caps"])]
         for t, pairs in enumerate(w["init"]):
             op = mxco.REQ_INSERT if t < w["num_maps"] else mxco.REQ_MM_INSERT
@@ -702,10 +706,13 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
                          o1.data_ptr() if o1 is not None else None, o2.data_ptr() if o2 is not None else None,
                          o1.numel() if r["op"] == "retrieve" else 0))
         torch.cuda.synchronize()
+        launches0 = srv.resident_stats()[1]
         results, lat = srv.replay(reqs, arrivals, plain=(mode == "plain"))
         ticks, folded = srv.stats()
+        resident, launches, _ = srv.resident_stats()
         srv.close()
         del srv
+        serve.kernel = {"resident_kernel": resident, "resident_launches": launches - launches0}
         return results, outs, np.array(lat) / 1e3, ticks, folded
 
     def same(a, b):
@@ -726,30 +733,35 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
         keys = sum(r["n"] for r in w["requests"])
         mm = np.array([r["op"] == "retrieve" for r in w["requests"]])
         res, runs = {}, {}
-        for mode in ("batched", "plain"):
+        for mode in ("batched", "launched", "plain"):
             results, outs, lat, ticks, folded = serve(w, mode, w["arrivals_us"])
             done = np.array(w["arrivals_us"]) / 1e3 + lat
             res[mode] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                          "map_p99_ms": pct(lat[~mm], 99), "multimap_p99_ms": pct(lat[mm], 99),
                          "gkeys_s": round(keys / (done.max() * 1e-3) / 1e9, 4),
                          "ticks": ticks, "requests_per_tick": round(folded / max(ticks, 1), 2)}
+            if mode != "plain":
+                res[mode].update(serve.kernel)
             runs[mode] = (results, outs)
-        res["identical_results"] = same(runs["batched"], runs["plain"])
+        res["identical_results"] = same(runs["batched"], runs["plain"]) and same(runs["launched"], runs["plain"])
         res["keys"] = keys
         out["qps_%d" % int(qps)] = res
         del runs
         if qps == qps_list[-1]:
             # closed loop: the same requests, all arriving at once
             closed, runs = {"requests": len(w["requests"]), "keys": keys}, {}
-            for mode in ("batched", "plain"):
+            for mode in ("batched", "launched", "plain"):
                 results, outs, lat, ticks, folded = serve(w, mode, [0.0] * len(w["requests"]))
                 closed[mode] = {"makespan_ms": round(float(lat.max()), 3),
                                 "gkeys_s": round(keys / (lat.max() * 1e-3) / 1e9, 4),
                                 "map_makespan_ms": round(float(lat[~mm].max()), 3),
                                 "multimap_makespan_ms": round(float(lat[mm].max()), 3),
                                 "requests_per_tick": round(folded / max(ticks, 1), 2)}
+                if mode != "plain":
+                    closed[mode].update(serve.kernel)
                 runs[mode] = (results, outs)
-            closed["identical_results"] = same(runs["batched"], runs["plain"])
+            closed["identical_results"] = same(runs["batched"], runs["plain"]) and same(runs["launched"],
+                                                                                          runs["plain"])
             out["closed_loop"] = closed
             del runs
         del w

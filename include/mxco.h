@@ -469,7 +469,12 @@ typedef struct mxco_server_config {
     uint64_t max_tick_keys;     /* keys folded into one tick (0 = 2^24); a tick also takes at
                                    most 4096 requests, and one larger request runs alone */
     uint32_t group_size;        /* lanes per key of the batched probes (0 = 4) */
-    uint32_t blocks_per_cu;     /* persistent grid (0 = 64) */
+    uint32_t blocks_per_cu;     /* grid of a launched tick (0 = 64) */
+    uint32_t tick_mode;         /* 0 = StaticMap ticks run in the resident kernel, fed through a
+                                   mailbox in host-coherent memory (4- and 8-byte keys and
+                                   payloads); 1 = one insert and one read launch per tick */
+    uint32_t resident_blocks_per_cu; /* workgroups per CU of the resident kernel (0 = all that fit
+                                        a CU but one) */
     uint8_t empty_key[8];       /* StaticMap sentinels, little-endian */
     uint8_t empty_value[8];
 } mxco_server_config;
@@ -525,6 +530,18 @@ MXCO_API mxco_status mxco_server_replay_plain(mxco_server* server, const mxco_re
                                               double* latency_us);
 MXCO_API mxco_status mxco_server_stats(mxco_server* server, uint64_t* ticks, uint64_t* tick_requests);
 MXCO_API mxco_status mxco_server_table(mxco_server* server, uint32_t table, mxco_oa** map, mxco_mm** multimap);
+/* Whether StaticMap ticks run in the resident kernel, how many launches of
+ * it have served them so far (one serves ticks until it is idle for 2 ms,
+ * 250 ms old, or stopped between ticks) and its grid. */
+MXCO_API mxco_status mxco_server_resident_stats(mxco_server* server, uint32_t* resident, uint64_t* launches,
+                                                uint32_t* grid);
+/* Diagnostic: waits for outstanding requests, ends the resident kernel and
+ * copies the timing trace of the last min(max_ticks, 1024) StaticMap ticks,
+ * oldest first, 4 words each: wall-clock times at which the kernel found the
+ * tick posted, opened it and published its results, then [insert chunks:32 |
+ * read chunks:32]; ticks_per_us converts the times. */
+MXCO_API mxco_status mxco_server_resident_trace(mxco_server* server, uint64_t* out, uint32_t max_ticks, uint32_t* n,
+                                                double* ticks_per_us);
 
 /* ---------------------------------------------------------------------- */
 /* BloomFilter (include/bloom_filter.h:37-133) under                      */

@@ -37,6 +37,7 @@
 #include "batch_kernels.h"
 #include "multimap_batch.h"
 #include "mxco_common.h"
+#include "serve_resident.h"
 
 using namespace mxco;
 
@@ -65,9 +66,9 @@ struct Table {
     bool live = false;
 };
 
-constexpr int kTickSlots = 4;
+constexpr int kTickSlots = kServeSlots;
 constexpr int kMmWorkers = 1;  // StaticMultimap tick threads (and streams)
-constexpr uint32_t kMaxTickRequests = 4096;
+constexpr uint32_t kMaxTickRequests = kServeMaxItems;
 constexpr uint64_t kMaxMmTickKeys = 1ull << 22;  // multimap queries per tick (one larger request runs alone)
 
 struct Tick {
@@ -83,6 +84,7 @@ struct Tick {
     std::vector<void*> retired_host, retired_dev;
     std::vector<Req*> reqs;
     std::vector<uint32_t> insert_items;  // request index of each insert item
+    uint64_t seq = 0;                    // resident mode: the tick's number
     bool busy = false;
 };
 
@@ -108,24 +110,57 @@ struct mxco_server {
     std::thread dispatcher, completer, mm_workers[kMmWorkers];
     bool stop = false;
     std::string worker_error;
+    // Resident mode: ticks go to the resident kernel through the mailbox
+    // (serve_resident.h) instead of one launch pair each.  The completion
+    // thread (re)launches the kernel when a posted tick is unfinished and no
+    // launch is running; res_* below are guarded by `mu`.
+    bool resident = false;
+    ServeMailbox* mb = nullptr;  // host-coherent
+    ServeCtl* ctl = nullptr;     // device
+    hipStream_t res_stream = nullptr;  // CU-masked over every CU: a hardware queue of its own
+    hipEvent_t res_done = nullptr;
+    bool res_running = false;
+    uint64_t res_posted = 0;  // seq of the last posted tick
+    uint64_t res_launches = 0;
+    unsigned res_grid = 0;
+    uint64_t res_idle = 0, res_life = 0, res_hard = 0;  // in wall-clock ticks
+    double res_ticks_per_us = 100.0;
+    uint32_t res_group = 1;  // lanes per key of the resident kernel
     // statistics
     uint64_t n_ticks = 0, n_tick_requests = 0;
 };
 
 namespace {
 
-mxco_status grow_tick(Tick& t, size_t bytes, uint32_t counters)
+// A stream with a hardware queue of its own, for the resident kernel: HIP
+// gives a stream with a CU mask (here every CU) its own queue, while
+// ordinary streams share the process's few queues, so a kernel queued on a
+// stream that shares the resident kernel's queue would wait until it exits.
+// Such a stream is a blocking one: work on the legacy default stream is
+// ordered with it.
+hipError_t queue_stream(hipStream_t* st)
+{
+    std::vector<uint32_t> mask(((unsigned)cu_count() + 31) / 32, 0xffffffffu);
+    return hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data());
+}
+
+mxco_status grow_tick(Tick& t, size_t bytes, uint32_t counters, bool resident)
 {
     // Sized at create for a full tick; a tick grows only for one request
     // larger than max_tick_keys.  The slot is idle here (its previous tick
-    // retired), but the old buffers are only freed at destroy.
+    // retired), but the old buffers are only freed at destroy.  The resident
+    // kernel reads the request list and writes the counters in place, so in
+    // that mode both are host-coherent and there is no device copy.
+    const unsigned host_flags = resident ? hipHostMallocCoherent : 0;
     if (bytes > t.blob_cap) {
         const size_t cap = std::max(bytes, 2 * t.blob_cap);
         uint8_t *h = nullptr, *d = nullptr;
-        MXCO_HIP(hipHostMalloc((void**)&h, cap, 0), "server: tick buffers");
-        if (hipError_t e = hipMalloc((void**)&d, cap); e != hipSuccess) {
-            (void)hipHostFree(h);
-            return hip_fail(e, "server: tick buffers");
+        MXCO_HIP(hipHostMalloc((void**)&h, cap, host_flags), "server: tick buffers");
+        if (!resident) {
+            if (hipError_t e = hipMalloc((void**)&d, cap); e != hipSuccess) {
+                (void)hipHostFree(h);
+                return hip_fail(e, "server: tick buffers");
+            }
         }
         if (t.blob_host) t.retired_host.push_back(t.blob_host);
         if (t.blob_dev) t.retired_dev.push_back(t.blob_dev);
@@ -136,7 +171,7 @@ mxco_status grow_tick(Tick& t, size_t bytes, uint32_t counters)
     if (counters > t.counters_cap) {
         const uint32_t cap = std::max(counters, 2 * t.counters_cap);
         unsigned long long* h = nullptr;
-        MXCO_HIP(hipHostMalloc((void**)&h, cap * sizeof(unsigned long long), 0), "server: counters");
+        MXCO_HIP(hipHostMalloc((void**)&h, cap * sizeof(unsigned long long), host_flags), "server: counters");
         if (t.counters_host) t.retired_host.push_back(t.counters_host);
         t.counters_host = h;
         t.counters_cap = cap;
@@ -217,7 +252,7 @@ mxco_status launch_tick(mxco_server* s, Tick& t)
     const uint32_t ninsert = (uint32_t)ins.size();
     const TickLayout lay = tick_layout(nitems, nchunks, ninsert);
     const size_t off_chunks = lay.off_chunks, off_counters = lay.off_counters, bytes = lay.bytes;
-    if (mxco_status e = grow_tick(t, bytes, 2 * ninsert + 2)) return e;
+    if (mxco_status e = grow_tick(t, bytes, 2 * ninsert + 2, false)) return e;
     ServeItem* items = (ServeItem*)t.blob_host;
     BatchChunk* chunks = (BatchChunk*)(t.blob_host + off_chunks);
     memset(t.blob_host + off_counters, 0, 2ull * ninsert * sizeof(unsigned long long));
@@ -259,6 +294,88 @@ mxco_status launch_tick(mxco_server* s, Tick& t)
                  "server: read counters");
     MXCO_HIP(hipEventRecord(t.done, st), "server: record");
     return MXCO_OK;
+}
+
+// Keys per chunk of a resident tick phase of `keys` keys: about two chunks
+// per worker, at least one key per lane group of a worker, at most kChunk.
+uint32_t resident_chunk(uint64_t keys, unsigned workers, uint32_t groups)
+{
+    const uint64_t target = keys / (2ull * std::max(1u, workers));
+    uint32_t c = groups;
+    while (c < kChunk && c * 2 <= target) c *= 2;
+    return c;
+}
+
+// Resident mode: writes one tick's request list (inserts first, each with
+// its keys per chunk and first chunk) into its host-coherent blob and posts
+// the tick into the mailbox; the resident kernel's leader copies it in.
+mxco_status post_tick(mxco_server* s, Tick& t)
+{
+    std::vector<uint32_t> ins, rd;
+    uint64_t ins_keys = 0, rd_keys = 0;
+    for (uint32_t i = 0; i < t.reqs.size(); ++i) {
+        const bool insert = t.reqs[i]->r.op == MXCO_REQ_INSERT;
+        (insert ? ins : rd).push_back(i);
+        (insert ? ins_keys : rd_keys) += t.reqs[i]->r.n;
+    }
+    const unsigned workers = s->res_grid - 1;
+    const uint32_t groups = kServeThreads / s->res_group;
+    const uint32_t csz_ins = resident_chunk(ins_keys, workers, groups), csz_rd = resident_chunk(rd_keys, workers, groups);
+    const uint32_t nreq = (uint32_t)t.reqs.size();
+    // the kernel's leader reads whole 16-byte words
+    if (mxco_status e = grow_tick(t, nreq * sizeof(ServeReq) + 16, 2 * (uint32_t)ins.size() + 2, true)) return e;
+    ServeReq* reqs = (ServeReq*)t.blob_host;
+    uint64_t chunk = 0;
+    uint32_t n_ins_chunks = 0, item = 0;
+    t.insert_items.clear();
+    for (int k = 0; k < 2; ++k) {
+        if (k == 1) n_ins_chunks = (uint32_t)chunk;
+        const uint32_t csz = k == 0 ? csz_ins : csz_rd;
+        for (uint32_t ri : (k == 0 ? ins : rd)) {
+            const mxco_request& r = t.reqs[ri]->r;
+            if (k == 0) t.insert_items.push_back(ri);
+            reqs[item++] = ServeReq{(const uint8_t*)r.in, (uint8_t*)r.out, r.n, r.table, r.op, csz, (uint32_t)chunk};
+            chunk += (r.n + csz - 1) / csz;
+            if (chunk >= kServeMaxChunks)
+                return fail(MXCO_LENGTH_ERROR, "server: tick too large for the resident kernel");
+        }
+    }
+    t.seq = ++s->res_posted;
+    ServeTickDesc& d = s->mb->desc[t.seq % kServeSlots];
+    d.reqs = reqs;
+    d.counters_host = t.counters_host;
+    d.seq = t.seq;
+    d.n_reqs = nreq;
+    d.n_insert_items = (uint32_t)ins.size();
+    d.n_ins_chunks = n_ins_chunks;
+    d.n_rd_chunks = (uint32_t)chunk - n_ins_chunks;
+    __atomic_store_n(&s->mb->posted, t.seq, __ATOMIC_RELEASE);
+    return MXCO_OK;
+}
+
+// Resident mode, under `mu`: starts the resident kernel at tick seq_base + 1.
+mxco_status res_launch(mxco_server* s, uint64_t seq_base)
+{
+    MXCO_HIP(hipMemsetAsync(s->ctl, 0, kServeCtlHeader, s->res_stream), "server: resident reset");
+    MXCO_HIP(launch_serve_resident(s->map_desc.key_size, s->map_desc.value_size, s->res_group, map_args(s),
+                                   s->tables_dev, s->mb, s->ctl, seq_base, s->res_idle, s->res_life, s->res_hard,
+                                   s->res_grid, s->res_stream),
+             "server: resident launch");
+    MXCO_HIP(hipEventRecord(s->res_done, s->res_stream), "server: resident record");
+    s->res_running = true;
+    ++s->res_launches;
+    return MXCO_OK;
+}
+
+// Resident mode, under `mu`, with no tick in flight: ends the resident
+// kernel (it exits at once between ticks) and waits for it.
+void res_quiesce(mxco_server* s)
+{
+    if (!s->resident || !s->res_running) return;
+    __atomic_store_n(&s->mb->stop, (uint64_t)1, __ATOMIC_RELEASE);
+    (void)hipEventSynchronize(s->res_done);
+    __atomic_store_n(&s->mb->stop, (uint64_t)0, __ATOMIC_RELEASE);
+    s->res_running = false;
 }
 
 void finish(mxco_server* s, Req* r, uint64_t result, mxco_status st)
@@ -322,7 +439,7 @@ void dispatcher_loop(mxco_server* s)
         ++s->n_ticks;
         s->n_tick_requests += t->reqs.size();
         lk.unlock();
-        mxco_status st = launch_tick(s, *t);
+        mxco_status st = s->resident ? post_tick(s, *t) : launch_tick(s, *t);
         lk.lock();
         if (st != MXCO_OK) {
             for (Req* r : t->reqs) finish(s, r, 0, st);
@@ -333,6 +450,35 @@ void dispatcher_loop(mxco_server* s)
         }
         s->inflight.push_back(t);
         s->cv_done.notify_all();  // wakes the completer
+    }
+}
+
+// Resident mode: waits for the tick's done word (polling without the lock)
+// and, whenever the resident kernel is not running while the tick is
+// unfinished, launches it at this tick (every earlier tick has finished:
+// ticks retire in order).  Called and returns with `lk` held.
+hipError_t res_wait(mxco_server* s, Tick& t, std::unique_lock<std::mutex>& lk)
+{
+    const uint64_t* done = &s->mb->done[t.seq % kServeSlots];
+    for (;;) {
+        if (__atomic_load_n(done, __ATOMIC_ACQUIRE) == t.seq) return hipSuccess;
+        if (s->res_running) {
+            const hipError_t q = hipEventQuery(s->res_done);
+            if (q != hipSuccess && q != hipErrorNotReady) return q;
+            if (q == hipSuccess) s->res_running = false;
+        }
+        if (!s->res_running) {
+            // the last launch has ended: its ticks are finished, or it gave this one up
+            if (__atomic_load_n(done, __ATOMIC_ACQUIRE) == t.seq) return hipSuccess;
+            if (__atomic_load_n(&s->mb->aborted, __ATOMIC_ACQUIRE)) {
+                __atomic_store_n(&s->mb->aborted, (uint64_t)0, __ATOMIC_RELEASE);
+                return hipErrorLaunchTimeOut;
+            }
+            if (res_launch(s, t.seq - 1) != MXCO_OK) return hipErrorLaunchFailure;
+        }
+        lk.unlock();
+        for (int i = 0; i < 256 && __atomic_load_n(done, __ATOMIC_ACQUIRE) != t.seq; ++i) std::this_thread::yield();
+        lk.lock();
     }
 }
 
@@ -347,13 +493,20 @@ void completer_loop(mxco_server* s)
             continue;
         }
         Tick* t = s->inflight.front();
-        lk.unlock();
-        hipError_t e;
-        // poll: completion is noticed within microseconds of the event
-        while ((e = hipEventQuery(t->done)) == hipErrorNotReady) std::this_thread::yield();
-        lk.lock();
+        hipError_t e = hipSuccess;
+        if (s->resident) {
+            e = res_wait(s, *t, lk);
+        } else {
+            lk.unlock();
+            // poll: completion is noticed within microseconds of the event
+            while ((e = hipEventQuery(t->done)) == hipErrorNotReady) std::this_thread::yield();
+            lk.lock();
+        }
         s->inflight.pop_front();
         const double tnow = now_us();
+        if (e != hipSuccess)
+            s->worker_error = e == hipErrorLaunchTimeOut ? "resident kernel: a tick exceeded its deadline"
+                                                         : std::string("tick failed: ") + hipGetErrorString(e);
         for (Req* r : t->reqs) {
             r->t_done = tnow;
             r->status = e == hipSuccess ? MXCO_OK : MXCO_RUNTIME_ERROR;
@@ -485,11 +638,17 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
     s->cfg = *cfg;
     s->map_desc = *map_desc;
     if (s->map_desc.group_size == 0) s->map_desc.group_size = 4;
+    // The resident kernel keeps two waves per SIMD, so by default it walks
+    // one key per lane (the most keys in flight); launched ticks take the
+    // tables' lane groups.
+    s->res_group = s->cfg.group_size ? s->cfg.group_size : 1;
     if (!s->cfg.group_size) s->cfg.group_size = s->map_desc.group_size;
     if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 64;
     if (!s->cfg.max_tick_keys) s->cfg.max_tick_keys = 1ull << 24;
     (void)hipGetDevice(&s->device);
     s->tables_cap = 4096;
+    const int occ = serve_resident_occupancy(s->map_desc.key_size, s->map_desc.value_size, s->res_group);
+    s->resident = s->cfg.tick_mode == 0 && env_int("MXCO_SERVE_RESIDENT", 1) != 0 && occ > 0;
     hipError_t e = hipMalloc((void**)&s->tables_dev, s->tables_cap * sizeof(BatchTable));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->map_stream, hipStreamNonBlocking);
     for (int w = 0; w < kMmWorkers && e == hipSuccess; ++w) {
@@ -500,15 +659,44 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
     for (int w = 0; w < kMmWorkers && e == hipSuccess; ++w)
         if (mm_batch_ws_reserve(s->mm_ws[w], kMaxTickRequests, kMaxMmTickKeys) != MXCO_OK) e = hipErrorOutOfMemory;
     for (int i = 0; i < kTickSlots && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&s->ticks[i].done, hipEventDisableTiming);
+    // Resident mode (the default) for the layouts with a resident kernel;
+    // tick_mode 1 or MXCO_SERVE_RESIDENT=0 keeps one launch pair per tick.
+    if (s->resident && e == hipSuccess) {
+        e = hipHostMalloc((void**)&s->mb, sizeof(ServeMailbox), hipHostMallocCoherent);
+        if (e == hipSuccess) {
+            memset((void*)s->mb, 0, sizeof(ServeMailbox));
+            e = hipMalloc((void**)&s->ctl, sizeof(ServeCtl));
+        }
+        // a queue of its own: the running kernel never holds back the work of
+        // another stream; work on the legacy default stream waits for the
+        // kernel's idle exit
+        if (e == hipSuccess) e = queue_stream(&s->res_stream);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->res_done, hipEventDisableTiming);
+        int khz = 0;
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device);
+        // The leader and one 512-thread worker per CU, which leaves room on
+        // every SIMD for the multimap kernels and copies on other streams;
+        // resident_blocks_per_cu > 1 asks for more workers where they fit.
+        const unsigned per_cu = std::max(1u, std::min<unsigned>(s->cfg.resident_blocks_per_cu, (unsigned)occ));
+        s->res_grid = 1 + (unsigned)cu_count() * per_cu;
+        const double ticks_per_us = khz > 0 ? khz / 1000.0 : 100.0;
+        s->res_ticks_per_us = ticks_per_us;
+        s->res_idle = (uint64_t)(ticks_per_us * env_int("MXCO_SERVE_IDLE_US", 300));
+        s->res_life = (uint64_t)(ticks_per_us * 50000.0);  // relaunched every 50 ms under load
+        s->res_hard = (uint64_t)(ticks_per_us * 5e6);       // a tick open for 5 s is given up
+    }
     if (e != hipSuccess) {
         (void)mxco_server_destroy(s);
         return hip_fail(e, "server: create");
     }
     // every tick slot sized for a full tick up front: no allocation on the request path
-    const TickLayout full = tick_layout(kMaxTickRequests, tick_chunk_bound(s->cfg.max_tick_keys, kMaxTickRequests),
-                                        kMaxTickRequests);
+    const size_t full_bytes =
+        s->resident ? kMaxTickRequests * sizeof(ServeReq) + 16
+                    : tick_layout(kMaxTickRequests, tick_chunk_bound(s->cfg.max_tick_keys, kMaxTickRequests),
+                                  kMaxTickRequests)
+                          .bytes;
     for (Tick& t : s->ticks)
-        if (mxco_status st = grow_tick(t, full.bytes, 2 * kMaxTickRequests + 2)) {
+        if (mxco_status st = grow_tick(t, full_bytes, 2 * kMaxTickRequests + 2, s->resident)) {
             (void)mxco_server_destroy(s);
             return st;
         }
@@ -542,6 +730,7 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
     if (s->map_stream) (void)hipStreamSynchronize(s->map_stream);
     for (hipStream_t st : s->mm_streams)
         if (st) (void)hipStreamSynchronize(st);
+    res_quiesce(s);
     for (Table& t : s->tables) {
         if (t.map) (void)mxco_oa_destroy(t.map);
         if (t.mm) (void)mxco_mm_destroy(t.mm);
@@ -555,6 +744,11 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* s)
         if (t.done) (void)hipEventDestroy(t.done);
     }
     for (auto& kv : s->reqs) delete kv.second;
+    if (s->res_stream) (void)hipStreamSynchronize(s->res_stream);
+    if (s->res_done) (void)hipEventDestroy(s->res_done);
+    if (s->res_stream) (void)hipStreamDestroy(s->res_stream);
+    if (s->ctl) (void)hipFree(s->ctl);
+    if (s->mb) (void)hipHostFree(s->mb);
     if (s->tables_dev) (void)hipFree(s->tables_dev);
     if (s->map_stream) (void)hipStreamDestroy(s->map_stream);
     for (hipStream_t st : s->mm_streams)
@@ -575,6 +769,12 @@ MXCO_API mxco_status mxco_server_add_map(mxco_server* s, uint64_t capacity, uint
     mxco_oa* m = nullptr;
     if (mxco_status e = mxco_oa_create(&s->map_desc, capacity, s->cfg.empty_key, s->cfg.empty_value, s->map_stream, &m))
         return e;
+    // the table's fill is done before any tick can reach it (the resident
+    // kernel runs on a stream of its own)
+    if (hipError_t e = hipStreamSynchronize(s->map_stream); e != hipSuccess) {
+        (void)mxco_oa_destroy(m);
+        return hip_fail(e, "server: add table");
+    }
     BatchTable bt;
     void* slots = nullptr;
     uint64_t c = 0;
@@ -584,8 +784,11 @@ MXCO_API mxco_status mxco_server_add_map(mxco_server* s, uint64_t capacity, uint
     bt.cap = (uint32_t)c;
     bt.nbuckets = (uint32_t)(c / s->map_desc.bucket_size);
     const uint32_t id = (uint32_t)s->tables.size();
-    // a new descriptor entry: no running kernel reads this index yet
-    MXCO_HIP(hipMemcpy(s->tables_dev + id, &bt, sizeof(bt), hipMemcpyHostToDevice), "server: add table");
+    // a new descriptor entry: no running kernel reads this index yet (on the
+    // map stream: the default stream would wait for the resident kernel)
+    MXCO_HIP(hipMemcpyAsync(s->tables_dev + id, &bt, sizeof(bt), hipMemcpyHostToDevice, s->map_stream),
+             "server: add table");
+    MXCO_HIP(hipStreamSynchronize(s->map_stream), "server: add table");
     Table t;
     t.kind = 0;
     t.map = m;
@@ -624,6 +827,7 @@ MXCO_API mxco_status mxco_server_drop_table(mxco_server* s, uint32_t table)
     (void)mxco_server_drain(s);
     std::lock_guard<std::mutex> lk(s->mu);
     if (table >= s->tables.size() || !s->tables[table].live) return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
+    res_quiesce(s);  // hipFree would wait for the resident kernel's idle exit
     Table& t = s->tables[table];
     if (t.map) (void)mxco_oa_destroy(t.map);
     if (t.mm) (void)mxco_mm_destroy(t.mm);
@@ -744,6 +948,46 @@ MXCO_API mxco_status mxco_server_stats(mxco_server* s, uint64_t* ticks, uint64_t
     return ok();
 }
 
+MXCO_API mxco_status mxco_server_resident_stats(mxco_server* s, uint32_t* resident, uint64_t* launches,
+                                                uint32_t* grid)
+{
+    if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (resident) *resident = s->resident;
+    if (launches) *launches = s->res_launches;
+    if (grid) *grid = s->res_grid;
+    return ok();
+}
+
+MXCO_API mxco_status mxco_server_resident_trace(mxco_server* s, uint64_t* out, uint32_t max_ticks, uint32_t* n,
+                                                double* ticks_per_us)
+{
+    if (!s || (max_ticks && !out)) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    if (mxco_status e = mxco_server_drain(s)) return e;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (n) *n = 0;
+    if (!s->resident) return ok();
+    res_quiesce(s);
+    const uint32_t k = (uint32_t)std::min<uint64_t>({(uint64_t)max_ticks, (uint64_t)kServeTrace, s->res_posted});
+    uint64_t* all = nullptr;
+    MXCO_HIP(hipHostMalloc((void**)&all, sizeof(uint64_t) * 4 * kServeTrace, 0), "server: trace");
+    hipError_t e = hipMemcpyAsync(all, s->ctl->trace, sizeof(uint64_t) * 4 * kServeTrace, hipMemcpyDeviceToHost,
+                                  s->map_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->map_stream);
+    if (e == hipSuccess) {
+        // the last k ticks, oldest first
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint64_t seq = s->res_posted - k + 1 + i;
+            memcpy(out + 4 * i, all + 4 * (seq % kServeTrace), 4 * sizeof(uint64_t));
+        }
+        if (n) *n = k;
+        if (ticks_per_us) *ticks_per_us = s->res_ticks_per_us;
+    }
+    (void)hipHostFree(all);
+    if (e != hipSuccess) return hip_fail(e, "server: trace");
+    return ok();
+}
+
 MXCO_API mxco_status mxco_server_table(mxco_server* s, uint32_t table, mxco_oa** map, mxco_mm** multimap)
 {
     if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
@@ -762,6 +1006,10 @@ MXCO_API mxco_status mxco_server_replay_plain(mxco_server* s, const mxco_request
 {
     if (!s || (n && (!reqs || !arrival_us))) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
     if (mxco_status e = mxco_server_drain(s)) return e;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        res_quiesce(s);  // the baseline runs without the resident kernel holding CUs
+    }
     hipStream_t st = s->map_stream;
     const double t0 = now_us();
     for (uint32_t i = 0; i < n; ++i) {

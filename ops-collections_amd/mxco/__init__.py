@@ -1099,8 +1099,8 @@ class Graph:
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_reserve_bytes", u64), ("hbm_budget_bytes", u64), ("max_tick_keys", u64),
-                ("group_size", u32), ("blocks_per_cu", u32), ("empty_key", ctypes.c_uint8 * 8),
-                ("empty_value", ctypes.c_uint8 * 8)]
+                ("group_size", u32), ("blocks_per_cu", u32), ("tick_mode", u32), ("resident_blocks_per_cu", u32),
+                ("empty_key", ctypes.c_uint8 * 8), ("empty_value", ctypes.c_uint8 * 8)]
 
 
 class Request(ctypes.Structure):
@@ -1122,6 +1122,8 @@ _sig("mxco_server_replay", [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c
 _sig("mxco_server_replay_plain", [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), u32, pu64,
                                   ctypes.POINTER(ctypes.c_double)])
 _sig("mxco_server_stats", [vp, pu64, pu64])
+_sig("mxco_server_resident_stats", [vp, ctypes.POINTER(u32), pu64, ctypes.POINTER(u32)])
+_sig("mxco_server_resident_trace", [vp, pu64, u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)])
 
 
 class Server:
@@ -1130,11 +1132,13 @@ class Server:
     threads)."""
 
     def __init__(self, empty_key=-1, empty_value=-1, key_dtype=torch.int64, value_dtype=torch.int64,
-                 hbm_reserve_bytes=1 << 30, hbm_budget_bytes=0, max_tick_keys=0, group_size=0, blocks_per_cu=0):
+                 hbm_reserve_bytes=1 << 30, hbm_budget_bytes=0, max_tick_keys=0, group_size=0, blocks_per_cu=0,
+                 resident=True, resident_blocks_per_cu=0):
         cfg = ServerConfig()
         cfg.hbm_reserve_bytes, cfg.hbm_budget_bytes, cfg.max_tick_keys = hbm_reserve_bytes, hbm_budget_bytes, \
             max_tick_keys
         cfg.group_size, cfg.blocks_per_cu = group_size, blocks_per_cu
+        cfg.tick_mode, cfg.resident_blocks_per_cu = (0 if resident else 1), resident_blocks_per_cu
         ek = _scalar_bytes(empty_key, key_dtype).raw
         ev = _scalar_bytes(empty_value, value_dtype).raw
         for j in range(len(ek)):
@@ -1207,6 +1211,32 @@ class Server:
         a, b = u64(), u64()
         check(_lib.mxco_server_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def resident_trace(self, max_ticks=1024):
+        """Timing trace of the last StaticMap ticks served by the resident
+        kernel (drains and ends the kernel first): per tick the microseconds
+        from finding it posted to opening it, from opening it to publishing
+        its results, from the previous tick's publication to finding this
+        one, and its insert and read chunk counts."""
+        import numpy as np
+        buf = (u64 * (4 * max_ticks))()
+        n, tpu = u32(), ctypes.c_double()
+        check(_lib.mxco_server_resident_trace(self._h, buf, max_ticks, ctypes.byref(n), ctypes.byref(tpu)))
+        a = np.frombuffer(buf, dtype=np.uint64, count=4 * n.value).reshape(-1, 4).astype(np.float64)
+        if not len(a):
+            return {}
+        seen, opened, done = a[:, 0], a[:, 1], a[:, 2]
+        cnt = a[:, 3].astype(np.uint64)
+        return {"open_us": (opened - seen) / tpu.value, "run_us": (done - opened) / tpu.value,
+                "gap_us": (seen[1:] - done[:-1]) / tpu.value,
+                "ins_chunks": (cnt >> np.uint64(32)).astype(np.int64),
+                "rd_chunks": (cnt & np.uint64(0xffffffff)).astype(np.int64)}
+
+    def resident_stats(self):
+        """(StaticMap ticks run in the resident kernel, its launches so far, its grid)."""
+        r, n, g = u32(), u64(), u32()
+        check(_lib.mxco_server_resident_stats(self._h, ctypes.byref(r), ctypes.byref(n), ctypes.byref(g)))
+        return bool(r.value), n.value, g.value
 
 
 # ---------------------------------------------------------------------------

@@ -56,12 +56,14 @@ def plan(w, tables, outs):
     return reqs
 
 
+@pytest.mark.parametrize("resident", [True, False], ids=["resident", "launched"])
 @pytest.mark.parametrize("qps", [2000.0, 200000.0])
-def test_batched_serving_equals_one_by_one(qps):
+def test_batched_serving_equals_one_by_one(qps, resident):
     from mxco import workloads
     w = workloads.serving(num_maps=6, num_multimaps=2, slots_log2=(12, 16), num_requests=300, qps=qps,
                           batch_log2=(0, 10), seed=11)
-    s1, t1 = build(w, max_tick_keys=1 << 14)
+    s1, t1 = build(w, max_tick_keys=1 << 14, resident=resident)
+    assert s1.resident_stats()[0] == resident
     s2, t2 = build(w)
     o1, o2 = outputs(w), outputs(w)
     r1, lat1 = s1.replay(plan(w, t1, o1), w["arrivals_us"])
@@ -233,4 +235,65 @@ def test_admission_by_budget():
         s.add_map(1 << 23)  # 128 MiB: over the budget
     t = s.add_map(1 << 20)
     s.drop_table(t)
+    s.close()
+
+
+def test_resident_kernel_serves_many_ticks_per_launch():
+    """Closed loop: every StaticMap tick goes through the resident kernel's
+    mailbox, one launch serves many ticks, and the results equal the
+    launched ticks' and the workload's closed form (a read's hit flag holds
+    whatever the inserts: present keys come from the initial fill, absent
+    ones from indices no insert reaches)."""
+    from mxco import workloads
+    w = workloads.serving(num_maps=4, num_multimaps=1, slots_log2=(14, 16), num_requests=400, qps=1e6,
+                          batch_log2=(0, 12), seed=23)
+    runs = []
+    for resident in (True, False):
+        s, t = build(w, resident=resident)
+        o = outputs(w)
+        res, _ = s.replay(plan(w, t, o), [0.0] * len(w["requests"]))
+        ticks, _ = s.stats()
+        on, launches, grid = s.resident_stats()
+        assert on == resident
+        if resident:
+            assert 1 <= launches < ticks, (launches, ticks)
+            assert grid > 1
+        runs.append((res, [(a.cpu() if a is not None else None, b.cpu() if b is not None else None) for a, b in o]))
+        s.close()
+    assert runs[0][0] == runs[1][0]
+    for (a, a2), (b, b2) in zip(runs[0][1], runs[1][1]):
+        if a is not None:
+            assert torch.equal(a, b)
+        if a2 is not None:
+            assert torch.equal(a2, b2)
+    for req, (a, _) in zip(w["requests"], runs[0][1]):
+        if req["op"] not in ("find", "contains"):
+            continue
+        hit, inp = req["hit"].cpu(), req["input"].cpu()
+        if req["op"] == "find":
+            assert torch.equal(a, torch.where(hit, inp * 7 + 1, torch.full_like(a, -1)))
+        elif req["op"] == "contains":
+            assert torch.equal(a.bool(), hit)
+
+
+def test_resident_kernel_relaunches_after_idle_exit():
+    """The resident kernel ends after 2 ms without a tick; the next request
+    launches it again and is answered like the first."""
+    import time
+    import mxco
+    s = mxco.Server()
+    t = s.add_map(1 << 12)
+    keys = torch.arange(1, 1001, dtype=torch.int64, device="cuda")
+    pairs = torch.stack([keys, keys * 7 + 1], 1).contiguous()
+    assert s.wait(s.submit(t, mxco.REQ_INSERT, pairs))[0] == 0
+    launches0 = s.resident_stats()[1]
+    assert launches0 >= 1
+    q = torch.arange(500, 1501, dtype=torch.int64, device="cuda")
+    exp = torch.where(q <= 1000, q * 7 + 1, torch.full_like(q, -1))
+    for rep in range(3):
+        time.sleep(0.05)  # well past the idle limit
+        out = torch.full_like(q, 3)
+        s.wait(s.submit(t, mxco.REQ_FIND, q, out=out))
+        assert torch.equal(out, exp), rep
+    assert s.resident_stats()[1] >= launches0 + 3
     s.close()
