@@ -684,6 +684,7 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
     def serve(w, mode, arrivals):
         # "batched": the default server (StaticMap ticks in the resident kernel);
         # "launched": one insert and one read launch per tick; "plain": one by one
+        torch.cuda.synchronize()  # requests carry device pointers: their inputs are written before submit
         srv = mxco.Server(resident=(mode != "launched"))
         tabs = [srv.add_map(c) if t < w["num_maps"] else srv.add_multimap(c) for t, c in enumerate(w["caps"])]
         for t, pairs in enumerate(w["init"]):

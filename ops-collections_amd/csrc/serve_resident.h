@@ -68,13 +68,16 @@ constexpr uint32_t kServeMaxItems = 4096;      // requests per tick
 constexpr uint32_t kServeMaxChunks = 1u << 23; // chunks per tick (the claim word's 24-bit fields)
 constexpr uint32_t kServeMaxEpochs = 60000;    // ticks per launch (the claim word's 16-bit epoch)
 constexpr uint32_t kServeTrace = 1024;         // ticks kept in the timing trace
+// ServeReq::op flag of a read whose table has an insert in the same tick:
+// its chunks wait for the tick's inserts (other reads start at once).
+constexpr uint32_t kServeAfterInsert = 0x80000000u;
 
 // One request of a tick; its chunks are [first, first + ceil(count / csz)).
 struct ServeReq {
     const uint8_t* in;  // device input (pairs or keys)
     uint8_t* out;       // device output (reads)
     uint64_t count;     // keys
-    uint32_t table, op; // op: MXCO_REQ_INSERT, MXCO_REQ_FIND or MXCO_REQ_CONTAINS
+    uint32_t table, op; // op: MXCO_REQ_INSERT, MXCO_REQ_FIND or MXCO_REQ_CONTAINS (| kServeAfterInsert)
     uint32_t csz;       // keys per chunk
     uint32_t first;     // the request's first chunk in the tick
 };
@@ -111,8 +114,10 @@ struct ServeCtl {
     __attribute__((aligned(128))) uint32_t finished;  // last epoch whose results are published
     uint32_t exit;
     __attribute__((aligned(128))) ServeTickDesc cur;  // descriptor of the open epoch
-    unsigned long long counters[2 * kServeMaxItems];
-    __attribute__((aligned(16))) ServeReq reqs[kServeMaxItems];  // the open epoch's request list
+    // Per epoch parity: the insert counters and the request list.  The leader
+    // stages the next tick into the other parity while the open one runs.
+    unsigned long long counters[2][2 * kServeMaxItems];
+    __attribute__((aligned(16))) ServeReq reqs[2][kServeMaxItems];
     // Timing trace of the last kServeTrace ticks, by seq % kServeTrace (wall
     // clock): the leader found the tick posted, opened it, the finisher
     // published it; [insert chunks:32 | read chunks:32].
@@ -150,33 +155,43 @@ __device__ __forceinline__ T srv_copy_sys(const T* p)
     return v;
 }
 
-// Workgroup 0: publishes posted ticks one at a time and decides the exit.
+// Workgroup 0: stages the next posted tick (descriptor and request list)
+// while the open one runs, opens it the moment the open one finishes, and
+// decides the exit.
 __device__ __forceinline__ void serve_lead(ServeMailbox* mb, ServeCtl* ctl, uint64_t seq_base, uint64_t idle,
                                            uint64_t life, uint64_t hard)
 {
-    __shared__ uint32_t s_act;
+    __shared__ uint32_t s_act, s_par;
     __shared__ ServeTickDesc s_d;
-    uint32_t e = 0;  // open epoch (thread 0)
+    uint32_t e = 0;       // open epoch (thread 0)
+    bool staged = false;  // epoch e + 1 is staged (thread 0)
+    ServeTickDesc next{};  // its descriptor (thread 0)
     const uint64_t t0 = wall_clock64();
     uint64_t last = t0, opened = t0;  // last moment a tick was open; when the open tick was published
     for (;;) {
         if (threadIdx.x == 0) {
-            uint32_t act = 0;  // 0 wait, 1 publish, 2 exit, 3 abort
+            uint32_t act = 0;  // 0 wait, 1 stage, 2 open, 3 exit, 4 abort
             const uint64_t now = wall_clock64();
             // relaxed polls: an acquire here would invalidate this XCD's L2 on every poll
-            if (srv_peek(&ctl->finished) == e) {
-                if (srv_peek_sys(&mb->stop) || e >= kServeMaxEpochs) act = 2;
-                else if (srv_peek_sys(&mb->posted) > seq_base + e) act = 1;
-                else if (now - last > idle || now - t0 > life) act = 2;
-            } else {
-                last = now;
-                if (now - opened > hard) act = 3;
+            const bool idle_now = srv_peek(&ctl->finished) == e;
+            if (!idle_now) last = now;
+            if (!staged && e + 1 < kServeMaxEpochs && srv_peek_sys(&mb->posted) > seq_base + e) {
+                act = 1;
+            } else if (idle_now) {
+                if (staged) act = 2;
+                else if (srv_peek_sys(&mb->stop) || e + 1 >= kServeMaxEpochs) act = 3;
+                else if (now - last > idle || now - t0 > life) act = 3;
+            } else if (now - opened > hard) {
+                act = 4;
             }
             if (act == 1) {
-                // the finisher's reads of the counters, and the host's writes, before the rewrite
+                // the host's descriptor and list; the parity's counters and list
+                // were last read by the finisher and workers of epoch e - 1
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
-                s_d = srv_copy_sys(&mb->desc[(seq_base + e + 1) % kServeSlots]);
-                ctl->trace[s_d.seq % kServeTrace][0] = now;
+                next = srv_copy_sys(&mb->desc[(seq_base + e + 1) % kServeSlots]);
+                s_d = next;
+                s_par = (e + 1) & 1;
+                ctl->trace[next.seq % kServeTrace][0] = now;
             }
             s_act = act;
         }
@@ -185,38 +200,44 @@ __device__ __forceinline__ void serve_lead(ServeMailbox* mb, ServeCtl* ctl, uint
         if (act == 1) {
             // the request list: coalesced 16-byte loads from host memory
             const ServeTickDesc d = s_d;
+            const uint32_t par = s_par;  // thread 0 alone counts the epochs
             const uint32_t n16 = (d.n_reqs * (uint32_t)sizeof(ServeReq) + 15) / 16;
             const uint4* src = reinterpret_cast<const uint4*>(d.reqs);
-            uint4* dst = reinterpret_cast<uint4*>(ctl->reqs);
+            uint4* dst = reinterpret_cast<uint4*>(ctl->reqs[par]);
             for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
-            for (uint32_t i = threadIdx.x; i < 2 * d.n_insert_items; i += blockDim.x) ctl->counters[i] = 0;
+            for (uint32_t i = threadIdx.x; i < 2 * d.n_insert_items; i += blockDim.x) ctl->counters[par][i] = 0;
             __syncthreads();
-            if (threadIdx.x == 0) {
-                ++e;
-                ctl->cur = d;
-                ctl->done_writers = 0;
-                ctl->ins_done = 0;
-                // the list, descriptor, counters and resets reach every XCD before the epoch opens
-                const uint64_t claim =
-                    ((uint64_t)e << 48) | ((uint64_t)(d.n_ins_chunks + d.n_rd_chunks) << 24);
-                last = opened = wall_clock64();
-                ctl->trace[d.seq % kServeTrace][1] = opened;
-                ctl->trace[d.seq % kServeTrace][3] = ((uint64_t)d.n_ins_chunks << 32) | d.n_rd_chunks;
-                __hip_atomic_store(&ctl->claim, claim, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            __syncthreads();
+            if (threadIdx.x == 0) staged = true;
             continue;
         }
         __syncthreads();  // every wave has read s_act before thread 0 writes it again
-        if (act >= 2) {
+        if (act == 2) {
             if (threadIdx.x == 0) {
-                if (act == 3)
+                ++e;
+                staged = false;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the finisher's reads before the resets
+                ctl->cur = next;
+                ctl->done_writers = 0;
+                ctl->ins_done = 0;
+                last = opened = wall_clock64();
+                ctl->trace[next.seq % kServeTrace][1] = opened;
+                ctl->trace[next.seq % kServeTrace][3] = ((uint64_t)next.n_ins_chunks << 32) | next.n_rd_chunks;
+                // the staged list, descriptor, counters and resets reach every XCD before the epoch opens
+                const uint64_t claim =
+                    ((uint64_t)e << 48) | ((uint64_t)(next.n_ins_chunks + next.n_rd_chunks) << 24);
+                __hip_atomic_store(&ctl->claim, claim, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            continue;
+        }
+        if (act >= 3) {
+            if (threadIdx.x == 0) {
+                if (act == 4)
                     __hip_atomic_store(&mb->aborted, (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
                 __hip_atomic_store(&ctl->exit, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
             return;
         }
-        __builtin_amdgcn_s_sleep(8);
+        __builtin_amdgcn_s_sleep(4);
     }
 }
 
@@ -250,7 +271,7 @@ __device__ __forceinline__ void serve_leave(ServeMailbox* mb, ServeCtl* ctl, con
     if (left != 0 || done != d.n_ins_chunks + d.n_rd_chunks) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
     for (uint32_t i = 0; i < 2 * d.n_insert_items; ++i)
-        __hip_atomic_store(d.counters_host + i, srv_peek(&ctl->counters[i]), __ATOMIC_RELAXED,
+        __hip_atomic_store(d.counters_host + i, srv_peek(&ctl->counters[epoch & 1][i]), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_SYSTEM);
     ctl->trace[d.seq % kServeTrace][2] = wall_clock64();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
@@ -264,7 +285,7 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
 {
     __shared__ uint32_t s_first[kServeMaxItems];  // first chunk of each request of the epoch
     __shared__ ServeReq s_req;
-    __shared__ uint32_t s_state, s_idx, s_nreq, s_nins, s_fresh;
+    __shared__ uint32_t s_state, s_idx, s_nreq, s_nins, s_fresh, s_epoch;
     // thread 0's state
     ServeTickDesc d{};
     uint32_t my_e = 0, ins_mine = 0, done_mine = 0, backoff = 1;
@@ -304,7 +325,7 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
                     joined = false;
                 }
                 for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
-                backoff = backoff < 32 ? 2 * backoff : 32;
+                backoff = backoff < 8 ? 2 * backoff : 8;
             }
             if (state == 1) {
                 if (!joined) {
@@ -312,26 +333,23 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
                                            __HIP_MEMORY_SCOPE_AGENT);
                     joined = true;
                 }
-                if (idx >= d.n_ins_chunks && !ins_seen) {
-                    // reads of the tick start after all of its inserts
-                    serve_release_inserts(ctl, ins_mine);
-                    while (srv_peek(&ctl->ins_done) < d.n_ins_chunks && !srv_peek(&ctl->exit))
-                        __builtin_amdgcn_s_sleep(2);
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    ins_seen = true;
-                }
+                // a read chunk: this workgroup's inserts of the tick are released for
+                // the reads that wait for them
+                if (idx >= d.n_ins_chunks) serve_release_inserts(ctl, ins_mine);
                 s_idx = idx;
                 s_nreq = d.n_reqs;
                 s_nins = d.n_ins_chunks;
                 s_fresh = fresh;
+                s_epoch = my_e;
             }
             s_state = state;
         }
         __syncthreads();
         if (s_state == 2) return;
         const uint32_t nreq = s_nreq;
+        const ServeReq* reqs = ctl->reqs[s_epoch & 1];
         if (s_fresh) {
-            for (uint32_t i = threadIdx.x; i < nreq; i += blockDim.x) s_first[i] = ctl->reqs[i].first;
+            for (uint32_t i = threadIdx.x; i < nreq; i += blockDim.x) s_first[i] = reqs[i].first;
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -343,8 +361,17 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
                 if (s_first[mid] <= idx) lo = mid;
                 else hi = mid;
             }
-            s_req = ctl->reqs[lo];
-            s_req.first = lo;  // from here on: the request's index (its insert counter pair)
+            ServeReq rq = reqs[lo];
+            if ((rq.op & kServeAfterInsert) && !ins_seen) {
+                // a read of a table with an insert in this tick: after all of the tick's inserts
+                while (srv_peek(&ctl->ins_done) < d.n_ins_chunks && !srv_peek(&ctl->exit))
+                    __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                ins_seen = true;
+            }
+            rq.op &= ~kServeAfterInsert;
+            rq.first = lo;  // from here on: the request's index (its insert counter pair)
+            s_req = rq;
         }
         __syncthreads();
         const uint32_t idx = s_idx;
@@ -364,7 +391,7 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
         unsigned long long c0 = 0, c1 = 0;
         if (ins) {
             oa_process<KS, VS, G, OP_INSERT>(a, start + gj, end, gpb, c0, c1);
-            flush_counters<OP_INSERT>(ctl->counters + 2ull * item, c0, c1);
+            flush_counters<OP_INSERT>(ctl->counters[s_epoch & 1] + 2ull * item, c0, c1);
         } else if (rq.op == MXCO_REQ_CONTAINS) {
             oa_process<KS, VS, G, OP_CONTAINS>(a, start + gj, end, gpb, c0, c1);
         } else {

@@ -318,6 +318,15 @@ mxco_status post_tick(mxco_server* s, Tick& t)
         (insert ? ins : rd).push_back(i);
         (insert ? ins_keys : rd_keys) += t.reqs[i]->r.n;
     }
+    // reads of tables without an insert in this tick first: they need not
+    // wait for the tick's inserts
+    std::vector<uint32_t> ins_tables;
+    for (uint32_t ri : ins) ins_tables.push_back(t.reqs[ri]->r.table);
+    std::sort(ins_tables.begin(), ins_tables.end());
+    auto after_insert = [&](uint32_t ri) {
+        return std::binary_search(ins_tables.begin(), ins_tables.end(), t.reqs[ri]->r.table);
+    };
+    std::stable_partition(rd.begin(), rd.end(), [&](uint32_t ri) { return !after_insert(ri); });
     const unsigned workers = s->res_grid - 1;
     const uint32_t groups = kServeThreads / s->res_group;
     const uint32_t csz_ins = resident_chunk(ins_keys, workers, groups), csz_rd = resident_chunk(rd_keys, workers, groups);
@@ -334,7 +343,8 @@ mxco_status post_tick(mxco_server* s, Tick& t)
         for (uint32_t ri : (k == 0 ? ins : rd)) {
             const mxco_request& r = t.reqs[ri]->r;
             if (k == 0) t.insert_items.push_back(ri);
-            reqs[item++] = ServeReq{(const uint8_t*)r.in, (uint8_t*)r.out, r.n, r.table, r.op, csz, (uint32_t)chunk};
+            const uint32_t op = r.op | (k == 1 && after_insert(ri) ? kServeAfterInsert : 0u);
+            reqs[item++] = ServeReq{(const uint8_t*)r.in, (uint8_t*)r.out, r.n, r.table, op, csz, (uint32_t)chunk};
             chunk += (r.n + csz - 1) / csz;
             if (chunk >= kServeMaxChunks)
                 return fail(MXCO_LENGTH_ERROR, "server: tick too large for the resident kernel");

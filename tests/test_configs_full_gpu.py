@@ -217,6 +217,7 @@ def test_cfg5_serving_64_tables_2p20_to_2p26_slots():
     assert min(w["caps"]) >= 1 << 20 and max(w["caps"]) <= 1 << 26
 
     def build():
+        torch.cuda.synchronize()  # the default stream's inputs are written before submit
         s = mxco.Server()
         tables = [s.add_map(c) if t < w["num_maps"] else s.add_multimap(c) for t, c in enumerate(w["caps"])]
         for t, pairs in enumerate(w["init"]):
@@ -246,10 +247,12 @@ def test_cfg5_serving_64_tables_2p20_to_2p26_slots():
 
     s1, t1 = build()
     o1 = outputs()
+    torch.cuda.synchronize()
     r1, lat = s1.replay(plan(t1, o1), w["arrivals_us"])
     s1.close()
     s2, t2 = build()
     o2 = outputs()
+    torch.cuda.synchronize()
     r2, _ = s2.replay(plan(t2, o2), w["arrivals_us"], plain=True)
     s2.close()
     assert r1 == r2

@@ -17,8 +17,16 @@ def mx():
 OPS = {"insert": 0, "find": 1, "contains": 2, "retrieve": 6}
 
 
+def ready():
+    """Requests carry device pointers: their inputs and outputs must be
+    written before submit.  The tensors here come from the default stream,
+    which does not order itself with the server's streams, so wait for it."""
+    torch.cuda.synchronize()
+
+
 def build(w, **cfg):
     import mxco
+    ready()
     s = mxco.Server(**cfg)
     tables = []
     for t, cap in enumerate(w["caps"]):
@@ -66,6 +74,7 @@ def test_batched_serving_equals_one_by_one(qps, resident):
     assert s1.resident_stats()[0] == resident
     s2, t2 = build(w)
     o1, o2 = outputs(w), outputs(w)
+    ready()
     r1, lat1 = s1.replay(plan(w, t1, o1), w["arrivals_us"])
     r2, lat2 = s2.replay(plan(w, t2, o2), w["arrivals_us"], plain=True)
     assert r1 == r2
@@ -93,6 +102,7 @@ def test_find_results_against_closed_form():
     w["arrivals_us"] = w["arrivals_us"][:len(w["requests"])]
     s, t = build(w)
     o = outputs(w)
+    ready()
     s.replay(plan(w, t, o), w["arrivals_us"])
     for req, (a, _) in zip(w["requests"], o):
         hit = req["hit"]
@@ -126,6 +136,7 @@ def test_same_table_inserts_keep_arrival_order():
         outs = torch.full((space,), 7, dtype=torch.int64, device="cuda")
         plan_ = [(t, mxco.REQ_INSERT, x.data_ptr(), per, None, None, 0) for x in inputs]
         plan_.append((t, mxco.REQ_FIND, q.data_ptr(), space, outs.data_ptr(), None, 0))
+        ready()
         res, _ = s.replay(plan_, [0.0] * len(plan_), plain=plain)  # all at once
         results.append((res, outs.cpu().numpy()))
         s.close()
@@ -183,6 +194,7 @@ def test_multimap_ticks_equal_plain_and_counts():
                 plan_.append((tabs[t], mxco.REQ_MM_FIND, q[t].data_ptr(), nq, f.data_ptr(), None, 0))
                 plan_.append((tabs[t], mxco.REQ_MM_COUNT, q[t].data_ptr(), nq, None, None, 0))
                 plan_.append((tabs[t], mxco.REQ_MM_RETRIEVE, q[t].data_ptr(), nq, po.data_ptr(), vo.data_ptr(), cap))
+        ready()
         res, _ = s.replay(plan_, [0.0] * len(plan_), plain=plain)
         results.append((res, [tuple(x.cpu().numpy() for x in o) for o in outs]))
         s.close()
@@ -210,6 +222,7 @@ def test_multimap_retrieve_too_small_fails_alone():
     s = mxco.Server()
     t = s.add_multimap(1024)
     pairs = torch.tensor([[5, 1], [5, 2], [5, 3], [6, 4]], dtype=torch.int64, device="cuda")
+    ready()
     s.wait(s.submit(t, mxco.REQ_MM_INSERT, pairs))
     k5 = torch.tensor([5], dtype=torch.int64, device="cuda")
     k6 = torch.tensor([6], dtype=torch.int64, device="cuda")
@@ -217,6 +230,7 @@ def test_multimap_retrieve_too_small_fails_alone():
     vo = torch.full((2,), 9, dtype=torch.int64, device="cuda")
     po6 = torch.zeros(2, dtype=torch.int64, device="cuda")
     vo6 = torch.zeros(2, dtype=torch.int64, device="cuda")
+    ready()
     bad = s.submit(t, mxco.REQ_MM_RETRIEVE, k5, out=po, out2=vo, out_capacity=2)
     good = s.submit(t, mxco.REQ_MM_RETRIEVE, k6, out=po6, out2=vo6, out_capacity=2)
     with pytest.raises(mxco.LengthError):
@@ -251,6 +265,7 @@ def test_resident_kernel_serves_many_ticks_per_launch():
     for resident in (True, False):
         s, t = build(w, resident=resident)
         o = outputs(w)
+        ready()
         res, _ = s.replay(plan(w, t, o), [0.0] * len(w["requests"]))
         ticks, _ = s.stats()
         on, launches, grid = s.resident_stats()
@@ -285,6 +300,7 @@ def test_resident_kernel_relaunches_after_idle_exit():
     t = s.add_map(1 << 12)
     keys = torch.arange(1, 1001, dtype=torch.int64, device="cuda")
     pairs = torch.stack([keys, keys * 7 + 1], 1).contiguous()
+    ready()
     assert s.wait(s.submit(t, mxco.REQ_INSERT, pairs))[0] == 0
     launches0 = s.resident_stats()[1]
     assert launches0 >= 1
@@ -293,6 +309,7 @@ def test_resident_kernel_relaunches_after_idle_exit():
     for rep in range(3):
         time.sleep(0.05)  # well past the idle limit
         out = torch.full_like(q, 3)
+        ready()
         s.wait(s.submit(t, mxco.REQ_FIND, q, out=out))
         assert torch.equal(out, exp), rep
     assert s.resident_stats()[1] >= launches0 + 3
