@@ -708,12 +708,20 @@ def run_serving(args, torch, mxco, qps_list=(20000.0, 100000.0, 200000.0), num_r
                          o1.numel() if r["op"] == "retrieve" else 0))
         torch.cuda.synchronize()
         launches0 = srv.resident_stats()[1]
+        ticks0 = srv.stats()[0]
         results, lat = srv.replay(reqs, arrivals, plain=(mode == "plain"))
         ticks, folded = srv.stats()
-        resident, launches, _ = srv.resident_stats()
+        resident, launches, grid = srv.resident_stats()
+        serve.kernel = {"resident_kernel": resident, "resident_launches": launches - launches0}
+        if resident and mode == "batched" and ticks > ticks0:
+            # the resident kernel's own timing of this replay's ticks
+            tr = srv.resident_trace(min(ticks - ticks0, 1024))
+            hand = tr["handoff_us"]  # ticks staged before the previous one finished
+            serve.kernel.update({"resident_grid": grid, "tick_run_us_p50": round(float(np.median(tr["run_us"])), 1),
+                                 "tick_handoff_us_p50": round(float(np.median(hand)), 2) if len(hand) else None,
+                                 "ticks_staged_ahead": int(len(hand))})
         srv.close()
         del srv
-        serve.kernel = {"resident_kernel": resident, "resident_launches": launches - launches0}
         return results, outs, np.array(lat) / 1e3, ticks, folded
 
     def same(a, b):

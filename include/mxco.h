@@ -537,9 +537,9 @@ MXCO_API mxco_status mxco_server_resident_stats(mxco_server* server, uint32_t* r
                                                 uint32_t* grid);
 /* Diagnostic: waits for outstanding requests, ends the resident kernel and
  * copies the timing trace of the last min(max_ticks, 1024) StaticMap ticks,
- * oldest first, 4 words each: wall-clock times at which the kernel found the
- * tick posted, opened it and published its results, then [insert chunks:32 |
- * read chunks:32]; ticks_per_us converts the times. */
+ * oldest first, 4 words each: wall-clock times at which the kernel began
+ * staging the tick, opened it and published its results, then [insert
+ * chunks:32 | read chunks:32]; ticks_per_us converts the times. */
 MXCO_API mxco_status mxco_server_resident_trace(mxco_server* server, uint64_t* out, uint32_t max_ticks, uint32_t* n,
                                                 double* ticks_per_us);
 

@@ -119,7 +119,7 @@ struct ServeCtl {
     unsigned long long counters[2][2 * kServeMaxItems];
     __attribute__((aligned(16))) ServeReq reqs[2][kServeMaxItems];
     // Timing trace of the last kServeTrace ticks, by seq % kServeTrace (wall
-    // clock): the leader found the tick posted, opened it, the finisher
+    // clock): the leader began staging the tick, opened it, the finisher
     // published it; [insert chunks:32 | read chunks:32].
     uint64_t trace[kServeTrace][4];
 };

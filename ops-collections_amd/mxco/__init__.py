@@ -1214,10 +1214,10 @@ class Server:
 
     def resident_trace(self, max_ticks=1024):
         """Timing trace of the last StaticMap ticks served by the resident
-        kernel (drains and ends the kernel first): per tick the microseconds
-        from finding it posted to opening it, from opening it to publishing
-        its results, from the previous tick's publication to finding this
-        one, and its insert and read chunk counts."""
+        kernel (drains and ends the kernel first), in microseconds: per tick
+        `run_us` from opening it to publishing its results; for each tick
+        that was staged before the previous one finished, `handoff_us` from
+        that finish to its opening; its insert and read chunk counts."""
         import numpy as np
         buf = (u64 * (4 * max_ticks))()
         n, tpu = u32(), ctypes.c_double()
@@ -1225,10 +1225,11 @@ class Server:
         a = np.frombuffer(buf, dtype=np.uint64, count=4 * n.value).reshape(-1, 4).astype(np.float64)
         if not len(a):
             return {}
-        seen, opened, done = a[:, 0], a[:, 1], a[:, 2]
+        staged, opened, done = a[:, 0], a[:, 1], a[:, 2]
         cnt = a[:, 3].astype(np.uint64)
-        return {"open_us": (opened - seen) / tpu.value, "run_us": (done - opened) / tpu.value,
-                "gap_us": (seen[1:] - done[:-1]) / tpu.value,
+        waiting = staged[1:] <= done[:-1]
+        return {"run_us": (done - opened) / tpu.value,
+                "handoff_us": ((opened[1:] - done[:-1]) / tpu.value)[waiting],
                 "ins_chunks": (cnt >> np.uint64(32)).astype(np.int64),
                 "rd_chunks": (cnt & np.uint64(0xffffffff)).astype(np.int64)}
 
