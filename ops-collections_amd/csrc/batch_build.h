@@ -352,10 +352,25 @@ constexpr bool place_fits()
 }
 constexpr uint32_t kPlaceGroup = 64;
 
+// The region's pairs part[s + tid + u * kBuildThreads], u < kPer, below e,
+// into registers (place_sorted's layout).
+template <int KS, int VS>
+__device__ __forceinline__ void load_region_pairs(const typename Lay<KS, VS>::Raw* part, uint32_t s, uint32_t e,
+                                                  typename Lay<KS, VS>::Raw (&p)[kPlaceMax / kBuildThreads])
+{
+#pragma unroll
+    for (uint32_t u = 0; u < kPlaceMax / kBuildThreads; ++u) {
+        const uint32_t i = s + threadIdx.x + u * kBuildThreads;
+        if (i < e) p[u] = part[i];
+    }
+}
+
 template <int KS, int VS>
 __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTable& t, uint32_t r0, uint32_t rlen,
                                              uint32_t s, uint32_t e, typename Lay<KS, VS>::Raw* ls,
-                                             typename Lay<KS, VS>::Raw* part, uint32_t& s_nov, uint32_t& fails)
+                                             typename Lay<KS, VS>::Raw* part,
+                                             const typename Lay<KS, VS>::Raw (&pin)[kPlaceMax / kBuildThreads],
+                                             uint32_t& s_nov, uint32_t& fails)
 {
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
@@ -384,7 +399,7 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
         const uint32_t i = s + tid + u * kBuildThreads;
         valid[u] = false;
         if (i < e) {
-            p[u] = part[i];
+            p[u] = pin[u];  // loaded by the caller (load_region_pairs)
             const uint64_t key = L::key(p[u]);
             if (key == ekey) {
                 ++fails;  // the reserved empty key is never inserted
@@ -503,60 +518,19 @@ __device__ __forceinline__ bool place_sorted(const BuildArgs& a, const BatchTabl
     return true;
 }
 
-// 4. One region per workgroup: build it in LDS and write it once.
-// kLoad: start from the table's slots (otherwise from the empty slot: a
-// batched plan, or a table known to be empty).
-template <int KS, int VS, bool kOne, bool kLoad>
-__global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
+// The CAS walk of the region's pairs part[s, e) in LDS (the rules of
+// open_addressing_ref_impl.h:490-532): each pair walks from its first slot to
+// the first empty slot or equal key; a walk past the region's end sends the
+// pair to the overflow kernel (back into part[s, ...)), and a one-region
+// table wraps.  Entered after a barrier.
+template <int KS, int VS>
+__device__ __forceinline__ void apply_walk(const BuildArgs& a, const BatchTable& t, uint32_t r0, uint32_t rlen,
+                                           bool whole, uint32_t s, uint32_t e, typename Lay<KS, VS>::Raw* ls,
+                                           typename Lay<KS, VS>::Raw* part, uint32_t& s_nov, uint32_t& fails)
 {
     using L = Lay<KS, VS>;
     using Raw = typename L::Raw;
-    constexpr uint32_t SS = L::SS;
-    constexpr uint32_t R = BuildShape<SS>::kRegion;
-    __shared__ __align__(16) Raw ls[R];
-    __shared__ uint32_t s_nov, s_fail;
-    const uint32_t r = blockIdx.x;
-    const uint32_t ti = kOne ? 0u : a.reg_table[r];
-    const BatchTable t = a.tables[ti];
-    const uint32_t r0 = (r - a.reg0[ti]) * R;
-    const uint32_t rlen = min(R, t.cap - r0);
-    const bool whole = rlen == t.cap;  // a one-region table: the walk wraps inside the region
-    const uint32_t s = a.offsets[r], e = kOne ? a.apply_end[r] : a.offsets[r + 1];
     const uint64_t ekey = a.empty_key;
-    uint8_t* dst = t.slots + (uint64_t)r0 * SS;
-    const uint32_t bytes = rlen * SS, n16 = bytes / 16;
-    if constexpr (kLoad) {
-        for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)ls)[j] = ((const u64x2*)dst)[j];
-        for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ls[j] = ((const Raw*)dst)[j];
-    } else {
-        const Raw empty = empty_slot<KS, VS>(ekey, a.empty_val);
-        for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) ls[j] = empty;
-    }
-    if (threadIdx.x == 0) {
-        s_nov = 0;
-        s_fail = 0;
-    }
-    Raw* part = (Raw*)a.part;
-    uint32_t fails = 0;
-    if constexpr (place_fits<KS, SS>()) {
-        // an empty region, integer keys: placed without walks
-        bool empty_region = true;
-        if constexpr (kLoad)
-            for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) empty_region &= L::key(ls[j]) == ekey;
-        if (__syncthreads_and(empty_region && !a.key_float && e - s <= kPlaceMax) &&
-            place_sorted<KS, VS>(a, t, r0, rlen, s, e, ls, part, s_nov, fails)) {
-            if (fails) atomicAdd(&s_fail, fails);
-            __syncthreads();
-            for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)dst)[j] = ((const u64x2*)ls)[j];
-            for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ((Raw*)dst)[j] = ls[j];
-            if (threadIdx.x == 0) {
-                a.ovc[r] = s_nov;
-                count_failures(a, ti, s_fail);
-            }
-            return;
-        }
-    }
-    __syncthreads();
     for (uint32_t b = s; b < e; b += blockDim.x) {
         const uint32_t j = b + threadIdx.x;
         Raw p;
@@ -596,6 +570,68 @@ __global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
             }
         }
     }
+}
+
+// 4. One region per workgroup: build it in LDS and write it once.
+// kLoad: start from the table's slots (otherwise from the empty slot: a
+// batched plan, or a table known to be empty).
+template <int KS, int VS, bool kOne, bool kLoad>
+__global__ __launch_bounds__(kBuildThreads) void build_apply_kernel(BuildArgs a)
+{
+    using L = Lay<KS, VS>;
+    using Raw = typename L::Raw;
+    constexpr uint32_t SS = L::SS;
+    constexpr uint32_t R = BuildShape<SS>::kRegion;
+    __shared__ __align__(16) Raw ls[R];
+    __shared__ uint32_t s_nov, s_fail;
+    const uint32_t r = blockIdx.x;
+    const uint32_t ti = kOne ? 0u : a.reg_table[r];
+    const BatchTable t = a.tables[ti];
+    const uint32_t r0 = (r - a.reg0[ti]) * R;
+    const uint32_t rlen = min(R, t.cap - r0);
+    const bool whole = rlen == t.cap;  // a one-region table: the walk wraps inside the region
+    const uint32_t s = a.offsets[r], e = kOne ? a.apply_end[r] : a.offsets[r + 1];
+    const uint64_t ekey = a.empty_key;
+    uint8_t* dst = t.slots + (uint64_t)r0 * SS;
+    const uint32_t bytes = rlen * SS, n16 = bytes / 16;
+    // regions of an empty table placed without walks are filled by place_sorted
+    constexpr bool kPlace = place_fits<KS, SS>();
+    const bool placing = !kLoad && kPlace && !a.key_float && e - s <= kPlaceMax;
+    if constexpr (kLoad) {
+        for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)ls)[j] = ((const u64x2*)dst)[j];
+        for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ls[j] = ((const Raw*)dst)[j];
+    } else if (!placing) {
+        const Raw empty = empty_slot<KS, VS>(ekey, a.empty_val);
+        for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) ls[j] = empty;
+    }
+    if (threadIdx.x == 0) {
+        s_nov = 0;
+        s_fail = 0;
+    }
+    Raw* part = (Raw*)a.part;
+    uint32_t fails = 0;
+    if constexpr (place_fits<KS, SS>()) {
+        // an empty region, integer keys: placed without walks
+        bool empty_region = true;
+        if constexpr (kLoad)
+            for (uint32_t j = threadIdx.x; j < rlen; j += blockDim.x) empty_region &= L::key(ls[j]) == ekey;
+        Raw pin[kPlaceMax / kBuildThreads];
+        const bool try_place = empty_region && !a.key_float && e - s <= kPlaceMax;
+        if (try_place) load_region_pairs<KS, VS>(part, s, e, pin);
+        if (__syncthreads_and(try_place) && place_sorted<KS, VS>(a, t, r0, rlen, s, e, ls, part, pin, s_nov, fails)) {
+            if (fails) atomicAdd(&s_fail, fails);
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)dst)[j] = ((const u64x2*)ls)[j];
+            for (uint32_t j = n16 * 16 / SS + threadIdx.x; j < rlen; j += blockDim.x) ((Raw*)dst)[j] = ls[j];
+            if (threadIdx.x == 0) {
+                a.ovc[r] = s_nov;
+                count_failures(a, ti, s_fail);
+            }
+            return;
+        }
+    }
+    __syncthreads();
+    apply_walk<KS, VS>(a, t, r0, rlen, whole, s, e, ls, part, s_nov, fails);
     if (fails) atomicAdd(&s_fail, fails);
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < n16; j += blockDim.x) ((u64x2*)dst)[j] = ((const u64x2*)ls)[j];

@@ -5,7 +5,10 @@
 //     buffer: 128 bytes over a table far larger than the caches is the
 //     probe walk's pattern (one line per window); 32 bytes over a 128 MiB
 //     buffer is the Bloom filter's block read, served by the MALL;
-//   - a streaming copy, the pattern of the region build and the fill.
+//   - a streaming copy, the pattern of the region build and the fill, with
+//     4 workgroups per CU (a sweep of 4-32 workgroups per CU and 1-8 loads
+//     in flight per lane read 4.7-5.5 TB/s; 4 per CU was the fastest, and
+//     hipMemcpyAsync device-to-device reads 4.9).
 // Reads are spread over lane groups of access/16 lanes (one 16-byte load
 // each), eight accesses in flight per group, offsets from a multiplicative
 // hash of the access index masked to the largest power-of-two count of units
@@ -88,7 +91,7 @@ MXCO_API mxco_status mxco_diag_stream_copy(const void* src, void* dst, uint64_t 
 {
     if (!src || !dst || bytes % 16 || ((uintptr_t)src | (uintptr_t)dst) % 16)
         return fail(MXCO_INVALID_ARGUMENT, "stream_copy: 16-byte aligned buffers and size");
-    stream_copy_kernel<<<grid_for(bytes / 16, 8), 256, 0, stream>>>((const u64x2*)src, (u64x2*)dst, bytes / 16);
+    stream_copy_kernel<<<grid_for(bytes / 16, 4), 256, 0, stream>>>((const u64x2*)src, (u64x2*)dst, bytes / 16);
     MXCO_HIP(hipGetLastError(), "stream_copy");
     return ok();
 }
