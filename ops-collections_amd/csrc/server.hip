@@ -693,7 +693,8 @@ MXCO_API mxco_status mxco_server_create(const mxco_server_config* cfg, const mxc
         s->res_ticks_per_us = ticks_per_us;
         s->res_idle = (uint64_t)(ticks_per_us * env_int("MXCO_SERVE_IDLE_US", 300));
         s->res_life = (uint64_t)(ticks_per_us * 50000.0);  // relaunched every 50 ms under load
-        s->res_hard = (uint64_t)(ticks_per_us * 5e6);       // a tick open for 5 s is given up
+        // a tick open for 5 s is given up (MXCO_SERVE_HARD_US shortens it, for tests of that path)
+        s->res_hard = (uint64_t)(ticks_per_us * env_int("MXCO_SERVE_HARD_US", 5000000));
     }
     if (e != hipSuccess) {
         (void)mxco_server_destroy(s);

@@ -314,3 +314,30 @@ def test_resident_kernel_relaunches_after_idle_exit():
         assert torch.equal(out, exp), rep
     assert s.resident_stats()[1] >= launches0 + 3
     s.close()
+
+
+def test_resident_kernel_deadline_fails_the_tick():
+    """A tick open past the resident kernel's deadline is given up: the
+    kernel exits normally, the request fails with an error instead of
+    waiting, and the server still closes.  The deadline is set to zero from
+    the host, so the first tick trips it."""
+    import os
+    import mxco
+    os.environ["MXCO_SERVE_HARD_US"] = "0"
+    try:
+        s = mxco.Server()
+    finally:
+        del os.environ["MXCO_SERVE_HARD_US"]
+    t = s.add_map(1 << 16)
+    keys = torch.arange(1, 1 << 15, dtype=torch.int64, device="cuda")
+    pairs = torch.stack([keys, keys * 7 + 1], 1).contiguous()
+    ready()
+    failed = 0
+    for _ in range(3):
+        try:
+            s.wait(s.submit(t, mxco.REQ_INSERT, pairs))
+        except RuntimeError as e:
+            assert "deadline" in str(e), e
+            failed += 1
+    assert failed >= 1  # a tick whose chunks all finish before the leader looks again may still complete
+    s.close()
