@@ -305,7 +305,9 @@ def run_hll(torch, mxco, sketches=64, n=1 << 28):
 def run_ceilings(torch, mxco, reps=5):
     """Measured ceilings of the containers' access patterns: random 128-,
     64- and 32-byte reads over an 8 GiB buffer (HBM; the probe walk reads
-    one 128-byte line per window), random 32- and 128-byte reads over a
+    one 128-byte line per window), random 256-byte reads (two adjacent
+    lines: whether a walk's next line is cheaper than a random one), random
+    32- and 128-byte reads over a
     128 MiB buffer (the Bloom filter's size: served from the 256 MB MALL),
     and a 4 GiB streaming copy.  Median of `reps` timed runs each."""
     import statistics
@@ -329,6 +331,7 @@ def run_ceilings(torch, mxco, reps=5):
     for name, nbytes, access, n in (("hbm_random_128B", 8 << 30, 128, 1 << 28),
                                     ("hbm_random_64B", 8 << 30, 64, 1 << 28),
                                     ("hbm_random_32B", 8 << 30, 32, 1 << 28),
+                                    ("hbm_random_256B", 8 << 30, 256, 1 << 27),
                                     ("mall_random_32B_128MiB", 128 << 20, 32, 1 << 27),
                                     ("mall_random_128B_128MiB", 128 << 20, 128, 1 << 27)):
         ms = timed(lambda: mxco.random_read(big, nbytes, access, n, sink))

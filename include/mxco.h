@@ -473,8 +473,8 @@ typedef struct mxco_server_config {
     uint32_t tick_mode;         /* 0 = StaticMap ticks run in the resident kernel, fed through a
                                    mailbox in host-coherent memory (4- and 8-byte keys and
                                    payloads); 1 = one insert and one read launch per tick */
-    uint32_t resident_blocks_per_cu; /* workgroups per CU of the resident kernel (0 = all that fit
-                                        a CU but one) */
+    uint32_t resident_blocks_per_cu; /* 512-thread workers per CU of the resident kernel, where they
+                                        fit (0 = 1) */
     uint8_t empty_key[8];       /* StaticMap sentinels, little-endian */
     uint8_t empty_value[8];
 } mxco_server_config;
@@ -635,7 +635,7 @@ MXCO_API mxco_status mxco_hll_estimate(const mxco_hll* sketch, hipStream_t strea
 MXCO_API mxco_status mxco_device_cu_count(int* cus);
 
 /* Measured ceilings (no reference counterpart; measurement entry points).
- * random_read: n reads of `access` bytes (16, 32, 64 or 128) at hashed,
+ * random_read: n reads of `access` bytes (16, 32, 64, 128 or 256) at hashed,
  * aligned offsets within the largest power-of-two number of `access`-byte
  * units in the first `bytes` of buf, eight in flight per lane group; sink
  * receives one word per lane (CUs * 8 * 256 words).

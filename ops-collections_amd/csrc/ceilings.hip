@@ -67,8 +67,8 @@ MXCO_API mxco_status mxco_diag_random_read(const void* buf, uint64_t bytes, uint
                                            uint64_t* sink, uint64_t sink_words, hipStream_t stream)
 {
     if (!buf || !sink) return fail(MXCO_INVALID_ARGUMENT, "random_read: null buffer");
-    if (access != 16 && access != 32 && access != 64 && access != 128)
-        return fail(MXCO_INVALID_ARGUMENT, "random_read: access must be 16, 32, 64 or 128 bytes");
+    if (access != 16 && access != 32 && access != 64 && access != 128 && access != 256)
+        return fail(MXCO_INVALID_ARGUMENT, "random_read: access must be 16, 32, 64, 128 or 256 bytes");
     if (((uintptr_t)buf % access) != 0 || bytes < access)
         return fail(MXCO_INVALID_ARGUMENT, "random_read: buffer not aligned to the access or too small");
     const unsigned grid = (unsigned)cu_count() * 8;
@@ -81,7 +81,8 @@ MXCO_API mxco_status mxco_diag_random_read(const void* buf, uint64_t bytes, uint
     case 16: random_read_kernel<1><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
     case 32: random_read_kernel<2><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
     case 64: random_read_kernel<4><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
-    default: random_read_kernel<8><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
+    case 128: random_read_kernel<8><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
+    default: random_read_kernel<16><<<grid, 256, 0, stream>>>(b, mask, n, sink); break;
     }
     MXCO_HIP(hipGetLastError(), "random_read");
     return ok();

@@ -25,12 +25,16 @@
 // itself (a claim past the count takes nothing; there is at most one such
 // claim per workgroup and epoch, so the next-chunk field never overflows).
 // A worker finds a chunk's request by binary search over the requests'
-// first-chunk indices, held in its LDS for the epoch, and runs the probe
-// state machine of the launched path (oa_process) on the chunk's keys.
-// Insert chunks come first; a workgroup that claims a read chunk first waits
-// until every insert chunk of the tick has been processed and released, so
-// reads see the tick's inserts, as they do after the launched path's insert
-// kernel.
+// first-chunk indices, held in its LDS for the epoch.  Insert chunks, and
+// read chunks of fewer than kServeStagedMin keys, run the probe state
+// machine of the launched path (oa_process) on the chunk's keys.  Larger read
+// chunks run the bulk Find's wave-staged loop (wave_staged_find): each
+// wavefront takes 128-key pieces of the chunk, with the keys copied into LDS
+// when a piece opens and the outputs written back as whole lines.  Insert
+// chunks come first.  A read of a table with an insert in the same tick waits
+// until every insert chunk of the tick has been processed and released, so it
+// sees the tick's inserts, as it does after the launched path's insert kernel;
+// other reads start at once.
 //
 // Memory ordering (on gfx950 an agent-scope release writes back this XCD's
 // L2 and an acquire invalidates it, so each is done once per workgroup and
@@ -71,15 +75,23 @@ constexpr uint32_t kServeTrace = 1024;         // ticks kept in the timing trace
 // ServeReq::op flag of a read whose table has an insert in the same tick:
 // its chunks wait for the tick's inserts (other reads start at once).
 constexpr uint32_t kServeAfterInsert = 0x80000000u;
+// Requests per tick that a worker keeps in LDS (larger ticks read them from
+// device memory).
+constexpr uint32_t kServeReqCache = 256;
+// Read chunks of at least this many keys run the wave-staged loop: every
+// wavefront of the worker gets at least one 128-key piece.
+constexpr uint32_t kServeStagedMin = (kServeThreads / 64) * kWaveChunk;
 
 // One request of a tick; its chunks are [first, first + ceil(count / csz)).
 struct ServeReq {
-    const uint8_t* in;  // device input (pairs or keys)
-    uint8_t* out;       // device output (reads)
-    uint64_t count;     // keys
-    uint32_t table, op; // op: MXCO_REQ_INSERT, MXCO_REQ_FIND or MXCO_REQ_CONTAINS (| kServeAfterInsert)
-    uint32_t csz;       // keys per chunk
-    uint32_t first;     // the request's first chunk in the tick
+    const uint8_t* in;    // device input (pairs or keys)
+    uint8_t* out;         // device output (reads)
+    uint64_t count;       // keys
+    uint8_t* slots;       // the table: slot array, capacity and bucket count
+    uint32_t cap, nbuckets;
+    uint32_t op;          // MXCO_REQ_INSERT, MXCO_REQ_FIND or MXCO_REQ_CONTAINS (| kServeAfterInsert)
+    uint32_t csz;         // keys per chunk
+    uint32_t first, pad;  // the request's first chunk in the tick
 };
 
 struct ServeTickDesc {
@@ -94,7 +106,7 @@ struct ServeTickDesc {
 struct ServeMailbox {
     uint64_t posted;                  // host: seq of the last tick posted
     uint64_t stop;                    // host: exit at the next moment between ticks
-    uint64_t aborted;                 // kernel: the hard deadline ended an unfinished tick
+    uint64_t aborted;                 // kernel: number of an unfinished tick the hard deadline ended
     uint64_t pad;
     uint64_t done[kServeSlots];       // kernel: seq of the last finished tick of each slot
     ServeTickDesc desc[kServeSlots];  // host: the tick posted into slot seq % kServeSlots
@@ -168,6 +180,7 @@ __device__ __forceinline__ void serve_lead(ServeMailbox* mb, ServeCtl* ctl, uint
     ServeTickDesc next{};  // its descriptor (thread 0)
     const uint64_t t0 = wall_clock64();
     uint64_t last = t0, opened = t0;  // last moment a tick was open; when the open tick was published
+    uint64_t open_seq = 0;            // number of the open tick (thread 0)
     for (;;) {
         if (threadIdx.x == 0) {
             uint32_t act = 0;  // 0 wait, 1 stage, 2 open, 3 exit, 4 abort
@@ -217,6 +230,7 @@ __device__ __forceinline__ void serve_lead(ServeMailbox* mb, ServeCtl* ctl, uint
                 staged = false;
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the finisher's reads before the resets
                 ctl->cur = next;
+                open_seq = next.seq;
                 ctl->done_writers = 0;
                 ctl->ins_done = 0;
                 last = opened = wall_clock64();
@@ -232,7 +246,7 @@ __device__ __forceinline__ void serve_lead(ServeMailbox* mb, ServeCtl* ctl, uint
         if (act >= 3) {
             if (threadIdx.x == 0) {
                 if (act == 4)
-                    __hip_atomic_store(&mb->aborted, (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_store(&mb->aborted, open_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
                 __hip_atomic_store(&ctl->exit, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
             return;
@@ -279,36 +293,65 @@ __device__ __forceinline__ void serve_leave(ServeMailbox* mb, ServeCtl* ctl, con
     __hip_atomic_store(&ctl->finished, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// 128-key pieces of one read chunk (no stencil), held by value: wavefront w
+// takes pieces w, w + waves, ...
+template <int KS, int OUTB>
+struct ServePieces {
+    static constexpr bool kOneTable = true;
+    uint8_t* table;
+    const uint8_t* in;
+    uint8_t* out;
+    uint64_t n, next, step;
+    uint32_t cap, nbuckets;
+    __device__ __forceinline__ WsChunk open()
+    {
+        WsChunk c;
+        const uint64_t base = next * kWaveChunk;
+        next += step;
+        c.table = table;
+        c.cap = cap;
+        c.nbuckets = nbuckets;
+        c.len = base < n ? (uint32_t)min((uint64_t)kWaveChunk, n - base) : 0u;
+        c.in = in + base * KS;
+        c.out = out + base * OUTB;
+        c.stencil = nullptr;
+        return c;
+    }
+};
+
 template <int KS, int VS, int G>
-__device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable* tables, ServeMailbox* mb,
-                                           ServeCtl* ctl)
+__device__ __forceinline__ void serve_work(const OaArgs& base, ServeMailbox* mb, ServeCtl* ctl)
 {
     __shared__ uint32_t s_first[kServeMaxItems];  // first chunk of each request of the epoch
+    __shared__ ServeReq s_reqs[kServeReqCache];   // the epoch's requests, when they fit
+    // per wavefront: output staging and key buffers of its two open pieces
+    __shared__ __align__(16) uint8_t s_out[kServeThreads / 64][2][kWaveChunk * 8];
+    __shared__ __align__(16) uint8_t s_keys[kServeThreads / 64][2][kWaveChunk * 8];
     __shared__ ServeReq s_req;
     __shared__ uint32_t s_state, s_idx, s_nreq, s_nins, s_fresh, s_epoch;
     // thread 0's state
     ServeTickDesc d{};
     uint32_t my_e = 0, ins_mine = 0, done_mine = 0, backoff = 1;
-    bool joined = false, ins_seen = false;
+    uint32_t dead_e = 0xffffffffu;  // epoch in which this workgroup's claim came back empty
+    uint64_t ahead = 0;             // claim issued while the previous chunk ran
+    uint32_t exit_ahead = 0;        // the exit flag, read with it
+    bool joined = false, ins_seen = false, have_ahead = false;
     const uint32_t gpb = blockDim.x / G;
     const uint32_t gj = threadIdx.x / G;
     for (;;) {
         if (threadIdx.x == 0) {
             uint32_t state = 0, idx = 0, fresh = 0;
+            uint64_t old = ahead;
+            bool got = have_ahead && !exit_ahead;  // after the exit flag: no further chunk
+            have_ahead = false;
             for (;;) {
-                if (srv_peek(&ctl->exit)) {
-                    state = 2;
-                    break;
-                }
-                const uint64_t v = srv_peek(&ctl->claim);
-                if (claim_next(v) < claim_count(v)) {
-                    const uint64_t old =
-                        __hip_atomic_fetch_add(&ctl->claim, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (got) {
+                    got = false;
                     if (claim_next(old) < claim_count(old)) {
                         idx = claim_next(old);
                         if (claim_epoch(old) != my_e) {
                             // a chunk of a new tick (this workgroup holds none of an older one):
-                            // inputs, table descriptors and the request list written before it opened
+                            // inputs and the request list written before it opened
                             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
                             d = ctl->cur;
                             my_e = claim_epoch(old);
@@ -319,15 +362,38 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
                         backoff = 1;
                         break;
                     }
+                    // the tick has no chunk left: this workgroup claims no more in it
+                    dead_e = claim_epoch(old);
                 }
                 if (joined) {
                     serve_leave(mb, ctl, d, my_e, ins_mine, done_mine);
                     joined = false;
                 }
+                if (srv_peek(&ctl->exit)) {
+                    state = 2;
+                    break;
+                }
+                const uint64_t v = srv_peek(&ctl->claim);
+                if (claim_epoch(v) != dead_e && claim_next(v) < claim_count(v)) {
+                    old = __hip_atomic_fetch_add(&ctl->claim, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    got = true;
+                    continue;
+                }
                 for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
                 backoff = backoff < 8 ? 2 * backoff : 8;
             }
             if (state == 1) {
+                // The next chunk's claim, issued now so that it returns while this
+                // chunk runs, when the tick has more chunks left than workers (in a
+                // smaller tick it would take a chunk an idle worker could start at
+                // once).  A claim past the tick's chunks is this workgroup's one
+                // empty claim of the tick.
+                if (claim_count(old) - claim_next(old) > gridDim.x) {
+                    ahead = __hip_atomic_fetch_add(&ctl->claim, (uint64_t)1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    exit_ahead = srv_peek(&ctl->exit);
+                    have_ahead = true;
+                }
                 if (!joined) {
                     __hip_atomic_fetch_add(&ctl->done_writers, (uint64_t)1, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
@@ -350,6 +416,12 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
         const ServeReq* reqs = ctl->reqs[s_epoch & 1];
         if (s_fresh) {
             for (uint32_t i = threadIdx.x; i < nreq; i += blockDim.x) s_first[i] = reqs[i].first;
+            if (nreq <= kServeReqCache) {
+                constexpr uint32_t W = sizeof(ServeReq) / 8;
+                const uint64_t* src = reinterpret_cast<const uint64_t*>(reqs);
+                uint64_t* dst = reinterpret_cast<uint64_t*>(s_reqs);
+                for (uint32_t i = threadIdx.x; i < nreq * W; i += blockDim.x) dst[i] = src[i];
+            }
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -361,7 +433,7 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
                 if (s_first[mid] <= idx) lo = mid;
                 else hi = mid;
             }
-            ServeReq rq = reqs[lo];
+            ServeReq rq = nreq <= kServeReqCache ? s_reqs[lo] : reqs[lo];
             if ((rq.op & kServeAfterInsert) && !ins_seen) {
                 // a read of a table with an insert in this tick: after all of the tick's inserts
                 while (srv_peek(&ctl->ins_done) < d.n_ins_chunks && !srv_peek(&ctl->exit))
@@ -380,11 +452,10 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
         const bool ins = idx < s_nins;
         const uint64_t start = (uint64_t)(idx - s_first[item]) * rq.csz;
         const uint64_t end = min(start + rq.csz, rq.count);
-        const BatchTable t = tables[rq.table];
         OaArgs a = base;
-        a.table = t.slots;
-        a.cap = t.cap;
-        a.nbuckets = t.nbuckets;
+        a.table = rq.slots;
+        a.cap = rq.cap;
+        a.nbuckets = rq.nbuckets;
         a.in = rq.in;
         a.out = rq.out;
         a.n = rq.count;
@@ -392,6 +463,22 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
         if (ins) {
             oa_process<KS, VS, G, OP_INSERT>(a, start + gj, end, gpb, c0, c1);
             flush_counters<OP_INSERT>(ctl->counters[s_epoch & 1] + 2ull * item, c0, c1);
+        } else if (end - start >= kServeStagedMin) {
+            // wavefront w takes the chunk's 128-key pieces w, w + waves, ...
+            constexpr int OUTF = VS ? VS : KS;
+            const uint32_t wib = threadIdx.x >> 6, waves = blockDim.x >> 6;
+            a.stencil = nullptr;
+            const uint64_t n = end - start;
+            const uint8_t* in = rq.in + start * KS;
+            if (rq.op == MXCO_REQ_CONTAINS) {
+                ServePieces<KS, 1> src{rq.slots, in, rq.out + start, n, wib, waves, rq.cap, rq.nbuckets};
+                wave_staged_find<KS, VS, G, OP_CONTAINS>(a, src, s_out[wib][0], s_out[wib][1], s_keys[wib][0],
+                                                        s_keys[wib][1]);
+            } else {
+                ServePieces<KS, OUTF> src{rq.slots, in, rq.out + start * OUTF, n, wib, waves, rq.cap, rq.nbuckets};
+                wave_staged_find<KS, VS, G, OP_FIND>(a, src, s_out[wib][0], s_out[wib][1], s_keys[wib][0],
+                                                    s_keys[wib][1]);
+            }
         } else if (rq.op == MXCO_REQ_CONTAINS) {
             oa_process<KS, VS, G, OP_CONTAINS>(a, start + gj, end, gpb, c0, c1);
         } else {
@@ -406,18 +493,17 @@ __device__ __forceinline__ void serve_work(const OaArgs& base, const BatchTable*
 }
 
 template <int KS, int VS, int G>
-__global__ __launch_bounds__(kServeThreads) void serve_resident_kernel(OaArgs base, const BatchTable* tables,
-                                                                       ServeMailbox* mb, ServeCtl* ctl,
+__global__ __launch_bounds__(kServeThreads) void serve_resident_kernel(OaArgs base, ServeMailbox* mb, ServeCtl* ctl,
                                                                        uint64_t seq_base, uint64_t idle, uint64_t life,
                                                                        uint64_t hard)
 {
     if (blockIdx.x == 0) serve_lead(mb, ctl, seq_base, idle, life, hard);
-    else serve_work<KS, VS, G>(base, tables, mb, ctl);
+    else serve_work<KS, VS, G>(base, mb, ctl);
 }
 
 // Launches the resident kernel for the map layout (defined in
 // serve_resident.hip); hipErrorInvalidValue for a layout without one.
-hipError_t launch_serve_resident(int ks, int vs, int g, const OaArgs& a, const BatchTable* tables, ServeMailbox* mb,
+hipError_t launch_serve_resident(int ks, int vs, int g, const OaArgs& a, ServeMailbox* mb,
                                  ServeCtl* ctl, uint64_t seq_base, uint64_t idle, uint64_t life, uint64_t hard,
                                  unsigned grid, hipStream_t s);
 

@@ -8,10 +8,10 @@ namespace mxco {
 namespace {
 
 template <int KS, int VS, int G>
-hipError_t launch_g(const OaArgs& a, const BatchTable* tables, ServeMailbox* mb, ServeCtl* ctl,
+hipError_t launch_g(const OaArgs& a, ServeMailbox* mb, ServeCtl* ctl,
                     uint64_t seq_base, uint64_t idle, uint64_t life, uint64_t hard, unsigned grid, hipStream_t s)
 {
-    serve_resident_kernel<KS, VS, G><<<grid, kServeThreads, 0, s>>>(a, tables, mb, ctl, seq_base, idle, life, hard);
+    serve_resident_kernel<KS, VS, G><<<grid, kServeThreads, 0, s>>>(a, mb, ctl, seq_base, idle, life, hard);
     return hipGetLastError();
 }
 
@@ -27,15 +27,15 @@ int occupancy_g()
 }
 
 template <int KS, int VS>
-hipError_t launch_l(int g, const OaArgs& a, const BatchTable* tables, ServeMailbox* mb,
+hipError_t launch_l(int g, const OaArgs& a, ServeMailbox* mb,
                     ServeCtl* ctl, uint64_t seq_base, uint64_t idle, uint64_t life, uint64_t hard, unsigned grid,
                     hipStream_t s)
 {
     switch (g) {
-    case 1: return launch_g<KS, VS, 1>(a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
-    case 2: return launch_g<KS, VS, 2>(a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
-    case 4: return launch_g<KS, VS, 4>(a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
-    case 8: return launch_g<KS, VS, 8>(a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
+    case 1: return launch_g<KS, VS, 1>(a, mb, ctl, seq_base, idle, life, hard, grid, s);
+    case 2: return launch_g<KS, VS, 2>(a, mb, ctl, seq_base, idle, life, hard, grid, s);
+    case 4: return launch_g<KS, VS, 4>(a, mb, ctl, seq_base, idle, life, hard, grid, s);
+    case 8: return launch_g<KS, VS, 8>(a, mb, ctl, seq_base, idle, life, hard, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -54,14 +54,14 @@ int occupancy_l(int g)
 
 }  // namespace
 
-hipError_t launch_serve_resident(int ks, int vs, int g, const OaArgs& a, const BatchTable* tables, ServeMailbox* mb,
+hipError_t launch_serve_resident(int ks, int vs, int g, const OaArgs& a, ServeMailbox* mb,
                                  ServeCtl* ctl, uint64_t seq_base, uint64_t idle, uint64_t life,
                                  uint64_t hard, unsigned grid, hipStream_t s)
 {
-    if (ks == 8 && vs == 8) return launch_l<8, 8>(g, a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
-    if (ks == 4 && vs == 4) return launch_l<4, 4>(g, a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
-    if (ks == 4 && vs == 8) return launch_l<4, 8>(g, a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
-    if (ks == 8 && vs == 4) return launch_l<8, 4>(g, a, tables, mb, ctl, seq_base, idle, life, hard, grid, s);
+    if (ks == 8 && vs == 8) return launch_l<8, 8>(g, a, mb, ctl, seq_base, idle, life, hard, grid, s);
+    if (ks == 4 && vs == 4) return launch_l<4, 4>(g, a, mb, ctl, seq_base, idle, life, hard, grid, s);
+    if (ks == 4 && vs == 8) return launch_l<4, 8>(g, a, mb, ctl, seq_base, idle, life, hard, grid, s);
+    if (ks == 8 && vs == 4) return launch_l<8, 4>(g, a, mb, ctl, seq_base, idle, life, hard, grid, s);
     return hipErrorInvalidValue;
 }
 

@@ -51,6 +51,7 @@ double now_us()
 
 struct Req {
     mxco_request r;
+    BatchTable bt{};  // a StaticMap request's table, taken at submit
     uint64_t ticket;
     double t_submit = 0, t_done = 0;
     uint64_t result = 0;
@@ -61,6 +62,7 @@ struct Req {
 struct Table {
     int kind;  // 0 map, 1 multimap
     mxco_oa* map = nullptr;
+    BatchTable bt{};  // a map's slot array, capacity and bucket count
     mxco_mm* mm = nullptr;
     uint64_t bytes = 0;
     bool live = false;
@@ -344,7 +346,9 @@ mxco_status post_tick(mxco_server* s, Tick& t)
             const mxco_request& r = t.reqs[ri]->r;
             if (k == 0) t.insert_items.push_back(ri);
             const uint32_t op = r.op | (k == 1 && after_insert(ri) ? kServeAfterInsert : 0u);
-            reqs[item++] = ServeReq{(const uint8_t*)r.in, (uint8_t*)r.out, r.n, r.table, op, csz, (uint32_t)chunk};
+            const BatchTable& bt = t.reqs[ri]->bt;
+            reqs[item++] = ServeReq{(const uint8_t*)r.in, (uint8_t*)r.out, r.n, bt.slots, bt.cap, bt.nbuckets,
+                                    op, csz, (uint32_t)chunk, 0u};
             chunk += (r.n + csz - 1) / csz;
             if (chunk >= kServeMaxChunks)
                 return fail(MXCO_LENGTH_ERROR, "server: tick too large for the resident kernel");
@@ -368,7 +372,7 @@ mxco_status res_launch(mxco_server* s, uint64_t seq_base)
 {
     MXCO_HIP(hipMemsetAsync(s->ctl, 0, kServeCtlHeader, s->res_stream), "server: resident reset");
     MXCO_HIP(launch_serve_resident(s->map_desc.key_size, s->map_desc.value_size, s->res_group, map_args(s),
-                                   s->tables_dev, s->mb, s->ctl, seq_base, s->res_idle, s->res_life, s->res_hard,
+                                   s->mb, s->ctl, seq_base, s->res_idle, s->res_life, s->res_hard,
                                    s->res_grid, s->res_stream),
              "server: resident launch");
     MXCO_HIP(hipEventRecord(s->res_done, s->res_stream), "server: resident record");
@@ -480,9 +484,11 @@ hipError_t res_wait(mxco_server* s, Tick& t, std::unique_lock<std::mutex>& lk)
         if (!s->res_running) {
             // the last launch has ended: its ticks are finished, or it gave this one up
             if (__atomic_load_n(done, __ATOMIC_ACQUIRE) == t.seq) return hipSuccess;
-            if (__atomic_load_n(&s->mb->aborted, __ATOMIC_ACQUIRE)) {
+            // the launch gave up this tick; a number of an older tick means that
+            // tick's chunks all finished after the deadline, and it completed
+            if (const uint64_t ab = __atomic_load_n(&s->mb->aborted, __ATOMIC_ACQUIRE)) {
                 __atomic_store_n(&s->mb->aborted, (uint64_t)0, __ATOMIC_RELEASE);
-                return hipErrorLaunchTimeOut;
+                if (ab == t.seq) return hipErrorLaunchTimeOut;
             }
             if (res_launch(s, t.seq - 1) != MXCO_OK) return hipErrorLaunchFailure;
         }
@@ -803,6 +809,7 @@ MXCO_API mxco_status mxco_server_add_map(mxco_server* s, uint64_t capacity, uint
     Table t;
     t.kind = 0;
     t.map = m;
+    t.bt = bt;
     t.bytes = c * ss;
     t.live = true;
     s->tables.push_back(t);
@@ -861,6 +868,7 @@ MXCO_API mxco_status mxco_server_submit(mxco_server* s, const mxco_request* req,
             delete r;
             return e;
         }
+        r->bt = s->tables[req->table].bt;
         r->ticket = s->next_ticket++;
         s->reqs[r->ticket] = r;
         if (r->r.n == 0) {

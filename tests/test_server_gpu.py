@@ -328,8 +328,11 @@ def test_resident_kernel_deadline_fails_the_tick():
         s = mxco.Server()
     finally:
         del os.environ["MXCO_SERVE_HARD_US"]
-    t = s.add_map(1 << 16)
-    keys = torch.arange(1, 1 << 15, dtype=torch.int64, device="cuda")
+    # 2^22 keys make 2048 chunks: far more than the workers claim before the
+    # deadline's exit flag reaches them (each runs at most the chunk it holds
+    # and the one claimed ahead)
+    t = s.add_map(1 << 23)
+    keys = torch.arange(1, 1 << 22, dtype=torch.int64, device="cuda")
     pairs = torch.stack([keys, keys * 7 + 1], 1).contiguous()
     ready()
     failed = 0
@@ -339,5 +342,56 @@ def test_resident_kernel_deadline_fails_the_tick():
         except RuntimeError as e:
             assert "deadline" in str(e), e
             failed += 1
-    assert failed >= 1  # a tick whose chunks all finish before the leader looks again may still complete
+    assert failed >= 1
     s.close()
+
+
+@pytest.mark.parametrize("dt,group", [(torch.int64, 1), (torch.int64, 2), (torch.int32, 1)],
+                         ids=["i64-g1", "i64-g2", "i32-g1"])
+def test_resident_large_reads_match_closed_form(dt, group):
+    """Ticks with millions of read keys: their read chunks (2048 keys) run the
+    wave-staged loop in the resident kernel, ragged last chunks of 1971 and
+    1000 keys run the staged and the per-lane loop.  One tick holds Find and
+    Contains of present and absent keys; the next holds an insert of new
+    keys and a Find of them, which waits for that insert.  Finds return
+    key*7+1 or -1, Contains the hit flag, as the launched ticks do."""
+    import mxco
+    g = torch.Generator(device="cuda").manual_seed(11)
+    nk = 1 << 21
+    perm = torch.randperm(1 << 24, device="cuda", generator=g)
+    keys = (perm[:nk] * 2 + 1).to(dt)          # odd: present after the first insert
+    new = (perm[nk:nk + (1 << 20)] * 2 + 1).to(dt)  # odd, inserted by the second tick
+    absent = (perm[:nk] * 2 + 2).to(dt)         # even: never inserted
+    nq = 1 << 20
+    hit = torch.rand(nq + 1000, device="cuda", generator=g) < 0.5
+    q = torch.where(hit, keys[:nq + 1000], absent[:nq + 1000])
+    q_c = q[:nq - 77]
+    runs = []
+    for resident in (True, False):
+        ready()
+        s = mxco.Server(key_dtype=dt, value_dtype=dt, group_size=group, resident=resident)
+        t = s.add_map(1 << 23)
+        res, _ = s.wait(s.submit(t, mxco.REQ_INSERT, mxco.make_pairs(keys, keys * 7 + 1)))
+        assert res == 0
+        f_out = torch.full((q.numel(),), 5, dtype=dt, device="cuda")
+        c_out = torch.full((q_c.numel(),), 5, dtype=torch.uint8, device="cuda")
+        n_out = torch.full((new.numel(),), 5, dtype=dt, device="cuda")
+        new_pairs = mxco.make_pairs(new, new * 7 + 1)
+        ready()
+        reqs = [(t, mxco.REQ_FIND, q.data_ptr(), q.numel(), f_out.data_ptr(), None, 0),
+                (t, mxco.REQ_CONTAINS, q_c.data_ptr(), q_c.numel(), c_out.data_ptr(), None, 0)]
+        res, _ = s.replay(reqs, [0.0, 0.0])
+        reqs = [(t, mxco.REQ_INSERT, new_pairs.data_ptr(), new.numel(), None, None, 0),
+                (t, mxco.REQ_FIND, new.data_ptr(), new.numel(), n_out.data_ptr(), None, 0)]
+        res2, _ = s.replay(reqs, [0.0, 0.0])
+        assert res2[0] == 0
+        assert s.resident_stats()[0] == resident
+        s.close()
+        torch.cuda.synchronize()
+        assert torch.equal(f_out, torch.where(hit, q * 7 + 1, torch.full_like(q, -1)))
+        assert torch.equal(c_out.bool(), hit[:nq - 77])
+        assert torch.equal(n_out, new * 7 + 1)
+        runs.append((f_out.cpu(), c_out.cpu(), n_out.cpu()))
+    for a, b in zip(*runs):
+        assert torch.equal(a, b)
+
