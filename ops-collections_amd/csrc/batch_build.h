@@ -670,8 +670,36 @@ __device__ __forceinline__ bool cas_walk(const BuildArgs& a, const BatchTable& t
     return true;
 }
 
+// A slot value of lane `src` of the wavefront.
+template <typename Raw>
+__device__ __forceinline__ Raw shfl_raw(Raw v, uint32_t src)
+{
+    if constexpr (sizeof(Raw) == 16) {
+        Raw r;
+        r.x = __shfl(v.x, (int)src);
+        r.y = __shfl(v.y, (int)src);
+        return r;
+    } else if constexpr (sizeof(Raw) == 8) {
+        return (Raw)__shfl((unsigned long long)v, (int)src);
+    } else {
+        return (Raw)__shfl((unsigned int)v, (int)src);
+    }
+}
+
 // 5. Keys whose walk left their region: the CAS walk on the table, resumed
-// at the slot after the region.  One wavefront per region.
+// at the slot after the region.  One wavefront per region, whose keys all
+// resume at the same slot, walks them together line by line: lane j loads
+// slot j of the line, and the wavefront inserts its keys as the sequential
+// walk inserts them in lane order.  A key found in the line is a duplicate
+// (no empty slot precedes it there: the key was inserted by a walk that
+// found every earlier slot full, and slots are never emptied during an
+// Insert); the other keys take the line's empty slots in lane order.  A
+// compare-and-swap lost to another wavefront's key makes that slot full, and
+// the keys still without a slot take the line's remaining empty slots: the
+// result is the sequential walk's for the order that puts the other key
+// first.  Keys of one region that are equal are inserted once, by the lowest
+// lane.  Each key keeps its walk's slot budget (a full cycle of the table
+// from its first slot).
 template <int KS, int VS, bool kOne>
 __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
 {
@@ -679,7 +707,10 @@ __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
     using Raw = typename L::Raw;
     constexpr uint32_t SS = L::SS;
     constexpr uint32_t R = BuildShape<SS>::kRegion;
+    constexpr uint32_t SPL = 128 / SS;  // slots of a 128-byte line (at most 32)
+    static_assert(SPL <= 64, "a line's slots fit a wavefront");
     const uint32_t lane = threadIdx.x & 63;
+    const uint64_t ekey = a.empty_key;
     const uint32_t waves = gridDim.x * (blockDim.x / 64);
     for (uint32_t r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < a.nreg; r += waves) {
         const uint32_t n = a.ovc[r];
@@ -688,14 +719,94 @@ __global__ __launch_bounds__(256) void build_overflow_kernel(BuildArgs a)
         const BatchTable t = a.tables[ti];
         const uint32_t r0 = (r - a.reg0[ti]) * R;
         const uint32_t rend = r0 + min(R, t.cap - r0);
+        const uint32_t start = rend == t.cap ? 0u : rend;
         const Raw* part = (const Raw*)a.part + a.offsets[r];
         uint32_t fails = 0;
-        for (uint32_t k = lane; k < n; k += 64) {
-            const Raw p = part[k];
-            const uint64_t key = L::key(p);
-            const uint32_t first = first_slot<KS>(a, t.nbuckets, key);
-            // the region's slots from `first` on are full and hold other keys
-            fails += cas_walk<KS, VS>(a, t, p, key, rend == t.cap ? 0 : rend, t.cap - (rend - first));
+        for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+            const bool in = b0 + lane < n;
+            Raw p{};
+            uint64_t key = 0;
+            uint32_t left = 0;  // slots this key's walk may still examine
+            if (in) {
+                p = part[b0 + lane];
+                key = L::key(p);
+                const uint32_t first = first_slot<KS>(a, t.nbuckets, key);
+                // the region's slots from `first` on are full and hold other keys
+                left = t.cap - (rend - first);
+            }
+            // a key equal to a lower lane's is a duplicate
+            const uint64_t inmask = __ballot(in);
+            bool dup = false;
+            for (uint32_t j = 0; j < 64 && (inmask >> j); ++j) {
+                const uint64_t kj = __shfl(key, (int)j);
+                if (j < lane && ((inmask >> j) & 1ull) && key_eq<KS>(kj, key, a.key_float)) dup = true;
+            }
+            bool active = in && !dup;
+            fails += in && dup;
+            uint32_t pos = start;
+            while (__ballot(active)) {
+                const uint32_t wlen = min(SPL - pos % SPL, t.cap - pos);
+                Raw* line = (Raw*)(t.slots + (uint64_t)pos * SS);
+                Raw sv{};
+                if (lane < wlen) sv = line[lane];
+                uint64_t empt = __ballot(lane < wlen && L::key(sv) == ekey);
+                // slots of the line this key may examine, and the first holding it
+                const uint32_t lim = active ? min(wlen, left) : 0u;
+                uint32_t eq = 64;
+                for (uint32_t j = 0; j < wlen; ++j) {
+                    const uint64_t kj = L::key(shfl_raw(sv, j));
+                    if (j < lim && eq == 64 && kj != ekey && key_eq<KS>(kj, key, a.key_float)) eq = j;
+                }
+                if (active && eq < 64) {
+                    ++fails;  // already in the table
+                    active = false;
+                }
+                // the other keys take the empty slots in lane order, until the line
+                // has none left or every key has a slot
+                for (;;) {
+                    uint64_t rem = __ballot(active);
+                    if (!rem || !empt) break;
+                    uint32_t mine = 64;
+                    uint64_t em = empt;
+                    while (rem && em) {
+                        const uint32_t l = (uint32_t)__builtin_ctzll(rem);
+                        rem &= rem - 1;
+                        const uint32_t e = (uint32_t)__builtin_ctzll(em);
+                        if (e < (uint32_t)__builtin_amdgcn_readlane((int)lim, (int)l)) {
+                            em &= em - 1;  // lane l takes slot e
+                            if (lane == l) mine = e;
+                        }
+                    }
+                    if (!__ballot(mine < 64)) break;  // the empty slots lie past every budget
+                    uint64_t kk = 0;
+                    const Raw ev = shfl_raw(sv, mine < 64 ? mine : 0u);
+                    if (mine < 64) kk = claim_slot<KS, VS>(&line[mine], ev, p, ekey);
+                    // the claimed slots are full now, won or lost
+                    const uint64_t took = __ballot(mine < 64);
+                    uint64_t taken = 0;
+                    for (uint64_t tk = took; tk; tk &= tk - 1) {
+                        const uint32_t l = (uint32_t)__builtin_ctzll(tk);
+                        taken |= 1ull << (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)l);
+                    }
+                    empt &= ~taken;
+                    if (mine < 64) {
+                        if (kk == ekey) {
+                            active = false;  // inserted
+                        } else if (key_eq<KS>(kk, key, a.key_float)) {
+                            ++fails;  // another wavefront inserted it first
+                            active = false;
+                        }
+                    }
+                }
+                // keys whose budget ends in this line: no empty slot on their walk
+                if (active && left <= wlen) {
+                    ++fails;
+                    active = false;
+                }
+                left -= active ? wlen : 0u;
+                pos += wlen;
+                if (pos == t.cap) pos = 0;
+            }
         }
         // one add per wavefront
         for (int o = 32; o > 0; o >>= 1) fails += __shfl_xor(fails, o);
