@@ -27,7 +27,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-CFG2_REQ = 1 << 28
+MAP_2P28_REQ = 1 << 28
 
 
 def mx():
@@ -52,14 +52,14 @@ def check_map(m, keys, q, hit):
 
 
 @pytest.mark.parametrize("lf", [0.5, 0.7, 0.8, 0.9])
-def test_cfg2_map_2p28_slots(lf):
+def test_map_2p28_slots(lf):
     from mxco.workloads import uniform_map
     mxco = mx()
-    m = mxco.StaticMap(CFG2_REQ, -1, -1, torch.int64, torch.int64)
+    m = mxco.StaticMap(MAP_2P28_REQ, -1, -1, torch.int64, torch.int64)
     cap = m.capacity()
     assert cap == 268435460
     n = int(round(lf * cap))
-    print("cfg2 LF %.1f: %d keys into %d slots" % (lf, n, cap), flush=True)
+    print("2^28-slot map, LF %.1f: %d keys into %d slots" % (lf, n, cap), flush=True)
     keys, q, hit = uniform_map(n, 1 << 28, 0.5, seed=int(lf * 10))
     pairs = mxco.make_pairs(keys, keys * 7 + 1)
     assert m.insert_uses_region_build(n)  # the default path at this size
@@ -75,11 +75,11 @@ def test_cfg2_map_2p28_slots(lf):
         assert torch.equal(stored_keys(m), torch.sort(keys).values)
 
 
-def test_cfg2_cas_walk_2p28_lf08(monkeypatch):
+def test_map_2p28_cas_walk_lf08(monkeypatch):
     from mxco.workloads import uniform_map
     mxco = mx()
     monkeypatch.setenv("MXCO_OA_REGION_INSERT", "0")
-    m = mxco.StaticMap(CFG2_REQ, -1, -1, torch.int64, torch.int64)
+    m = mxco.StaticMap(MAP_2P28_REQ, -1, -1, torch.int64, torch.int64)
     n = int(round(0.8 * m.capacity()))
     keys, q, hit = uniform_map(n, 1 << 28, 0.5, seed=81)
     assert not m.insert_uses_region_build(n)
@@ -88,7 +88,7 @@ def test_cfg2_cas_walk_2p28_lf08(monkeypatch):
     check_map(m, keys, q, hit)
 
 
-def test_cfg2_zipf_stream_2p27_draws():
+def test_map_zipf_stream_2p27_draws():
     """2^27 draws of Zipf(s=1.1) over 2^24 keys into the 2^28-slot table:
     failures equal the oracle's sequential count (every repeat fails), the
     table holds each drawn key once with value key*7+1."""
@@ -98,7 +98,7 @@ def test_cfg2_zipf_stream_2p27_draws():
     ranks = z.draw(1 << 27, seed=11)
     keys = Zipf.keys(ranks)
     del z, ranks
-    m = mxco.StaticMap(CFG2_REQ, -1, -1, torch.int64, torch.int64)
+    m = mxco.StaticMap(MAP_2P28_REQ, -1, -1, torch.int64, torch.int64)
     failed = m.insert(mxco.make_pairs(keys, keys * 7 + 1))
     kh = keys.cpu().numpy()
     ref = O.OpenAddressing(np.int64, np.int64, 1 << 26, -1, -1)
@@ -109,7 +109,7 @@ def test_cfg2_zipf_stream_2p27_draws():
     assert torch.equal(m.find(uk), uk * 7 + 1)
 
 
-def test_cfg3_bloom_2p22_blocks_2p27_keys():
+def test_bloom_2p22_blocks_2p27_keys():
     nb, n = 1 << 22, 1 << 27
     g = torch.Generator(device="cuda")
     g.manual_seed(21)
@@ -126,7 +126,7 @@ def test_cfg3_bloom_2p22_blocks_2p27_keys():
     assert np.array_equal(out.cpu().numpy().astype(bool), ref)
     assert ref[: n // 2].all()
     fpr = ref[n // 2:].mean()
-    print("cfg3 bloom FPR %.4f" % fpr)
+    print("bloom 2^22 blocks: FPR %.4f" % fpr)
     assert 0.02 < fpr < 0.05  # Poisson(32) keys per block, 8 bits each: about 3.3 %
 
 
@@ -138,7 +138,7 @@ def _hll_oracle_estimate(regs, p):
                                         float(d["threshold"])))
 
 
-def test_cfg3_hll_2p28_zipf_keys_and_64_sketch_merge():
+def test_hll_2p28_zipf_keys_and_64_sketch_merge():
     from mxco.workloads import Zipf
     mxco = mx()
     universe, n = 10 ** 8, 1 << 28
@@ -173,7 +173,7 @@ def test_cfg3_hll_2p28_zipf_keys_and_64_sketch_merge():
     assert abs(est - exact) / exact < 3 * 1.04 / math.sqrt(1 << p)
 
 
-def test_cfg4_1024_tables_2p10_to_2p22_slots():
+def test_batched_1024_tables_2p10_to_2p22_slots():
     mxco = mx()
     from mxco import workloads
     w = workloads.batched_tables(1024, (10, 22), (0.5, 0.85), (6, 20), seed=4,
@@ -209,7 +209,7 @@ def test_cfg4_1024_tables_2p10_to_2p22_slots():
 OPS = {"insert": 0, "find": 1, "contains": 2, "retrieve": 6}
 
 
-def test_cfg5_serving_64_tables_2p20_to_2p26_slots():
+def test_serving_64_tables_2p20_to_2p26_slots():
     mxco = mx()
     from mxco import workloads
     w = workloads.serving(num_maps=48, num_multimaps=16, slots_log2=(20, 26), num_requests=2000, qps=200000.0,

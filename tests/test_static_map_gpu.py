@@ -42,7 +42,7 @@ def absent_keys(rng, n, hi, present, dtype):
     return np.concatenate(out)[:n]
 
 
-def cfg1_inputs():
+def i32_map_2p20_inputs():
     """Plain StaticMap int32->int32: 2^19 distinct keys in [0, 2^31-2],
     value = key*7+1 in uint32 arithmetic, 2^19 queries half present, shuffled."""
     rng = np.random.default_rng(0)
@@ -55,8 +55,8 @@ def cfg1_inputs():
 
 
 @pytest.mark.parametrize("group", GROUPS)
-def test_cfg1_find_contains_bit_exact(group):
-    keys, vals, q = cfg1_inputs()
+def test_i32_map_2p20_find_contains_bit_exact(group):
+    keys, vals, q = i32_map_2p20_inputs()
     ref = O.OpenAddressing(np.int32, np.int32, 1 << 20, -1, -1)
     assert ref.insert(keys, vals) == 0
     exp_v, exp_c = ref.find(q)
