@@ -509,7 +509,8 @@ MXCO_API mxco_status mxco_server_destroy(mxco_server* server);
 MXCO_API mxco_status mxco_server_add_map(mxco_server* server, uint64_t capacity, uint32_t* table);
 MXCO_API mxco_status mxco_server_add_multimap(mxco_server* server, uint64_t capacity, const void* empty_key,
                                               const void* empty_value, uint32_t* table);
-/* Waits for outstanding requests, then frees the table's HBM. */
+/* Refuses further requests to the table, waits for those already submitted,
+   then frees the table's HBM. */
 MXCO_API mxco_status mxco_server_drop_table(mxco_server* server, uint32_t table);
 /* Queues a request and returns at once. */
 MXCO_API mxco_status mxco_server_submit(mxco_server* server, const mxco_request* request, uint64_t* ticket);

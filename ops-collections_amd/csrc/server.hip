@@ -842,9 +842,16 @@ MXCO_API mxco_status mxco_server_add_multimap(mxco_server* s, uint64_t capacity,
 MXCO_API mxco_status mxco_server_drop_table(mxco_server* s, uint32_t table)
 {
     if (!s) return fail(MXCO_INVALID_ARGUMENT, "server: null argument");
+    {
+        // from here on submits to the table fail, so the drain below also
+        // waits for every request that reached it before
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (table >= s->tables.size() || !s->tables[table].live)
+            return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
+        s->tables[table].live = false;
+    }
     (void)mxco_server_drain(s);
     std::lock_guard<std::mutex> lk(s->mu);
-    if (table >= s->tables.size() || !s->tables[table].live) return fail(MXCO_INVALID_ARGUMENT, "server: unknown table");
     res_quiesce(s);  // hipFree would wait for the resident kernel's idle exit
     Table& t = s->tables[table];
     if (t.map) (void)mxco_oa_destroy(t.map);

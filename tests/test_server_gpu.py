@@ -252,6 +252,29 @@ def test_admission_by_budget():
     s.close()
 
 
+def test_drop_table_waits_for_its_requests_and_refuses_new_ones():
+    """drop_table lets every request already submitted to the table finish,
+    then frees it; later submits to it, and a second drop, fail."""
+    import mxco
+    s = mxco.Server()
+    t = s.add_map(1 << 20)
+    keys = torch.arange(1, 1 << 18, dtype=torch.int64, device="cuda")
+    pairs = mxco.make_pairs(keys, keys * 7 + 1)
+    out = torch.full((keys.numel(),), 5, dtype=torch.int64, device="cuda")
+    ready()
+    tickets = [s.submit(t, mxco.REQ_INSERT, pairs), s.submit(t, mxco.REQ_FIND, keys, out=out)]
+    s.drop_table(t)
+    assert s.wait(tickets[0])[0] == 0
+    s.wait(tickets[1])
+    torch.cuda.synchronize()
+    assert torch.equal(out, keys * 7 + 1)
+    with pytest.raises(Exception, match="unknown table"):
+        s.submit(t, mxco.REQ_FIND, keys, out=out)
+    with pytest.raises(Exception, match="unknown table"):
+        s.drop_table(t)
+    s.close()
+
+
 def test_resident_kernel_serves_many_ticks_per_launch():
     """Closed loop: every StaticMap tick goes through the resident kernel's
     mailbox, one launch serves many ticks, and the results equal the
