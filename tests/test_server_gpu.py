@@ -418,3 +418,45 @@ def test_resident_large_reads_match_closed_form(dt, group):
     for a, b in zip(*runs):
         assert torch.equal(a, b)
 
+
+
+def test_resident_tick_of_many_requests():
+    """A tick of more requests than a resident worker keeps in LDS (256)
+    reads its requests from device memory: 600 small Finds and Contains on
+    two tables give the closed-form results.  They are submitted while an
+    insert of 2^24 keys into a third table runs, so that most of them wait
+    and are served in one tick."""
+    import mxco
+    ready()
+    s = mxco.Server()
+    tabs = [s.add_map(1 << 16), s.add_map(1 << 16)]
+    big = s.add_map(1 << 25)
+    keys = torch.arange(1, 1 << 14, dtype=torch.int64, device="cuda") * 3
+    for t in tabs:
+        assert s.wait(s.submit(t, mxco.REQ_INSERT, mxco.make_pairs(keys, keys * 7 + 1)))[0] == 0
+    bkeys = torch.arange(1, (1 << 24) + 1, dtype=torch.int64, device="cuda")
+    bpairs = mxco.make_pairs(bkeys, bkeys * 7 + 1)
+    g = torch.Generator(device="cuda").manual_seed(2)
+    reqs, checks = [], []
+    for i in range(600):
+        n = 1 + (i * 37) % 50
+        q = keys[torch.randint(0, keys.numel(), (n,), device="cuda", generator=g)] + (i % 2)  # odd i: absent (not a multiple of 3)
+        contains = i % 3 == 0
+        out = torch.full((n,), 5, dtype=torch.uint8 if contains else torch.int64, device="cuda")
+        reqs.append((tabs[i % 2], mxco.REQ_CONTAINS if contains else mxco.REQ_FIND, q.data_ptr(), n, out.data_ptr(), None, 0))
+        checks.append((q, out, contains, i % 2 == 0))
+    ready()
+    ticket = s.submit(big, mxco.REQ_INSERT, bpairs)
+    s.replay(reqs, [0.0] * len(reqs))
+    assert s.wait(ticket)[0] == 0
+    # each request of at most 50 keys is one chunk: a tick's read chunks count its requests
+    largest = int(s.resident_trace()["rd_chunks"].max())
+    s.close()
+    torch.cuda.synchronize()
+    assert largest > 256, largest
+    for q, out, contains, present in checks:
+        if contains:
+            assert bool(out.bool().all()) == present and bool(out.bool().any()) == present
+        else:
+            exp = q * 7 + 1 if present else torch.full_like(q, -1)
+            assert torch.equal(out, exp)
