@@ -369,9 +369,12 @@ def test_resident_kernel_deadline_fails_the_tick():
     s.close()
 
 
-@pytest.mark.parametrize("dt,group", [(torch.int64, 1), (torch.int64, 2), (torch.int32, 1)],
-                         ids=["i64-g1", "i64-g2", "i32-g1"])
-def test_resident_large_reads_match_closed_form(dt, group):
+@pytest.mark.parametrize("dt,vt,group", [(torch.int64, torch.int64, 1), (torch.int64, torch.int64, 2),
+                                         (torch.int64, torch.int64, 4), (torch.int64, torch.int64, 8),
+                                         (torch.int32, torch.int32, 1), (torch.int32, torch.int64, 1),
+                                         (torch.int64, torch.int32, 1)],
+                         ids=["i64-g1", "i64-g2", "i64-g4", "i64-g8", "i32-g1", "i32i64-g1", "i64i32-g1"])
+def test_resident_large_reads_match_closed_form(dt, vt, group):
     """Ticks with millions of read keys: their read chunks (2048 keys) run the
     wave-staged loop in the resident kernel, ragged last chunks of 1971 and
     1000 keys run the staged and the per-lane loop.  One tick holds Find and
@@ -392,14 +395,14 @@ def test_resident_large_reads_match_closed_form(dt, group):
     runs = []
     for resident in (True, False):
         ready()
-        s = mxco.Server(key_dtype=dt, value_dtype=dt, group_size=group, resident=resident)
+        s = mxco.Server(key_dtype=dt, value_dtype=vt, group_size=group, resident=resident)
         t = s.add_map(1 << 23)
-        res, _ = s.wait(s.submit(t, mxco.REQ_INSERT, mxco.make_pairs(keys, keys * 7 + 1)))
+        res, _ = s.wait(s.submit(t, mxco.REQ_INSERT, mxco.make_pairs(keys, (keys * 7 + 1).to(vt))))
         assert res == 0
-        f_out = torch.full((q.numel(),), 5, dtype=dt, device="cuda")
+        f_out = torch.full((q.numel(),), 5, dtype=vt, device="cuda")
         c_out = torch.full((q_c.numel(),), 5, dtype=torch.uint8, device="cuda")
-        n_out = torch.full((new.numel(),), 5, dtype=dt, device="cuda")
-        new_pairs = mxco.make_pairs(new, new * 7 + 1)
+        n_out = torch.full((new.numel(),), 5, dtype=vt, device="cuda")
+        new_pairs = mxco.make_pairs(new, (new * 7 + 1).to(vt))
         ready()
         reqs = [(t, mxco.REQ_FIND, q.data_ptr(), q.numel(), f_out.data_ptr(), None, 0),
                 (t, mxco.REQ_CONTAINS, q_c.data_ptr(), q_c.numel(), c_out.data_ptr(), None, 0)]
@@ -411,9 +414,9 @@ def test_resident_large_reads_match_closed_form(dt, group):
         assert s.resident_stats()[0] == resident
         s.close()
         torch.cuda.synchronize()
-        assert torch.equal(f_out, torch.where(hit, q * 7 + 1, torch.full_like(q, -1)))
+        assert torch.equal(f_out, torch.where(hit, q * 7 + 1, torch.full_like(q, -1)).to(vt))
         assert torch.equal(c_out.bool(), hit[:nq - 77])
-        assert torch.equal(n_out, new * 7 + 1)
+        assert torch.equal(n_out, (new * 7 + 1).to(vt))
         runs.append((f_out.cpu(), c_out.cpu(), n_out.cpu()))
     for a, b in zip(*runs):
         assert torch.equal(a, b)
