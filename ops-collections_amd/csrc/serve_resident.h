@@ -24,8 +24,12 @@
 // returned names the epoch and its chunk count, so a claim is valid by
 // itself (a claim past the count takes nothing; there is at most one such
 // claim per workgroup and epoch, so the next-chunk field never overflows).
-// A worker finds a chunk's request by binary search over the requests'
-// first-chunk indices, held in its LDS for the epoch.  Insert chunks, and
+// While the tick has more chunks left than there are workers, a worker
+// issues its next claim when a chunk starts, so the claim's round trip is
+// hidden by the chunk's work.  A worker finds a chunk's request by binary
+// search over the requests' first-chunk indices, held in its LDS for the
+// epoch with the requests themselves (up to kServeReqCache; each carries its
+// table's slot array, capacity and bucket count).  Insert chunks, and
 // read chunks of fewer than kServeStagedMin keys, run the probe state
 // machine of the launched path (oa_process) on the chunk's keys.  Larger read
 // chunks run the bulk Find's wave-staged loop (wave_staged_find): each
@@ -46,8 +50,8 @@
 // copies the insert counters to the host, stores the tick's number into the
 // mailbox's done word of the slot and marks the epoch finished for the
 // leader.  A worker acquires at system scope when it first claims a chunk of
-// a new epoch, so request inputs and table descriptors written before the
-// tick was posted are read fresh.
+// a new epoch, so request inputs written before the tick was posted are
+// read fresh.
 //
 // Lifetime.  Every wait ends: workers wait only for chunks that running
 // workgroups have claimed, or for the leader's exit flag; the leader sets
@@ -55,8 +59,9 @@
 // no tick has come for `idle` clock ticks, or when the launch is older than
 // `life` (or has opened kServeMaxEpochs ticks), and in the middle of a tick
 // only when the tick has been open for `hard` clock ticks (it then reports
-// the tick as aborted).  The host relaunches the kernel whenever a posted
-// tick is unfinished and the kernel has ended.
+// the tick as aborted, by its number).  A worker that saw the exit flag
+// when it claimed ahead drops that claim.  The host relaunches the kernel
+// whenever a posted tick is unfinished and the kernel has ended.
 #pragma once
 #include <hip/hip_runtime.h>
 
