@@ -39,6 +39,7 @@ struct mxco_mm {
     uint64_t empty_key, empty_val;
     // scratch, grown on demand
     uint32_t* status;
+    unsigned long long* counters;  // failure counters of the lane-group insert (oa_kernel)
     uint64_t* host;  // pinned, 4 words
     void* buf[6];
     size_t buf_size[6];
@@ -143,6 +144,39 @@ __global__ __launch_bounds__(256) void mm_insert_kernel(uint8_t* table, uint64_t
         }
         if (!done) atomicOr(status, 2u);
     }
+}
+
+// The multimap's walk as the map probe kernel's policy: linear probing from
+// fmix64 of the sign-extended key (bucket 1 over the power-of-two table), and
+// no key compares equal, so every pair takes the first empty slot of its walk.
+struct MmFns {
+    template <int KS>
+    static __device__ __forceinline__ uint32_t hash1(const OaArgs&, uint64_t key)
+    {
+        return (uint32_t)fmix64(sext<KS>(key));
+    }
+    template <int KS>
+    static __device__ __forceinline__ uint32_t hash2(const OaArgs&, uint64_t)
+    {
+        return 0u;
+    }
+    template <int KS>
+    static __device__ __forceinline__ bool equal(const OaArgs&, uint64_t, uint64_t)
+    {
+        return false;
+    }
+    template <int KS>
+    static __device__ __forceinline__ bool sentinel(uint64_t key, uint64_t sentinel_key)
+    {
+        return key == sentinel_key;
+    }
+};
+
+// Status bit 2 when the lane-group insert left a pair out (its walk found no
+// empty slot).
+__global__ void mm_counters_status_kernel(const unsigned long long* counters, uint32_t* status)
+{
+    if (__ballot(counters[2 * threadIdx.x] != 0) && threadIdx.x == 0) atomicOr(status, 2u);
 }
 
 enum { Q_CONTAINS = 0, Q_FIND = 1, Q_COUNT = 2, Q_GATHER = 3 };
@@ -457,6 +491,7 @@ MXCO_API mxco_status mxco_mm_create(uint32_t key_size, uint32_t value_size, uint
     memcpy(&m->empty_val, empty_value, value_size);
     hipError_t e = hipMalloc((void**)&m->table, p * m->ss);
     if (e == hipSuccess) e = hipMalloc((void**)&m->status, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->counters, 2 * kCounterSlots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->host, 4 * sizeof(uint64_t), 0);
     mxco_status st = e == hipSuccess ? clear_table(m, stream) : hip_fail(e, "StaticMultimap: allocation");
     if (st != MXCO_OK) {
@@ -475,6 +510,7 @@ MXCO_API mxco_status mxco_mm_destroy(mxco_mm* m)
     if (!m) return ok();
     if (m->table) (void)hipFree(m->table);
     if (m->status) (void)hipFree(m->status);
+    if (m->counters) (void)hipFree(m->counters);
     if (m->host) (void)hipHostFree(m->host);
     for (int i = 0; i < 6; ++i)
         if (m->buf[i]) (void)hipFree(m->buf[i]);
@@ -544,12 +580,35 @@ MXCO_API mxco_status mxco_mm_insert(mxco_mm* m, const void* pairs, const mxco_st
         return fail(MXCO_INVALID_ARGUMENT, "StaticMultimap: selected input contains a reserved key or value");
     const uint64_t room = m->capacity - m->size;
     const uint64_t accepted = selected < room ? selected : room;
-    if (accepted) {
+    if (accepted && !ranks && m->slots_pow2 <= (1ull << 31)) {
+        // the first `accepted` pairs: the map probe kernel with lane groups of 4
+        // (one 128-byte line per window), a walk that never finds its key
+        OaArgs a;
+        memset(&a, 0, sizeof(a));
+        a.table = m->table;
+        a.in = (const uint8_t*)pairs;
+        a.counters = m->counters;
+        a.n = accepted;
+        a.empty_key = m->empty_key;
+        a.empty_val = m->empty_val;
+        a.erased_key = m->empty_key;
+        a.cap = (uint32_t)m->slots_pow2;
+        a.bucket = 1;
+        a.nbuckets = (uint32_t)m->slots_pow2;
+        zero_counters_kernel<<<1, 128, 0, s>>>(m->counters, 2 * kCounterSlots);
+        const unsigned ggrid = grid_for(accepted * 4, 256);
+#define INS(K, V) oa_kernel<K, V, 4, OP_INSERT, 1, MmFns><<<ggrid, 256, 0, s>>>(a)
+        MM_DISPATCH(m, INS);
+#undef INS
+        mm_counters_status_kernel<<<1, kCounterSlots, 0, s>>>(m->counters, m->status);
+    } else if (accepted) {
 #define INS(K, V)                                                                                                  \
     mm_insert_kernel<K, V><<<grid, 256, 0, s>>>(m->table, m->slots_pow2 - 1, (const uint8_t*)pairs, sd, esz, pred, \
                                                 ranks, n, accepted, m->empty_key, m->empty_val, m->status)
         MM_DISPATCH(m, INS);
 #undef INS
+    }
+    if (accepted) {
         MXCO_HIP(hipGetLastError(), "StaticMultimap: insert");
         MXCO_HIP(hipMemcpyAsync(m->host + 1, m->status, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "status");
         MXCO_HIP(hipStreamSynchronize(s), "StaticMultimap: insert");
